@@ -1,0 +1,748 @@
+// cfx_api.hip -- the C ABI (include/cfx.h) over the device objects.
+#include "../../include/cfx.h"
+
+#include "dev/fvops.h"
+#include "dev/krylov.h"
+#include "dev/mesh.h"
+
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+
+using namespace cfx;
+
+struct cfx_ctx
+{
+    std::unique_ptr<Context> ctx;
+    std::string err;
+    std::map<int, std::unique_ptr<Krylov>> krylov;  // workspace per system size
+
+    Krylov& solver(int n, int rowBlocks)
+    {
+        auto& k = krylov[n];
+        // a workspace is sized for a row-block count; grow it for a matrix
+        // of the same size with more CSR-stream blocks
+        if (!k || k->maxBlocks() < rowBlocks) k = std::make_unique<Krylov>(*ctx, n, std::max(rowBlocks, 1));
+        return *k;
+    }
+};
+
+struct cfx_mesh
+{
+    cfx_ctx* owner = nullptr;
+    std::unique_ptr<DevMesh> m;
+    DevBuf faceTmp, gradTmp, cellTmp, result;
+};
+
+struct cfx_matrix
+{
+    cfx_mesh* mesh = nullptr;
+    int ncomp = 1;
+    DevBuf diag, rhs, rhsBase, upper, lower, val, val32, tmp;
+    Ldu ldu() const
+    {
+        Ldu A;
+        A.ncomp = ncomp;
+        A.diag = diag.as<double>();
+        A.rhs = rhs.as<double>();
+        A.upper = upper.as<double>();
+        A.lower = lower.as<double>();
+        return A;
+    }
+};
+
+struct cfx_csr
+{
+    cfx_ctx* owner = nullptr;
+    DevPattern pattern;
+    DevBuf val, val32, b, x, resHist;
+    double innerTol = 1e-4;
+    int lastIters = -1;
+    bool val32Valid = false;
+};
+
+namespace
+{
+
+int fail(cfx_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+template <typename F>
+int guard(cfx_ctx* c, F&& f)
+{
+    if (!c) return CFX_E_INVALID;
+    try
+    {
+        CFX_HIP(hipSetDevice(c->ctx->device()));
+        f();
+        return CFX_OK;
+    }
+    catch (const HipError& e)
+    {
+        return fail(c, CFX_E_HIP, e.what());
+    }
+    catch (const std::invalid_argument& e)
+    {
+        return fail(c, CFX_E_INVALID, e.what());
+    }
+    catch (const std::exception& e)
+    {
+        return fail(c, CFX_E_INTERNAL, e.what());
+    }
+}
+
+void toReport(const SolveReport& s, cfx_report* r)
+{
+    if (!r) return;
+    r->iterations = s.iterations;
+    r->initialResidual = s.initialResidual;
+    r->finalResidual = s.finalResidual;
+    r->converged = s.converged;
+    r->breakdown = s.breakdown;
+    r->l2Residual = s.l2Residual;
+    r->trueResidual = s.trueResidual;
+    r->outerIterations = s.outerIterations;
+    r->precondFallbacks = s.precondFallbacks;
+    r->ms = s.ms;
+}
+
+void need(bool ok, const char* what)
+{
+    if (!ok) throw std::invalid_argument(what);
+}
+
+__global__ void k_zero_bytes(unsigned char* p, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+
+__global__ void k_explicit_op(int n, const double* __restrict__ rhs, const double* __restrict__ ax,
+                              const double* __restrict__ diag, const double* __restrict__ x,
+                              double* __restrict__ h)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) h[i] = rhs[i] - (ax[i] - diag[i] * x[i]);
+}
+
+__global__ void k_add_vec(int n, const double* __restrict__ v, double* __restrict__ y)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += v[i];
+}
+
+SolveReport solveOn(cfx_ctx* c, const CsrView& A, const double* val, DevBuf& val32, bool& val32Valid,
+                    const double* b, double* x, double tol, int maxIter, int solver, int precond,
+                    int precision, double innerTol, double* resHist)
+{
+    need(solver == CFX_SOLVER_CG || solver == CFX_SOLVER_BICGSTAB, "unknown solver");
+    need(precision == CFX_FP64 || precision == CFX_FP32_IR, "unknown precision");
+    need(precond == CFX_PRECOND_JACOBI, "preconditioner not available in this build");
+    Krylov& K = c->solver(A.n, A.nRowBlocks);
+    if (solver == CFX_SOLVER_BICGSTAB)
+    {
+        need(precision == CFX_FP64, "BiCGStab runs in FP64");
+        return K.bicgstab(A, val, b, x, tol, maxIter);
+    }
+    if (precision == CFX_FP64) return K.cg(A, val, b, x, tol, maxIter, resHist);
+    if (!val32Valid)
+    {
+        if (!val32.p || val32.bytes < sizeof(float) * A.nnz) val32.alloc(sizeof(float) * A.nnz);
+        castToFloat(*c->ctx, val, val32.as<float>(), A.nnz);
+        val32Valid = true;
+    }
+    return K.cgMixed(A, val, val32.as<float>(), b, x, tol, maxIter, innerTol, false);
+}
+
+} // namespace
+
+extern "C" {
+
+int cfx_create(int device, cfx_ctx** out)
+{
+    if (!out) return CFX_E_INVALID;
+    *out = nullptr;
+    auto* c = new cfx_ctx;
+    try
+    {
+        c->ctx = std::make_unique<Context>(device);
+    }
+    catch (const std::exception& e)
+    {
+        delete c;
+        return CFX_E_HIP;
+    }
+    *out = c;
+    return CFX_OK;
+}
+
+int cfx_destroy(cfx_ctx* c)
+{
+    if (!c) return CFX_E_INVALID;
+    try
+    {
+        c->krylov.clear();
+        c->ctx.reset();
+    }
+    catch (...)
+    {
+    }
+    delete c;
+    return CFX_OK;
+}
+
+const char* cfx_last_error(const cfx_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int cfx_synchronize(cfx_ctx* c)
+{
+    return guard(c, [&] { c->ctx->sync(); });
+}
+
+int cfx_device_info(cfx_ctx* c, char* name, size_t len, int* cus)
+{
+    return guard(c, [&] {
+        if (name && len) { std::strncpy(name, c->ctx->deviceName.c_str(), len - 1); name[len - 1] = 0; }
+        if (cus) *cus = c->ctx->numCUs();
+    });
+}
+
+int cfx_malloc(cfx_ctx* c, size_t bytes, void** out)
+{
+    return guard(c, [&] {
+        need(out != nullptr, "null out");
+        CFX_HIP(hipMalloc(out, bytes ? bytes : 1));
+    });
+}
+
+int cfx_free(cfx_ctx* c, void* p)
+{
+    return guard(c, [&] {
+        c->ctx->sync();
+        CFX_HIP(hipFree(p));
+    });
+}
+
+int cfx_memcpy_htod(cfx_ctx* c, void* dst, const void* src, size_t bytes)
+{
+    return guard(c, [&] {
+        CFX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->ctx->stream()));
+        c->ctx->sync();
+    });
+}
+
+int cfx_memcpy_dtoh(cfx_ctx* c, void* dst, const void* src, size_t bytes)
+{
+    return guard(c, [&] {
+        CFX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->ctx->stream()));
+        c->ctx->sync();
+    });
+}
+
+int cfx_memcpy_dtod(cfx_ctx* c, void* dst, const void* src, size_t bytes)
+{
+    return guard(c, [&] { CFX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->ctx->stream())); });
+}
+
+int cfx_zero(cfx_ctx* c, void* p, size_t bytes)
+{
+    return guard(c, [&] {
+        if (!bytes) return;
+        k_zero_bytes<<<std::max(1, std::min(4096, divUp((long long)bytes, 256))), 256, 0, c->ctx->stream()>>>(
+            static_cast<unsigned char*>(p), bytes);
+        CFX_CHECK_LAUNCH();
+    });
+}
+
+// ----------------------------------------------------------------- mesh
+int cfx_mesh_create(cfx_ctx* c, int64_t nCells, int64_t nInternal, int64_t nFaces, const int64_t* owner,
+                    const int64_t* neighbour, const double* Sf, const double* Cf, const double* C,
+                    const double* V, const double* d, const double* magSf, int64_t nPatches,
+                    const int64_t* patchStart, const int64_t* patchSize, cfx_mesh** out)
+{
+    return guard(c, [&] {
+        need(out && owner && Sf && Cf && C && V && d && magSf, "null mesh array");
+        need(nInternal == 0 || neighbour, "null neighbour");
+        MeshDesc desc{nCells, nInternal, nFaces, reinterpret_cast<const long long*>(owner),
+                      reinterpret_cast<const long long*>(neighbour), Sf, Cf, C, V, d, magSf, nPatches,
+                      reinterpret_cast<const long long*>(patchStart), reinterpret_cast<const long long*>(patchSize)};
+        auto* m = new cfx_mesh;
+        m->owner = c;
+        try
+        {
+            m->m = createDevMesh(*c->ctx, desc);
+            m->faceTmp.alloc(sizeof(double) * nFaces);
+            m->gradTmp.alloc(sizeof(double) * 3 * nCells);
+            m->cellTmp.alloc(sizeof(double) * nCells);
+            m->result.alloc(256);
+            CFX_HIP(hipMemsetAsync(m->result.p, 0, 256, c->ctx->stream()));
+            c->ctx->sync();
+        }
+        catch (...)
+        {
+            delete m;
+            throw;
+        }
+        *out = m;
+    });
+}
+
+int cfx_mesh_destroy(cfx_mesh* m)
+{
+    if (!m) return CFX_E_INVALID;
+    cfx_ctx* c = m->owner;
+    return guard(c, [&] {
+        c->ctx->sync();
+        delete m;
+    });
+}
+
+int cfx_mesh_sizes(cfx_mesh* m, int64_t s[5])
+{
+    if (!m) return CFX_E_INVALID;
+    return guard(m->owner, [&] {
+        s[0] = m->m->n;
+        s[1] = m->m->nI;
+        s[2] = m->m->nF;
+        s[3] = m->m->pattern.nnz;
+        s[4] = m->m->pattern.nRowBlocks;
+    });
+}
+
+int cfx_mesh_pattern(cfx_mesh* m, int32_t* rowPtr, int32_t* cols)
+{
+    if (!m) return CFX_E_INVALID;
+    return guard(m->owner, [&] {
+        const auto& p = m->m->pattern;
+        if (rowPtr) std::memcpy(rowPtr, p.hRowPtr.data(), sizeof(int) * p.hRowPtr.size());
+        if (cols) std::memcpy(cols, p.hCol.data(), sizeof(int) * p.hCol.size());
+    });
+}
+
+int cfx_owner_weights(cfx_mesh* m, double* d_w)
+{
+    if (!m) return CFX_E_INVALID;
+    return guard(m->owner, [&] {
+        if (m->m->nI)
+            CFX_HIP(hipMemcpyAsync(d_w, m->m->ow.p, sizeof(double) * m->m->nI, hipMemcpyDeviceToDevice,
+                                   m->owner->ctx->stream()));
+    });
+}
+
+// ------------------------------------------------------------ field ops
+int cfx_grad(cfx_mesh* m, const double* x, const double* linc, const double* link, int ncomp, int comp,
+             int corrected, int quirk, double* g)
+{
+    if (!m) return CFX_E_INVALID;
+    return guard(m->owner, [&] {
+        need(ncomp >= 1 && comp >= 0 && comp < ncomp, "bad component");
+        gradGauss(*m->m, x, linc, link, ncomp, comp, corrected != 0, quirk != 0, g, m->faceTmp.as<double>());
+    });
+}
+
+int cfx_harmonic_gamma_uniform(cfx_mesh* m, double gamma, double* gf)
+{
+    if (!m) return CFX_E_INVALID;
+    return guard(m->owner, [&] { harmonicGammaUniform(*m->m, gamma, gf); });
+}
+
+int cfx_interpolate_to_faces(cfx_mesh* m, const double* cell, double* face)
+{
+    if (!m) return CFX_E_INVALID;
+    return guard(m->owner, [&] { interpolateToFaces(*m->m, cell, face); });
+}
+
+int cfx_update_phi_internal(cfx_mesh* m, const double* U3, double* phi)
+{
+    if (!m) return CFX_E_INVALID;
+    return guard(m->owner, [&] { updatePhiInternal(*m->m, U3, phi); });
+}
+
+int cfx_correct_boundary_flux(cfx_mesh* m, const double* U3, const double* linc, const double* link, double* phi)
+{
+    if (!m) return CFX_E_INVALID;
+    return guard(m->owner, [&] { correctBoundaryFlux(*m->m, U3, linc, link, phi); });
+}
+
+int cfx_correct_fluxes(cfx_mesh* m, const double* p, const double* g, const double* rAUf, const double* linc,
+                       const double* link, int corrected, double* phi)
+{
+    if (!m) return CFX_E_INVALID;
+    return guard(m->owner, [&] { correctFluxes(*m->m, p, g, rAUf, linc, link, corrected != 0, phi); });
+}
+
+int cfx_div_phi(cfx_mesh* m, const double* phi, double* out)
+{
+    if (!m) return CFX_E_INVALID;
+    return guard(m->owner, [&] { divPhi(*m->m, phi, out); });
+}
+
+int cfx_continuity_error(cfx_mesh* m, const double* phi, double* err)
+{
+    if (!m) return CFX_E_INVALID;
+    return guard(m->owner, [&] {
+        double* r = m->result.as<double>();
+        continuityError(*m->m, phi, m->cellTmp.as<double>(), r, m->owner->ctx->partials(0));
+        CFX_HIP(hipMemcpyAsync(err, r, sizeof(double), hipMemcpyDeviceToHost, m->owner->ctx->stream()));
+        m->owner->ctx->sync();
+    });
+}
+
+// ------------------------------------------------------------- fvMatrix
+int cfx_matrix_create(cfx_mesh* m, int ncomp, cfx_matrix** out)
+{
+    if (!m) return CFX_E_INVALID;
+    return guard(m->owner, [&] {
+        need(out && ncomp >= 1 && ncomp <= 3, "bad matrix arguments");
+        auto* A = new cfx_matrix;
+        A->mesh = m;
+        A->ncomp = ncomp;
+        const std::size_t n = m->m->n, nI = std::max(1, m->m->nI), nnz = m->m->pattern.nnz;
+        A->diag.alloc(sizeof(double) * ncomp * n);
+        A->rhs.alloc(sizeof(double) * ncomp * n);
+        A->rhsBase.alloc(sizeof(double) * ncomp * n);
+        A->tmp.alloc(sizeof(double) * n);
+        A->upper.alloc(sizeof(double) * nI);
+        A->lower.alloc(sizeof(double) * nI);
+        A->val.alloc(sizeof(double) * ncomp * nnz);
+        lduReset(*m->m, A->ldu());
+        *out = A;
+    });
+}
+
+int cfx_matrix_destroy(cfx_matrix* A)
+{
+    if (!A) return CFX_E_INVALID;
+    return guard(A->mesh->owner, [&] {
+        A->mesh->owner->ctx->sync();
+        delete A;
+    });
+}
+
+int cfx_matrix_reset(cfx_matrix* A)
+{
+    if (!A) return CFX_E_INVALID;
+    return guard(A->mesh->owner, [&] { lduReset(*A->mesh->m, A->ldu()); });
+}
+
+int cfx_assemble_laplacian(cfx_matrix* A, const double* gf, const double* linc, const double* link, int sign)
+{
+    if (!A) return CFX_E_INVALID;
+    return guard(A->mesh->owner, [&] { assembleLaplacian(*A->mesh->m, A->ldu(), gf, linc, link, sign); });
+}
+
+int cfx_assemble_div(cfx_matrix* A, const double* phi, int scheme, double blend, const double* linc,
+                     const double* link, int sign)
+{
+    if (!A) return CFX_E_INVALID;
+    return guard(A->mesh->owner, [&] {
+        need(scheme >= 0 && scheme <= 2, "div scheme must be 0 upwind, 1 linear, 2 blend");
+        assembleDiv(*A->mesh->m, A->ldu(), phi, scheme, blend, linc, link, sign);
+    });
+}
+
+int cfx_assemble_ddt(cfx_matrix* A, const double* old, double dt, int sign)
+{
+    if (!A) return CFX_E_INVALID;
+    return guard(A->mesh->owner, [&] {
+        need(dt > 0, "fvm::ddt: non-positive time step");
+        assembleDdt(*A->mesh->m, A->ldu(), old, dt, sign);
+    });
+}
+
+int cfx_matrix_add_rhs(cfx_matrix* A, int comp, const double* v)
+{
+    if (!A) return CFX_E_INVALID;
+    return guard(A->mesh->owner, [&] {
+        need(comp >= 0 && comp < A->ncomp, "bad component");
+        const int n = A->mesh->m->n;
+        k_add_vec<<<divUp(n, 256), 256, 0, A->mesh->owner->ctx->stream()>>>(n, v, A->rhs.as<double>() + (long long)comp * n);
+        CFX_CHECK_LAUNCH();
+    });
+}
+
+int cfx_matrix_get(cfx_matrix* A, double* diag, double* rhs, double* upper, double* lower)
+{
+    if (!A) return CFX_E_INVALID;
+    return guard(A->mesh->owner, [&] {
+        hipStream_t s = A->mesh->owner->ctx->stream();
+        const std::size_t n = A->mesh->m->n, nI = A->mesh->m->nI;
+        if (diag) CFX_HIP(hipMemcpyAsync(diag, A->diag.p, sizeof(double) * A->ncomp * n, hipMemcpyDeviceToHost, s));
+        if (rhs) CFX_HIP(hipMemcpyAsync(rhs, A->rhs.p, sizeof(double) * A->ncomp * n, hipMemcpyDeviceToHost, s));
+        if (upper && nI) CFX_HIP(hipMemcpyAsync(upper, A->upper.p, sizeof(double) * nI, hipMemcpyDeviceToHost, s));
+        if (lower && nI) CFX_HIP(hipMemcpyAsync(lower, A->lower.p, sizeof(double) * nI, hipMemcpyDeviceToHost, s));
+        A->mesh->owner->ctx->sync();
+    });
+}
+
+int cfx_matrix_device_ptrs(cfx_matrix* A, double** d_diag, double** d_rhs)
+{
+    if (!A) return CFX_E_INVALID;
+    return guard(A->mesh->owner, [&] {
+        if (d_diag) *d_diag = A->diag.as<double>();
+        if (d_rhs) *d_rhs = A->rhs.as<double>();
+    });
+}
+
+int cfx_nonorth_correct(cfx_mesh* m, const double* gf, const double* x, const double* linc, const double* link,
+                        int ncomp, int comp, int gradCorrected, int sign, double* rhs)
+{
+    if (!m) return CFX_E_INVALID;
+    return guard(m->owner, [&] {
+        nonOrthCorrect(*m->m, gf, x, linc, link, ncomp, comp, gradCorrected != 0, sign, rhs,
+                       m->gradTmp.as<double>(), m->faceTmp.as<double>());
+    });
+}
+
+int cfx_matrix_solve(cfx_matrix* A, double* x, int solver, double tol, int maxIter, int nonOrth,
+                     const double* gf, const double* linc, const double* link, int sign, int gradCorrected,
+                     int precond, int precision, cfx_report* rep)
+{
+    if (!A) return CFX_E_INVALID;
+    cfx_ctx* c = A->mesh->owner;
+    return guard(c, [&] {
+        DevMesh& m = *A->mesh->m;
+        hipStream_t s = c->ctx->stream();
+        const int n = m.n, nnz = m.pattern.nnz;
+        const std::size_t bytes = sizeof(double) * A->ncomp * n;
+        for (int k = 0; k < A->ncomp; ++k) lduToCsr(m, A->ldu(), k, A->val.as<double>() + (long long)k * nnz);
+        CFX_HIP(hipMemcpyAsync(A->rhsBase.p, A->rhs.p, bytes, hipMemcpyDeviceToDevice, s));
+        const int passes = gf ? std::max(1, nonOrth + 1) : 1;
+        const CsrView view = m.pattern.view();
+        SolveReport last;
+        bool v32 = false;
+        for (int pass = 0; pass < passes; ++pass)
+        {
+            CFX_HIP(hipMemcpyAsync(A->rhs.p, A->rhsBase.p, bytes, hipMemcpyDeviceToDevice, s));
+            if (gf)
+                for (int k = 0; k < A->ncomp; ++k)
+                    nonOrthCorrect(m, gf, x + (long long)k * n, linc, link, A->ncomp, k, gradCorrected != 0, sign,
+                                   A->rhs.as<double>() + (long long)k * n, A->mesh->gradTmp.as<double>(),
+                                   A->mesh->faceTmp.as<double>());
+            for (int k = 0; k < A->ncomp; ++k)
+            {
+                // each component has its own values: refresh the FP32 copy per component
+                v32 = false;
+                DevBuf& v32buf = A->val32;
+                last = solveOn(c, view, A->val.as<double>() + (long long)k * nnz, v32buf, v32,
+                               A->rhs.as<double>() + (long long)k * n, x + (long long)k * n, tol, maxIter, solver,
+                               precond, precision, 1e-4, nullptr);
+            }
+        }
+        toReport(last, rep);
+    });
+}
+
+int cfx_matrix_explicit_operator(cfx_matrix* A, const double* x, int comp, double* h)
+{
+    if (!A) return CFX_E_INVALID;
+    cfx_ctx* c = A->mesh->owner;
+    return guard(c, [&] {
+        DevMesh& m = *A->mesh->m;
+        need(comp >= 0 && comp < A->ncomp, "bad component");
+        Krylov& K = c->solver(m.n, m.pattern.nRowBlocks);
+        K.spmv(m.pattern.view(), A->val.as<double>() + (long long)comp * m.pattern.nnz, x, A->tmp.as<double>());
+        k_explicit_op<<<divUp(m.n, 256), 256, 0, c->ctx->stream()>>>(
+            m.n, A->rhs.as<double>() + (long long)comp * m.n, A->tmp.as<double>(),
+            A->diag.as<double>() + (long long)comp * m.n, x, h);
+        CFX_CHECK_LAUNCH();
+    });
+}
+
+int cfx_matrix_spmv(cfx_matrix* A, int comp, const double* x, double* y)
+{
+    if (!A) return CFX_E_INVALID;
+    cfx_ctx* c = A->mesh->owner;
+    return guard(c, [&] {
+        DevMesh& m = *A->mesh->m;
+        need(comp >= 0 && comp < A->ncomp, "bad component");
+        c->solver(m.n, m.pattern.nRowBlocks).spmv(m.pattern.view(), A->val.as<double>() + (long long)comp * m.pattern.nnz, x, y);
+    });
+}
+
+// ------------------------------------------------------ linear-solver seam
+int cfx_csr_create(cfx_ctx* c, int64_t n, const int64_t* rowStart, const int64_t* cols, const double* vals,
+                   cfx_csr** out)
+{
+    return guard(c, [&] {
+        need(out && rowStart && cols && vals && n > 0, "bad CSR arguments");
+        need(n < (1LL << 30) && rowStart[n] < (1LL << 31) - 1, "CSR too large for int32 indices");
+        std::vector<int> rp(n + 1), cl(rowStart[n]);
+        for (int64_t i = 0; i <= n; ++i) rp[i] = static_cast<int>(rowStart[i]);
+        for (int64_t r = 0; r < n; ++r)
+        {
+            need(rowStart[r + 1] >= rowStart[r], "rowStart not monotone");
+            for (int64_t k = rowStart[r]; k < rowStart[r + 1]; ++k)
+            {
+                need(cols[k] >= 0 && cols[k] < n, "column out of range");
+                need(k == rowStart[r] || cols[k] > cols[k - 1], "columns not sorted within a row");
+                cl[k] = static_cast<int>(cols[k]);
+            }
+        }
+        auto* A = new cfx_csr;
+        A->owner = c;
+        try
+        {
+            A->pattern.build(std::move(rp), std::move(cl), c->ctx->stream());
+            A->val.alloc(sizeof(double) * std::max(1, A->pattern.nnz));
+            A->b.alloc(sizeof(double) * n);
+            A->x.alloc(sizeof(double) * n);
+            CFX_HIP(hipMemcpyAsync(A->val.p, vals, sizeof(double) * A->pattern.nnz, hipMemcpyHostToDevice,
+                                   c->ctx->stream()));
+            c->ctx->sync();
+        }
+        catch (...)
+        {
+            delete A;
+            throw;
+        }
+        *out = A;
+    });
+}
+
+int cfx_csr_destroy(cfx_csr* A)
+{
+    if (!A) return CFX_E_INVALID;
+    return guard(A->owner, [&] {
+        A->owner->ctx->sync();
+        delete A;
+    });
+}
+
+int cfx_csr_set_values(cfx_csr* A, const double* vals)
+{
+    if (!A) return CFX_E_INVALID;
+    return guard(A->owner, [&] {
+        CFX_HIP(hipMemcpyAsync(A->val.p, vals, sizeof(double) * A->pattern.nnz, hipMemcpyHostToDevice,
+                               A->owner->ctx->stream()));
+        A->val32Valid = false;
+        A->owner->ctx->sync();
+    });
+}
+
+int cfx_csr_spmv(cfx_csr* A, const double* x, double* y)
+{
+    if (!A) return CFX_E_INVALID;
+    return guard(A->owner, [&] {
+        A->owner->solver(A->pattern.n, A->pattern.nRowBlocks).spmv(A->pattern.view(), A->val.as<double>(), x, y);
+    });
+}
+
+int cfx_csr_solve_device(cfx_csr* A, const double* b, double* x, double tol, int maxIter, int solver, int precond,
+                         int precision, cfx_report* rep)
+{
+    if (!A) return CFX_E_INVALID;
+    return guard(A->owner, [&] {
+        need(maxIter >= 0, "maxIter < 0");
+        double* hist = nullptr;
+        if (solver == CFX_SOLVER_CG && precision == CFX_FP64)
+        {
+            if (!A->resHist.p || A->resHist.bytes < sizeof(double) * (maxIter + 1))
+                A->resHist.alloc(sizeof(double) * (maxIter + 1));
+            hist = A->resHist.as<double>();
+        }
+        const SolveReport s = solveOn(A->owner, A->pattern.view(), A->val.as<double>(), A->val32, A->val32Valid, b, x,
+                                      tol, maxIter, solver, precond, precision, A->innerTol, hist);
+        A->lastIters = hist ? s.iterations : -1;
+        toReport(s, rep);
+    });
+}
+
+int cfx_csr_solve(cfx_csr* A, const double* b, double* x, double tol, int maxIter, int solver, int precond,
+                  int precision, cfx_report* rep)
+{
+    if (!A) return CFX_E_INVALID;
+    const std::size_t bytes = sizeof(double) * A->pattern.n;
+    int rc = guard(A->owner, [&] {
+        CFX_HIP(hipMemcpyAsync(A->b.p, b, bytes, hipMemcpyHostToDevice, A->owner->ctx->stream()));
+        CFX_HIP(hipMemcpyAsync(A->x.p, x, bytes, hipMemcpyHostToDevice, A->owner->ctx->stream()));
+    });
+    if (rc) return rc;
+    rc = cfx_csr_solve_device(A, A->b.as<double>(), A->x.as<double>(), tol, maxIter, solver, precond, precision, rep);
+    if (rc) return rc;
+    return guard(A->owner, [&] {
+        CFX_HIP(hipMemcpyAsync(x, A->x.p, bytes, hipMemcpyDeviceToHost, A->owner->ctx->stream()));
+        A->owner->ctx->sync();
+    });
+}
+
+int cfx_csr_residual_history(cfx_csr* A, double* hist, int maxLen)
+{
+    if (!A) return CFX_E_INVALID;
+    return guard(A->owner, [&] {
+        need(A->lastIters >= 0, "no FP64 CG history recorded");
+        const int len = std::min(maxLen, A->lastIters + 1);
+        // entry 0 is the initial residual: filled from the report path
+        CFX_HIP(hipMemcpyAsync(hist, A->resHist.p, sizeof(double) * len, hipMemcpyDeviceToHost, A->owner->ctx->stream()));
+        A->owner->ctx->sync();
+    });
+}
+
+int cfx_csr_set_inner_tol(cfx_csr* A, double t)
+{
+    if (!A) return CFX_E_INVALID;
+    return guard(A->owner, [&] {
+        need(t > 0 && t < 1, "inner tolerance must be in (0,1)");
+        A->innerTol = t;
+    });
+}
+
+namespace
+{
+double timeSpmv(cfx_ctx* c, const CsrView& A, const double* val, DevBuf& val32, bool& v32ok, const void* x, void* y,
+                int reps, int precision)
+{
+    need(reps > 0, "reps must be positive");
+    Krylov& K = c->solver(A.n, A.nRowBlocks);
+    hipStream_t s = c->ctx->stream();
+    if (precision == CFX_FP32_IR && !v32ok)
+    {
+        if (!val32.p || val32.bytes < sizeof(float) * A.nnz) val32.alloc(sizeof(float) * A.nnz);
+        castToFloat(*c->ctx, val, val32.as<float>(), A.nnz);
+        v32ok = true;
+    }
+    hipEvent_t e0, e1;
+    CFX_HIP(hipEventCreate(&e0));
+    CFX_HIP(hipEventCreate(&e1));
+    CFX_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < reps; ++i)
+    {
+        if (precision == CFX_FP32_IR)
+            K.spmv32(A, val32.as<float>(), static_cast<const float*>(x), static_cast<float*>(y));
+        else
+            K.spmv(A, val, static_cast<const double*>(x), static_cast<double*>(y));
+    }
+    CFX_HIP(hipEventRecord(e1, s));
+    CFX_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    CFX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms / reps;
+}
+} // namespace
+
+int cfx_csr_time_spmv(cfx_csr* A, const void* x, void* y, int reps, int precision, double* ms)
+{
+    if (!A || !ms) return CFX_E_INVALID;
+    return guard(A->owner, [&] {
+        *ms = timeSpmv(A->owner, A->pattern.view(), A->val.as<double>(), A->val32, A->val32Valid, x, y, reps,
+                       precision);
+    });
+}
+
+int cfx_matrix_time_spmv(cfx_matrix* A, int comp, const void* x, void* y, int reps, int precision, double* ms)
+{
+    if (!A || !ms) return CFX_E_INVALID;
+    return guard(A->mesh->owner, [&] {
+        need(comp >= 0 && comp < A->ncomp, "bad component");
+        DevMesh& m = *A->mesh->m;
+        bool ok = false;
+        *ms = timeSpmv(A->mesh->owner, m.pattern.view(), A->val.as<double>() + (long long)comp * m.pattern.nnz,
+                       A->val32, ok, x, y, reps, precision);
+    });
+}
+
+} // extern "C"

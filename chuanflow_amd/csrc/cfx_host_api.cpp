@@ -1,0 +1,186 @@
+// cfx_host_api.cpp -- C ABI of the host mesh layer (include/cfx_host.h).
+#include "../../include/cfx_host.h"
+
+#include "cf/PolyMesh.h"
+
+#include <cstring>
+#include <string>
+
+struct cfx_hmesh
+{
+    cf::polyMesh mesh;
+};
+
+namespace
+{
+thread_local std::string g_err;
+
+template <typename F>
+int guardHost(F&& f)
+{
+    try
+    {
+        f();
+        return 0;
+    }
+    catch (const std::invalid_argument& e)
+    {
+        g_err = e.what();
+        return 1;
+    }
+    catch (const cf::Error& e)
+    {
+        g_err = e.what();
+        return 1;
+    }
+    catch (const std::exception& e)
+    {
+        g_err = e.what();
+        return 5;
+    }
+}
+} // namespace
+
+extern "C" {
+
+const char* cfx_hmesh_last_error(void) { return g_err.c_str(); }
+
+int cfx_hmesh_box(int64_t nx, int64_t ny, int64_t nz, const double mn[3], const double mx[3], double jitter,
+                  uint64_t seed, int tets, cfx_hmesh** out)
+{
+    return guardHost([&] {
+        if (!out || !mn || !mx) throw std::invalid_argument("null argument");
+        const cf::Vec3 a{mn[0], mn[1], mn[2]}, b{mx[0], mx[1], mx[2]};
+        auto* h = new cfx_hmesh;
+        try
+        {
+            h->mesh = tets ? cf::BlockMesh::createTets(a, b, nx, ny, nz, {}, jitter, seed)
+                           : cf::BlockMesh::create(a, b, nx, ny, nz, {}, jitter, seed);
+        }
+        catch (...)
+        {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int cfx_hmesh_from_arrays(int64_t nPoints, const double* points, int64_t nFaces, const int64_t* faceStart,
+                          const int64_t* faceVerts, int64_t nInternal, const int64_t* owner,
+                          const int64_t* neighbour, int64_t nPatches, const int64_t* patchStart,
+                          const int64_t* patchSize, cfx_hmesh** out)
+{
+    return guardHost([&] {
+        if (!out || !points || !faceStart || !faceVerts || !owner) throw std::invalid_argument("null argument");
+        std::vector<double> p(points, points + 3 * nPoints);
+        std::vector<cf::label> fs(faceStart, faceStart + nFaces + 1), fv(faceVerts, faceVerts + faceStart[nFaces]),
+            ow(owner, owner + nFaces), nb(neighbour, neighbour + nInternal);
+        for (cf::label v : fv)
+            if (v < 0 || v >= nPoints) throw std::invalid_argument("face vertex out of range");
+        std::vector<cf::polyMesh::Patch> patches;
+        for (int64_t i = 0; i < nPatches; ++i)
+            patches.push_back({"patch" + std::to_string(i), "patch", patchStart[i], patchSize[i]});
+        auto* h = new cfx_hmesh;
+        try
+        {
+            h->mesh = cf::polyMesh(std::move(p), std::move(fs), std::move(fv), std::move(ow), std::move(nb),
+                                   std::move(patches));
+        }
+        catch (...)
+        {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int cfx_hmesh_free(cfx_hmesh* h)
+{
+    delete h;
+    return 0;
+}
+
+int cfx_hmesh_sizes(const cfx_hmesh* h, int64_t s[6])
+{
+    return guardHost([&] {
+        if (!h) throw std::invalid_argument("null mesh");
+        const cf::polyMesh& m = h->mesh;
+        s[0] = m.nPoints();
+        s[1] = m.nFaces();
+        s[2] = m.nInternalFaces();
+        s[3] = m.nCells();
+        s[4] = m.nPatches();
+        s[5] = static_cast<int64_t>(m.faceVerts().size());
+    });
+}
+
+int cfx_hmesh_get(const cfx_hmesh* h, double* points, int64_t* faceStart, int64_t* faceVerts, int64_t* owner,
+                  int64_t* neighbour, int64_t* patchStart, int64_t* patchSize)
+{
+    return guardHost([&] {
+        if (!h) throw std::invalid_argument("null mesh");
+        const cf::polyMesh& m = h->mesh;
+        auto cp = [](auto* dst, const auto& v) {
+            if (dst && !v.empty()) std::memcpy(dst, v.data(), sizeof(v[0]) * v.size());
+        };
+        cp(points, m.points());
+        cp(faceStart, m.faceStart());
+        cp(faceVerts, m.faceVerts());
+        cp(owner, m.owner());
+        cp(neighbour, m.neighbour());
+        for (cf::label i = 0; i < m.nPatches(); ++i)
+        {
+            if (patchStart) patchStart[i] = m.patches()[i].start;
+            if (patchSize) patchSize[i] = m.patches()[i].size;
+        }
+    });
+}
+
+int cfx_hmesh_geometry(const cfx_hmesh* h, double* Sf, double* Cf, double* magSf, double* C, double* V, double* d,
+                       double* invDist)
+{
+    return guardHost([&] {
+        if (!h) throw std::invalid_argument("null mesh");
+        const cf::polyMesh& m = h->mesh;
+        auto cp = [](double* dst, const std::vector<double>& v) {
+            if (dst && !v.empty()) std::memcpy(dst, v.data(), sizeof(double) * v.size());
+        };
+        cp(Sf, m.faceSf());
+        cp(Cf, m.faceCentres());
+        cp(magSf, m.faceAreas());
+        cp(C, m.cellCentres());
+        cp(V, m.cellVolumes());
+        cp(d, m.deltaCoeffVec());
+        cp(invDist, m.deltaCoeff());
+    });
+}
+
+int cfx_hmesh_shuffle(cfx_hmesh* h, uint64_t seed)
+{
+    return guardHost([&] {
+        if (!h) throw std::invalid_argument("null mesh");
+        h->mesh.shuffle(seed);
+    });
+}
+
+int cfx_hmesh_renumber_rcm(cfx_hmesh* h, int64_t* newId)
+{
+    return guardHost([&] {
+        if (!h) throw std::invalid_argument("null mesh");
+        const std::vector<cf::label> id = h->mesh.rcmOrdering();
+        h->mesh.renumberCells(id);
+        if (newId) std::memcpy(newId, id.data(), sizeof(int64_t) * id.size());
+    });
+}
+
+int cfx_hmesh_bandwidth(const cfx_hmesh* h, int64_t* bw)
+{
+    return guardHost([&] {
+        if (!h || !bw) throw std::invalid_argument("null argument");
+        *bw = h->mesh.bandwidth();
+    });
+}
+
+} // extern "C"

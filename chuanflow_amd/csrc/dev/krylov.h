@@ -1,0 +1,98 @@
+// krylov.h -- SpMV and Krylov solvers on the device.
+//
+// solveCG / solveBiCGStab follow the reference algorithms step for step
+// (src/common/src/Matrix.cpp:153-222, :224-319): Jacobi preconditioning with
+// the same guards, the same infinity-norm stop ||r||inf / max(||b||inf,
+// 1e-300) < tol on the recurrence residual, the same breakdown exits.  The
+// SpMV sums each row in column order, so A*x is bit-identical to the
+// reference SparseMatrix::product; only the dot products differ (tree
+// instead of serial summation), which moves iterates by rounding only.
+#pragma once
+
+#include "types.h"
+
+namespace cfx
+{
+
+struct SolveReport
+{
+    int iterations = 0;
+    double initialResidual = 0, finalResidual = 0;
+    int converged = 0;
+    int breakdown = 0;
+    // extensions
+    double l2Residual = 0;        // ||b - A x||2 / ||b||2 (true FP64 residual)
+    double trueResidual = 0;      // ||b - A x||inf / ||b||inf (true FP64 residual)
+    int outerIterations = 0;      // refinement sweeps (mixed precision)
+    int precondFallbacks = 0;     // learned preconditioner rejected -> Jacobi
+    double ms = 0;                // device time of the solve
+};
+
+enum Precond { kJacobi = 0, kDIC = 1, kLearned = 2 };
+enum Precision { kFP64 = 0, kFP32IR = 1 };
+
+class Krylov
+{
+public:
+    Krylov(Context& ctx, int n, int maxRowBlocks);
+    ~Krylov();
+    int n() const { return n_; }
+    int maxBlocks() const { return maxBlocks_; }
+
+    // y = A x (bit-identical to the reference product)
+    void spmv(const CsrView& A, const double* val, const double* x, double* y);
+    void spmv32(const CsrView& A, const float* val, const float* x, float* y);
+
+    // Jacobi-PCG in FP64 (solveCG).  x is the initial guess and the result.
+    // resHist (device, optional, [maxIter+1]) receives ||r||inf per iteration.
+    SolveReport cg(const CsrView& A, const double* val, const double* b, double* x, double tol,
+                   int maxIter, double* resHist = nullptr);
+
+    // Jacobi-BiCGStab in FP64 (solveBiCGStab).
+    SolveReport bicgstab(const CsrView& A, const double* val, const double* b, double* x, double tol,
+                         int maxIter);
+
+    // FP32-inner Jacobi-PCG with FP64 iterative refinement: the outer loop
+    // keeps x, r = b - A x and the stop test in FP64; each inner sweep solves
+    // A d = r in FP32 to innerTol (relative), then x += d.  `val32` is the
+    // FP32 copy of the matrix.  `project` removes the mean of each inner
+    // right-hand side (pure-Neumann systems).
+    SolveReport cgMixed(const CsrView& A, const double* val, const float* val32, const double* b,
+                        double* x, double tol, int maxIter, double innerTol, bool project);
+
+    // Inner FP32 PCG on its own (used by the refinement and by benchmarks).
+    SolveReport cg32(const CsrView& A, const float* val32, const float* b, float* x, double tol,
+                     int maxIter, bool project);
+
+    // True residual ||b - A x|| in FP64: returns {inf-norm rel, 2-norm rel}.
+    void trueResidual(const CsrView& A, const double* val, const double* b, const double* x,
+                      double& infRel, double& l2Rel);
+
+    // Device timing of the last solve (ms), from HIP events on the stream.
+    double lastMs() const { return lastMs_; }
+
+    // Batch control: iterations launched between host polls.
+    int minBatch = 8, maxBatch = 256;
+
+private:
+    template <typename T>
+    SolveReport runCG(const CsrView& A, const T* val, const T* invD, const T* b, T* x, double tol,
+                      int maxIter, double* resHist, T* r, T* p, T* q, bool project);
+    void poll();
+    int nextBatch(int done, double r0, double rNow, double tolAbs, int maxIter);
+
+    Context& ctx_;
+    int n_, maxBlocks_;
+    DevBuf state_, partials_;
+    DevBuf r_, p_, q_, r0_, v_, s_, sh_, t_, z_, invD_, invDb_;
+    DevBuf r32_, p32_, q32_, d32_, b32_, invD32_;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    double lastMs_ = 0;
+};
+
+// invD from a CSR diagonal: CG guard |d| > 1e-300 (Matrix.cpp:160-165),
+// BiCGStab guard d > 1e-300 (Matrix.cpp:231-236).
+void computeInvD(Context& ctx, const CsrView& A, const double* val, double* invD, bool bicgGuard);
+void castToFloat(Context& ctx, const double* in, float* out, long long n);
+
+} // namespace cfx

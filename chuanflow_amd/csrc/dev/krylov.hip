@@ -1,0 +1,877 @@
+// krylov.hip -- CSR-stream SpMV, Jacobi-PCG, Jacobi-BiCGStab and the
+// FP32-inner / FP64-refined PCG for gfx950.
+//
+// Per CG iteration three kernels run back to back on one stream:
+//   spmv_dot : q = A p  (LDS-staged CSR-stream)        + partial p.q
+//   update   : x += a p, r -= a q, z = invD r          + partial r.z, max|r|
+//   direction: p = invD r + b p
+// Each reduction finishes in the last workgroup to arrive (ticket), which
+// writes the new scalars into the device-resident SolveState; the host only
+// polls `done` between batches of iterations, sized from the observed
+// convergence rate.
+#include "krylov.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace cfx
+{
+
+std::vector<int> buildRowBlocks(const std::vector<int>& rowPtr)
+{
+    const int n = static_cast<int>(rowPtr.size()) - 1;
+    std::vector<int> b{0};
+    int r = 0;
+    while (r < n)
+    {
+        int rows = 0;
+        long long nnz = 0;
+        while (r < n && rows < kSpmvBlock)
+        {
+            const int len = rowPtr[r + 1] - rowPtr[r];
+            if (rows > 0 && nnz + len > kSpmvLds) break;
+            nnz += len;
+            ++rows;
+            ++r;
+            if (nnz > kSpmvLds) break;  // one long row alone in its block
+        }
+        b.push_back(r);
+    }
+    return b;
+}
+
+void DevPattern::build(std::vector<int> rowPtrH, std::vector<int> colH, hipStream_t s)
+{
+    n = static_cast<int>(rowPtrH.size()) - 1;
+    nnz = rowPtrH.back();
+    std::vector<int> diag(static_cast<std::size_t>(n), -1);
+    for (int r = 0; r < n; ++r)
+        for (int k = rowPtrH[r]; k < rowPtrH[r + 1]; ++k)
+            if (colH[k] == r) { diag[r] = k; break; }
+    const std::vector<int> blocks = buildRowBlocks(rowPtrH);
+    nRowBlocks = static_cast<int>(blocks.size()) - 1;
+    rowPtr = upload(rowPtrH, s);
+    col = upload(colH, s);
+    diagSlot = upload(diag, s);
+    rowBlocks = upload(blocks, s);
+    hRowPtr = std::move(rowPtrH);
+    hCol = std::move(colH);
+    CFX_HIP(hipStreamSynchronize(s));
+}
+
+namespace
+{
+
+constexpr int kVecGridMax = 2048;
+
+// Consecutive workgroup ids are dealt round-robin to the 8 XCDs; remap so
+// each XCD walks one contiguous chunk of rows and its L2 holds that chunk's
+// gathered x entries.
+__device__ __forceinline__ int xcdRemap(int h, int G)
+{
+    constexpr int X = 8;
+    const int q = G / X, r = G % X;
+    const int x = h % X, idx = h / X;
+    return (x < r) ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
+}
+
+// Row sum of the CSR-stream block `blk` for the row this thread owns.
+// Products are formed once (coalesced val/col reads, gathered x) and staged
+// in LDS; each row is then summed sequentially in column order from zero,
+// which is exactly SparseMatrix::product's arithmetic.
+template <typename T>
+__device__ __forceinline__ bool spmvRowSum(const CsrView& A, const T* __restrict__ val,
+                                           const T* __restrict__ x, int blk, T* lds, int& row,
+                                           T& sum)
+{
+    const int r0 = A.rowBlocks[blk], r1 = A.rowBlocks[blk + 1];
+    const int k0 = A.rowPtr[r0];
+    const int nnz = A.rowPtr[r1] - k0;
+    row = r0 + static_cast<int>(threadIdx.x);
+    if (nnz <= kSpmvLds)
+    {
+        for (int k = threadIdx.x; k < nnz; k += blockDim.x) lds[k] = val[k0 + k] * x[A.col[k0 + k]];
+        __syncthreads();
+        if (row < r1)
+        {
+            T s = T(0);
+            const int e = A.rowPtr[row + 1] - k0;
+            for (int k = A.rowPtr[row] - k0; k < e; ++k) s += lds[k];
+            sum = s;
+            return true;
+        }
+        return false;
+    }
+    if (row < r1)
+    {
+        T s = T(0);
+        for (int k = A.rowPtr[row]; k < A.rowPtr[row + 1]; ++k) s += val[k] * x[A.col[k]];
+        sum = s;
+        return true;
+    }
+    return false;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kSpmvBlock) void k_spmv(CsrView A, const T* __restrict__ val,
+                                                     const T* __restrict__ x, T* __restrict__ y)
+{
+    __shared__ T lds[kSpmvLds];
+    int row;
+    T s;
+    if (spmvRowSum(A, val, x, xcdRemap(blockIdx.x, gridDim.x), lds, row, s)) y[row] = s;
+}
+
+// r = b - A x with reductions max|r|, max|b| (NaN-flagged), sum r^2, sum b^2.
+// mode 0: initial residual of a solve (sets initialResidual, bnorm and the
+//         early-convergence exit, solveCG :167-180);
+// mode 1: refinement residual (sets rnorm; converged if rnorm/bnorm < tol);
+// mode 2: reductions only (sum0 = rnorm, rho = ||r||2 / ||b||2).
+template <typename T>
+__global__ __launch_bounds__(kSpmvBlock) void k_residual(CsrView A, const T* __restrict__ val,
+                                                         const T* __restrict__ x,
+                                                         const T* __restrict__ b, T* __restrict__ r,
+                                                         SolveState* st, double* part, int* ipart,
+                                                         int mode)
+{
+    __shared__ T lds[kSpmvLds];
+    __shared__ double red[kSpmvBlock / kWave];
+    __shared__ int ired[kSpmvBlock / kWave];
+    __shared__ int flag;
+    const int G = gridDim.x;
+    const int blk = xcdRemap(blockIdx.x, G);
+    int row;
+    T s;
+    double rm = 0, bm = 0, r2 = 0, b2 = 0;
+    int nan = 0;
+    if (spmvRowSum(A, val, x, blk, lds, row, s))
+    {
+        const T bb = b[row];
+        const T rr = bb - s;
+        r[row] = rr;
+        const double rd = rr, bd = bb;
+        rm = fabs(rd);
+        bm = fabs(bd);
+        r2 = rd * rd;
+        b2 = bd * bd;
+        nan = (rd != rd ? 1 : 0) | (bd != bd ? 2 : 0);
+    }
+    rm = blockMax(rm, red);
+    bm = blockMax(bm, red);
+    r2 = blockSum(r2, red);
+    b2 = blockSum(b2, red);
+    nan = blockOr(nan, ired);
+    if (threadIdx.x == 0)
+    {
+        part[blk] = rm;
+        part[G + blk] = bm;
+        part[2 * G + blk] = r2;
+        part[3 * G + blk] = b2;
+        ipart[blk] = nan;
+    }
+    if (!lastBlockDone(&st->ticket[0], &flag)) return;
+    const double RM = maxPartials(part, G, red);
+    const double BM = maxPartials(part + G, G, red);
+    const double R2 = sumPartials(part + 2 * G, G, red);
+    const double B2 = sumPartials(part + 3 * G, G, red);
+    const int NAN_ = orPartials(ipart, G, ired);
+    if (threadIdx.x == 0)
+    {
+        const double rnorm = (NAN_ & 1) ? NAN : RM;
+        const double binf = (NAN_ & 2) ? NAN : BM;
+        const double l2 = sqrt(R2) / (sqrt(B2) > 1e-300 ? sqrt(B2) : 1e-300);
+        if (mode == 0)
+        {
+            const double bnorm = (binf < 1e-300) ? 1e-300 : binf;  // std::max(binf, 1e-300)
+            st->initialResidual = rnorm;
+            st->rnorm = rnorm;
+            st->bnorm = bnorm;
+            st->iter = 0;
+            st->rho = l2;
+            if (rnorm / bnorm < st->tol)
+            {
+                st->converged = 1;
+                st->done = 1;
+            }
+        }
+        else if (mode == 1)
+        {
+            st->rnorm = rnorm;
+            st->rho = l2;
+            if (rnorm / st->bnorm < st->tol)
+            {
+                st->converged = 1;
+                st->done = 1;
+            }
+        }
+        else
+        {
+            st->sum0 = rnorm / ((binf < 1e-300) ? 1e-300 : binf);
+            st->rho = l2;
+        }
+    }
+}
+
+// z = invD r; p = z; rz = r.z   (solveCG :182-185)
+template <typename T>
+__global__ __launch_bounds__(kVecBlock) void k_cg_init(int n, const T* __restrict__ invD,
+                                                       const T* __restrict__ r, T* __restrict__ p,
+                                                       SolveState* st, double* part)
+{
+    __shared__ double red[kVecBlock / kWave];
+    __shared__ int flag;
+    if (st->done) return;
+    double rz = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const T z = invD[i] * r[i];
+        p[i] = z;
+        rz += static_cast<double>(r[i]) * static_cast<double>(z);
+    }
+    rz = blockSum(rz, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = rz;
+    if (!lastBlockDone(&st->ticket[0], &flag)) return;
+    const double RZ = sumPartials(part, gridDim.x, red);
+    if (threadIdx.x == 0) st->rz = RZ;
+}
+
+// q = A p; pAp; alpha  (solveCG :189-195)
+template <typename T>
+__global__ __launch_bounds__(kSpmvBlock) void k_cg_spmv_dot(CsrView A, const T* __restrict__ val,
+                                                            const T* __restrict__ p,
+                                                            T* __restrict__ q, SolveState* st,
+                                                            double* part)
+{
+    __shared__ T lds[kSpmvLds];
+    __shared__ double red[kSpmvBlock / kWave];
+    __shared__ int flag;
+    if (st->done) return;
+    const int G = gridDim.x;
+    const int blk = xcdRemap(blockIdx.x, G);
+    int row;
+    T s;
+    double pq = 0;
+    if (spmvRowSum(A, val, p, blk, lds, row, s))
+    {
+        q[row] = s;
+        pq = static_cast<double>(p[row]) * static_cast<double>(s);
+    }
+    pq = blockSum(pq, red);
+    if (threadIdx.x == 0) part[blk] = pq;
+    if (!lastBlockDone(&st->ticket[0], &flag)) return;
+    const double pAp = sumPartials(part, G, red);
+    if (threadIdx.x == 0)
+    {
+        st->pAp = pAp;
+        if (pAp <= 0 || fabs(st->rz) < 1e-300)
+        {
+            st->done = 1;  // matrix not sufficiently SPD (reference: break)
+            st->breakdown = 1;
+        }
+        else
+        {
+            st->alpha = st->rz / pAp;
+        }
+    }
+}
+
+// x += alpha p; r -= alpha q; z = invD r; rz' = r.z; ||r||inf
+// (solveCG :196-211)
+template <typename T>
+__global__ __launch_bounds__(kVecBlock) void k_cg_update(int n, const T* __restrict__ invD,
+                                                         T* __restrict__ x, T* __restrict__ r,
+                                                         const T* __restrict__ p,
+                                                         const T* __restrict__ q, SolveState* st,
+                                                         double* part, int* ipart,
+                                                         double* resHist)
+{
+    __shared__ double red[kVecBlock / kWave];
+    __shared__ int ired[kVecBlock / kWave];
+    __shared__ int flag;
+    if (st->done) return;
+    const T alpha = static_cast<T>(st->alpha);
+    double rz = 0, rm = 0;
+    int nan = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        x[i] += alpha * p[i];
+        const T ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        const T z = invD[i] * ri;
+        rz += static_cast<double>(ri) * static_cast<double>(z);
+        const double a = fabs(static_cast<double>(ri));
+        rm = fmax(rm, a);
+        nan |= (ri != ri);
+    }
+    const int G = gridDim.x;
+    rz = blockSum(rz, red);
+    rm = blockMax(rm, red);
+    nan = blockOr(nan, ired);
+    if (threadIdx.x == 0)
+    {
+        part[blockIdx.x] = rz;
+        part[G + blockIdx.x] = rm;
+        ipart[blockIdx.x] = nan;
+    }
+    if (!lastBlockDone(&st->ticket[0], &flag)) return;
+    const double RZ = sumPartials(part, G, red);
+    const double RM = maxPartials(part + G, G, red);
+    const int NAN_ = orPartials(ipart, G, ired);
+    if (threadIdx.x == 0)
+    {
+        const int it = st->iter + 1;
+        const double rnorm = NAN_ ? NAN : RM;
+        st->iter = it;
+        st->rnorm = rnorm;
+        if (resHist) resHist[it] = rnorm;
+        if (rnorm / st->bnorm < st->tol)
+        {
+            st->converged = 1;
+            st->done = 1;
+        }
+        else
+        {
+            st->beta = RZ / st->rz;
+            st->rz = RZ;
+            if (it >= st->maxIter) st->done = 1;
+        }
+    }
+}
+
+// p = invD r + beta p  (solveCG :209-217)
+template <typename T>
+__global__ __launch_bounds__(kVecBlock) void k_cg_direction(int n, const T* __restrict__ invD,
+                                                            const T* __restrict__ r,
+                                                            T* __restrict__ p, const SolveState* st)
+{
+    if (st->done) return;
+    const T beta = static_cast<T>(st->beta);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const T z = invD[i] * r[i];
+        p[i] = z + beta * p[i];
+    }
+}
+
+// ---------------------------------------------------------------- BiCGStab
+// r0 = r, p = v = 0, rho/beta of iteration 1 (solveBiCGStab :253-266)
+__global__ __launch_bounds__(kVecBlock) void k_bicg_init(int n, const double* __restrict__ r,
+                                                         double* __restrict__ r0, double* __restrict__ p,
+                                                         double* __restrict__ v, SolveState* st,
+                                                         double* part)
+{
+    __shared__ double red[kVecBlock / kWave];
+    __shared__ int flag;
+    if (st->done) return;
+    double s = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const double ri = r[i];
+        r0[i] = ri;
+        p[i] = 0.0;
+        v[i] = 0.0;
+        s += ri * ri;
+    }
+    s = blockSum(s, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+    if (!lastBlockDone(&st->ticket[0], &flag)) return;
+    const double rhoNew = sumPartials(part, gridDim.x, red);
+    if (threadIdx.x == 0)
+    {
+        st->rho = 1.0;
+        st->alpha = 1.0;
+        st->omega = 1.0;
+        if (fabs(rhoNew) < 1e-300)
+        {
+            st->done = 1;
+        }
+        else
+        {
+            st->beta = (rhoNew / st->rho) * (st->alpha / st->omega);
+            st->rho = rhoNew;
+        }
+    }
+}
+
+// p = r + beta (p - omega v); z = p invD   (:267-272)
+__global__ __launch_bounds__(kVecBlock) void k_bicg_p(int n, const double* __restrict__ invD,
+                                                      const double* __restrict__ r,
+                                                      double* __restrict__ p,
+                                                      const double* __restrict__ v,
+                                                      double* __restrict__ z, const SolveState* st)
+{
+    if (st->done) return;
+    const double beta = st->beta, omega = st->omega;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const double pi = r[i] + beta * (p[i] - omega * v[i]);
+        p[i] = pi;
+        z[i] = pi * invD[i];
+    }
+}
+
+// y = A x, then a dot with `w` (r0.v) or two dots (t.t, t.s) in the epilogue.
+// which == 0: v = A z, denom = r0.v, alpha = rho/denom   (:273-277)
+// which == 1: t = A sh, tt = t.t, ts = t.s, omega = ts/tt (:290-300)
+__global__ __launch_bounds__(kSpmvBlock) void k_bicg_spmv(CsrView A, const double* __restrict__ val,
+                                                          const double* __restrict__ x,
+                                                          double* __restrict__ y,
+                                                          const double* __restrict__ w,
+                                                          SolveState* st, double* part, int which)
+{
+    __shared__ double lds[kSpmvLds];
+    __shared__ double red[kSpmvBlock / kWave];
+    __shared__ int flag;
+    if (st->done) return;
+    const int G = gridDim.x;
+    const int blk = xcdRemap(blockIdx.x, G);
+    int row;
+    double s;
+    double d0 = 0, d1 = 0;
+    if (spmvRowSum(A, val, x, blk, lds, row, s))
+    {
+        y[row] = s;
+        if (which == 0) d0 = w[row] * s;
+        else { d0 = s * s; d1 = s * w[row]; }
+    }
+    d0 = blockSum(d0, red);
+    d1 = blockSum(d1, red);
+    if (threadIdx.x == 0) { part[blk] = d0; part[G + blk] = d1; }
+    if (!lastBlockDone(&st->ticket[0], &flag)) return;
+    const double D0 = sumPartials(part, G, red);
+    const double D1 = sumPartials(part + G, G, red);
+    if (threadIdx.x == 0)
+    {
+        if (which == 0)
+        {
+            st->denom = D0;
+            if (fabs(D0) < 1e-300) st->done = 1;
+            else st->alpha = st->rho / D0;
+        }
+        else
+        {
+            st->tt = D0;
+            st->ts = D1;
+            if (D0 < 1e-300) st->breakdown = 2;  // r = s and stop (:292-299)
+            else st->omega = D1 / D0;
+        }
+    }
+}
+
+// x += alpha z; s = r - alpha v; sh = s invD   (:278-289)
+__global__ __launch_bounds__(kVecBlock) void k_bicg_s(int n, const double* __restrict__ invD,
+                                                      double* __restrict__ x,
+                                                      const double* __restrict__ z,
+                                                      const double* __restrict__ r,
+                                                      const double* __restrict__ v,
+                                                      double* __restrict__ s, double* __restrict__ sh,
+                                                      const SolveState* st)
+{
+    if (st->done) return;
+    const double alpha = st->alpha;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        x[i] += alpha * z[i];
+        const double si = r[i] - alpha * v[i];
+        s[i] = si;
+        sh[i] = si * invD[i];
+    }
+}
+
+// x += omega sh; r = s - omega t; ||r||inf; next rho = r0.r   (:301-314, :263-266)
+__global__ __launch_bounds__(kVecBlock) void k_bicg_r(int n, double* __restrict__ x,
+                                                      const double* __restrict__ sh,
+                                                      const double* __restrict__ s,
+                                                      const double* __restrict__ t,
+                                                      const double* __restrict__ r0,
+                                                      double* __restrict__ r, SolveState* st,
+                                                      double* part, int* ipart)
+{
+    __shared__ double red[kVecBlock / kWave];
+    __shared__ int ired[kVecBlock / kWave];
+    __shared__ int flag;
+    if (st->done) return;
+    const bool tsmall = st->breakdown == 2;
+    const double omega = st->omega;
+    double rm = 0, rr0 = 0;
+    int nan = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        double ri;
+        if (tsmall)
+        {
+            ri = s[i];
+        }
+        else
+        {
+            x[i] += omega * sh[i];
+            ri = s[i] - omega * t[i];
+        }
+        r[i] = ri;
+        rm = fmax(rm, fabs(ri));
+        nan |= (ri != ri);
+        rr0 += r0[i] * ri;
+    }
+    const int G = gridDim.x;
+    rm = blockMax(rm, red);
+    rr0 = blockSum(rr0, red);
+    nan = blockOr(nan, ired);
+    if (threadIdx.x == 0) { part[blockIdx.x] = rm; part[G + blockIdx.x] = rr0; ipart[blockIdx.x] = nan; }
+    if (!lastBlockDone(&st->ticket[0], &flag)) return;
+    const double RM = maxPartials(part, G, red);
+    const double RR0 = sumPartials(part + G, G, red);
+    const int NAN_ = orPartials(ipart, G, ired);
+    if (threadIdx.x == 0)
+    {
+        const int it = st->iter + 1;
+        const double rnorm = NAN_ ? NAN : RM;
+        st->iter = it;
+        st->rnorm = rnorm;
+        if (tsmall)
+        {
+            st->converged = rnorm / st->bnorm < st->tol;
+            st->done = 1;
+        }
+        else if (rnorm / st->bnorm < st->tol)
+        {
+            st->converged = 1;
+            st->done = 1;
+        }
+        else if (it >= st->maxIter)
+        {
+            st->done = 1;
+        }
+        else if (fabs(RR0) < 1e-300)
+        {
+            st->done = 1;  // rho breakdown at the top of the next iteration
+        }
+        else
+        {
+            st->beta = (RR0 / st->rho) * (st->alpha / st->omega);
+            st->rho = RR0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- helpers
+__global__ void k_invD(CsrView A, const double* __restrict__ val, double* __restrict__ invD,
+                       int bicgGuard)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    const int k = A.diagSlot[i];
+    const double d = k >= 0 ? val[k] : 0.0;
+    invD[i] = bicgGuard ? ((d > 1e-300) ? 1.0 / d : 1.0) : ((fabs(d) > 1e-300) ? 1.0 / d : 1.0);
+}
+
+__global__ void k_cast(const double* __restrict__ in, float* __restrict__ out, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        out[i] = static_cast<float>(in[i]);
+}
+
+template <typename T>
+__global__ void k_fill(T* __restrict__ p, T v, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// b32 = float(r64) (mean-projected when asked): pass 0 sums r in double
+__global__ __launch_bounds__(kVecBlock) void k_sum(int n, const double* __restrict__ v,
+                                                   SolveState* st, double* part)
+{
+    __shared__ double red[kVecBlock / kWave];
+    __shared__ int flag;
+    double s = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += v[i];
+    s = blockSum(s, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+    if (!lastBlockDone(&st->ticket[0], &flag)) return;
+    const double S = sumPartials(part, gridDim.x, red);
+    if (threadIdx.x == 0) st->sum0 = S;
+}
+
+__global__ void k_to_inner_rhs(int n, const double* __restrict__ r, float* __restrict__ b32,
+                               float* __restrict__ d32, const SolveState* st, int project)
+{
+    const double mean = project ? st->sum0 / n : 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        b32[i] = static_cast<float>(r[i] - mean);
+        d32[i] = 0.0f;
+    }
+}
+
+__global__ void k_ir_add(int n, double* __restrict__ x, const float* __restrict__ d32)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        x[i] += static_cast<double>(d32[i]);
+}
+
+int vecGridFor(long long n) { return std::max(1, std::min(kVecGridMax, divUp(n, 4LL * kVecBlock))); }
+
+} // namespace
+
+void computeInvD(Context& ctx, const CsrView& A, const double* val, double* invD, bool bicgGuard)
+{
+    k_invD<<<divUp(A.n, 256), 256, 0, ctx.stream()>>>(A, val, invD, bicgGuard ? 1 : 0);
+    CFX_CHECK_LAUNCH();
+}
+
+void castToFloat(Context& ctx, const double* in, float* out, long long n)
+{
+    k_cast<<<std::max(1, std::min(4096, divUp(n, 256))), 256, 0, ctx.stream()>>>(in, out, n);
+    CFX_CHECK_LAUNCH();
+}
+
+Krylov::Krylov(Context& ctx, int n, int maxRowBlocks) : ctx_(ctx), n_(n)
+{
+    maxBlocks_ = std::max(maxRowBlocks, kVecGridMax);
+    const std::size_t d = sizeof(double) * static_cast<std::size_t>(n), f = sizeof(float) * static_cast<std::size_t>(n);
+    state_.alloc(2 * sizeof(SolveState));
+    CFX_HIP(hipMemsetAsync(state_.p, 0, 2 * sizeof(SolveState), ctx.stream()));
+    partials_.alloc(sizeof(double) * 4 * maxBlocks_ + sizeof(int) * maxBlocks_);
+    for (DevBuf* b : {&r_, &p_, &q_, &r0_, &v_, &s_, &sh_, &t_, &z_, &invD_, &invDb_}) b->alloc(d);
+    for (DevBuf* b : {&r32_, &p32_, &q32_, &d32_, &b32_, &invD32_}) b->alloc(f);
+    CFX_HIP(hipEventCreate(&ev0_));
+    CFX_HIP(hipEventCreate(&ev1_));
+    ctx.sync();
+}
+
+Krylov::~Krylov()
+{
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+}
+
+void Krylov::poll()
+{
+    CFX_HIP(hipMemcpyAsync(ctx_.hostState(), state_.p, 2 * sizeof(SolveState), hipMemcpyDeviceToHost,
+                           ctx_.stream()));
+    CFX_HIP(hipStreamSynchronize(ctx_.stream()));
+}
+
+// Iterations to launch before the next poll: extrapolate the observed
+// geometric residual decay to the tolerance, undershooting by 10%.
+int Krylov::nextBatch(int done, double r0, double rNow, double tolAbs, int maxIter)
+{
+    int b = minBatch;
+    if (done > 0 && rNow > 0 && r0 > 0 && rNow < r0 && tolAbs > 0)
+    {
+        const double rate = std::log(rNow / r0) / done;
+        const double need = std::log(tolAbs / rNow) / rate;
+        if (std::isfinite(need)) b = static_cast<int>(0.9 * need);
+    }
+    b = std::max(minBatch, std::min(maxBatch, b));
+    return std::max(1, std::min(b, maxIter - done));
+}
+
+void Krylov::spmv(const CsrView& A, const double* val, const double* x, double* y)
+{
+    k_spmv<double><<<A.nRowBlocks, kSpmvBlock, 0, ctx_.stream()>>>(A, val, x, y);
+    CFX_CHECK_LAUNCH();
+}
+
+void Krylov::spmv32(const CsrView& A, const float* val, const float* x, float* y)
+{
+    k_spmv<float><<<A.nRowBlocks, kSpmvBlock, 0, ctx_.stream()>>>(A, val, x, y);
+    CFX_CHECK_LAUNCH();
+}
+
+template <typename T>
+SolveReport Krylov::runCG(const CsrView& A, const T* val, const T* invD, const T* b, T* x,
+                          double tol, int maxIter, double* resHist, T* r, T* p, T* q, bool project)
+{
+    hipStream_t s = ctx_.stream();
+    SolveState h{};
+    h.tol = tol;
+    h.maxIter = maxIter;
+    SolveState* st = state_.as<SolveState>() + 1;  // slot 1: Krylov iteration
+    CFX_HIP(hipMemcpyAsync(st, &h, offsetof(SolveState, ticket), hipMemcpyHostToDevice, s));
+    double* part = partials_.as<double>();
+    int* ipart = reinterpret_cast<int*>(part + 4 * maxBlocks_);
+    const int Gv = vecGridFor(n_);
+    (void)project;
+    k_residual<T><<<A.nRowBlocks, kSpmvBlock, 0, s>>>(A, val, x, b, r, st, part, ipart, 0);
+    k_cg_init<T><<<Gv, kVecBlock, 0, s>>>(n_, invD, r, p, st, part);
+    CFX_CHECK_LAUNCH();
+    poll();
+    SolveState* hs = ctx_.hostState() + 1;
+    const double r0 = hs->initialResidual, bnorm = hs->bnorm;
+    if (resHist) CFX_HIP(hipMemcpyAsync(resHist, &hs->initialResidual, sizeof(double), hipMemcpyHostToDevice, s));
+    int launched = 0;
+    while (!hs->done && launched < maxIter)
+    {
+        const int batch = nextBatch(hs->iter, r0, hs->rnorm, tol * bnorm, maxIter);
+        for (int k = 0; k < batch; ++k)
+        {
+            k_cg_spmv_dot<T><<<A.nRowBlocks, kSpmvBlock, 0, s>>>(A, val, p, q, st, part);
+            k_cg_update<T><<<Gv, kVecBlock, 0, s>>>(n_, invD, x, r, p, q, st, part, ipart, resHist);
+            k_cg_direction<T><<<Gv, kVecBlock, 0, s>>>(n_, invD, r, p, st);
+        }
+        CFX_CHECK_LAUNCH();
+        launched += batch;
+        poll();
+    }
+    SolveReport rep;
+    rep.iterations = hs->iter;
+    rep.initialResidual = hs->initialResidual;
+    rep.finalResidual = hs->iter > 0 ? hs->rnorm : hs->initialResidual;
+    rep.converged = hs->converged;
+    rep.breakdown = hs->breakdown;
+    return rep;
+}
+
+SolveReport Krylov::cg(const CsrView& A, const double* val, const double* b, double* x, double tol,
+                       int maxIter, double* resHist)
+{
+    CFX_HIP(hipEventRecord(ev0_, ctx_.stream()));
+    computeInvD(ctx_, A, val, invD_.as<double>(), false);
+    SolveReport rep = runCG<double>(A, val, invD_.as<double>(), b, x, tol, maxIter, resHist,
+                                    r_.as<double>(), p_.as<double>(), q_.as<double>(), false);
+    CFX_HIP(hipEventRecord(ev1_, ctx_.stream()));
+    CFX_HIP(hipEventSynchronize(ev1_));
+    float ms = 0;
+    CFX_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+    lastMs_ = rep.ms = ms;
+    return rep;
+}
+
+SolveReport Krylov::cg32(const CsrView& A, const float* val32, const float* b, float* x, double tol,
+                         int maxIter, bool project)
+{
+    return runCG<float>(A, val32, invD32_.as<float>(), b, x, tol, maxIter, nullptr, r32_.as<float>(),
+                        p32_.as<float>(), q32_.as<float>(), project);
+}
+
+void Krylov::trueResidual(const CsrView& A, const double* val, const double* b, const double* x,
+                          double& infRel, double& l2Rel)
+{
+    hipStream_t s = ctx_.stream();
+    SolveState* st = state_.as<SolveState>();
+    double* part = partials_.as<double>();
+    int* ipart = reinterpret_cast<int*>(part + 4 * maxBlocks_);
+    k_residual<double><<<A.nRowBlocks, kSpmvBlock, 0, s>>>(A, val, x, b, r0_.as<double>(), st, part,
+                                                           ipart, 2);
+    CFX_CHECK_LAUNCH();
+    poll();
+    infRel = ctx_.hostState()[0].sum0;
+    l2Rel = ctx_.hostState()[0].rho;
+}
+
+SolveReport Krylov::cgMixed(const CsrView& A, const double* val, const float* val32, const double* b,
+                            double* x, double tol, int maxIter, double innerTol, bool project)
+{
+    hipStream_t s = ctx_.stream();
+    CFX_HIP(hipEventRecord(ev0_, s));
+    const int Gv = vecGridFor(n_);
+    // FP32 Jacobi scaling from the FP64 diagonal (CG guard)
+    computeInvD(ctx_, A, val, invD_.as<double>(), false);
+    castToFloat(ctx_, invD_.as<double>(), invD32_.as<float>(), n_);
+    SolveState h{};
+    h.tol = tol;
+    h.maxIter = maxIter;
+    SolveState* st = state_.as<SolveState>();  // slot 0: refinement
+    CFX_HIP(hipMemcpyAsync(st, &h, offsetof(SolveState, ticket), hipMemcpyHostToDevice, s));
+    double* part = partials_.as<double>();
+    int* ipart = reinterpret_cast<int*>(part + 4 * maxBlocks_);
+    double* r = r0_.as<double>();
+    k_residual<double><<<A.nRowBlocks, kSpmvBlock, 0, s>>>(A, val, x, b, r, st, part, ipart, 0);
+    CFX_CHECK_LAUNCH();
+    poll();
+    SolveReport rep;
+    rep.initialResidual = ctx_.hostState()[0].initialResidual;
+    int total = 0, outer = 0;
+    while (!ctx_.hostState()[0].converged && total < maxIter)
+    {
+        if (project) k_sum<<<Gv, kVecBlock, 0, s>>>(n_, r, st, part);
+        k_to_inner_rhs<<<Gv, kVecBlock, 0, s>>>(n_, r, b32_.as<float>(), d32_.as<float>(), st,
+                                                 project ? 1 : 0);
+        CFX_CHECK_LAUNCH();
+        const SolveReport in = runCG<float>(A, val32, invD32_.as<float>(), b32_.as<float>(),
+                                            d32_.as<float>(), innerTol, maxIter - total, nullptr,
+                                            r32_.as<float>(), p32_.as<float>(), q32_.as<float>(), false);
+        total += in.iterations;
+        ++outer;
+        k_ir_add<<<Gv, kVecBlock, 0, s>>>(n_, x, d32_.as<float>());
+        k_residual<double><<<A.nRowBlocks, kSpmvBlock, 0, s>>>(A, val, x, b, r, st, part, ipart, 1);
+        CFX_CHECK_LAUNCH();
+        poll();
+        if (in.iterations == 0) break;  // no progress possible in FP32
+    }
+    const SolveState& hs = ctx_.hostState()[0];
+    rep.iterations = total;
+    rep.outerIterations = outer;
+    rep.finalResidual = hs.rnorm;
+    rep.trueResidual = hs.rnorm / hs.bnorm;
+    rep.l2Residual = hs.rho;
+    rep.converged = hs.converged;
+    CFX_HIP(hipEventRecord(ev1_, s));
+    CFX_HIP(hipEventSynchronize(ev1_));
+    float ms = 0;
+    CFX_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+    lastMs_ = rep.ms = ms;
+    return rep;
+}
+
+SolveReport Krylov::bicgstab(const CsrView& A, const double* val, const double* b, double* x,
+                             double tol, int maxIter)
+{
+    hipStream_t s = ctx_.stream();
+    CFX_HIP(hipEventRecord(ev0_, s));
+    computeInvD(ctx_, A, val, invDb_.as<double>(), true);
+    SolveState h{};
+    h.tol = tol;
+    h.maxIter = maxIter;
+    SolveState* st = state_.as<SolveState>() + 1;
+    CFX_HIP(hipMemcpyAsync(st, &h, offsetof(SolveState, ticket), hipMemcpyHostToDevice, s));
+    double* part = partials_.as<double>();
+    int* ipart = reinterpret_cast<int*>(part + 4 * maxBlocks_);
+    const int Gv = vecGridFor(n_);
+    double *r = r_.as<double>(), *r0 = r0_.as<double>(), *p = p_.as<double>(), *v = v_.as<double>();
+    double *z = z_.as<double>(), *sv = s_.as<double>(), *sh = sh_.as<double>(), *t = t_.as<double>();
+    const double* invD = invDb_.as<double>();
+    k_residual<double><<<A.nRowBlocks, kSpmvBlock, 0, s>>>(A, val, x, b, r, st, part, ipart, 0);
+    k_bicg_init<<<Gv, kVecBlock, 0, s>>>(n_, r, r0, p, v, st, part);
+    CFX_CHECK_LAUNCH();
+    poll();
+    SolveState* hs = ctx_.hostState() + 1;
+    const double rInit = hs->initialResidual, bnorm = hs->bnorm;
+    int launched = 0;
+    while (!hs->done && launched < maxIter)
+    {
+        const int batch = nextBatch(hs->iter, rInit, hs->rnorm, tol * bnorm, maxIter);
+        for (int k = 0; k < batch; ++k)
+        {
+            k_bicg_p<<<Gv, kVecBlock, 0, s>>>(n_, invD, r, p, v, z, st);
+            k_bicg_spmv<<<A.nRowBlocks, kSpmvBlock, 0, s>>>(A, val, z, v, r0, st, part, 0);
+            k_bicg_s<<<Gv, kVecBlock, 0, s>>>(n_, invD, x, z, r, v, sv, sh, st);
+            k_bicg_spmv<<<A.nRowBlocks, kSpmvBlock, 0, s>>>(A, val, sh, t, sv, st, part, 1);
+            k_bicg_r<<<Gv, kVecBlock, 0, s>>>(n_, x, sh, sv, t, r0, r, st, part, ipart);
+        }
+        CFX_CHECK_LAUNCH();
+        launched += batch;
+        poll();
+    }
+    SolveReport rep;
+    rep.iterations = hs->iter;
+    rep.initialResidual = hs->initialResidual;
+    rep.finalResidual = hs->iter > 0 ? hs->rnorm : hs->initialResidual;
+    rep.converged = hs->converged;
+    CFX_HIP(hipEventRecord(ev1_, s));
+    CFX_HIP(hipEventSynchronize(ev1_));
+    float ms = 0;
+    CFX_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+    lastMs_ = rep.ms = ms;
+    return rep;
+}
+
+template SolveReport Krylov::runCG<double>(const CsrView&, const double*, const double*, const double*,
+                                           double*, double, int, double*, double*, double*, double*, bool);
+template SolveReport Krylov::runCG<float>(const CsrView&, const float*, const float*, const float*,
+                                          float*, double, int, double*, float*, float*, float*, bool);
+
+} // namespace cfx

@@ -1,0 +1,162 @@
+// types.h -- device-resident objects of the MI355X hot path.
+//
+// Layout in HBM (one GPU, everything resident for the whole solve):
+//  * cell arrays SoA, [n] per component;
+//  * face arrays SoA in the mesh's face order (RCM-renumbered meshes keep
+//    internal faces sorted by (owner, neighbour), so face streams and the
+//    cell gathers they feed are coalesced);
+//  * the matrix pattern is CSR with int32 columns, rows sorted by column
+//    (the reference SparseMatrix layout), plus face->slot maps for assembly;
+//  * CSR-stream row blocks: each workgroup owns a contiguous row range whose
+//    nonzeros fit in LDS, so the val/col streams are read fully coalesced
+//    and each row is then summed from LDS in column order.
+#pragma once
+
+#include "common.h"
+
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace cfx
+{
+
+constexpr int kSpmvBlock = 256;      // threads = max rows per CSR-stream block
+constexpr int kSpmvLds = 2048;       // max nonzeros staged per block
+constexpr int kVecBlock = 256;
+constexpr int kMaxPartials = 4096;   // grid size cap for reduction kernels
+
+// Device allocation that frees itself.
+struct DevBuf
+{
+    void* p = nullptr;
+    std::size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(std::size_t b) { alloc(b); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept
+    {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(std::size_t b)
+    {
+        release();
+        if (b == 0) b = 256;
+        CFX_HIP(hipMalloc(&p, b));
+        bytes = b;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+template <typename T>
+DevBuf upload(const std::vector<T>& h, hipStream_t s)
+{
+    DevBuf b(h.size() * sizeof(T));
+    if (!h.empty()) CFX_HIP(hipMemcpyAsync(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    return b;
+}
+
+// Scalars of one Krylov solve, device-resident so the iteration never waits
+// on the host; the host polls `done` between batches of iterations.
+struct SolveState
+{
+    double rz, pAp, alpha, beta, rnorm, bnorm, initialResidual, tol;
+    double rho, omega, denom, tt, ts;  // BiCGStab
+    double sum0;                       // scratch reduction result
+    int iter, maxIter, done, converged, breakdown, nan;
+    unsigned int ticket[8];
+};
+
+// Non-owning view of a CSR pattern on the device.
+struct CsrView
+{
+    int n = 0, nnz = 0, nRowBlocks = 0;
+    const int* rowPtr = nullptr;
+    const int* col = nullptr;
+    const int* diagSlot = nullptr;
+    const int* rowBlocks = nullptr;
+};
+
+// Owning CSR pattern (int32 columns, rows sorted by column).
+struct DevPattern
+{
+    int n = 0, nnz = 0, nRowBlocks = 0;
+    DevBuf rowPtr, col, diagSlot, rowBlocks;
+    std::vector<int> hRowPtr, hCol;
+    CsrView view() const
+    {
+        return {n, nnz, nRowBlocks, rowPtr.as<int>(), col.as<int>(), diagSlot.as<int>(), rowBlocks.as<int>()};
+    }
+    void build(std::vector<int> rowPtrH, std::vector<int> colH, hipStream_t s);
+};
+
+// Builds rowBlocks for CSR-stream from a host row pointer: each block holds
+// at most kSpmvBlock rows and kSpmvLds nonzeros (a longer row gets a block
+// of its own and is summed directly from global memory).
+std::vector<int> buildRowBlocks(const std::vector<int>& rowPtr);
+
+class Context;
+
+// Mesh on the device: face streams, cell->face gathers, geometry-derived
+// face coefficients and the matrix pattern with face->slot maps.
+struct DevMesh
+{
+    Context* ctx = nullptr;
+    int n = 0, nI = 0, nF = 0, nB = 0;
+    int nPatches = 0;
+    std::vector<long long> patchStart, patchSize;
+    DevBuf owner, neighbour;               // int[nF], int[nI]
+    DevBuf cellFaceStart, cellFace;        // int[n+1], int[2nI+nB]: face<<1 | isNeighbour
+    DevBuf Sf, Cf, d, C;                   // SoA double[3][nF] / [3][n]
+    DevBuf V, magSf, localScale;           // double[n], [nF], [n]
+    DevBuf ow;                             // double[nI] owner weight
+    DevBuf sfd, d2;                        // double[nF]: Sf.d and d.d
+    DevBuf skew;                           // double[3][nI]
+    DevBuf kf;                             // double[3][nF]: Sf - dh (Sf.dh)
+    DevBuf faceGeom;                       // reserved
+    // matrix pattern (face graph + diagonal)
+    DevPattern pattern;
+    DevBuf slotU, slotL;                   // int[nI]
+    std::vector<double> hV;                // host copy of volumes
+};
+
+// Context: one HIP stream, one device, scratch for reductions.
+class Context
+{
+public:
+    explicit Context(int device);
+    ~Context();
+    int device() const { return device_; }
+    hipStream_t stream() const { return stream_; }
+    void sync() { CFX_HIP(hipStreamSynchronize(stream_)); }
+    std::string lastError;
+
+    // reduction scratch: partials + tickets (zeroed once)
+    double* partials(int slot) { return partials_.as<double>() + slot * kMaxPartials; }
+    int* ipartials() { return ipartials_.as<int>(); }
+    unsigned int* tickets() { return tickets_.as<unsigned int>(); }
+    SolveState* hostState() { return hostState_; }
+
+    int numCUs() const { return numCUs_; }
+    std::string deviceName;
+
+private:
+    int device_ = 0;
+    int numCUs_ = 0;
+    hipStream_t stream_ = nullptr;
+    DevBuf partials_, ipartials_, tickets_;
+    SolveState* hostState_ = nullptr;  // pinned
+};
+
+} // namespace cfx

@@ -1,0 +1,519 @@
+"""ctypes binding of the native library (include/cfx.h, include/cfx_host.h).
+
+The library is required: importing this module on a machine where the
+library has not been built raises ImportError, and every compute call goes
+to the HIP kernels -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libchuanflow_amd.so")
+INCLUDE_DIR = os.path.join(os.path.dirname(HERE), "include")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"chuanflow_amd native library not built: {LIB_PATH} "
+                      "(run `python -m chuanflow_amd.build`)")
+
+_L = C.CDLL(LIB_PATH)
+
+CFX_OK, CFX_E_INVALID, CFX_E_HIP, CFX_E_STATE, CFX_E_NUMERIC, CFX_E_INTERNAL = range(6)
+PRECOND_JACOBI, PRECOND_DIC, PRECOND_LEARNED = 0, 1, 2
+FP64, FP32_IR = 0, 1
+SOLVER_CG, SOLVER_BICGSTAB = 0, 1
+
+_vp = C.c_void_p
+_i64 = C.c_int64
+_dp = C.POINTER(C.c_double)
+
+
+class Report(C.Structure):
+    """cfx_report: SolverReport plus FP64 true residuals and timing."""
+    _fields_ = [("iterations", C.c_int), ("initialResidual", C.c_double),
+                ("finalResidual", C.c_double), ("converged", C.c_int), ("breakdown", C.c_int),
+                ("l2Residual", C.c_double), ("trueResidual", C.c_double),
+                ("outerIterations", C.c_int), ("precondFallbacks", C.c_int), ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class CfxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"cfx error {code}: {msg}")
+        self.code = code
+
+
+def _sig(name, restype, *args):
+    f = getattr(_L, name)
+    f.restype = restype
+    f.argtypes = list(args)
+    return f
+
+
+# signatures -------------------------------------------------------------
+_sig("cfx_create", C.c_int, C.c_int, C.POINTER(_vp))
+_sig("cfx_destroy", C.c_int, _vp)
+_sig("cfx_last_error", C.c_char_p, _vp)
+_sig("cfx_synchronize", C.c_int, _vp)
+_sig("cfx_device_info", C.c_int, _vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_int))
+_sig("cfx_malloc", C.c_int, _vp, C.c_size_t, C.POINTER(_vp))
+_sig("cfx_free", C.c_int, _vp, _vp)
+_sig("cfx_memcpy_htod", C.c_int, _vp, _vp, _vp, C.c_size_t)
+_sig("cfx_memcpy_dtoh", C.c_int, _vp, _vp, _vp, C.c_size_t)
+_sig("cfx_memcpy_dtod", C.c_int, _vp, _vp, _vp, C.c_size_t)
+_sig("cfx_zero", C.c_int, _vp, _vp, C.c_size_t)
+_sig("cfx_mesh_create", C.c_int, _vp, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp,
+     _vp, C.POINTER(_vp))
+_sig("cfx_mesh_destroy", C.c_int, _vp)
+_sig("cfx_mesh_sizes", C.c_int, _vp, _vp)
+_sig("cfx_mesh_pattern", C.c_int, _vp, _vp, _vp)
+_sig("cfx_owner_weights", C.c_int, _vp, _vp)
+_sig("cfx_grad", C.c_int, _vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp)
+_sig("cfx_harmonic_gamma_uniform", C.c_int, _vp, C.c_double, _vp)
+_sig("cfx_interpolate_to_faces", C.c_int, _vp, _vp, _vp)
+_sig("cfx_update_phi_internal", C.c_int, _vp, _vp, _vp)
+_sig("cfx_correct_boundary_flux", C.c_int, _vp, _vp, _vp, _vp, _vp)
+_sig("cfx_correct_fluxes", C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp)
+_sig("cfx_div_phi", C.c_int, _vp, _vp, _vp)
+_sig("cfx_continuity_error", C.c_int, _vp, _vp, _dp)
+_sig("cfx_matrix_create", C.c_int, _vp, C.c_int, C.POINTER(_vp))
+_sig("cfx_matrix_destroy", C.c_int, _vp)
+_sig("cfx_matrix_reset", C.c_int, _vp)
+_sig("cfx_assemble_laplacian", C.c_int, _vp, _vp, _vp, _vp, C.c_int)
+_sig("cfx_assemble_div", C.c_int, _vp, _vp, C.c_int, C.c_double, _vp, _vp, C.c_int)
+_sig("cfx_assemble_ddt", C.c_int, _vp, _vp, C.c_double, C.c_int)
+_sig("cfx_matrix_add_rhs", C.c_int, _vp, C.c_int, _vp)
+_sig("cfx_matrix_get", C.c_int, _vp, _vp, _vp, _vp, _vp)
+_sig("cfx_matrix_device_ptrs", C.c_int, _vp, C.POINTER(_vp), C.POINTER(_vp))
+_sig("cfx_nonorth_correct", C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp)
+_sig("cfx_matrix_solve", C.c_int, _vp, _vp, C.c_int, C.c_double, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int,
+     C.c_int, C.c_int, C.c_int, C.POINTER(Report))
+_sig("cfx_matrix_explicit_operator", C.c_int, _vp, _vp, C.c_int, _vp)
+_sig("cfx_matrix_spmv", C.c_int, _vp, C.c_int, _vp, _vp)
+_sig("cfx_csr_create", C.c_int, _vp, _i64, _vp, _vp, _vp, C.POINTER(_vp))
+_sig("cfx_csr_destroy", C.c_int, _vp)
+_sig("cfx_csr_set_values", C.c_int, _vp, _vp)
+_sig("cfx_csr_spmv", C.c_int, _vp, _vp, _vp)
+_sig("cfx_csr_solve", C.c_int, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
+     C.POINTER(Report))
+_sig("cfx_csr_solve_device", C.c_int, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
+     C.POINTER(Report))
+_sig("cfx_csr_residual_history", C.c_int, _vp, _vp, C.c_int)
+_sig("cfx_csr_set_inner_tol", C.c_int, _vp, C.c_double)
+_sig("cfx_csr_time_spmv", C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, _dp)
+_sig("cfx_matrix_time_spmv", C.c_int, _vp, C.c_int, _vp, _vp, C.c_int, C.c_int, _dp)
+# host mesh
+_sig("cfx_hmesh_box", C.c_int, _i64, _i64, _i64, _vp, _vp, C.c_double, C.c_uint64, C.c_int, C.POINTER(_vp))
+_sig("cfx_hmesh_from_arrays", C.c_int, _i64, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp,
+     C.POINTER(_vp))
+_sig("cfx_hmesh_free", C.c_int, _vp)
+_sig("cfx_hmesh_sizes", C.c_int, _vp, _vp)
+_sig("cfx_hmesh_get", C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp)
+_sig("cfx_hmesh_geometry", C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp)
+_sig("cfx_hmesh_shuffle", C.c_int, _vp, C.c_uint64)
+_sig("cfx_hmesh_renumber_rcm", C.c_int, _vp, _vp)
+_sig("cfx_hmesh_bandwidth", C.c_int, _vp, C.POINTER(_i64))
+_sig("cfx_hmesh_last_error", C.c_char_p)
+
+
+def lib():
+    return _L
+
+
+def declared_symbols():
+    """Function names declared in include/*.h."""
+    import re
+    names = []
+    for h in sorted(os.listdir(INCLUDE_DIR)):
+        if h.endswith(".h"):
+            txt = open(os.path.join(INCLUDE_DIR, h)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names += re.findall(r"\b(cfx_[a-z0-9_]+)\s*\(", txt)
+    return sorted(set(names))
+
+
+def _p(a):
+    if a is None:
+        return None
+    if isinstance(a, DeviceArray):
+        return a.ptr
+    return a.ctypes.data_as(_vp)
+
+
+def _host_check(rc):
+    if rc != CFX_OK:
+        raise CfxError(rc, _L.cfx_hmesh_last_error().decode())
+
+
+# --------------------------------------------------------------- context
+class Context:
+    """A HIP context (one device, one stream)."""
+
+    def __init__(self, device=0):
+        h = _vp()
+        rc = _L.cfx_create(device, C.byref(h))
+        if rc != CFX_OK:
+            raise CfxError(rc, f"cannot create a context on device {device}")
+        self.h = h
+
+    def check(self, rc):
+        if rc != CFX_OK:
+            raise CfxError(rc, _L.cfx_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            _L.cfx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        self.check(_L.cfx_synchronize(self.h))
+
+    def device_info(self):
+        buf = C.create_string_buffer(64)
+        cus = C.c_int(0)
+        self.check(_L.cfx_device_info(self.h, buf, 64, C.byref(cus)))
+        return buf.value.decode(), cus.value
+
+    def empty(self, shape, dtype=np.float64):
+        return DeviceArray(self, shape, dtype)
+
+    def zeros(self, shape, dtype=np.float64):
+        a = DeviceArray(self, shape, dtype)
+        self.check(_L.cfx_zero(self.h, a.ptr, a.nbytes))
+        return a
+
+    def array(self, host, dtype=None):
+        host = np.ascontiguousarray(host, dtype=dtype)
+        a = DeviceArray(self, host.shape, host.dtype)
+        self.check(_L.cfx_memcpy_htod(self.h, a.ptr, _p(host), a.nbytes))
+        return a
+
+
+class DeviceArray:
+    """Device buffer owned by a Context."""
+
+    def __init__(self, ctx: Context, shape, dtype=np.float64):
+        self.ctx = ctx
+        self.shape = tuple(np.atleast_1d(shape).tolist()) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = _vp()
+        ctx.check(_L.cfx_malloc(ctx.h, max(self.nbytes, 1), C.byref(p)))
+        self.ptr = p
+
+    def get(self):
+        out = np.empty(self.shape, self.dtype)
+        self.ctx.check(_L.cfx_memcpy_dtoh(self.ctx.h, _p(out), self.ptr, self.nbytes))
+        return out
+
+    def set(self, host):
+        host = np.ascontiguousarray(host, dtype=self.dtype)
+        assert host.nbytes == self.nbytes
+        self.ctx.check(_L.cfx_memcpy_htod(self.ctx.h, self.ptr, _p(host), self.nbytes))
+
+    def copy_from(self, other: "DeviceArray"):
+        assert other.nbytes == self.nbytes
+        self.ctx.check(_L.cfx_memcpy_dtod(self.ctx.h, self.ptr, other.ptr, self.nbytes))
+
+    def offset(self, elems):
+        """raw device pointer elems into the buffer"""
+        return _vp(self.ptr.value + int(elems) * self.dtype.itemsize)
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.h:
+                _L.cfx_free(self.ctx.h, self.ptr)
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------ host mesh
+class HostMesh:
+    """The library's host polyMesh (BlockMesh, jitter, Kuhn tets, RCM)."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    @classmethod
+    def box(cls, nx, ny, nz, mn=(0, 0, 0), mx=(1, 1, 1), jitter=0.0, seed=0, tets=False):
+        h = _vp()
+        _host_check(_L.cfx_hmesh_box(nx, ny, nz, _p(np.asarray(mn, np.float64)), _p(np.asarray(mx, np.float64)),
+                                     jitter, seed, int(tets), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, points, faceStart, faceVerts, owner, neighbour, patchStart, patchSize):
+        h = _vp()
+        pts = np.ascontiguousarray(points, np.float64)
+        arrs = [np.ascontiguousarray(a, np.int64) for a in (faceStart, faceVerts, owner, neighbour, patchStart,
+                                                             patchSize)]
+        _host_check(_L.cfx_hmesh_from_arrays(len(pts), _p(pts), len(arrs[2]), _p(arrs[0]), _p(arrs[1]),
+                                             len(arrs[3]), _p(arrs[2]), _p(arrs[3]), len(arrs[4]), _p(arrs[4]),
+                                             _p(arrs[5]), C.byref(h)))
+        return cls(h)
+
+    def __del__(self):
+        try:
+            if self.h:
+                _L.cfx_hmesh_free(self.h)
+        except Exception:
+            pass
+
+    def sizes(self):
+        s = np.zeros(6, np.int64)
+        _host_check(_L.cfx_hmesh_sizes(self.h, _p(s)))
+        return dict(nPoints=int(s[0]), nFaces=int(s[1]), nInternal=int(s[2]), nCells=int(s[3]),
+                    nPatches=int(s[4]), nFaceVerts=int(s[5]))
+
+    def arrays(self):
+        s = self.sizes()
+        a = dict(points=np.zeros((s["nPoints"], 3)), faceStart=np.zeros(s["nFaces"] + 1, np.int64),
+                 faceVerts=np.zeros(s["nFaceVerts"], np.int64), owner=np.zeros(s["nFaces"], np.int64),
+                 neighbour=np.zeros(max(1, s["nInternal"]), np.int64),
+                 patchStart=np.zeros(s["nPatches"], np.int64), patchSize=np.zeros(s["nPatches"], np.int64))
+        _host_check(_L.cfx_hmesh_get(self.h, *[_p(a[k]) for k in ("points", "faceStart", "faceVerts", "owner",
+                                                                   "neighbour", "patchStart", "patchSize")]))
+        a["neighbour"] = a["neighbour"][:s["nInternal"]]
+        a["nCells"] = s["nCells"]
+        return a
+
+    def geometry(self):
+        s = self.sizes()
+        nF, nC = s["nFaces"], s["nCells"]
+        g = dict(Sf=np.zeros((nF, 3)), Cf=np.zeros((nF, 3)), magSf=np.zeros(nF), C=np.zeros((nC, 3)),
+                 V=np.zeros(nC), dvec=np.zeros((nF, 3)), invDist=np.zeros(nF))
+        _host_check(_L.cfx_hmesh_geometry(self.h, *[_p(g[k]) for k in ("Sf", "Cf", "magSf", "C", "V", "dvec",
+                                                                        "invDist")]))
+        return g
+
+    def shuffle(self, seed):
+        _host_check(_L.cfx_hmesh_shuffle(self.h, seed))
+
+    def renumber_rcm(self):
+        newId = np.zeros(self.sizes()["nCells"], np.int64)
+        _host_check(_L.cfx_hmesh_renumber_rcm(self.h, _p(newId)))
+        return newId
+
+    def bandwidth(self):
+        bw = _i64(0)
+        _host_check(_L.cfx_hmesh_bandwidth(self.h, C.byref(bw)))
+        return bw.value
+
+
+# ------------------------------------------------------------ device mesh
+class Mesh:
+    """Device mesh: face streams, gathers, geometry coefficients, CSR pattern."""
+
+    def __init__(self, ctx: Context, arrays: dict, geom: dict):
+        self.ctx = ctx
+        self.nCells = int(arrays["nCells"])
+        self.nInternal = len(arrays["neighbour"])
+        self.nFaces = len(arrays["owner"])
+        self.nB = self.nFaces - self.nInternal
+        self.patchStart = np.asarray(arrays["patchStart"], np.int64)
+        self.patchSize = np.asarray(arrays["patchSize"], np.int64)
+        own = np.ascontiguousarray(arrays["owner"], np.int64)
+        nb = np.ascontiguousarray(arrays["neighbour"], np.int64)
+        if len(nb) == 0:
+            nb = np.zeros(1, np.int64)
+        g = {k: np.ascontiguousarray(v, np.float64) for k, v in geom.items()}
+        h = _vp()
+        ctx.check(_L.cfx_mesh_create(ctx.h, self.nCells, self.nInternal, self.nFaces, _p(own), _p(nb), _p(g["Sf"]),
+                                     _p(g["Cf"]), _p(g["C"]), _p(g["V"]), _p(g["dvec"]), _p(g["magSf"]),
+                                     len(self.patchStart), _p(self.patchStart), _p(self.patchSize), C.byref(h)))
+        self.h = h
+        self.V = g["V"]
+
+    @classmethod
+    def from_host(cls, ctx, hm: HostMesh):
+        return cls(ctx, hm.arrays(), hm.geometry())
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                _L.cfx_mesh_destroy(self.h)
+        except Exception:
+            pass
+
+    def sizes(self):
+        s = np.zeros(5, np.int64)
+        self.ctx.check(_L.cfx_mesh_sizes(self.h, _p(s)))
+        return dict(nCells=int(s[0]), nInternal=int(s[1]), nFaces=int(s[2]), nnz=int(s[3]), nRowBlocks=int(s[4]))
+
+    def pattern(self):
+        s = self.sizes()
+        rp = np.zeros(s["nCells"] + 1, np.int32)
+        cl = np.zeros(s["nnz"], np.int32)
+        self.ctx.check(_L.cfx_mesh_pattern(self.h, _p(rp), _p(cl)))
+        return rp, cl
+
+    # field ops (device arrays in/out)
+    def grad(self, x, linc, link, ncomp=1, comp=0, corrected=True, quirk=True, out=None):
+        out = out if out is not None else self.ctx.empty((3, self.nCells))
+        self.ctx.check(_L.cfx_grad(self.h, _p(x), _p(linc), _p(link), ncomp, comp, int(corrected), int(quirk),
+                                   _p(out)))
+        return out
+
+    def harmonic_gamma_uniform(self, gamma, out=None):
+        out = out if out is not None else self.ctx.empty(self.nFaces)
+        self.ctx.check(_L.cfx_harmonic_gamma_uniform(self.h, gamma, _p(out)))
+        return out
+
+    def interpolate_to_faces(self, cell, out=None):
+        out = out if out is not None else self.ctx.empty(self.nFaces)
+        self.ctx.check(_L.cfx_interpolate_to_faces(self.h, _p(cell), _p(out)))
+        return out
+
+    def update_phi_internal(self, U3, phi):
+        self.ctx.check(_L.cfx_update_phi_internal(self.h, _p(U3), _p(phi)))
+
+    def correct_boundary_flux(self, U3, linc3, link3, phi):
+        self.ctx.check(_L.cfx_correct_boundary_flux(self.h, _p(U3), _p(linc3), _p(link3), _p(phi)))
+
+    def correct_fluxes(self, p, gradP3, rAUf, linc, link, corrected, phi):
+        self.ctx.check(_L.cfx_correct_fluxes(self.h, _p(p), _p(gradP3), _p(rAUf), _p(linc), _p(link),
+                                             int(corrected), _p(phi)))
+
+    def div_phi(self, phi, out=None):
+        out = out if out is not None else self.ctx.empty(self.nCells)
+        self.ctx.check(_L.cfx_div_phi(self.h, _p(phi), _p(out)))
+        return out
+
+    def continuity_error(self, phi):
+        e = C.c_double(0)
+        self.ctx.check(_L.cfx_continuity_error(self.h, _p(phi), C.byref(e)))
+        return e.value
+
+    def nonorth_correct(self, gammaFace, x, linc, link, rhs, ncomp=1, comp=0, gradCorrected=True, sign=-1):
+        self.ctx.check(_L.cfx_nonorth_correct(self.h, _p(gammaFace), _p(x), _p(linc), _p(link), ncomp, comp,
+                                              int(gradCorrected), sign, _p(rhs)))
+
+
+class Matrix:
+    """Device fvMatrix on the mesh's fixed pattern."""
+
+    def __init__(self, mesh: Mesh, ncomp=1):
+        self.mesh, self.ctx, self.ncomp = mesh, mesh.ctx, ncomp
+        h = _vp()
+        self.ctx.check(_L.cfx_matrix_create(mesh.h, ncomp, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                _L.cfx_matrix_destroy(self.h)
+        except Exception:
+            pass
+
+    def reset(self):
+        self.ctx.check(_L.cfx_matrix_reset(self.h))
+
+    def laplacian(self, gammaFace, linc, link, sign=-1):
+        self.ctx.check(_L.cfx_assemble_laplacian(self.h, _p(gammaFace), _p(linc), _p(link), sign))
+
+    def div(self, phi, linc, link, scheme=0, blend=0.9, sign=1):
+        self.ctx.check(_L.cfx_assemble_div(self.h, _p(phi), scheme, blend, _p(linc), _p(link), sign))
+
+    def ddt(self, old, dt, sign=1):
+        self.ctx.check(_L.cfx_assemble_ddt(self.h, _p(old), dt, sign))
+
+    def add_rhs(self, comp, v):
+        self.ctx.check(_L.cfx_matrix_add_rhs(self.h, comp, _p(v)))
+
+    def get(self):
+        n, nI = self.mesh.nCells, self.mesh.nInternal
+        d = np.zeros((self.ncomp, n))
+        r = np.zeros((self.ncomp, n))
+        u = np.zeros(max(1, nI))
+        lo = np.zeros(max(1, nI))
+        self.ctx.check(_L.cfx_matrix_get(self.h, _p(d), _p(r), _p(u), _p(lo)))
+        return d, r, u[:nI], lo[:nI]
+
+    def solve(self, x, solver=SOLVER_CG, tol=1e-8, maxIter=2000, nonOrth=2, gammaFace=None, linc=None, link=None,
+              sign=-1, gradCorrected=True, precond=PRECOND_JACOBI, precision=FP64):
+        rep = Report()
+        self.ctx.check(_L.cfx_matrix_solve(self.h, _p(x), solver, tol, maxIter, nonOrth, _p(gammaFace), _p(linc),
+                                           _p(link), sign, int(gradCorrected), precond, precision, C.byref(rep)))
+        return rep
+
+    def time_spmv(self, comp, x, y, reps=20, precision=FP64):
+        ms = C.c_double(0)
+        self.ctx.check(_L.cfx_matrix_time_spmv(self.h, comp, _p(x), _p(y), reps, precision, C.byref(ms)))
+        return ms.value
+
+    def explicit_operator(self, x, comp, out=None):
+        out = out if out is not None else self.ctx.empty(self.mesh.nCells)
+        self.ctx.check(_L.cfx_matrix_explicit_operator(self.h, _p(x), comp, _p(out)))
+        return out
+
+    def spmv(self, comp, x, out=None):
+        out = out if out is not None else self.ctx.empty(self.mesh.nCells)
+        self.ctx.check(_L.cfx_matrix_spmv(self.h, comp, _p(x), _p(out)))
+        return out
+
+
+class Csr:
+    """Device CSR matrix for the linear-solver seam (solveCG / solveBiCGStab)."""
+
+    def __init__(self, ctx: Context, rowStart, cols, vals):
+        self.ctx = ctx
+        rs = np.ascontiguousarray(rowStart, np.int64)
+        cl = np.ascontiguousarray(cols, np.int64)
+        vv = np.ascontiguousarray(vals, np.float64)
+        self.n = len(rs) - 1
+        h = _vp()
+        ctx.check(_L.cfx_csr_create(ctx.h, self.n, _p(rs), _p(cl), _p(vv), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                _L.cfx_csr_destroy(self.h)
+        except Exception:
+            pass
+
+    def set_values(self, vals):
+        self.ctx.check(_L.cfx_csr_set_values(self.h, _p(np.ascontiguousarray(vals, np.float64))))
+
+    def spmv(self, x, out=None):
+        out = out if out is not None else self.ctx.empty(self.n)
+        self.ctx.check(_L.cfx_csr_spmv(self.h, _p(x), _p(out)))
+        return out
+
+    def solve(self, b, x0=None, tol=1e-8, maxIter=2000, solver=SOLVER_CG, precond=PRECOND_JACOBI, precision=FP64):
+        """Host b / x (the reference solveCG signature); returns (x, Report)."""
+        x = np.zeros(self.n) if x0 is None else np.array(x0, np.float64)
+        rep = Report()
+        self.ctx.check(_L.cfx_csr_solve(self.h, _p(np.ascontiguousarray(b, np.float64)), _p(x), tol, maxIter, solver,
+                                        precond, precision, C.byref(rep)))
+        return x, rep
+
+    def solve_device(self, b, x, tol=1e-8, maxIter=2000, solver=SOLVER_CG, precond=PRECOND_JACOBI, precision=FP64):
+        rep = Report()
+        self.ctx.check(_L.cfx_csr_solve_device(self.h, _p(b), _p(x), tol, maxIter, solver, precond, precision,
+                                               C.byref(rep)))
+        return rep
+
+    def residual_history(self, maxLen):
+        h = np.zeros(maxLen)
+        self.ctx.check(_L.cfx_csr_residual_history(self.h, _p(h), maxLen))
+        return h
+
+    def time_spmv(self, x, y, reps=20, precision=FP64):
+        ms = C.c_double(0)
+        self.ctx.check(_L.cfx_csr_time_spmv(self.h, _p(x), _p(y), reps, precision, C.byref(ms)))
+        return ms.value
+
+    def set_inner_tol(self, t):
+        self.ctx.check(_L.cfx_csr_set_inner_tol(self.h, t))

@@ -1,0 +1,194 @@
+/*
+ * cfx.h -- C ABI of the MI355X SIMPLE pressure-velocity hot path.
+ *
+ * Plain pointers and sizes only.  Conventions:
+ *  - every function returns an int status: CFX_OK (0) or a CFX_E_* code;
+ *    cfx_last_error(ctx) returns the message of the last failure on ctx;
+ *  - pointers named d_* are DEVICE pointers (cfx_malloc or any HIP
+ *    allocation on the context's device); everything else is host memory,
+ *    borrowed for the duration of the call;
+ *  - cell vectors on the device are SoA [3][nCells]; boundary
+ *    linearisations are lin_c/lin_k [nBoundaryFaces][ncomp] with
+ *    Tf = c * Tp + k (the reference's volScalarField/volVectorField::linearise);
+ *  - one HIP stream per context; a context is not thread safe, separate
+ *    contexts are independent.
+ *
+ * Each declaration names the reference interface it replaces
+ * (reference file:line).
+ */
+#ifndef CFX_H
+#define CFX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CFX_OK 0
+#define CFX_E_INVALID 1     /* bad argument / shape */
+#define CFX_E_HIP 2         /* HIP runtime error */
+#define CFX_E_STATE 3       /* object in the wrong state */
+#define CFX_E_NUMERIC 4     /* non-finite values where the reference throws */
+#define CFX_E_INTERNAL 5
+
+typedef struct cfx_ctx cfx_ctx;
+typedef struct cfx_mesh cfx_mesh;
+typedef struct cfx_matrix cfx_matrix;
+typedef struct cfx_csr cfx_csr;
+
+/* SolverReport (src/common/include/cf/common/Matrix.h:57-63) plus
+ * extensions: true FP64 residuals, refinement sweeps, preconditioner
+ * fallbacks, device time. */
+typedef struct cfx_report {
+    int iterations;
+    double initialResidual;
+    double finalResidual;
+    int converged;
+    int breakdown;          /* 1: pAp <= 0 (not SPD); 2: BiCGStab t.t < 1e-300 */
+    double l2Residual;      /* ||b - A x||2 / ||b||2, FP64 (mixed precision) */
+    double trueResidual;    /* ||b - A x||inf / ||b||inf, FP64 (mixed precision) */
+    int outerIterations;    /* FP64 refinement sweeps */
+    int precondFallbacks;   /* learned preconditioner rejected -> Jacobi */
+    double ms;              /* device time of the solve */
+} cfx_report;
+
+enum { CFX_PRECOND_JACOBI = 0, CFX_PRECOND_DIC = 1, CFX_PRECOND_LEARNED = 2 };
+enum { CFX_FP64 = 0, CFX_FP32_IR = 1 };
+enum { CFX_SOLVER_CG = 0, CFX_SOLVER_BICGSTAB = 1 };
+
+/* ------------------------------------------------------------ context */
+/* No reference counterpart (the reference is single-process CPU code). */
+int cfx_create(int device, cfx_ctx** out);
+int cfx_destroy(cfx_ctx* ctx);
+const char* cfx_last_error(const cfx_ctx* ctx);
+int cfx_synchronize(cfx_ctx* ctx);
+/* name: gcnArch string, cus: compute units */
+int cfx_device_info(cfx_ctx* ctx, char* name, size_t nameLen, int* cus);
+int cfx_malloc(cfx_ctx* ctx, size_t bytes, void** d_out);
+int cfx_free(cfx_ctx* ctx, void* d_ptr);
+int cfx_memcpy_htod(cfx_ctx* ctx, void* d_dst, const void* src, size_t bytes);
+int cfx_memcpy_dtoh(cfx_ctx* ctx, void* dst, const void* d_src, size_t bytes);
+int cfx_memcpy_dtod(cfx_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
+/* zero-fill by kernel (safe inside captured graphs) */
+int cfx_zero(cfx_ctx* ctx, void* d_ptr, size_t bytes);
+
+/* ------------------------------------------------------------- mesh */
+/* Device mesh from a reference polyMesh's arrays
+ * (src/3d/include/cf/fvm/PolyMesh.h:17-103): owner/neighbour (int64 label),
+ * faceSf, faceCentres, cellCentres, cellVolumes, deltaCoeffVec, faceAreas
+ * (Field<Vector> AoS [n][3]), patches as start/size.  Builds the face
+ * streams, cell->face gathers, geometry-derived face coefficients and the
+ * fixed CSR pattern (the reference rebuilds it by a sort on every solve,
+ * src/common/src/Matrix.cpp:56-117). */
+int cfx_mesh_create(cfx_ctx* ctx, int64_t nCells, int64_t nInternal, int64_t nFaces,
+                    const int64_t* owner, const int64_t* neighbour, const double* Sf,
+                    const double* Cf, const double* C, const double* V, const double* d,
+                    const double* magSf, int64_t nPatches, const int64_t* patchStart,
+                    const int64_t* patchSize, cfx_mesh** out);
+int cfx_mesh_destroy(cfx_mesh* mesh);
+/* sizes: {nCells, nInternal, nFaces, nnz, nRowBlocks} */
+int cfx_mesh_sizes(cfx_mesh* mesh, int64_t sizes[5]);
+/* host copy of the CSR pattern: rowPtr[nCells+1], cols[nnz] (int32) */
+int cfx_mesh_pattern(cfx_mesh* mesh, int32_t* rowPtr, int32_t* cols);
+/* makeInternalOwnerWeights (src/3d/src/discrete/interpolate.cpp:28-35) */
+int cfx_owner_weights(cfx_mesh* mesh, double* d_w);
+
+/* ------------------------------------------------------- field ops */
+/* fvc::grad (src/3d/include/cf/fvm/FvMatrix.h:122-125, gradScheme.cpp:15-96):
+ * corrected != 0 -> "corrected" (2 skew passes); else "linear", where
+ * quirk != 0 reproduces volScalarField::faceValue on internal faces
+ * (volFields.cpp:101-111) as fvc::grad(volScalarField) does. */
+int cfx_grad(cfx_mesh* mesh, const double* d_x, const double* d_linc, const double* d_link,
+             int ncomp, int comp, int corrected, int quirk, double* d_grad3);
+/* harmonicFaceGamma with a uniform value (laplacianScheme.cpp:37-62) */
+int cfx_harmonic_gamma_uniform(cfx_mesh* mesh, double gamma, double* d_gammaFace);
+/* interpolateToFaces (src/3d/include/cf/fvm/PressureCorrection.h:28) */
+int cfx_interpolate_to_faces(cfx_mesh* mesh, const double* d_cell, double* d_face);
+/* updatePhiInternal (PressureCorrection.h:23) */
+int cfx_update_phi_internal(cfx_mesh* mesh, const double* d_U3, double* d_phi);
+/* surfaceScalarField::correctBoundaryFlux (src/3d/include/cf/fvm/Fields.h:144) */
+int cfx_correct_boundary_flux(cfx_mesh* mesh, const double* d_U3, const double* d_linc3,
+                              const double* d_link3, double* d_phi);
+/* correctFluxes, the Rhie-Chow closure (PressureCorrection.h:15-20) */
+int cfx_correct_fluxes(cfx_mesh* mesh, const double* d_p, const double* d_gradP3,
+                       const double* d_rAUf, const double* d_linc, const double* d_link,
+                       int corrected, double* d_phi);
+/* fvc::divPhi (FvMatrix.h:131) */
+int cfx_div_phi(cfx_mesh* mesh, const double* d_phi, double* d_out);
+/* continuityError (PressureCorrection.h:31) */
+int cfx_continuity_error(cfx_mesh* mesh, const double* d_phi, double* err);
+
+/* ----------------------------------------------------------- fvMatrix */
+/* fvMatrix (src/3d/include/cf/fvm/FvMatrix.h:42-101) on the fixed pattern. */
+int cfx_matrix_create(cfx_mesh* mesh, int ncomp, cfx_matrix** out);
+int cfx_matrix_destroy(cfx_matrix* A);
+int cfx_matrix_reset(cfx_matrix* A);
+/* fvm::laplacian / fvm::laplacianFace (FvMatrix.h:170-175, laplacianScheme.cpp:67-104) */
+int cfx_assemble_laplacian(cfx_matrix* A, const double* d_gammaFace, const double* d_linc,
+                           const double* d_link, int sign);
+/* fvm::div (FvMatrix.h:166, divScheme.cpp:11-62): scheme 0 upwind, 1 linear, 2 blend */
+int cfx_assemble_div(cfx_matrix* A, const double* d_phi, int scheme, double blendCoeff,
+                     const double* d_linc, const double* d_link, int sign);
+/* fvm::ddt (FvMatrix.h:162-163, divScheme.cpp:64-94); d_old is [ncomp][n] */
+int cfx_assemble_ddt(cfx_matrix* A, const double* d_old, double dt, int sign);
+/* fvMatrix::addRhs over all cells of one component: rhs[comp] += d_v */
+int cfx_matrix_add_rhs(cfx_matrix* A, int comp, const double* d_v);
+/* host copies of diag/rhs [ncomp][n] and upper/lower [nInternal] */
+int cfx_matrix_get(cfx_matrix* A, double* diag, double* rhs, double* upper, double* lower);
+/* device pointers of diag/rhs [ncomp][n] */
+int cfx_matrix_device_ptrs(cfx_matrix* A, double** d_diag, double** d_rhs);
+/* the laplacian corrector run by fvMatrix::solve before each pass
+ * (laplacianScheme.cpp:111-154, fvMatrix.cpp:86-91): adds into d_rhs */
+int cfx_nonorth_correct(cfx_mesh* mesh, const double* d_gammaFace, const double* d_x,
+                        const double* d_linc, const double* d_link, int ncomp, int comp,
+                        int gradCorrected, int sign, double* d_rhs);
+/* fvMatrix::solve (FvMatrix.h:77, fvMatrix.cpp:73-129): builds the CSR
+ * values, then (nonOrth+1 passes when d_gammaFace != NULL, else 1) restores
+ * the rhs, applies the corrector from the current solution and solves every
+ * component (CG if solver == CFX_SOLVER_CG else BiCGStab) from d_x
+ * ([ncomp][n], initial guess and result).  rep gets the last solve. */
+int cfx_matrix_solve(cfx_matrix* A, double* d_x, int solver, double tol, int maxIter, int nonOrth,
+                     const double* d_gammaFace, const double* d_linc, const double* d_link,
+                     int sign, int gradCorrected, int precond, int precision, cfx_report* rep);
+/* explicitOperator (FvMatrix.h:86, fvMatrix.cpp:131-146): h = rhs - (A x - D x) */
+int cfx_matrix_explicit_operator(cfx_matrix* A, const double* d_x, int comp, double* d_h);
+/* y = A x with the matrix's current CSR values of one component */
+int cfx_matrix_spmv(cfx_matrix* A, int comp, const double* d_x, double* d_y);
+
+/* ------------------------------------------------- linear-solver seam */
+/* SparseMatrix (Matrix.h:13-55) uploaded from reference CSR arrays
+ * (int64 rowStart/cols, rows sorted by column). */
+int cfx_csr_create(cfx_ctx* ctx, int64_t n, const int64_t* rowStart, const int64_t* cols,
+                   const double* vals, cfx_csr** out);
+int cfx_csr_destroy(cfx_csr* A);
+/* replace the values (same pattern) */
+int cfx_csr_set_values(cfx_csr* A, const double* vals);
+/* SparseMatrix::product (Matrix.h:47, Matrix.cpp:119-133) */
+int cfx_csr_spmv(cfx_csr* A, const double* d_x, double* d_y);
+/* solveCG / solveBiCGStab / solveWithBackend (Matrix.h:67-89): host b, x
+ * (x = initial guess and result), tol/maxIter as the reference. */
+int cfx_csr_solve(cfx_csr* A, const double* b, double* x, double tol, int maxIter, int solver,
+                  int precond, int precision, cfx_report* rep);
+/* the same with device-resident b and x */
+int cfx_csr_solve_device(cfx_csr* A, const double* d_b, double* d_x, double tol, int maxIter,
+                         int solver, int precond, int precision, cfx_report* rep);
+/* per-iteration ||r||inf of the last FP64 CG solve: hist[0..iterations] */
+int cfx_csr_residual_history(cfx_csr* A, double* hist, int maxLen);
+/* inner tolerance of the FP32 sweeps of CFX_FP32_IR (default 1e-4) */
+int cfx_csr_set_inner_tol(cfx_csr* A, double innerTol);
+
+/* ----------------------------------------------------------- timing */
+/* Average device time of one SpMV launch (y = A x) over `reps` launches,
+ * measured with HIP events on the context's stream.  precision CFX_FP64
+ * uses the FP64 values, CFX_FP32_IR the FP32 copy (x, y then float).
+ * No reference counterpart (measurement only). */
+int cfx_csr_time_spmv(cfx_csr* A, const void* d_x, void* d_y, int reps, int precision, double* msPerLaunch);
+int cfx_matrix_time_spmv(cfx_matrix* A, int comp, const void* d_x, void* d_y, int reps, int precision,
+                         double* msPerLaunch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,54 @@
+/*
+ * cfx_host.h -- host-side mesh layer of the MI355X hot path (no GPU needed).
+ *
+ * A cfx_hmesh is the library's C++ polyMesh: the reference's mesh model
+ * (src/3d/include/cf/fvm/PolyMesh.h:17-103) with the same geometry, plus the
+ * generators of the benchmark workloads.  Same status codes as cfx.h.
+ */
+#ifndef CFX_HOST_H
+#define CFX_HOST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cfx_hmesh cfx_hmesh;
+
+/* BlockMesh::create (src/3d/include/cf/fvm/BlockMesh.h:31-53,
+ * blockMesh.cpp:36-189): same points, faces, face order and patches
+ * (xmin xmax ymin ymax zmin zmax).  jitter > 0 moves strictly interior
+ * vertices by (2u-1)*jitter*h per component, u from MT19937-64(seed).
+ * tets != 0 splits every hex into 6 Kuhn tetrahedra (no reference
+ * counterpart). */
+int cfx_hmesh_box(int64_t nx, int64_t ny, int64_t nz, const double mn[3], const double mx[3],
+                  double jitter, uint64_t seed, int tets, cfx_hmesh** out);
+/* polyMesh(points, faces, owner, neighbour, patches) (PolyMesh.h:29-33) with
+ * faces as CSR (faceStart[nFaces+1], faceVerts). */
+int cfx_hmesh_from_arrays(int64_t nPoints, const double* points, int64_t nFaces,
+                          const int64_t* faceStart, const int64_t* faceVerts, int64_t nInternal,
+                          const int64_t* owner, const int64_t* neighbour, int64_t nPatches,
+                          const int64_t* patchStart, const int64_t* patchSize, cfx_hmesh** out);
+int cfx_hmesh_free(cfx_hmesh* h);
+/* {nPoints, nFaces, nInternal, nCells, nPatches, nFaceVerts} */
+int cfx_hmesh_sizes(const cfx_hmesh* h, int64_t sizes[6]);
+int cfx_hmesh_get(const cfx_hmesh* h, double* points, int64_t* faceStart, int64_t* faceVerts,
+                  int64_t* owner, int64_t* neighbour, int64_t* patchStart, int64_t* patchSize);
+/* polyMesh geometry (polyMesh.cpp:83-288), AoS [n][3] */
+int cfx_hmesh_geometry(const cfx_hmesh* h, double* Sf, double* Cf, double* magSf, double* C,
+                       double* V, double* d, double* invDist);
+/* seeded shuffle of internal faces, boundary faces within patches, and
+ * cell ids (no reference counterpart) */
+int cfx_hmesh_shuffle(cfx_hmesh* h, uint64_t seed);
+/* reverse Cuthill-McKee renumbering; newId[old] written if non-NULL; faces
+ * re-sorted by (owner, neighbour) (no reference counterpart) */
+int cfx_hmesh_renumber_rcm(cfx_hmesh* h, int64_t* newId);
+int cfx_hmesh_bandwidth(const cfx_hmesh* h, int64_t* bw);
+const char* cfx_hmesh_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
