@@ -142,6 +142,8 @@ def _p(a):
         return None
     if isinstance(a, DeviceArray):
         return a.ptr
+    if isinstance(a, _vp):
+        return a
     return a.ctypes.data_as(_vp)
 
 
