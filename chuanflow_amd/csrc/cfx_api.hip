@@ -57,7 +57,7 @@ struct cfx_csr
     cfx_ctx* owner = nullptr;
     DevPattern pattern;
     DevBuf val, val32, b, x, resHist;
-    double innerTol = 1e-4;
+    double innerTol = 1e-2;  // reliable-update factor of the mixed-precision PCG
     int lastIters = -1;
     bool val32Valid = false;
 };
@@ -528,7 +528,7 @@ int cfx_matrix_solve(cfx_matrix* A, double* x, int solver, double tol, int maxIt
                 DevBuf& v32buf = A->val32;
                 last = solveOn(c, view, A->val.as<double>() + (long long)k * nnz, v32buf, v32,
                                A->rhs.as<double>() + (long long)k * n, x + (long long)k * n, tol, maxIter, solver,
-                               precond, precision, 1e-4, nullptr);
+                               precond, precision, 1e-2, nullptr);
             }
         }
         toReport(last, rep);

@@ -52,17 +52,14 @@ public:
     SolveReport bicgstab(const CsrView& A, const double* val, const double* b, double* x, double tol,
                          int maxIter);
 
-    // FP32-inner Jacobi-PCG with FP64 iterative refinement: the outer loop
-    // keeps x, r = b - A x and the stop test in FP64; each inner sweep solves
-    // A d = r in FP32 to innerTol (relative), then x += d.  `val32` is the
-    // FP32 copy of the matrix.  `project` removes the mean of each inner
-    // right-hand side (pure-Neumann systems).
+    // FP32-inner Jacobi-PCG with FP64 iterative refinement by reliable
+    // updates: the Krylov recurrence runs in FP32 (matrix `val32`) and
+    // accumulates a correction; each time its residual has fallen by `delta`
+    // below the last true residual, the correction is folded into the FP64 x
+    // and r = b - A x is recomputed in FP64 and replaces the FP32 residual.
+    // Convergence is declared on true FP64 residuals only.
     SolveReport cgMixed(const CsrView& A, const double* val, const float* val32, const double* b,
-                        double* x, double tol, int maxIter, double innerTol, bool project);
-
-    // Inner FP32 PCG on its own (used by the refinement and by benchmarks).
-    SolveReport cg32(const CsrView& A, const float* val32, const float* b, float* x, double tol,
-                     int maxIter, bool project);
+                        double* x, double tol, int maxIter, double delta, bool project);
 
     // True residual ||b - A x|| in FP64: returns {inf-norm rel, 2-norm rel}.
     void trueResidual(const CsrView& A, const double* val, const double* b, const double* x,
@@ -76,8 +73,9 @@ public:
 
 private:
     template <typename T>
-    SolveReport runCG(const CsrView& A, const T* val, const T* invD, const T* b, T* x, double tol,
-                      int maxIter, double* resHist, T* r, T* p, T* q, bool project);
+    SolveReport pcgRun(const CsrView& A, const double* val64, const T* valT, const T* invD,
+                       const double* b, double* x64, T* xT, T* r, double tol, int maxIter,
+                       double* resHist, bool mixed, double delta);
     void poll();
     int nextBatch(int done, double r0, double rNow, double tolAbs, int maxIter);
 

@@ -212,147 +212,6 @@ __global__ __launch_bounds__(kSpmvBlock) void k_residual(CsrView A, const T* __r
     }
 }
 
-// z = invD r; p = z; rz = r.z   (solveCG :182-185)
-template <typename T>
-__global__ __launch_bounds__(kVecBlock) void k_cg_init(int n, const T* __restrict__ invD,
-                                                       const T* __restrict__ r, T* __restrict__ p,
-                                                       SolveState* st, double* part)
-{
-    __shared__ double red[kVecBlock / kWave];
-    __shared__ int flag;
-    if (st->done) return;
-    double rz = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    {
-        const T z = invD[i] * r[i];
-        p[i] = z;
-        rz += static_cast<double>(r[i]) * static_cast<double>(z);
-    }
-    rz = blockSum(rz, red);
-    if (threadIdx.x == 0) part[blockIdx.x] = rz;
-    if (!lastBlockDone(&st->ticket[0], &flag)) return;
-    const double RZ = sumPartials(part, gridDim.x, red);
-    if (threadIdx.x == 0) st->rz = RZ;
-}
-
-// q = A p; pAp; alpha  (solveCG :189-195)
-template <typename T>
-__global__ __launch_bounds__(kSpmvBlock) void k_cg_spmv_dot(CsrView A, const T* __restrict__ val,
-                                                            const T* __restrict__ p,
-                                                            T* __restrict__ q, SolveState* st,
-                                                            double* part)
-{
-    __shared__ T lds[kSpmvLds];
-    __shared__ double red[kSpmvBlock / kWave];
-    __shared__ int flag;
-    if (st->done) return;
-    const int G = gridDim.x;
-    const int blk = xcdRemap(blockIdx.x, G);
-    int row;
-    T s;
-    double pq = 0;
-    if (spmvRowSum(A, val, p, blk, lds, row, s))
-    {
-        q[row] = s;
-        pq = static_cast<double>(p[row]) * static_cast<double>(s);
-    }
-    pq = blockSum(pq, red);
-    if (threadIdx.x == 0) part[blk] = pq;
-    if (!lastBlockDone(&st->ticket[0], &flag)) return;
-    const double pAp = sumPartials(part, G, red);
-    if (threadIdx.x == 0)
-    {
-        st->pAp = pAp;
-        if (pAp <= 0 || fabs(st->rz) < 1e-300)
-        {
-            st->done = 1;  // matrix not sufficiently SPD (reference: break)
-            st->breakdown = 1;
-        }
-        else
-        {
-            st->alpha = st->rz / pAp;
-        }
-    }
-}
-
-// x += alpha p; r -= alpha q; z = invD r; rz' = r.z; ||r||inf
-// (solveCG :196-211)
-template <typename T>
-__global__ __launch_bounds__(kVecBlock) void k_cg_update(int n, const T* __restrict__ invD,
-                                                         T* __restrict__ x, T* __restrict__ r,
-                                                         const T* __restrict__ p,
-                                                         const T* __restrict__ q, SolveState* st,
-                                                         double* part, int* ipart,
-                                                         double* resHist)
-{
-    __shared__ double red[kVecBlock / kWave];
-    __shared__ int ired[kVecBlock / kWave];
-    __shared__ int flag;
-    if (st->done) return;
-    const T alpha = static_cast<T>(st->alpha);
-    double rz = 0, rm = 0;
-    int nan = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    {
-        x[i] += alpha * p[i];
-        const T ri = r[i] - alpha * q[i];
-        r[i] = ri;
-        const T z = invD[i] * ri;
-        rz += static_cast<double>(ri) * static_cast<double>(z);
-        const double a = fabs(static_cast<double>(ri));
-        rm = fmax(rm, a);
-        nan |= (ri != ri);
-    }
-    const int G = gridDim.x;
-    rz = blockSum(rz, red);
-    rm = blockMax(rm, red);
-    nan = blockOr(nan, ired);
-    if (threadIdx.x == 0)
-    {
-        part[blockIdx.x] = rz;
-        part[G + blockIdx.x] = rm;
-        ipart[blockIdx.x] = nan;
-    }
-    if (!lastBlockDone(&st->ticket[0], &flag)) return;
-    const double RZ = sumPartials(part, G, red);
-    const double RM = maxPartials(part + G, G, red);
-    const int NAN_ = orPartials(ipart, G, ired);
-    if (threadIdx.x == 0)
-    {
-        const int it = st->iter + 1;
-        const double rnorm = NAN_ ? NAN : RM;
-        st->iter = it;
-        st->rnorm = rnorm;
-        if (resHist) resHist[it] = rnorm;
-        if (rnorm / st->bnorm < st->tol)
-        {
-            st->converged = 1;
-            st->done = 1;
-        }
-        else
-        {
-            st->beta = RZ / st->rz;
-            st->rz = RZ;
-            if (it >= st->maxIter) st->done = 1;
-        }
-    }
-}
-
-// p = invD r + beta p  (solveCG :209-217)
-template <typename T>
-__global__ __launch_bounds__(kVecBlock) void k_cg_direction(int n, const T* __restrict__ invD,
-                                                            const T* __restrict__ r,
-                                                            T* __restrict__ p, const SolveState* st)
-{
-    if (st->done) return;
-    const T beta = static_cast<T>(st->beta);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    {
-        const T z = invD[i] * r[i];
-        p[i] = z + beta * p[i];
-    }
-}
-
 // ---------------------------------------------------------------- BiCGStab
 // r0 = r, p = v = 0, rho/beta of iteration 1 (solveBiCGStab :253-266)
 __global__ __launch_bounds__(kVecBlock) void k_bicg_init(int n, const double* __restrict__ r,
@@ -579,38 +438,6 @@ __global__ void k_fill(T* __restrict__ p, T v, long long n)
         p[i] = v;
 }
 
-// b32 = float(r64) (mean-projected when asked): pass 0 sums r in double
-__global__ __launch_bounds__(kVecBlock) void k_sum(int n, const double* __restrict__ v,
-                                                   SolveState* st, double* part)
-{
-    __shared__ double red[kVecBlock / kWave];
-    __shared__ int flag;
-    double s = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += v[i];
-    s = blockSum(s, red);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
-    if (!lastBlockDone(&st->ticket[0], &flag)) return;
-    const double S = sumPartials(part, gridDim.x, red);
-    if (threadIdx.x == 0) st->sum0 = S;
-}
-
-__global__ void k_to_inner_rhs(int n, const double* __restrict__ r, float* __restrict__ b32,
-                               float* __restrict__ d32, const SolveState* st, int project)
-{
-    const double mean = project ? st->sum0 / n : 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    {
-        b32[i] = static_cast<float>(r[i] - mean);
-        d32[i] = 0.0f;
-    }
-}
-
-__global__ void k_ir_add(int n, double* __restrict__ x, const float* __restrict__ d32)
-{
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        x[i] += static_cast<double>(d32[i]);
-}
-
 int vecGridFor(long long n) { return std::max(1, std::min(kVecGridMax, divUp(n, 4LL * kVecBlock))); }
 
 } // namespace
@@ -631,8 +458,8 @@ Krylov::Krylov(Context& ctx, int n, int maxRowBlocks) : ctx_(ctx), n_(n)
 {
     maxBlocks_ = std::max(maxRowBlocks, kVecGridMax);
     const std::size_t d = sizeof(double) * static_cast<std::size_t>(n), f = sizeof(float) * static_cast<std::size_t>(n);
-    state_.alloc(2 * sizeof(SolveState));
-    CFX_HIP(hipMemsetAsync(state_.p, 0, 2 * sizeof(SolveState), ctx.stream()));
+    state_.alloc(3 * sizeof(SolveState));  // slots: refinement, Krylov, PCG state
+    CFX_HIP(hipMemsetAsync(state_.p, 0, 3 * sizeof(SolveState), ctx.stream()));
     partials_.alloc(sizeof(double) * 4 * maxBlocks_ + sizeof(int) * maxBlocks_);
     for (DevBuf* b : {&r_, &p_, &q_, &r0_, &v_, &s_, &sh_, &t_, &z_, &invD_, &invDb_}) b->alloc(d);
     for (DevBuf* b : {&r32_, &p32_, &q32_, &d32_, &b32_, &invD32_}) b->alloc(f);
@@ -681,72 +508,6 @@ void Krylov::spmv32(const CsrView& A, const float* val, const float* x, float* y
     CFX_CHECK_LAUNCH();
 }
 
-template <typename T>
-SolveReport Krylov::runCG(const CsrView& A, const T* val, const T* invD, const T* b, T* x,
-                          double tol, int maxIter, double* resHist, T* r, T* p, T* q, bool project)
-{
-    hipStream_t s = ctx_.stream();
-    SolveState h{};
-    h.tol = tol;
-    h.maxIter = maxIter;
-    SolveState* st = state_.as<SolveState>() + 1;  // slot 1: Krylov iteration
-    CFX_HIP(hipMemcpyAsync(st, &h, offsetof(SolveState, ticket), hipMemcpyHostToDevice, s));
-    double* part = partials_.as<double>();
-    int* ipart = reinterpret_cast<int*>(part + 4 * maxBlocks_);
-    const int Gv = vecGridFor(n_);
-    (void)project;
-    k_residual<T><<<A.nRowBlocks, kSpmvBlock, 0, s>>>(A, val, x, b, r, st, part, ipart, 0);
-    k_cg_init<T><<<Gv, kVecBlock, 0, s>>>(n_, invD, r, p, st, part);
-    CFX_CHECK_LAUNCH();
-    poll();
-    SolveState* hs = ctx_.hostState() + 1;
-    const double r0 = hs->initialResidual, bnorm = hs->bnorm;
-    if (resHist) CFX_HIP(hipMemcpyAsync(resHist, &hs->initialResidual, sizeof(double), hipMemcpyHostToDevice, s));
-    int launched = 0;
-    while (!hs->done && launched < maxIter)
-    {
-        const int batch = nextBatch(hs->iter, r0, hs->rnorm, tol * bnorm, maxIter);
-        for (int k = 0; k < batch; ++k)
-        {
-            k_cg_spmv_dot<T><<<A.nRowBlocks, kSpmvBlock, 0, s>>>(A, val, p, q, st, part);
-            k_cg_update<T><<<Gv, kVecBlock, 0, s>>>(n_, invD, x, r, p, q, st, part, ipart, resHist);
-            k_cg_direction<T><<<Gv, kVecBlock, 0, s>>>(n_, invD, r, p, st);
-        }
-        CFX_CHECK_LAUNCH();
-        launched += batch;
-        poll();
-    }
-    SolveReport rep;
-    rep.iterations = hs->iter;
-    rep.initialResidual = hs->initialResidual;
-    rep.finalResidual = hs->iter > 0 ? hs->rnorm : hs->initialResidual;
-    rep.converged = hs->converged;
-    rep.breakdown = hs->breakdown;
-    return rep;
-}
-
-SolveReport Krylov::cg(const CsrView& A, const double* val, const double* b, double* x, double tol,
-                       int maxIter, double* resHist)
-{
-    CFX_HIP(hipEventRecord(ev0_, ctx_.stream()));
-    computeInvD(ctx_, A, val, invD_.as<double>(), false);
-    SolveReport rep = runCG<double>(A, val, invD_.as<double>(), b, x, tol, maxIter, resHist,
-                                    r_.as<double>(), p_.as<double>(), q_.as<double>(), false);
-    CFX_HIP(hipEventRecord(ev1_, ctx_.stream()));
-    CFX_HIP(hipEventSynchronize(ev1_));
-    float ms = 0;
-    CFX_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
-    lastMs_ = rep.ms = ms;
-    return rep;
-}
-
-SolveReport Krylov::cg32(const CsrView& A, const float* val32, const float* b, float* x, double tol,
-                         int maxIter, bool project)
-{
-    return runCG<float>(A, val32, invD32_.as<float>(), b, x, tol, maxIter, nullptr, r32_.as<float>(),
-                        p32_.as<float>(), q32_.as<float>(), project);
-}
-
 void Krylov::trueResidual(const CsrView& A, const double* val, const double* b, const double* x,
                           double& infRel, double& l2Rel)
 {
@@ -760,61 +521,6 @@ void Krylov::trueResidual(const CsrView& A, const double* val, const double* b, 
     poll();
     infRel = ctx_.hostState()[0].sum0;
     l2Rel = ctx_.hostState()[0].rho;
-}
-
-SolveReport Krylov::cgMixed(const CsrView& A, const double* val, const float* val32, const double* b,
-                            double* x, double tol, int maxIter, double innerTol, bool project)
-{
-    hipStream_t s = ctx_.stream();
-    CFX_HIP(hipEventRecord(ev0_, s));
-    const int Gv = vecGridFor(n_);
-    // FP32 Jacobi scaling from the FP64 diagonal (CG guard)
-    computeInvD(ctx_, A, val, invD_.as<double>(), false);
-    castToFloat(ctx_, invD_.as<double>(), invD32_.as<float>(), n_);
-    SolveState h{};
-    h.tol = tol;
-    h.maxIter = maxIter;
-    SolveState* st = state_.as<SolveState>();  // slot 0: refinement
-    CFX_HIP(hipMemcpyAsync(st, &h, offsetof(SolveState, ticket), hipMemcpyHostToDevice, s));
-    double* part = partials_.as<double>();
-    int* ipart = reinterpret_cast<int*>(part + 4 * maxBlocks_);
-    double* r = r0_.as<double>();
-    k_residual<double><<<A.nRowBlocks, kSpmvBlock, 0, s>>>(A, val, x, b, r, st, part, ipart, 0);
-    CFX_CHECK_LAUNCH();
-    poll();
-    SolveReport rep;
-    rep.initialResidual = ctx_.hostState()[0].initialResidual;
-    int total = 0, outer = 0;
-    while (!ctx_.hostState()[0].converged && total < maxIter)
-    {
-        if (project) k_sum<<<Gv, kVecBlock, 0, s>>>(n_, r, st, part);
-        k_to_inner_rhs<<<Gv, kVecBlock, 0, s>>>(n_, r, b32_.as<float>(), d32_.as<float>(), st,
-                                                 project ? 1 : 0);
-        CFX_CHECK_LAUNCH();
-        const SolveReport in = runCG<float>(A, val32, invD32_.as<float>(), b32_.as<float>(),
-                                            d32_.as<float>(), innerTol, maxIter - total, nullptr,
-                                            r32_.as<float>(), p32_.as<float>(), q32_.as<float>(), false);
-        total += in.iterations;
-        ++outer;
-        k_ir_add<<<Gv, kVecBlock, 0, s>>>(n_, x, d32_.as<float>());
-        k_residual<double><<<A.nRowBlocks, kSpmvBlock, 0, s>>>(A, val, x, b, r, st, part, ipart, 1);
-        CFX_CHECK_LAUNCH();
-        poll();
-        if (in.iterations == 0) break;  // no progress possible in FP32
-    }
-    const SolveState& hs = ctx_.hostState()[0];
-    rep.iterations = total;
-    rep.outerIterations = outer;
-    rep.finalResidual = hs.rnorm;
-    rep.trueResidual = hs.rnorm / hs.bnorm;
-    rep.l2Residual = hs.rho;
-    rep.converged = hs.converged;
-    CFX_HIP(hipEventRecord(ev1_, s));
-    CFX_HIP(hipEventSynchronize(ev1_));
-    float ms = 0;
-    CFX_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
-    lastMs_ = rep.ms = ms;
-    return rep;
 }
 
 SolveReport Krylov::bicgstab(const CsrView& A, const double* val, const double* b, double* x,
@@ -868,10 +574,5 @@ SolveReport Krylov::bicgstab(const CsrView& A, const double* val, const double* 
     lastMs_ = rep.ms = ms;
     return rep;
 }
-
-template SolveReport Krylov::runCG<double>(const CsrView&, const double*, const double*, const double*,
-                                           double*, double, int, double*, double*, double*, double*, bool);
-template SolveReport Krylov::runCG<float>(const CsrView&, const float*, const float*, const float*,
-                                          float*, double, int, double*, float*, float*, float*, bool);
 
 } // namespace cfx

@@ -1,0 +1,503 @@
+// pcg.hip -- Jacobi-PCG with two kernels per iteration and no atomics.
+//
+// Iteration k runs
+//   A(k): reduce the previous update's partials (r.z, max|r|) -- every
+//         workgroup redundantly, in one fixed order, so all agree bit for
+//         bit -- take the stop decision, beta = rz_k / rz_{k-1}, and compute
+//         q = A p with p = invD r + beta p_old formed on the fly while the
+//         products are staged in LDS (CSR-stream), writing p for its own
+//         rows; partial p.q
+//   B(k): reduce p.q, alpha = rz / p.q, x += alpha p, r -= alpha q,
+//         partials r.(invD r) and max|r|
+// Workgroup 0 records the scalars for the host, which only polls `done`
+// between batches of iterations.  Scalars that a kernel reads while its own
+// workgroup 0 writes the next value are double-buffered by iteration parity.
+//
+// Mixed precision (reliable updates): the Krylov iteration runs in FP32 on
+// an FP32 copy of the matrix and accumulates a correction x32; whenever the
+// FP32 recurrence residual has dropped by `delta` below the last true
+// residual (or claims convergence) the iteration pauses, the host folds x32
+// into the FP64 solution, the true residual r = b - A x is recomputed in
+// FP64 and replaces the FP32 residual, and the iteration resumes with the
+// same search direction.  Convergence is only ever declared on a true FP64
+// residual.
+#include "krylov.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace cfx
+{
+namespace
+{
+
+constexpr int kRedMax = 2048;  // partial count cap (grids are capped to this)
+constexpr int kB = 256;
+
+struct CgState
+{
+    double rz[2];        // r.z after iteration k (slot k & 1)
+    double bnorm, initialResidual, rnorm, alpha, pAp, updAbs, beta;
+    int iter, done, converged, breakdown, paused, pad;
+};
+
+__device__ __forceinline__ int xcdRemap(int h, int G)
+{
+    constexpr int X = 8;
+    const int q = G / X, r = G % X;
+    const int x = h % X, idx = h / X;
+    return (x < r) ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
+}
+
+// Deterministic all-workgroup reduction of G <= 2048 partials (blockDim 256):
+// 8 independent loads per thread, a fixed tree, broadcast through LDS.
+__device__ __forceinline__ double allSum(const double* __restrict__ p, int G, double* lds)
+{
+    double v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+    {
+        const int i = threadIdx.x + j * kB;
+        v[j] = i < G ? p[i] : 0.0;
+    }
+    double s = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+    s = blockSum(s, lds);
+    if (threadIdx.x == 0) lds[0] = s;
+    __syncthreads();
+    s = lds[0];
+    __syncthreads();
+    return s;
+}
+
+// max over partials, where a partial of -1 marks a NaN in that workgroup
+__device__ __forceinline__ double allMaxNan(const double* __restrict__ p, int G, double* lds)
+{
+    double m = 0.0, mn = 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+    {
+        const int i = threadIdx.x + j * kB;
+        const double v = i < G ? p[i] : 0.0;
+        m = fmax(m, v);
+        mn = fmin(mn, v);
+    }
+    m = blockMax(m, lds);
+    const double anyNan = blockMax(-mn, lds);  // 1 if some partial was -1
+    if (threadIdx.x == 0) lds[0] = anyNan > 0 ? NAN : m;
+    __syncthreads();
+    m = lds[0];
+    __syncthreads();
+    return m;
+}
+
+// ------------------------------------------------------------------ kernels
+// r = b - A x (FP64); rT = (T) r; partials: max|r| (NaN -> -1), max|b|,
+// rT.(invD rT).  Also used after a reliable update.
+template <typename T>
+__global__ __launch_bounds__(kB) void k_pcg_residual(CsrView A, int rbPerWg, const double* __restrict__ val,
+                                                     const double* __restrict__ x, const double* __restrict__ b,
+                                                     double* __restrict__ r64, T* __restrict__ rT,
+                                                     const T* __restrict__ invD, double* __restrict__ partRZ,
+                                                     double* __restrict__ partRM, double* __restrict__ partBM)
+{
+    __shared__ double lds[kSpmvLds];
+    __shared__ double red[kB / kWave];
+    const int w = xcdRemap(blockIdx.x, gridDim.x);
+    double rm = 0, bm = 0, rz = 0;
+    int nan = 0;
+    for (int j = 0; j < rbPerWg; ++j)
+    {
+        const int blk = w * rbPerWg + j;
+        if (blk >= A.nRowBlocks) break;
+        const int r0 = A.rowBlocks[blk], r1 = A.rowBlocks[blk + 1];
+        const int k0 = A.rowPtr[r0], nnz = A.rowPtr[r1] - k0;
+        const int row = r0 + static_cast<int>(threadIdx.x);
+        double s = 0;
+        if (nnz <= kSpmvLds)
+        {
+            for (int k = threadIdx.x; k < nnz; k += kB) lds[k] = val[k0 + k] * x[A.col[k0 + k]];
+            __syncthreads();
+            if (row < r1)
+                for (int k = A.rowPtr[row] - k0, e = A.rowPtr[row + 1] - k0; k < e; ++k) s += lds[k];
+            __syncthreads();
+        }
+        else if (row < r1)
+        {
+            for (int k = A.rowPtr[row]; k < A.rowPtr[row + 1]; ++k) s += val[k] * x[A.col[k]];
+        }
+        if (row < r1)
+        {
+            const double bb = b[row];
+            const double rr = bb - s;
+            if (r64) r64[row] = rr;
+            const T rt = static_cast<T>(rr);
+            rT[row] = rt;
+            const T z = invD[row] * rt;
+            rz += static_cast<double>(rt) * static_cast<double>(z);
+            rm = fmax(rm, fabs(rr));
+            bm = fmax(bm, fabs(bb));
+            nan |= (rr != rr) | ((bb != bb) << 1);
+        }
+    }
+    rz = blockSum(rz, red);
+    rm = blockMax(rm, red);
+    bm = blockMax(bm, red);
+    __shared__ int ired[kB / kWave];
+    nan = blockOr(nan, ired);
+    if (threadIdx.x == 0)
+    {
+        partRZ[blockIdx.x] = rz;
+        partRM[blockIdx.x] = (nan & 1) ? -1.0 : rm;
+        partBM[blockIdx.x] = (nan & 2) ? -1.0 : bm;
+    }
+}
+
+// A(k): q = A p (CSR-stream, LDS-staged), partial p.q.
+template <typename T>
+__global__ __launch_bounds__(kB) void k_pcg_A(CsrView A, int rbPerWg, const T* __restrict__ val,
+                                              const T* __restrict__ p, T* __restrict__ q, const CgState* st,
+                                              double* __restrict__ partPQ)
+{
+    __shared__ T lds[kSpmvLds];
+    __shared__ double red[kB / kWave];
+    if (st->done) return;
+    const int w = xcdRemap(blockIdx.x, gridDim.x);
+    double pq = 0;
+    for (int j = 0; j < rbPerWg; ++j)
+    {
+        const int blk = w * rbPerWg + j;
+        if (blk >= A.nRowBlocks) break;
+        const int r0 = A.rowBlocks[blk], r1 = A.rowBlocks[blk + 1];
+        const int k0 = A.rowPtr[r0], nnz = A.rowPtr[r1] - k0;
+        const int row = r0 + static_cast<int>(threadIdx.x);
+        T s = T(0);
+        if (nnz <= kSpmvLds)
+        {
+            for (int kk = threadIdx.x; kk < nnz; kk += kB) lds[kk] = val[k0 + kk] * p[A.col[k0 + kk]];
+            __syncthreads();
+            if (row < r1)
+                for (int kk = A.rowPtr[row] - k0, e = A.rowPtr[row + 1] - k0; kk < e; ++kk) s += lds[kk];
+            __syncthreads();
+        }
+        else if (row < r1)
+        {
+            for (int kk = A.rowPtr[row]; kk < A.rowPtr[row + 1]; ++kk) s += val[kk] * p[A.col[kk]];
+        }
+        if (row < r1)
+        {
+            q[row] = s;
+            pq += static_cast<double>(p[row]) * static_cast<double>(s);
+        }
+    }
+    pq = blockSum(pq, red);
+    if (threadIdx.x == 0) partPQ[blockIdx.x] = pq;
+}
+
+// B(k): reduce p.q, alpha = rz / p.q; x += alpha p, r -= alpha q;
+// partials r.(invD r), max|r|   (solveCG :189-211)
+template <typename T>
+__global__ __launch_bounds__(kB) void k_pcg_B(int n, const T* __restrict__ invD, T* __restrict__ x,
+                                              T* __restrict__ r, const T* __restrict__ p,
+                                              const T* __restrict__ q, CgState* st,
+                                              const double* __restrict__ partPQ, int Gs, double* __restrict__ partRZ,
+                                              double* __restrict__ partRM, int k)
+{
+    __shared__ double red[kB / kWave];
+    __shared__ int ired[kB / kWave];
+    if (st->done) return;
+    const double PAP = allSum(partPQ, Gs, red);
+    const double rz = st->rz[(k - 1) & 1];
+    if (PAP <= 0 || fabs(rz) < 1e-300)
+    {
+        // matrix not sufficiently SPD (solveCG breaks here, Matrix.cpp:191-194)
+        if (blockIdx.x == 0 && threadIdx.x == 0) { st->breakdown = 1; st->done = 1; st->pAp = PAP; }
+        return;
+    }
+    const double alphaD = rz / PAP;
+    if (blockIdx.x == 0 && threadIdx.x == 0) { st->alpha = alphaD; st->pAp = PAP; }
+    const T alpha = static_cast<T>(alphaD);
+    double rzp = 0, rm = 0;
+    int nan = 0;
+    for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB)
+    {
+        x[i] += alpha * p[i];
+        const T ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        const T z = invD[i] * ri;
+        rzp += static_cast<double>(ri) * static_cast<double>(z);
+        rm = fmax(rm, fabs(static_cast<double>(ri)));
+        nan |= (ri != ri);
+    }
+    rzp = blockSum(rzp, red);
+    rm = blockMax(rm, red);
+    nan = blockOr(nan, ired);
+    if (threadIdx.x == 0)
+    {
+        partRZ[blockIdx.x] = rzp;
+        partRM[blockIdx.x] = nan ? -1.0 : rm;
+    }
+}
+
+// C(k): stop decision on the residual of iteration k (k = 0: the initial
+// residual), beta = rz_k / rz_{k-1}, p = invD r + beta p  (solveCG :202-217).
+// flags bit0: the partials hold a TRUE residual (after a residual kernel);
+//       bit1: mixed precision (pause instead of converging on sloppy residuals)
+template <typename T>
+__global__ __launch_bounds__(kB) void k_pcg_C(int n, const T* __restrict__ invD, const T* __restrict__ r,
+                                              T* __restrict__ p, CgState* st, const double* __restrict__ partRZ,
+                                              const double* __restrict__ partRM, const double* __restrict__ partBM,
+                                              int nPart, int k, double tol, int maxIter, double delta, int flags,
+                                              double* resHist)
+{
+    __shared__ double red[kB / kWave];
+    if (st->done) return;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const double RZ = allSum(partRZ, nPart, red);
+    const double RM = allMaxNan(partRM, nPart, red);
+    const bool trueRes = flags & 1, mixed = flags & 2;
+    if (k == 0)
+    {
+        const double BM = allMaxNan(partBM, nPart, red);
+        const double bnorm = (BM < 1e-300) ? 1e-300 : BM;  // std::max(||b||inf, 1e-300)
+        if (lead)
+        {
+            st->bnorm = bnorm;
+            st->initialResidual = RM;
+            st->rnorm = RM;
+            st->iter = 0;
+            if (resHist) resHist[0] = RM;
+        }
+        if (RM / bnorm < tol)
+        {
+            if (lead) { st->converged = 1; st->done = 1; }
+            return;
+        }
+        if (maxIter <= 0)
+        {
+            if (lead) st->done = 1;
+            return;
+        }
+    }
+    else
+    {
+        const double bnorm = st->bnorm;
+        if (lead)
+        {
+            st->iter = k;
+            st->rnorm = RM;
+            if (resHist) resHist[k] = RM;
+        }
+        const bool conv = RM / bnorm < tol;
+        if (conv && (!mixed || trueRes))
+        {
+            if (lead) { st->converged = 1; st->done = 1; }
+            return;
+        }
+        if (k >= maxIter)
+        {
+            if (lead) st->done = 1;
+            return;
+        }
+        if (mixed && !trueRes && (conv || RM < st->updAbs))
+        {
+            if (lead) { st->paused = 1; st->done = 1; }
+            return;
+        }
+    }
+    if (mixed && trueRes && lead) st->updAbs = delta * RM;
+    const double beta = k == 0 ? 0.0 : RZ / st->rz[(k - 1) & 1];
+    if (lead)
+    {
+        st->rz[k & 1] = RZ;
+        st->beta = beta;
+    }
+    const T tb = static_cast<T>(beta);
+    for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB)
+    {
+        const T z = invD[i] * r[i];
+        p[i] = k == 0 ? z : z + tb * p[i];
+    }
+}
+
+// reliable update: x64 += x32, x32 = 0
+__global__ void k_fold(int n, double* __restrict__ x, float* __restrict__ x32)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        x[i] += static_cast<double>(x32[i]);
+        x32[i] = 0.0f;
+    }
+}
+
+__global__ void k_resume(CgState* st)
+{
+    st->done = 0;
+    st->paused = 0;
+}
+
+__global__ void k_inv_diag32(CsrView A, const double* __restrict__ val, float* __restrict__ invD32,
+                             double* __restrict__ invD64)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    const int k = A.diagSlot[i];
+    const double d = k >= 0 ? val[k] : 0.0;
+    const double inv = (fabs(d) > 1e-300) ? 1.0 / d : 1.0;
+    invD64[i] = inv;
+    invD32[i] = static_cast<float>(inv);
+}
+
+} // namespace
+
+// ------------------------------------------------------------------- host
+struct PcgLaunch
+{
+    int Gs, rbPerWg, Gv;
+};
+
+static PcgLaunch pcgGrid(const CsrView& A, int n)
+{
+    PcgLaunch g;
+    g.Gs = std::min(A.nRowBlocks, kRedMax);
+    g.rbPerWg = divUp(A.nRowBlocks, g.Gs);
+    g.Gs = divUp(A.nRowBlocks, g.rbPerWg);
+    g.Gv = std::max(1, std::min(kRedMax, divUp(n, 4 * kB)));
+    return g;
+}
+
+template <typename T>
+SolveReport Krylov::pcgRun(const CsrView& A, const double* val64, const T* valT, const T* invD, const double* b,
+                           double* x64, T* xT, T* r, double tol, int maxIter, double* resHist, bool mixed,
+                           double delta)
+{
+    hipStream_t s = ctx_.stream();
+    const PcgLaunch g = pcgGrid(A, n_);
+    if (g.Gs > kRedMax || g.Gv > kRedMax) throw std::invalid_argument("pcg: grid exceeds the partial buffer");
+    double* part = partials_.as<double>();
+    double *partRZ = part, *partRM = part + kRedMax, *partBM = part + 2 * kRedMax, *partPQ = part + 3 * kRedMax;
+    CgState* st = reinterpret_cast<CgState*>(state_.as<char>() + 2 * sizeof(SolveState));
+    CFX_HIP(hipMemsetAsync(st, 0, sizeof(CgState), s));
+    T* p = reinterpret_cast<T*>(mixed ? p32_.p : p_.p);
+    T* q = reinterpret_cast<T*>(mixed ? q32_.p : q_.p);
+    double* r64 = mixed ? r0_.as<double>() : nullptr;
+
+    // initial residual (FP64); for FP64 runs r is the residual itself
+    k_pcg_residual<T><<<g.Gs, kB, 0, s>>>(A, g.rbPerWg, val64, x64, b, r64, r, invD, partRZ, partRM, partBM);
+    CFX_CHECK_LAUNCH();
+    CgState* hs = reinterpret_cast<CgState*>(ctx_.hostState() + 2);
+    auto pollState = [&] {
+        CFX_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+        CFX_HIP(hipStreamSynchronize(s));
+    };
+    int k = 0, outer = 1;
+    int flags = mixed ? 3 : 0;  // the C(k) after a residual kernel sees a true residual
+    double r0 = -1;
+    // C(0) decides on the initial residual and forms p; then A,B,C per iteration
+    k_pcg_C<T><<<g.Gv, kB, 0, s>>>(n_, invD, r, p, st, partRZ, partRM, partBM, g.Gs, 0, tol, maxIter, delta, flags,
+                                   resHist);
+    flags &= ~1;
+    for (;;)
+    {
+        int batch = minBatch;
+        if (r0 > 0 && hs->iter > 0 && hs->rnorm > 0 && hs->rnorm < r0)
+        {
+            const double rate = std::log(hs->rnorm / r0) / hs->iter;
+            const double target = mixed ? std::max(tol * hs->bnorm, hs->updAbs) : tol * hs->bnorm;
+            const double need = std::log(target / hs->rnorm) / rate;
+            if (std::isfinite(need)) batch = static_cast<int>(0.9 * need);
+        }
+        batch = std::max(minBatch, std::min(maxBatch, batch));
+        batch = std::min(batch, maxIter - k);
+        if (batch <= 0) break;  // C(maxIter) has been launched and stops
+        for (int j = 0; j < batch; ++j)
+        {
+            ++k;
+            k_pcg_A<T><<<g.Gs, kB, 0, s>>>(A, g.rbPerWg, valT, p, q, st, partPQ);
+            k_pcg_B<T><<<g.Gv, kB, 0, s>>>(n_, invD, xT, r, p, q, st, partPQ, g.Gs, partRZ, partRM, k);
+            k_pcg_C<T><<<g.Gv, kB, 0, s>>>(n_, invD, r, p, st, partRZ, partRM, partBM, g.Gv, k, tol, maxIter, delta,
+                                           flags, resHist);
+        }
+        CFX_CHECK_LAUNCH();
+        pollState();
+        if (r0 < 0) r0 = hs->initialResidual;
+        if (!hs->done) continue;
+        if (mixed && hs->paused && !hs->converged)
+        {
+            // reliable update at iteration hs->iter: fold, true residual, decide again there
+            k = hs->iter;
+            k_fold<<<g.Gv, kB, 0, s>>>(n_, x64, reinterpret_cast<float*>(xT));
+            k_pcg_residual<T><<<g.Gs, kB, 0, s>>>(A, g.rbPerWg, val64, x64, b, r64, r, invD, partRZ, partRM, partBM);
+            k_resume<<<1, 1, 0, s>>>(st);
+            k_pcg_C<T><<<g.Gv, kB, 0, s>>>(n_, invD, r, p, st, partRZ, partRM, partBM, g.Gs, k, tol, maxIter, delta,
+                                           3, resHist);
+            CFX_CHECK_LAUNCH();
+            ++outer;
+            continue;
+        }
+        break;
+    }
+    if (mixed && !hs->converged)
+    {
+        k_fold<<<g.Gv, kB, 0, s>>>(n_, x64, reinterpret_cast<float*>(xT));
+        CFX_CHECK_LAUNCH();
+    }
+    SolveReport rep;
+    rep.iterations = hs->iter;
+    rep.initialResidual = hs->initialResidual;
+    rep.finalResidual = hs->iter > 0 ? hs->rnorm : hs->initialResidual;
+    rep.converged = hs->converged;
+    rep.breakdown = hs->breakdown;
+    rep.outerIterations = mixed ? outer : 0;
+    if (mixed) rep.trueResidual = hs->converged ? hs->rnorm / hs->bnorm : -1;
+    return rep;
+}
+
+SolveReport Krylov::cg(const CsrView& A, const double* val, const double* b, double* x, double tol, int maxIter,
+                       double* resHist)
+{
+    hipStream_t s = ctx_.stream();
+    CFX_HIP(hipEventRecord(ev0_, s));
+    computeInvD(ctx_, A, val, invD_.as<double>(), false);
+    SolveReport rep = pcgRun<double>(A, val, val, invD_.as<double>(), b, x, x, r_.as<double>(), tol, maxIter,
+                                     resHist, false, 0.0);
+    CFX_HIP(hipEventRecord(ev1_, s));
+    CFX_HIP(hipEventSynchronize(ev1_));
+    float ms = 0;
+    CFX_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+    lastMs_ = rep.ms = ms;
+    return rep;
+}
+
+SolveReport Krylov::cgMixed(const CsrView& A, const double* val, const float* val32, const double* b, double* x,
+                            double tol, int maxIter, double delta, bool /*project*/)
+{
+    hipStream_t s = ctx_.stream();
+    CFX_HIP(hipEventRecord(ev0_, s));
+    k_inv_diag32<<<divUp(A.n, 256), 256, 0, s>>>(A, val, invD32_.as<float>(), invD_.as<double>());
+    CFX_HIP(hipMemsetAsync(d32_.p, 0, sizeof(float) * n_, s));
+    CFX_CHECK_LAUNCH();
+    SolveReport rep = pcgRun<float>(A, val, val32, invD32_.as<float>(), b, x, d32_.as<float>(), r32_.as<float>(),
+                                    tol, maxIter, nullptr, true, delta);
+    if (rep.converged)
+    {
+        double inf = 0, l2 = 0;
+        trueResidual(A, val, b, x, inf, l2);
+        rep.trueResidual = inf;
+        rep.l2Residual = l2;
+    }
+    CFX_HIP(hipEventRecord(ev1_, s));
+    CFX_HIP(hipEventSynchronize(ev1_));
+    float ms = 0;
+    CFX_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+    lastMs_ = rep.ms = ms;
+    return rep;
+}
+
+template SolveReport Krylov::pcgRun<double>(const CsrView&, const double*, const double*, const double*,
+                                            const double*, double*, double*, double*, double, int, double*, bool,
+                                            double);
+template SolveReport Krylov::pcgRun<float>(const CsrView&, const double*, const float*, const float*,
+                                           const double*, double*, float*, float*, double, int, double*, bool,
+                                           double);
+
+} // namespace cfx

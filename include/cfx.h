@@ -176,7 +176,9 @@ int cfx_csr_solve_device(cfx_csr* A, const double* d_b, double* d_x, double tol,
                          int solver, int precond, int precision, cfx_report* rep);
 /* per-iteration ||r||inf of the last FP64 CG solve: hist[0..iterations] */
 int cfx_csr_residual_history(cfx_csr* A, double* hist, int maxLen);
-/* inner tolerance of the FP32 sweeps of CFX_FP32_IR (default 1e-4) */
+/* CFX_FP32_IR: the factor by which the FP32 recurrence residual must fall
+ * below the last true FP64 residual before the next reliable update
+ * (default 1e-2) */
 int cfx_csr_set_inner_tol(cfx_csr* A, double innerTol);
 
 /* ----------------------------------------------------------- timing */
