@@ -4,6 +4,7 @@
 #include "dev/fvops.h"
 #include "dev/krylov.h"
 #include "dev/mesh.h"
+#include "dev/simple.h"
 
 #include <cstring>
 #include <map>
@@ -504,33 +505,23 @@ int cfx_matrix_solve(cfx_matrix* A, double* x, int solver, double tol, int maxIt
     cfx_ctx* c = A->mesh->owner;
     return guard(c, [&] {
         DevMesh& m = *A->mesh->m;
-        hipStream_t s = c->ctx->stream();
-        const int n = m.n, nnz = m.pattern.nnz;
-        const std::size_t bytes = sizeof(double) * A->ncomp * n;
-        for (int k = 0; k < A->ncomp; ++k) lduToCsr(m, A->ldu(), k, A->val.as<double>() + (long long)k * nnz);
-        CFX_HIP(hipMemcpyAsync(A->rhsBase.p, A->rhs.p, bytes, hipMemcpyDeviceToDevice, s));
-        const int passes = gf ? std::max(1, nonOrth + 1) : 1;
-        const CsrView view = m.pattern.view();
-        SolveReport last;
-        bool v32 = false;
-        for (int pass = 0; pass < passes; ++pass)
-        {
-            CFX_HIP(hipMemcpyAsync(A->rhs.p, A->rhsBase.p, bytes, hipMemcpyDeviceToDevice, s));
-            if (gf)
-                for (int k = 0; k < A->ncomp; ++k)
-                    nonOrthCorrect(m, gf, x + (long long)k * n, linc, link, A->ncomp, k, gradCorrected != 0, sign,
-                                   A->rhs.as<double>() + (long long)k * n, A->mesh->gradTmp.as<double>(),
-                                   A->mesh->faceTmp.as<double>());
-            for (int k = 0; k < A->ncomp; ++k)
-            {
-                // each component has its own values: refresh the FP32 copy per component
-                v32 = false;
-                DevBuf& v32buf = A->val32;
-                last = solveOn(c, view, A->val.as<double>() + (long long)k * nnz, v32buf, v32,
-                               A->rhs.as<double>() + (long long)k * n, x + (long long)k * n, tol, maxIter, solver,
-                               precond, precision, 1e-2, nullptr);
-            }
-        }
+        need(solver == CFX_SOLVER_CG || solver == CFX_SOLVER_BICGSTAB, "unknown solver");
+        need(precision == CFX_FP64 || (precision == CFX_FP32_IR && solver == CFX_SOLVER_CG), "unknown precision");
+        need(precond == CFX_PRECOND_JACOBI, "preconditioner not available in this build");
+        FvSolveArgs a;
+        a.solver = solver;
+        a.tol = tol;
+        a.maxIter = maxIter;
+        a.nonOrth = nonOrth;
+        a.gammaFace = gf;
+        a.linc = linc;
+        a.link = link;
+        a.sign = sign;
+        a.gradCorrected = gradCorrected != 0;
+        a.precision = precision;
+        const SolveReport last = fvSolve(m, c->solver(m.n, m.pattern.nRowBlocks), A->ldu(), A->val.as<double>(),
+                                         A->rhsBase.as<double>(), A->val32, x, a, A->mesh->gradTmp.as<double>(),
+                                         A->mesh->faceTmp.as<double>());
         toReport(last, rep);
     });
 }
@@ -742,6 +733,103 @@ int cfx_matrix_time_spmv(cfx_matrix* A, int comp, const void* x, void* y, int re
         bool ok = false;
         *ms = timeSpmv(A->mesh->owner, m.pattern.view(), A->val.as<double>() + (long long)comp * m.pattern.nnz,
                        A->val32, ok, x, y, reps, precision);
+    });
+}
+
+int cfx_linearise(cfx_mesh* m, int ncomp, const int32_t* type, const double* value, double* linc, double* link)
+{
+    if (!m) return CFX_E_INVALID;
+    return guard(m->owner, [&] {
+        need(ncomp == 1 || ncomp == 3, "ncomp must be 1 or 3");
+        const int nB = m->m->nB;
+        DevBuf t(sizeof(int) * std::max(1, nB)), v(sizeof(double) * ncomp * std::max(1, nB));
+        hipStream_t s = m->owner->ctx->stream();
+        CFX_HIP(hipMemcpyAsync(t.p, type, sizeof(int) * nB, hipMemcpyHostToDevice, s));
+        CFX_HIP(hipMemcpyAsync(v.p, value, sizeof(double) * ncomp * nB, hipMemcpyHostToDevice, s));
+        need(lineariseBc(*m->m, ncomp, t.as<int>(), v.as<double>(), linc, link),
+             "fields: symmetry normal not axis aligned, or unknown bc type");
+    });
+}
+
+namespace
+{
+Schemes toSchemes(const cfx_schemes* s)
+{
+    Schemes o;
+    if (s)
+    {
+        o.div = s->div;
+        o.laplacianCorrected = s->laplacianCorrected;
+        o.gradCorrected = s->gradCorrected;
+        o.blendCoeff = s->blendCoeff;
+    }
+    return o;
+}
+} // namespace
+
+int cfx_simple_solve(cfx_mesh* m, double nu, const cfx_schemes* schemes, const cfx_simple_controls* ctl,
+                     const int32_t* Utype, const double* Uvalue, const int32_t* ptype, const double* pvalue,
+                     double* U, double* p, double* phi, cfx_simple_result* result, double* hist)
+{
+    if (!m || !ctl) return CFX_E_INVALID;
+    return guard(m->owner, [&] {
+        need(m->m->nB == 0 || (Utype && Uvalue && ptype && pvalue), "null boundary conditions");
+        need(ctl->pPrecision == CFX_FP64 || ctl->pPrecision == CFX_FP32_IR, "unknown pressure precision");
+        SimpleControls c;
+        c.maxIterations = ctl->maxIterations;
+        c.continuityTol = ctl->continuityTol;
+        c.pcorrTol = ctl->pcorrTol;
+        c.momentumTol = ctl->momentumTol;
+        c.linearTol = ctl->linearTol;
+        c.linearMaxIter = ctl->linearMaxIter;
+        c.nonOrthCorrectors = ctl->nonOrthCorrectors;
+        c.relaxU = ctl->relaxU;
+        c.relaxP = ctl->relaxP;
+        c.pRelaxStart = ctl->pRelaxStart;
+        for (int k = 0; k < 3; ++k) c.bodyForce[k] = ctl->bodyForce[k];
+        c.pPrecision = ctl->pPrecision;
+        const SimpleResult r = simpleSolve(*m->m, m->owner->solver(m->m->n, m->m->pattern.nRowBlocks), nu,
+                                           toSchemes(schemes), c, Utype, Uvalue, ptype, pvalue, U, p, phi, hist);
+        if (result)
+        {
+            result->iterations = r.iterations;
+            result->finalContinuityError = r.finalContinuityError;
+            result->finalMomentumResidual = r.finalMomentumResidual;
+            result->converged = r.converged;
+            result->diverged = r.diverged;
+            result->ms = r.ms;
+        }
+    });
+}
+
+int cfx_piso_step(cfx_mesh* m, double nu, const cfx_schemes* schemes, const cfx_piso_controls* ctl,
+                  const int32_t* Utype, const double* Uvalue, const int32_t* ptype, const double* pvalue, double* U,
+                  double* p, double* phi, cfx_piso_result* result)
+{
+    if (!m || !ctl) return CFX_E_INVALID;
+    return guard(m->owner, [&] {
+        need(m->m->nB == 0 || (Utype && Uvalue && ptype && pvalue), "null boundary conditions");
+        need(ctl->pPrecision == CFX_FP64 || ctl->pPrecision == CFX_FP32_IR, "unknown pressure precision");
+        PisoControls c;
+        c.dt = ctl->dt;
+        c.correctors = ctl->correctors;
+        c.continuityTol = ctl->continuityTol;
+        c.linearTol = ctl->linearTol;
+        c.linearMaxIter = ctl->linearMaxIter;
+        c.nonOrthCorrectors = ctl->nonOrthCorrectors;
+        for (int k = 0; k < 3; ++k) c.bodyForce[k] = ctl->bodyForce[k];
+        c.pPrecision = ctl->pPrecision;
+        const PisoResult r = pisoStep(*m->m, m->owner->solver(m->m->n, m->m->pattern.nRowBlocks), nu,
+                                      toSchemes(schemes), c, Utype, Uvalue, ptype, pvalue, U, p, phi);
+        if (result)
+        {
+            result->correctorsUsed = r.correctorsUsed;
+            result->predictedContinuityError = r.predictedContinuityError;
+            result->continuityError = r.continuityError;
+            result->pressureCorrectionMax = r.pressureCorrectionMax;
+            result->converged = r.converged;
+            result->ms = r.ms;
+        }
     });
 }
 

@@ -1,0 +1,98 @@
+// simple.h -- the pressure-velocity drivers on the device.
+#pragma once
+
+#include "fvops.h"
+#include "krylov.h"
+
+namespace cfx
+{
+
+struct Schemes
+{
+    int div = 0;              // 0 upwind, 1 linear, 2 blend
+    int laplacianCorrected = 1;
+    int gradCorrected = 1;
+    double blendCoeff = 0.9;
+};
+
+// fvMatrix::solve (fvMatrix.cpp:73-129) on the fixed pattern: CSR values
+// of every component from the LDU, base rhs kept, then per pass the rhs is
+// restored, the laplacian corrector (if gammaFace) adds its terms from the
+// current solution, and every component is solved (CG or BiCGStab) from x.
+// On return A.rhs holds the last pass' rhs (what explicitOperator reads).
+struct FvSolveArgs
+{
+    int solver = 0;  // 0 CG, 1 BiCGStab
+    double tol = 1e-7;
+    int maxIter = 2000;
+    int nonOrth = 2;
+    const double* gammaFace = nullptr;  // corrector registered when non-null
+    const double* linc = nullptr;
+    const double* link = nullptr;
+    int sign = -1;
+    bool gradCorrected = true;
+    int precision = kFP64;
+    double delta = 1e-2;
+};
+SolveReport fvSolve(DevMesh& m, Krylov& K, const Ldu& A, double* val, double* rhsBase, DevBuf& val32,
+                    double* x, const FvSolveArgs& a, double* gradTmp, double* faceTmp);
+
+// volScalarField/volVectorField::linearise (volFields.cpp:117-147, :246-283)
+// on the device: types/values per boundary face (value [nB][ncomp]) ->
+// lin_c/lin_k [nB][ncomp].  Returns false if a symmetry/wall normal is not
+// axis aligned (the reference throws there).
+bool lineariseBc(DevMesh& m, int ncomp, const int* dType, const double* dValue, double* linc, double* link);
+
+struct SimpleControls
+{
+    int maxIterations = 200;
+    double continuityTol = 1e-6, pcorrTol = 0, momentumTol = 0;
+    double linearTol = 1e-7;
+    int linearMaxIter = 2000, nonOrthCorrectors = 2;
+    double relaxU = 0.7, relaxP = 0.3, pRelaxStart = 0;
+    double bodyForce[3] = {0, 0, 0};
+    int pPrecision = kFP64;  // pressure solve: FP64 or FP32 inner + FP64 refinement
+};
+
+struct SimpleResult
+{
+    int iterations = 0;
+    double finalContinuityError = 0, finalMomentumResidual = 0;
+    int converged = 0, diverged = 0;
+    double ms = 0;
+};
+
+// solveIncompressibleSimple (simpleSolver.cpp:33-394), laminar, steady.
+// U [3][n], p [n], phi [nF] device, in/out.  Boundary conditions per
+// boundary face (host): type codes 0 fixedValue, 1 fixedGradient,
+// 2 zeroGradient/outlet, 3 symmetry (vectors: symmetry or wall).
+// hist (host, optional) gets {contErr, errAfter, pDeltaMax, uRes} per iteration.
+SimpleResult simpleSolve(DevMesh& m, Krylov& K, double nu, const Schemes& s, const SimpleControls& c,
+                         const int* Utype, const double* Uvalue, const int* ptype, const double* pvalue, double* U,
+                         double* p, double* phi, double* hist);
+
+struct PisoControls
+{
+    double dt = 0.01;
+    int correctors = 2;
+    double continuityTol = 1e-10, linearTol = 1e-8;
+    int linearMaxIter = 2000, nonOrthCorrectors = 2;
+    double bodyForce[3] = {0, 0, 0};
+    int pPrecision = kFP64;
+};
+
+struct PisoResult
+{
+    int correctorsUsed = 0;
+    double predictedContinuityError = 0, continuityError = 0, pressureCorrectionMax = 0;
+    int converged = 0;
+    double ms = 0;
+};
+
+class PisoWorkspace;  // buffers reused across time steps
+
+// solveTransientIncompressiblePiso (pisoSolver.cpp:13-235), laminar, one step.
+PisoResult pisoStep(DevMesh& m, Krylov& K, double nu, const Schemes& s, const PisoControls& c, const int* Utype,
+                    const double* Uvalue, const int* ptype, const double* pvalue, double* U, double* p, double* phi);
+
+} // namespace cfx

@@ -42,6 +42,37 @@ class Report(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Schemes(C.Structure):
+    """cfx_schemes: div 0 upwind / 1 linear / 2 blend; laplacian and grad corrected flags."""
+    _fields_ = [("div", C.c_int), ("laplacianCorrected", C.c_int), ("gradCorrected", C.c_int),
+                ("blendCoeff", C.c_double)]
+
+
+class SimpleControls(C.Structure):
+    _fields_ = [("maxIterations", C.c_int), ("continuityTol", C.c_double), ("pcorrTol", C.c_double),
+                ("momentumTol", C.c_double), ("linearTol", C.c_double), ("linearMaxIter", C.c_int),
+                ("nonOrthCorrectors", C.c_int), ("relaxU", C.c_double), ("relaxP", C.c_double),
+                ("pRelaxStart", C.c_double), ("bodyForce", C.c_double * 3), ("pPrecision", C.c_int)]
+
+
+class SimpleResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("finalContinuityError", C.c_double),
+                ("finalMomentumResidual", C.c_double), ("converged", C.c_int), ("diverged", C.c_int),
+                ("ms", C.c_double)]
+
+
+class PisoControls(C.Structure):
+    _fields_ = [("dt", C.c_double), ("correctors", C.c_int), ("continuityTol", C.c_double),
+                ("linearTol", C.c_double), ("linearMaxIter", C.c_int), ("nonOrthCorrectors", C.c_int),
+                ("bodyForce", C.c_double * 3), ("pPrecision", C.c_int)]
+
+
+class PisoResult(C.Structure):
+    _fields_ = [("correctorsUsed", C.c_int), ("predictedContinuityError", C.c_double),
+                ("continuityError", C.c_double), ("pressureCorrectionMax", C.c_double), ("converged", C.c_int),
+                ("ms", C.c_double)]
+
+
 class CfxError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"cfx error {code}: {msg}")
@@ -105,6 +136,11 @@ _sig("cfx_csr_solve_device", C.c_int, _vp, _vp, _vp, C.c_double, C.c_int, C.c_in
      C.POINTER(Report))
 _sig("cfx_csr_residual_history", C.c_int, _vp, _vp, C.c_int)
 _sig("cfx_csr_set_inner_tol", C.c_int, _vp, C.c_double)
+_sig("cfx_linearise", C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp)
+_sig("cfx_simple_solve", C.c_int, _vp, C.c_double, C.POINTER(Schemes), C.POINTER(SimpleControls), _vp, _vp, _vp, _vp,
+     _vp, _vp, _vp, C.POINTER(SimpleResult), _vp)
+_sig("cfx_piso_step", C.c_int, _vp, C.c_double, C.POINTER(Schemes), C.POINTER(PisoControls), _vp, _vp, _vp, _vp,
+     _vp, _vp, _vp, C.POINTER(PisoResult))
 _sig("cfx_csr_time_spmv", C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, _dp)
 _sig("cfx_matrix_time_spmv", C.c_int, _vp, C.c_int, _vp, _vp, C.c_int, C.c_int, _dp)
 # host mesh
@@ -396,6 +432,34 @@ class Mesh:
         e = C.c_double(0)
         self.ctx.check(_L.cfx_continuity_error(self.h, _p(phi), C.byref(e)))
         return e.value
+
+    def linearise(self, ncomp, types, values):
+        """device (lin_c, lin_k) [nB][ncomp] of per-boundary-face BC types/values"""
+        t = np.ascontiguousarray(types, np.int32)
+        v = np.ascontiguousarray(values, np.float64).reshape(self.nB, ncomp)
+        lc, lk = self.ctx.empty((max(1, self.nB), ncomp)), self.ctx.empty((max(1, self.nB), ncomp))
+        self.ctx.check(_L.cfx_linearise(self.h, ncomp, _p(t), _p(v), _p(lc), _p(lk)))
+        return lc, lk
+
+    def simple(self, nu, schemes: "Schemes", ctl: "SimpleControls", Ubc, pbc, U, p, phi, hist=False):
+        """solveIncompressibleSimple on device fields U [3][n], p [n], phi [nFaces]"""
+        ut, uv = np.ascontiguousarray(Ubc[0], np.int32), np.ascontiguousarray(Ubc[1], np.float64)
+        pt, pv = np.ascontiguousarray(pbc[0], np.int32), np.ascontiguousarray(pbc[1], np.float64)
+        res = SimpleResult()
+        h = np.zeros((ctl.maxIterations, 4)) if hist else None
+        self.ctx.check(_L.cfx_simple_solve(self.h, nu, C.byref(schemes), C.byref(ctl), _p(ut), _p(uv), _p(pt),
+                                           _p(pv), _p(U), _p(p), _p(phi), C.byref(res), _p(h)))
+        if hist:
+            return res, h[:res.iterations]
+        return res
+
+    def piso_step(self, nu, schemes: "Schemes", ctl: "PisoControls", Ubc, pbc, U, p, phi):
+        ut, uv = np.ascontiguousarray(Ubc[0], np.int32), np.ascontiguousarray(Ubc[1], np.float64)
+        pt, pv = np.ascontiguousarray(pbc[0], np.int32), np.ascontiguousarray(pbc[1], np.float64)
+        res = PisoResult()
+        self.ctx.check(_L.cfx_piso_step(self.h, nu, C.byref(schemes), C.byref(ctl), _p(ut), _p(uv), _p(pt), _p(pv),
+                                        _p(U), _p(p), _p(phi), C.byref(res)))
+        return res
 
     def nonorth_correct(self, gammaFace, x, linc, link, rhs, ncomp=1, comp=0, gradCorrected=True, sign=-1):
         self.ctx.check(_L.cfx_nonorth_correct(self.h, _p(gammaFace), _p(x), _p(linc), _p(link), ncomp, comp,

@@ -181,6 +181,73 @@ int cfx_csr_residual_history(cfx_csr* A, double* hist, int maxLen);
  * (default 1e-2) */
 int cfx_csr_set_inner_tol(cfx_csr* A, double innerTol);
 
+/* ------------------------------------------------------------ drivers */
+/* Schemes (src/3d/include/cf/fvm/FvMatrix.h:30-38): div 0 upwind, 1 linear,
+ * 2 blend; laplacian "corrected" (1) or "none" (0); grad "corrected" (1)
+ * or "linear" (0). */
+typedef struct cfx_schemes {
+    int div;
+    int laplacianCorrected;
+    int gradCorrected;
+    double blendCoeff;
+} cfx_schemes;
+
+/* SimpleControls (src/3d/include/cf/fvm/SimpleSolver.h:13-64), steady,
+ * laminar; pPrecision selects the pressure solve (CFX_FP64 = the
+ * reference's Jacobi-PCG, CFX_FP32_IR = FP32 inner + FP64 refinement). */
+typedef struct cfx_simple_controls {
+    int maxIterations;
+    double continuityTol, pcorrTol, momentumTol, linearTol;
+    int linearMaxIter, nonOrthCorrectors;
+    double relaxU, relaxP, pRelaxStart;
+    double bodyForce[3];
+    int pPrecision;
+} cfx_simple_controls;
+
+/* SimpleResult (SimpleSolver.h:66-72) + divergence flag and device time */
+typedef struct cfx_simple_result {
+    int iterations;
+    double finalContinuityError, finalMomentumResidual;
+    int converged, diverged;
+    double ms;
+} cfx_simple_result;
+
+/* PisoControls / PisoStepResult (src/3d/include/cf/fvm/PisoSolver.h:12-49) */
+typedef struct cfx_piso_controls {
+    double dt;
+    int correctors;
+    double continuityTol, linearTol;
+    int linearMaxIter, nonOrthCorrectors;
+    double bodyForce[3];
+    int pPrecision;
+} cfx_piso_controls;
+
+typedef struct cfx_piso_result {
+    int correctorsUsed;
+    double predictedContinuityError, continuityError, pressureCorrectionMax;
+    int converged;
+    double ms;
+} cfx_piso_result;
+
+/* volScalarField/volVectorField::linearise (src/3d/src/fields/volFields.cpp:
+ * 117-147, 246-283) for every boundary face: type (host, [nB]: 0 fixedValue,
+ * 1 fixedGradient, 2 zeroGradient/outlet, 3 symmetry; for vectors 3 also
+ * covers "wall") and value (host, [nB][ncomp]) -> d_linc, d_link. */
+int cfx_linearise(cfx_mesh* mesh, int ncomp, const int32_t* type, const double* value, double* d_linc,
+                  double* d_link);
+/* solveIncompressibleSimple (src/3d/include/cf/fvm/SimpleSolver.h:119-127):
+ * d_U [3][n], d_p [n], d_phi [nFaces] in/out on the device; boundary
+ * conditions per boundary face (host) as for cfx_linearise; hist (host,
+ * optional, [maxIterations][4]) gets {continuity, corrected continuity,
+ * max |dp|, dU} per iteration (the reference's log line). */
+int cfx_simple_solve(cfx_mesh* mesh, double nu, const cfx_schemes* schemes, const cfx_simple_controls* ctl,
+                     const int32_t* Utype, const double* Uvalue, const int32_t* ptype, const double* pvalue,
+                     double* d_U, double* d_p, double* d_phi, cfx_simple_result* result, double* hist);
+/* solveTransientIncompressiblePiso (src/3d/include/cf/fvm/PisoSolver.h:92-100), one step */
+int cfx_piso_step(cfx_mesh* mesh, double nu, const cfx_schemes* schemes, const cfx_piso_controls* ctl,
+                  const int32_t* Utype, const double* Uvalue, const int32_t* ptype, const double* pvalue,
+                  double* d_U, double* d_p, double* d_phi, cfx_piso_result* result);
+
 /* ----------------------------------------------------------- timing */
 /* Average device time of one SpMV launch (y = A x) over `reps` launches,
  * measured with HIP events on the context's stream.  precision CFX_FP64
