@@ -9,7 +9,7 @@ Workload (one "step" = one pressure solve from x0 = 0):
   boundaries zeroGradient (pure Neumann); rhs N(0,1) (seed 2024) projected to
   zero mean.  Solved to ||r||inf / ||b||inf < 1e-8 (the reference stop test)
   by plain FP64 Jacobi-PCG and by FP32-inner Jacobi-PCG with FP64 iterative
-  refinement (true FP64 residual as the judge).
+  refinement (convergence decided on the true FP64 residual).
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
