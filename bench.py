@@ -69,13 +69,14 @@ def build_system(N, ctx, n):
     return hm, arrays, geom, dm, A, b, bw_shuffled
 
 
-def time_solve(N, ctx, A, d_x, precision, steps, warmup, tol, maxIter):
+def time_solve(N, ctx, A, d_x, precision, steps, warmup, tol, maxIter, precond=None):
     reps = []
+    precond = N.PRECOND_JACOBI if precond is None else precond
     for i in range(warmup + steps):
         ctx.check(N.lib().cfx_zero(ctx.h, d_x.ptr, d_x.nbytes))
         ctx.sync()
         t0 = time.perf_counter()
-        rep = A.solve(d_x, N.SOLVER_CG, tol, maxIter, 0, None, None, None, -1, True, N.PRECOND_JACOBI, precision)
+        rep = A.solve(d_x, N.SOLVER_CG, tol, maxIter, 0, None, None, None, -1, True, precond, precision)
         ctx.sync()
         wall = (time.perf_counter() - t0) * 1e3
         if i >= warmup:
@@ -94,7 +95,7 @@ def cpu_info():
     return model, os.cpu_count()
 
 
-def cpu_baseline_and_check(arrays, b, x64_gpu, xmix_gpu, gpu_iters, sample_iters, A_dev):
+def cpu_baseline_and_check(arrays, b, x64_gpu, xmix_gpu, gpu_iters, sample_iters, A_dev, xmg_gpu=None):
     """Checker + CPU baseline: the oracle restates the reference; the
     reference build itself (oracle/_ref) is timed when present."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -116,6 +117,10 @@ def cpu_baseline_and_check(arrays, b, x64_gpu, xmix_gpu, gpu_iters, sample_iters
     out["mixed_true_residual"] = float(np.abs(b - O.spmv(csr, xmix_gpu)).max() / bn)
     out["mixed_vs_fp64_rel_err"] = float(np.abs((xmix_gpu - xmix_gpu.mean()) - (x64_gpu - x64_gpu.mean())).max()
                                          / np.abs(x64_gpu - x64_gpu.mean()).max())
+    if xmg_gpu is not None:
+        out["mg_true_residual"] = float(np.abs(b - O.spmv(csr, xmg_gpu)).max() / bn)
+        out["mg_vs_fp64_rel_err"] = float(np.abs((xmg_gpu - xmg_gpu.mean()) - (x64_gpu - x64_gpu.mean())).max()
+                                          / np.abs(x64_gpu - x64_gpu.mean()).max())
     log(f"checker: assembly bit-exact {out['assembly_bitwise_equal']}, true residuals "
         f"fp64 {out['fp64_true_residual']:.2e} mixed {out['mixed_true_residual']:.2e} ({time.perf_counter()-t0:.1f}s)")
 
@@ -162,6 +167,7 @@ def main():
     ap.add_argument("--cpu-sample-iters", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--mg-scale", type=float, default=1.8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is described, "
@@ -183,6 +189,10 @@ def main():
     log("FP32-inner Jacobi-PCG + FP64 refinement")
     rmix = time_solve(N, ctx, A, d_x, N.FP32_IR, args.steps, args.warmup, args.tol, maxIter)
     xmix = d_x.get()
+    log("multigrid-preconditioned flexible CG (FP32 + FP64 refinement), tuned constants")
+    A.set_mg(scale=args.mg_scale, coarseScale=args.mg_scale)
+    rmg = time_solve(N, ctx, A, d_x, N.FP32_IR, args.steps, args.warmup, args.tol, maxIter, N.PRECOND_LEARNED)
+    xmg = d_x.get()
 
     # SpMV roofline: average launch time over reps on the library stream (HIP events)
     xs = ctx.array(np.random.default_rng(1).standard_normal(n))
@@ -200,7 +210,8 @@ def main():
     t64 = float(np.mean([w for w, _ in r64]))
     tmix = float(np.mean([w for w, _ in rmix]))
     it64 = r64[-1][1]["iterations"]
-    best = min(t64, tmix)
+    tmg = float(np.mean([w for w, _ in rmg]))
+    best = min(t64, tmix, tmg)
     result = {
         "metric": METRIC,
         "value": round(best, 3),
@@ -212,7 +223,7 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 inner + f64 refinement" if tmix <= t64 else "f64",
+        "dtype": "f32 inner + f64 refinement" if min(tmix, tmg) <= t64 else "f64",
         "data": "synthetic",
         "config": {
             "workload": f"pressure-Poisson {args.n}^3 jittered hex (+-15%, seed {SEED_MESH}), faces/cells shuffled "
@@ -228,6 +239,11 @@ def main():
         "mixed_outer_iterations": rmix[-1][1]["outerIterations"],
         "mixed_true_residual": rmix[-1][1]["trueResidual"],
         "speedup_mixed_vs_plain": round(t64 / tmix, 3),
+        "mg_fcg_ms": round(tmg, 3),
+        "mg_fcg_iterations": rmg[-1][1]["iterations"],
+        "mg_fcg_outer_iterations": rmg[-1][1]["outerIterations"],
+        "mg_fcg_true_residual": rmg[-1][1]["trueResidual"],
+        "speedup_mg_vs_plain": round(t64 / tmg, 3),
         "learned_preconditioner": "not in this build",
         "spmv_fp64_us": round(ms64 * 1e3, 2),
         "spmv_fp64_GBps": round(gbs64, 1),
@@ -238,7 +254,7 @@ def main():
                      "kernel": "k_spmv<double> (CSR-stream SpMV, FP64)", "bytes_per_launch": b64},
     }
     if not args.no_cpu:
-        chk = cpu_baseline_and_check(arrays, b, x64, xmix, it64, args.cpu_sample_iters, A)
+        chk = cpu_baseline_and_check(arrays, b, x64, xmix, it64, args.cpu_sample_iters, A, xmg)
         result["cpu_baseline"] = chk.pop("cpu_baseline")
         result["checks"] = chk
     print(json.dumps(result), flush=True)

@@ -4,6 +4,7 @@
 #include "dev/fvops.h"
 #include "dev/krylov.h"
 #include "dev/mesh.h"
+#include "dev/mg.h"
 #include "dev/simple.h"
 
 #include <cstring>
@@ -41,6 +42,7 @@ struct cfx_matrix
     cfx_mesh* mesh = nullptr;
     int ncomp = 1;
     DevBuf diag, rhs, rhsBase, upper, lower, val, val32, tmp;
+    MgCache mg;
     Ldu ldu() const
     {
         Ldu A;
@@ -61,6 +63,7 @@ struct cfx_csr
     double innerTol = 1e-2;  // reliable-update factor of the mixed-precision PCG
     int lastIters = -1;
     bool val32Valid = false;
+    MgCache mg;
 };
 
 namespace
@@ -507,8 +510,12 @@ int cfx_matrix_solve(cfx_matrix* A, double* x, int solver, double tol, int maxIt
         DevMesh& m = *A->mesh->m;
         need(solver == CFX_SOLVER_CG || solver == CFX_SOLVER_BICGSTAB, "unknown solver");
         need(precision == CFX_FP64 || (precision == CFX_FP32_IR && solver == CFX_SOLVER_CG), "unknown precision");
-        need(precond == CFX_PRECOND_JACOBI, "preconditioner not available in this build");
+        need(precond == CFX_PRECOND_JACOBI || precond == CFX_PRECOND_LEARNED,
+             "preconditioner not available in this build");
+        need(precond != CFX_PRECOND_LEARNED || (solver == CFX_SOLVER_CG && precision == CFX_FP32_IR),
+             "the multigrid preconditioner runs inside flexible CG in FP32 with FP64 refinement");
         FvSolveArgs a;
+        if (precond == CFX_PRECOND_LEARNED) a.mg = &A->mg;
         a.solver = solver;
         a.tol = tol;
         a.maxIter = maxIter;
@@ -609,6 +616,7 @@ int cfx_csr_set_values(cfx_csr* A, const double* vals)
         CFX_HIP(hipMemcpyAsync(A->val.p, vals, sizeof(double) * A->pattern.nnz, hipMemcpyHostToDevice,
                                A->owner->ctx->stream()));
         A->val32Valid = false;
+        A->mg.valid = false;
         A->owner->ctx->sync();
     });
 }
@@ -634,8 +642,19 @@ int cfx_csr_solve_device(cfx_csr* A, const double* b, double* x, double tol, int
                 A->resHist.alloc(sizeof(double) * (maxIter + 1));
             hist = A->resHist.as<double>();
         }
-        const SolveReport s = solveOn(A->owner, A->pattern.view(), A->val.as<double>(), A->val32, A->val32Valid, b, x,
-                                      tol, maxIter, solver, precond, precision, A->innerTol, hist);
+        SolveReport s;
+        if (precond == CFX_PRECOND_LEARNED)
+        {
+            need(solver == CFX_SOLVER_CG, "the multigrid preconditioner runs inside flexible CG");
+            need(precision == CFX_FP32_IR, "the multigrid preconditioner runs in FP32 with FP64 refinement");
+            Context& ctx = *A->owner->ctx;
+            Multigrid& mg = A->mg.ready(ctx, A->pattern, A->val.as<double>());
+            s = A->owner->solver(A->pattern.n, A->pattern.nRowBlocks)
+                    .fcgMixed(A->pattern.view(), A->val.as<double>(), mg, b, x, tol, maxIter, A->innerTol, nullptr);
+        }
+        else
+            s = solveOn(A->owner, A->pattern.view(), A->val.as<double>(), A->val32, A->val32Valid, b, x, tol, maxIter,
+                        solver, precond, precision, A->innerTol, hist);
         A->lastIters = hist ? s.iterations : -1;
         toReport(s, rep);
     });
@@ -668,6 +687,115 @@ int cfx_csr_residual_history(cfx_csr* A, double* hist, int maxLen)
         // entry 0 is the initial residual: filled from the report path
         CFX_HIP(hipMemcpyAsync(hist, A->resHist.p, sizeof(double) * len, hipMemcpyDeviceToHost, A->owner->ctx->stream()));
         A->owner->ctx->sync();
+    });
+}
+
+int cfx_mg_default_params(cfx_mg_params* p)
+{
+    if (!p) return CFX_E_INVALID;
+    const MgParams d;
+    *p = cfx_mg_params{d.omegaPre, d.scale, d.omegaPost, d.restrictScale, d.coarseOmega, d.coarseScale,
+                       d.coarseSweeps, d.coarsestOmega, d.maxLevels, d.coarseMax};
+    return CFX_OK;
+}
+
+namespace
+{
+void setMgParams(MgCache& mg, const cfx_mg_params* p)
+{
+    need(p->maxLevels >= 1 && p->maxLevels <= 16, "maxLevels must be in [1,16]");
+    need(p->coarseMax >= 1 && p->coarseMax <= 2048, "coarseMax must be in [1,2048]");
+    need(p->coarseSweeps >= 0, "coarseSweeps < 0");
+    MgParams& m = mg.p;
+    m.omegaPre = p->omegaPre;
+    m.scale = p->scale;
+    m.omegaPost = p->omegaPost;
+    m.restrictScale = p->restrictScale;
+    m.coarseOmega = p->coarseOmega;
+    m.coarseScale = p->coarseScale;
+    m.coarseSweeps = p->coarseSweeps;
+    m.coarsestOmega = p->coarsestOmega;
+    m.maxLevels = p->maxLevels;
+    m.coarseMax = p->coarseMax;
+    mg.valid = false;
+}
+
+void setMgWeights(MgCache& mg, const float* pre, const float* scale, const float* post, const float* restr)
+{
+    mg.pre = pre;
+    mg.scale = scale;
+    mg.post = post;
+    mg.restr = restr;
+    mg.valid = false;
+}
+} // namespace
+
+int cfx_csr_set_mg(cfx_csr* A, const cfx_mg_params* p)
+{
+    if (!A || !p) return CFX_E_INVALID;
+    return guard(A->owner, [&] { setMgParams(A->mg, p); });
+}
+
+int cfx_csr_set_mg_weights(cfx_csr* A, const float* pre, const float* scale, const float* post, const float* restr)
+{
+    if (!A) return CFX_E_INVALID;
+    return guard(A->owner, [&] { setMgWeights(A->mg, pre, scale, post, restr); });
+}
+
+int cfx_matrix_set_mg(cfx_matrix* A, const cfx_mg_params* p)
+{
+    if (!A || !p) return CFX_E_INVALID;
+    return guard(A->mesh->owner, [&] { setMgParams(A->mg, p); });
+}
+
+int cfx_matrix_set_mg_weights(cfx_matrix* A, const float* pre, const float* scale, const float* post,
+                              const float* restr)
+{
+    if (!A) return CFX_E_INVALID;
+    return guard(A->mesh->owner, [&] { setMgWeights(A->mg, pre, scale, post, restr); });
+}
+
+int cfx_csr_mg_info(cfx_csr* A, int* levels, int64_t* rows, int64_t* nnz, int maxOut)
+{
+    if (!A || !levels) return CFX_E_INVALID;
+    return guard(A->owner, [&] {
+        Multigrid& mg = A->mg.ready(*A->owner->ctx, A->pattern, A->val.as<double>());
+        *levels = mg.levels();
+        for (int l = 0; l < std::min(maxOut, mg.levels()); ++l)
+        {
+            if (rows) rows[l] = mg.rows(l);
+            if (nnz) nnz[l] = mg.nnz(l);
+        }
+    });
+}
+
+int cfx_csr_mg_level(cfx_csr* A, int l, int* rowPtr, int* col, double* val, int* agg)
+{
+    if (!A) return CFX_E_INVALID;
+    return guard(A->owner, [&] {
+        Context& ctx = *A->owner->ctx;
+        Multigrid& mg = A->mg.ready(ctx, A->pattern, A->val.as<double>());
+        need(l >= 0 && l < mg.levels(), "level out of range");
+        MgLevel& L = mg.level(l);
+        const DevPattern& P = l == 0 ? A->pattern : L.pat;
+        hipStream_t s = ctx.stream();
+        if (rowPtr) std::memcpy(rowPtr, P.hRowPtr.data(), sizeof(int) * (L.n + 1));
+        if (col) std::memcpy(col, P.hCol.data(), sizeof(int) * L.nnz);
+        if (val)
+            CFX_HIP(hipMemcpyAsync(val, l == 0 ? A->val.p : L.val64.p, sizeof(double) * L.nnz, hipMemcpyDeviceToHost, s));
+        if (agg && l + 1 < mg.levels()) std::memcpy(agg, L.hAgg.data(), sizeof(int) * L.n);
+        ctx.sync();
+    });
+}
+
+int cfx_csr_mg_apply(cfx_csr* A, const float* r, float* z)
+{
+    if (!A) return CFX_E_INVALID;
+    return guard(A->owner, [&] {
+        Context& ctx = *A->owner->ctx;
+        Multigrid& mg = A->mg.ready(ctx, A->pattern, A->val.as<double>());
+        mg.apply(r, z, nullptr, nullptr, A->pattern.nRowBlocks, 1, nullptr);
+        ctx.sync();
     });
 }
 

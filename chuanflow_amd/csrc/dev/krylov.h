@@ -14,6 +14,8 @@
 namespace cfx
 {
 
+class Multigrid;
+
 struct SolveReport
 {
     int iterations = 0;
@@ -61,6 +63,15 @@ public:
     SolveReport cgMixed(const CsrView& A, const double* val, const float* val32, const double* b,
                         double* x, double tol, int maxIter, double delta, bool project);
 
+    // Flexible CG (FCG(1): alpha = p.r / p.q, beta = -z.q / p.q) with one
+    // multigrid V-cycle as the preconditioner, FP32 Krylov vectors and the
+    // same reliable FP64 updates as cgMixed.  `mg` must hold the hierarchy
+    // of `val` (Multigrid::setMatrix).  On a breakdown or a non-finite value
+    // the solve continues from the last FP64 iterate with cgMixed (Jacobi)
+    // and reports precondFallbacks = 1.
+    SolveReport fcgMixed(const CsrView& A, const double* val, Multigrid& mg, const double* b, double* x,
+                         double tol, int maxIter, double delta, double* resHist = nullptr);
+
     // True residual ||b - A x|| in FP64: returns {inf-norm rel, 2-norm rel}.
     void trueResidual(const CsrView& A, const double* val, const double* b, const double* x,
                       double& infRel, double& l2Rel);
@@ -84,6 +95,7 @@ private:
     DevBuf state_, partials_;
     DevBuf r_, p_, q_, r0_, v_, s_, sh_, t_, z_, invD_, invDb_;
     DevBuf r32_, p32_, q32_, d32_, b32_, invD32_;
+    DevBuf z32_, fcgPart_;  // allocated by the first fcgMixed
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     double lastMs_ = 0;
 };

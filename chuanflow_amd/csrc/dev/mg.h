@@ -1,0 +1,122 @@
+// mg.h -- aggregation multigrid V-cycle used as the (learned) preconditioner
+// of flexible CG.
+//
+// Hierarchy (built once per matrix pattern, on the host): greedy aggregation
+// of the face graph (a root and its whole neighbourhood, then attachment of
+// leftovers to an adjacent aggregate, then leftover groups), repeated until the
+// coarsest level is small; coarse operators are the Galerkin products
+// P^T A P with the piecewise-constant prolongation P, evaluated on the device
+// as fixed-order sums of fine nonzeros (no atomics), so a new matrix on the
+// same pattern only costs a few streaming kernels.
+//
+// V-cycle in FP32: on level l, x = Wpre b, r = b - A x (one SpMV on the
+// column-scaled values A Wpre), restrict, recurse, x += S P e, then
+// z = x + Wpost (b - A x); the coarsest level runs damped Jacobi sweeps in
+// one workgroup.  Wpre = diag(a) D^-1, Wpost = diag(c) D^-1, the
+// prolongation scale S = diag(s) and the restriction scale R = diag(t)
+// (restrict P^T R r) are per-row: on the finest level they may come from the
+// learned network, on coarse levels they are constants (a = c, t = 1).  With
+// a = c and s = t the cycle is a symmetric operator; with a = 1, s = 0, c = 0
+// it is exactly z = D^-1 b.
+#pragma once
+
+#include "types.h"
+
+#include <memory>
+#include <utility>
+#include <vector>
+
+namespace cfx
+{
+
+struct MgParams
+{
+    float omegaPre = 2.0f / 3.0f;   // finest-level constants (used where no per-row values are given)
+    float scale = 1.0f;
+    float omegaPost = 2.0f / 3.0f;
+    float restrictScale = 1.0f;
+    float coarseOmega = 2.0f / 3.0f;  // coarse levels: pre = post smoother weight
+    float coarseScale = 1.0f;
+    int coarseSweeps = 30;            // damped Jacobi sweeps on the coarsest level
+    float coarsestOmega = 2.0f / 3.0f;
+    int maxLevels = 8, coarseMax = 2048;
+};
+
+struct MgLevel
+{
+    int n = 0, nnz = 0;
+    DevPattern pat;
+    DevBuf val64, val32, valS32;   // FP64 Galerkin values, FP32 copy, A Wpre (FP32)
+    DevBuf invD32, wPre32, wPost32, s32, t32;  // D^-1, a D^-1, c D^-1, prolongation / restriction scale
+    DevBuf b, x, r;                // FP32 work vectors
+    // transfer to level l+1
+    DevBuf agg;                    // int[n]: coarse row of each row
+    DevBuf aggStart, aggMem;       // members of each coarse row (ascending)
+    DevBuf gStart, gPerm;          // coarse nonzero c = sum of fine nonzeros gPerm[gStart[c]..gStart[c+1])
+    std::vector<int> hAgg;
+};
+
+class Multigrid
+{
+public:
+    // Builds the hierarchy from a fine pattern (host copies of rowPtr/col).
+    Multigrid(Context& ctx, const DevPattern& fine, int maxLevels = 8, int coarseMax = 2048);
+    const MgParams& params() const { return p_; }
+    int levels() const { return static_cast<int>(lv_.size()); }
+    int rows(int l) const { return lv_[l].n; }
+    int nnz(int l) const { return lv_[l].nnz; }
+    const std::vector<int>& aggregation(int l) const { return lv_[l].hAgg; }
+    // device vectors of a level (training and tests)
+    MgLevel& level(int l) { return lv_[l]; }
+    const float* fineVal32() const { return lv_[0].val32.as<float>(); }
+    const float* fineInvD32() const { return lv_[0].invD32.as<float>(); }
+
+    // Galerkin values of every level from the fine FP64 values, FP32 copies,
+    // inverse diagonals and the scaled values A Wpre.  pre/scale/post/restr
+    // (device, FP32, per fine row; each may be null) are the finest-level
+    // a, s, c, t; null takes the constant from `p`.
+    void setMatrix(const double* fineVal64, const float* pre, const float* scale, const float* post,
+                   const float* restr, const MgParams& p);
+    // z = V-cycle(r) on the finest level.  When partZQ is non-null also writes
+    // per-workgroup partials of z.q (grid Gs with rbPerWg row blocks each).
+    void apply(const float* r, float* z, const float* q, double* partZQ, int Gs, int rbPerWg, const int* done);
+
+private:
+    Context& ctx_;
+    const DevPattern& fine_;
+    MgParams p_;
+    std::vector<MgLevel> lv_;  // lv_[0] uses the fine pattern (pat unused)
+    void recurse(int l, const int* done);
+};
+
+// The multigrid preconditioner of one matrix pattern: the hierarchy (rebuilt
+// when maxLevels / coarseMax change), its parameters, the caller's per-row
+// finest-level weights (device pointers, read at each setup) and whether the
+// Galerkin values match the current matrix values.
+struct MgCache
+{
+    std::unique_ptr<Multigrid> mg;
+    MgParams p;
+    const float *pre = nullptr, *scale = nullptr, *post = nullptr, *restr = nullptr;
+    bool valid = false;
+    std::pair<int, int> built{-1, -1};
+
+    Multigrid& ready(Context& ctx, const DevPattern& pat, const double* val)
+    {
+        if (!mg || built != std::make_pair(p.maxLevels, p.coarseMax))
+        {
+            mg.reset();
+            mg = std::make_unique<Multigrid>(ctx, pat, p.maxLevels, p.coarseMax);
+            built = {p.maxLevels, p.coarseMax};
+            valid = false;
+        }
+        if (!valid)
+        {
+            mg->setMatrix(val, pre, scale, post, restr, p);
+            valid = true;
+        }
+        return *mg;
+    }
+};
+
+} // namespace cfx

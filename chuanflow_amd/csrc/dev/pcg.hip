@@ -22,6 +22,7 @@
 // same search direction.  Convergence is only ever declared on a true FP64
 // residual.
 #include "krylov.h"
+#include "mg.h"
 
 #include <algorithm>
 #include <cmath>
@@ -38,7 +39,7 @@ struct CgState
 {
     double rz[2];        // r.z after iteration k (slot k & 1)
     double bnorm, initialResidual, rnorm, alpha, pAp, updAbs, beta;
-    int iter, done, converged, breakdown, paused, pad;
+    int iter, done, converged, breakdown, paused, nan;
 };
 
 __device__ __forceinline__ int xcdRemap(int h, int G)
@@ -347,6 +348,187 @@ __global__ void k_inv_diag32(CsrView A, const double* __restrict__ val, float* _
     invD32[i] = static_cast<float>(inv);
 }
 
+
+// ------------------------------------------------------- flexible CG (FP32)
+// FCG A(k): q = A p with the FP64 matrix values (FP32 values would perturb
+// the near-null modes of a (nearly) singular pressure matrix by more than
+// their eigenvalues), q rounded to FP32; partials p.q and p.r
+__global__ __launch_bounds__(kB) void k_fcg_A(CsrView A, int rbPerWg, const double* __restrict__ val,
+                                              const float* __restrict__ p, const float* __restrict__ r,
+                                              float* __restrict__ q, const CgState* st, double* __restrict__ partPQ,
+                                              double* __restrict__ partPR)
+{
+    __shared__ double lds[kSpmvLds];
+    __shared__ double red[kB / kWave];
+    if (st->done) return;
+    const int w = xcdRemap(blockIdx.x, gridDim.x);
+    double pq = 0, pr = 0;
+    for (int j = 0; j < rbPerWg; ++j)
+    {
+        const int blk = w * rbPerWg + j;
+        if (blk >= A.nRowBlocks) break;
+        const int r0 = A.rowBlocks[blk], r1 = A.rowBlocks[blk + 1];
+        const int k0 = A.rowPtr[r0], nnz = A.rowPtr[r1] - k0;
+        const int row = r0 + static_cast<int>(threadIdx.x);
+        double s = 0.0;
+        if (nnz <= kSpmvLds)
+        {
+            for (int kk = threadIdx.x; kk < nnz; kk += kB)
+                lds[kk] = val[k0 + kk] * static_cast<double>(p[A.col[k0 + kk]]);
+            __syncthreads();
+            if (row < r1)
+                for (int kk = A.rowPtr[row] - k0, e = A.rowPtr[row + 1] - k0; kk < e; ++kk) s += lds[kk];
+            __syncthreads();
+        }
+        else if (row < r1)
+        {
+            for (int kk = A.rowPtr[row]; kk < A.rowPtr[row + 1]; ++kk) s += val[kk] * static_cast<double>(p[A.col[kk]]);
+        }
+        if (row < r1)
+        {
+            const float qs = static_cast<float>(s);
+            q[row] = qs;
+            const double pi = p[row];
+            pq += pi * static_cast<double>(qs);
+            pr += pi * static_cast<double>(r[row]);
+        }
+    }
+    pq = blockSum(pq, red);
+    pr = blockSum(pr, red);
+    if (threadIdx.x == 0)
+    {
+        partPQ[blockIdx.x] = pq;
+        partPR[blockIdx.x] = pr;
+    }
+}
+
+// FCG B(k): alpha = p.r / p.q; x += alpha p, r -= alpha q; partial max|r|
+__global__ __launch_bounds__(kB) void k_fcg_B(int n, float* __restrict__ x, float* __restrict__ r,
+                                              const float* __restrict__ p, const float* __restrict__ q, CgState* st,
+                                              const double* __restrict__ partPQ, const double* __restrict__ partPR,
+                                              int Gs, double* __restrict__ partRM)
+{
+    __shared__ double red[kB / kWave];
+    __shared__ int ired[kB / kWave];
+    if (st->done) return;
+    const double PQ = allSum(partPQ, Gs, red);
+    const double PR = allSum(partPR, Gs, red);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (!(PQ > 0) || !(fabs(PR) < INFINITY))
+    {
+        // A p.p <= 0 (not SPD) or a non-finite value from the preconditioner
+        if (lead)
+        {
+            st->breakdown = PQ <= 0;
+            st->nan = !(PQ <= 0);
+            st->done = 1;
+            st->pAp = PQ;
+        }
+        return;
+    }
+    const double alphaD = PR / PQ;
+    if (lead)
+    {
+        st->alpha = alphaD;
+        st->pAp = PQ;
+    }
+    const float alpha = static_cast<float>(alphaD);
+    double rm = 0;
+    int nan = 0;
+    for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB)
+    {
+        x[i] += alpha * p[i];
+        const float ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        rm = fmax(rm, fabs(static_cast<double>(ri)));
+        nan |= (ri != ri);
+    }
+    rm = blockMax(rm, red);
+    nan = blockOr(nan, ired);
+    if (threadIdx.x == 0) partRM[blockIdx.x] = nan ? -1.0 : rm;
+}
+
+// FCG C(k): stop decision on max|r_k| (k = 0: initial residual),
+// beta = -z.q / p.q, p = z + beta p.  flags as in k_pcg_C (bit0 true residual,
+// bit1 mixed).
+__global__ __launch_bounds__(kB) void k_fcg_C(int n, const float* __restrict__ z, float* __restrict__ p, CgState* st,
+                                              const double* __restrict__ partZQ, int nZQ,
+                                              const double* __restrict__ partRM, const double* __restrict__ partBM,
+                                              int nRM, int k, double tol, int maxIter, double delta, int flags,
+                                              double* resHist)
+{
+    __shared__ double red[kB / kWave];
+    if (st->done) return;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const double RM = allMaxNan(partRM, nRM, red);
+    const double ZQ = k == 0 ? 0.0 : allSum(partZQ, nZQ, red);
+    const bool trueRes = flags & 1;
+    if (k == 0)
+    {
+        const double BM = allMaxNan(partBM, nRM, red);
+        const double bnorm = (BM < 1e-300) ? 1e-300 : BM;
+        if (lead)
+        {
+            st->bnorm = bnorm;
+            st->initialResidual = RM;
+            st->rnorm = RM;
+            st->iter = 0;
+            if (resHist) resHist[0] = RM;
+        }
+        if (RM != RM || BM != BM)
+        {
+            if (lead) { st->nan = 1; st->done = 1; }
+            return;
+        }
+        if (RM / bnorm < tol)
+        {
+            if (lead) { st->converged = 1; st->done = 1; }
+            return;
+        }
+        if (maxIter <= 0)
+        {
+            if (lead) st->done = 1;
+            return;
+        }
+    }
+    else
+    {
+        if (lead)
+        {
+            st->iter = k;
+            st->rnorm = RM;
+            if (resHist) resHist[k] = RM;
+        }
+        if (RM != RM || ZQ != ZQ || !(fabs(ZQ) < INFINITY))
+        {
+            if (lead) { st->nan = 1; st->done = 1; }
+            return;
+        }
+        const bool conv = RM / st->bnorm < tol;
+        if (conv && trueRes)
+        {
+            if (lead) { st->converged = 1; st->done = 1; }
+            return;
+        }
+        if (k >= maxIter)
+        {
+            if (lead) st->done = 1;
+            return;
+        }
+        if (!trueRes && (conv || RM < st->updAbs))
+        {
+            if (lead) { st->paused = 1; st->done = 1; }
+            return;
+        }
+    }
+    if (trueRes && lead) st->updAbs = delta * RM;
+    const double beta = k == 0 ? 0.0 : -ZQ / st->pAp;
+    if (lead) st->beta = beta;
+    const float tb = static_cast<float>(beta);
+    for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB)
+        p[i] = k == 0 ? z[i] : z[i] + tb * p[i];
+}
+
 } // namespace
 
 // ------------------------------------------------------------------- host
@@ -478,6 +660,131 @@ SolveReport Krylov::cgMixed(const CsrView& A, const double* val, const float* va
     CFX_CHECK_LAUNCH();
     SolveReport rep = pcgRun<float>(A, val, val32, invD32_.as<float>(), b, x, d32_.as<float>(), r32_.as<float>(),
                                     tol, maxIter, nullptr, true, delta);
+    if (rep.converged)
+    {
+        double inf = 0, l2 = 0;
+        trueResidual(A, val, b, x, inf, l2);
+        rep.trueResidual = inf;
+        rep.l2Residual = l2;
+    }
+    CFX_HIP(hipEventRecord(ev1_, s));
+    CFX_HIP(hipEventSynchronize(ev1_));
+    float ms = 0;
+    CFX_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+    lastMs_ = rep.ms = ms;
+    return rep;
+}
+
+
+SolveReport Krylov::fcgMixed(const CsrView& A, const double* val, Multigrid& mg, const double* b, double* x,
+                             double tol, int maxIter, double delta, double* resHist)
+{
+    hipStream_t s = ctx_.stream();
+    CFX_HIP(hipEventRecord(ev0_, s));
+    const PcgLaunch g = pcgGrid(A, n_);
+    if (g.Gs > kRedMax || g.Gv > kRedMax) throw std::invalid_argument("fcg: grid exceeds the partial buffer");
+    if (mg.rows(0) != A.n) throw std::invalid_argument("fcg: multigrid hierarchy does not match the matrix");
+    if (!fcgPart_.p)
+    {
+        fcgPart_.alloc(sizeof(double) * 5 * kRedMax);
+        z32_.alloc(sizeof(float) * static_cast<std::size_t>(n_));
+    }
+    double* part = fcgPart_.as<double>();
+    double *partPQ = part, *partPR = part + kRedMax, *partRM = part + 2 * kRedMax, *partBM = part + 3 * kRedMax,
+           *partZQ = part + 4 * kRedMax;
+    double* partRZ = partials_.as<double>();  // written by the residual kernel, unused here
+    CgState* st = reinterpret_cast<CgState*>(state_.as<char>() + 2 * sizeof(SolveState));
+    CFX_HIP(hipMemsetAsync(st, 0, sizeof(CgState), s));
+    CFX_HIP(hipMemsetAsync(d32_.p, 0, sizeof(float) * n_, s));
+    float *p = p32_.as<float>(), *q = q32_.as<float>(), *r = r32_.as<float>(), *z = z32_.as<float>(),
+          *x32 = d32_.as<float>();
+    const float* val32 = mg.fineVal32();
+    const float* invD32 = mg.fineInvD32();
+    const int* done = &st->done;
+
+    k_pcg_residual<float><<<g.Gs, kB, 0, s>>>(A, g.rbPerWg, val, x, b, nullptr, r, invD32, partRZ, partRM, partBM);
+    mg.apply(r, z, nullptr, nullptr, g.Gs, g.rbPerWg, done);
+    k_fcg_C<<<g.Gv, kB, 0, s>>>(n_, z, p, st, partZQ, g.Gs, partRM, partBM, g.Gs, 0, tol, maxIter, delta, 3, resHist);
+    CFX_CHECK_LAUNCH();
+    CgState* hs = reinterpret_cast<CgState*>(ctx_.hostState() + 2);
+    auto pollState = [&] {
+        CFX_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+        CFX_HIP(hipStreamSynchronize(s));
+    };
+    int k = 0, outer = 1;
+    double r0 = -1;
+    for (;;)
+    {
+        int batch = 2;
+        if (r0 > 0 && hs->iter > 0 && hs->rnorm > 0 && hs->rnorm < r0)
+        {
+            const double rate = std::log(hs->rnorm / r0) / hs->iter;
+            const double target = std::max(tol * hs->bnorm, hs->updAbs);
+            const double need = std::log(target / hs->rnorm) / rate;
+            if (std::isfinite(need)) batch = static_cast<int>(0.9 * need);
+        }
+        batch = std::max(2, std::min(64, batch));
+        batch = std::min(batch, maxIter - k);
+        if (batch <= 0) break;
+        for (int j = 0; j < batch; ++j)
+        {
+            ++k;
+            k_fcg_A<<<g.Gs, kB, 0, s>>>(A, g.rbPerWg, val, p, r, q, st, partPQ, partPR);
+            k_fcg_B<<<g.Gv, kB, 0, s>>>(n_, x32, r, p, q, st, partPQ, partPR, g.Gs, partRM);
+            mg.apply(r, z, q, partZQ, g.Gs, g.rbPerWg, done);
+            k_fcg_C<<<g.Gv, kB, 0, s>>>(n_, z, p, st, partZQ, g.Gs, partRM, partBM, g.Gv, k, tol, maxIter, delta, 2,
+                                        resHist);
+        }
+        CFX_CHECK_LAUNCH();
+        pollState();
+        if (r0 < 0) r0 = hs->initialResidual;
+        if (!hs->done) continue;
+        if (hs->paused && !hs->converged)
+        {
+            // reliable update at iteration hs->iter; the direction p_{k-1} and q = A p_{k-1} are kept
+            k = hs->iter;
+            k_fold<<<g.Gv, kB, 0, s>>>(n_, x, x32);
+            k_pcg_residual<float><<<g.Gs, kB, 0, s>>>(A, g.rbPerWg, val, x, b, nullptr, r, invD32, partRZ, partRM,
+                                                      partBM);
+            k_resume<<<1, 1, 0, s>>>(st);
+            mg.apply(r, z, q, partZQ, g.Gs, g.rbPerWg, done);
+            k_fcg_C<<<g.Gv, kB, 0, s>>>(n_, z, p, st, partZQ, g.Gs, partRM, partBM, g.Gs, k, tol, maxIter, delta, 3,
+                                        resHist);
+            CFX_CHECK_LAUNCH();
+            ++outer;
+            continue;
+        }
+        break;
+    }
+    const bool failed = hs->nan || hs->breakdown;
+    if (!hs->converged && !failed)
+    {
+        k_fold<<<g.Gv, kB, 0, s>>>(n_, x, x32);
+        CFX_CHECK_LAUNCH();
+    }
+    SolveReport rep;
+    rep.iterations = hs->iter;
+    rep.initialResidual = hs->initialResidual;
+    rep.finalResidual = hs->iter > 0 ? hs->rnorm : hs->initialResidual;
+    rep.converged = hs->converged;
+    rep.breakdown = hs->breakdown;
+    rep.outerIterations = outer;
+    rep.trueResidual = hs->converged ? hs->rnorm / hs->bnorm : -1;
+    if (failed)
+    {
+        // the preconditioner produced a non-finite value or lost positivity:
+        // continue from the last FP64 iterate with Jacobi-PCG (FP32 + reliable updates)
+        const int used = rep.iterations;
+        const double init = rep.initialResidual;
+        k_inv_diag32<<<divUp(A.n, 256), 256, 0, s>>>(A, val, invD32_.as<float>(), invD_.as<double>());
+        CFX_HIP(hipMemsetAsync(d32_.p, 0, sizeof(float) * n_, s));
+        CFX_CHECK_LAUNCH();
+        rep = pcgRun<float>(A, val, val32, invD32_.as<float>(), b, x, d32_.as<float>(), r32_.as<float>(), tol,
+                            std::max(0, maxIter - used), nullptr, true, delta);
+        rep.iterations += used;
+        rep.initialResidual = init;
+        rep.precondFallbacks = 1;
+    }
     if (rep.converged)
     {
         double inf = 0, l2 = 0;

@@ -3,6 +3,7 @@
 
 #include "fvops.h"
 #include "krylov.h"
+#include "mg.h"
 
 namespace cfx
 {
@@ -33,6 +34,7 @@ struct FvSolveArgs
     bool gradCorrected = true;
     int precision = kFP64;
     double delta = 1e-2;
+    MgCache* mg = nullptr;  // non-null: flexible CG with the multigrid preconditioner (FP32 + FP64 refinement)
 };
 SolveReport fvSolve(DevMesh& m, Krylov& K, const Ldu& A, double* val, double* rhsBase, DevBuf& val32,
                     double* x, const FvSolveArgs& a, double* gradTmp, double* faceTmp);

@@ -388,6 +388,12 @@ SolveReport fvSolve(DevMesh& m, Krylov& K, const Ldu& A, double* val, double* rh
             {
                 last = K.bicgstab(view, v, b, xk, a.tol, a.maxIter);
             }
+            else if (a.mg)
+            {
+                a.mg->valid = false;  // new values every call
+                Multigrid& M = a.mg->ready(*m.ctx, m.pattern, v);
+                last = K.fcgMixed(view, v, M, b, xk, a.tol, a.maxIter, a.delta, nullptr);
+            }
             else if (a.precision == kFP64)
             {
                 last = K.cg(view, v, b, xk, a.tol, a.maxIter);

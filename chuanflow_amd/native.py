@@ -73,6 +73,13 @@ class PisoResult(C.Structure):
                 ("ms", C.c_double)]
 
 
+class MgParams(C.Structure):
+    """cfx_mg_params: finest-level a (pre), s (scale), c (post); coarse-level constants; hierarchy caps."""
+    _fields_ = [("omegaPre", C.c_float), ("scale", C.c_float), ("omegaPost", C.c_float),
+                ("restrictScale", C.c_float), ("coarseOmega", C.c_float), ("coarseScale", C.c_float), ("coarseSweeps", C.c_int),
+                ("coarsestOmega", C.c_float), ("maxLevels", C.c_int), ("coarseMax", C.c_int)]
+
+
 class CfxError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"cfx error {code}: {msg}")
@@ -136,6 +143,14 @@ _sig("cfx_csr_solve_device", C.c_int, _vp, _vp, _vp, C.c_double, C.c_int, C.c_in
      C.POINTER(Report))
 _sig("cfx_csr_residual_history", C.c_int, _vp, _vp, C.c_int)
 _sig("cfx_csr_set_inner_tol", C.c_int, _vp, C.c_double)
+_sig("cfx_mg_default_params", C.c_int, C.POINTER(MgParams))
+_sig("cfx_csr_set_mg", C.c_int, _vp, C.POINTER(MgParams))
+_sig("cfx_csr_set_mg_weights", C.c_int, _vp, _vp, _vp, _vp, _vp)
+_sig("cfx_matrix_set_mg", C.c_int, _vp, C.POINTER(MgParams))
+_sig("cfx_matrix_set_mg_weights", C.c_int, _vp, _vp, _vp, _vp, _vp)
+_sig("cfx_csr_mg_info", C.c_int, _vp, C.POINTER(C.c_int), _vp, _vp, C.c_int)
+_sig("cfx_csr_mg_level", C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp)
+_sig("cfx_csr_mg_apply", C.c_int, _vp, _vp, _vp)
 _sig("cfx_linearise", C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp)
 _sig("cfx_simple_solve", C.c_int, _vp, C.c_double, C.POINTER(Schemes), C.POINTER(SimpleControls), _vp, _vp, _vp, _vp,
      _vp, _vp, _vp, C.POINTER(SimpleResult), _vp)
@@ -482,6 +497,21 @@ class Matrix:
         except Exception:
             pass
 
+    def set_mg(self, **kw):
+        """Multigrid constants for precond=PRECOND_LEARNED (cfx_mg_params fields)."""
+        p = mg_default_params()
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise KeyError(k)
+            setattr(p, k, v)
+        self.mesh.ctx.check(_L.cfx_matrix_set_mg(self.h, C.byref(p)))
+        return p
+
+    def set_mg_weights(self, pre=None, scale=None, post=None, restrict=None):
+        self._mgw = (pre, scale, post, restrict)
+        self.mesh.ctx.check(_L.cfx_matrix_set_mg_weights(self.h, *[_p(a) if a is not None else None
+                                                                   for a in self._mgw]))
+
     def reset(self):
         self.ctx.check(_L.cfx_matrix_reset(self.h))
 
@@ -583,3 +613,46 @@ class Csr:
 
     def set_inner_tol(self, t):
         self.ctx.check(_L.cfx_csr_set_inner_tol(self.h, t))
+
+    def set_mg(self, **kw):
+        """Multigrid constants (cfx_mg_params fields); unspecified fields keep the library defaults."""
+        p = mg_default_params()
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise KeyError(k)
+            setattr(p, k, v)
+        self.ctx.check(_L.cfx_csr_set_mg(self.h, C.byref(p)))
+        return p
+
+    def set_mg_weights(self, pre=None, scale=None, post=None, restrict=None):
+        """Per-row finest-level a, s, c, t: device FP32 arrays (kept referenced here) or None."""
+        self._mgw = (pre, scale, post, restrict)
+        self.ctx.check(_L.cfx_csr_set_mg_weights(self.h, *[_p(a) if a is not None else None for a in self._mgw]))
+
+    def mg_info(self):
+        lv = C.c_int(0)
+        rows = np.zeros(16, np.int64)
+        nnz = np.zeros(16, np.int64)
+        self.ctx.check(_L.cfx_csr_mg_info(self.h, C.byref(lv), _p(rows), _p(nnz), 16))
+        return rows[:lv.value].tolist(), nnz[:lv.value].tolist()
+
+    def mg_level(self, l):
+        """(rowPtr, col, FP64 values, agg or None) of hierarchy level l."""
+        rows, nnz = self.mg_info()
+        n, z = rows[l], nnz[l]
+        rp = np.zeros(n + 1, np.int32)
+        cl = np.zeros(z, np.int32)
+        vv = np.zeros(z)
+        agg = np.zeros(n, np.int32) if l + 1 < len(rows) else None
+        self.ctx.check(_L.cfx_csr_mg_level(self.h, l, _p(rp), _p(cl), _p(vv), _p(agg) if agg is not None else None))
+        return rp, cl, vv, agg
+
+    def mg_apply(self, r, z):
+        self.ctx.check(_L.cfx_csr_mg_apply(self.h, _p(r), _p(z)))
+
+
+def mg_default_params():
+    p = MgParams()
+    if _L.cfx_mg_default_params(C.byref(p)):
+        raise CfxError(-1, "cfx_mg_default_params")
+    return p

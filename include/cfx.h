@@ -181,6 +181,50 @@ int cfx_csr_residual_history(cfx_csr* A, double* hist, int maxLen);
  * (default 1e-2) */
 int cfx_csr_set_inner_tol(cfx_csr* A, double innerTol);
 
+/* ------------------------------- multigrid / learned preconditioner */
+/* precond = CFX_PRECOND_LEARNED (solver CFX_SOLVER_CG, precision
+ * CFX_FP32_IR): flexible CG (alpha = p.r/p.q, beta = -z.q/p.q) preconditioned
+ * by one aggregation-multigrid V-cycle, FP32 Krylov vectors with FP64
+ * reliable updates and the reference's stop test on true FP64 residuals; on
+ * a breakdown or a non-finite value it continues with Jacobi-PCG
+ * (report.precondFallbacks = 1).  The finest level takes per-row weights
+ * a (pre-smoother a/D), s (prolongation scale), c (post-smoother c/D) and
+ * t (restriction scale, coarse rhs = P^T diag(t) r): constants from
+ * cfx_mg_params, or per-row values from the message-passing network
+ * (cfx_mpnn_*) or any device array.  a = c and s = t give a symmetric
+ * cycle; a = 1, s = 0, c = 0 gives z = D^-1 r exactly (Jacobi).  The reference has no counterpart: it offers
+ * Jacobi-PCG and Jacobi-BiCGStab only (Matrix.cpp:153-319). */
+typedef struct cfx_mg_params {
+    float omegaPre, scale, omegaPost;   /* finest-level a, s, c */
+    float restrictScale;                /* finest-level t */
+    float coarseOmega, coarseScale;     /* coarse levels: a = c, s */
+    int coarseSweeps;                   /* damped Jacobi sweeps on the coarsest level */
+    float coarsestOmega;
+    int maxLevels, coarseMax;           /* hierarchy depth cap, coarsest size cap (<= 2048) */
+} cfx_mg_params;
+int cfx_mg_default_params(cfx_mg_params* p);
+int cfx_csr_set_mg(cfx_csr* A, const cfx_mg_params* p);
+/* per-row finest-level a, s, c, t (device FP32 [n] each, or null for the
+ * constant); the pointers are kept and read whenever the hierarchy's values
+ * are set up (after cfx_csr_set_values or this call) */
+int cfx_csr_set_mg_weights(cfx_csr* A, const float* d_pre, const float* d_scale, const float* d_post,
+                           const float* d_restrict);
+/* the same for an fvMatrix: cfx_matrix_solve with precond
+ * CFX_PRECOND_LEARNED (CG, CFX_FP32_IR) rebuilds the Galerkin values from
+ * the assembled values at each call */
+int cfx_matrix_set_mg(cfx_matrix* A, const cfx_mg_params* p);
+int cfx_matrix_set_mg_weights(cfx_matrix* A, const float* d_pre, const float* d_scale, const float* d_post,
+                              const float* d_restrict);
+/* hierarchy sizes: rows[l], nnz[l] for l < min(maxOut, levels) */
+int cfx_csr_mg_info(cfx_csr* A, int* levels, int64_t* rows, int64_t* nnz, int maxOut);
+
+/* level l of the hierarchy (host outputs, each may be null): CSR rowPtr
+ * [rows+1], col [nnz], FP64 Galerkin values [nnz], and agg [rows] = the
+ * level-(l+1) row of each row (not written on the coarsest level) */
+int cfx_csr_mg_level(cfx_csr* A, int l, int* rowPtr, int* col, double* val, int* agg);
+/* one V-cycle z = M r on device FP32 vectors */
+int cfx_csr_mg_apply(cfx_csr* A, const float* d_r, float* d_z);
+
 /* ------------------------------------------------------------ drivers */
 /* Schemes (src/3d/include/cf/fvm/FvMatrix.h:30-38): div 0 upwind, 1 linear,
  * 2 blend; laplacian "corrected" (1) or "none" (0); grad "corrected" (1)

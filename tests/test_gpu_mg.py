@@ -1,0 +1,151 @@
+"""The multigrid preconditioner and flexible CG on the GPU against the
+multigrid oracle: hierarchy (aggregation bit-exact, Galerkin values to
+1e-13), one V-cycle in FP32 against FP64 (rel 1e-4), FCG convergence to a
+true FP64 residual of 1e-8, the Jacobi limit, and the Jacobi-PCG fallback
+on a non-finite preconditioner."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import mg_oracle as MG
+import oracle as O
+from channel import neumann_bcs
+from helpers import host_and_oracle
+from test_oracle_mg import laplacian3d
+
+pytestmark = pytest.mark.gpu
+
+
+def fv_pressure_system(nx, ny, nz, seed):
+    """Jittered, shuffled, RCM-renumbered FV Laplacian (gamma = 1, Neumann)
+    with one Dirichlet-pinned cell so the matrix is SPD, and a zero-mean
+    N(0,1) right-hand side."""
+    hm, a, om = host_and_oracle(nx, ny, nz, (0, 0, 0), (1, 1, 1), jitter=0.15, seed=seed, shuffle=seed + 1,
+                                rcm=True)
+    t, v = neumann_bcs(om.nB)
+    lc, lk = O.linearise(om, 1, t, v)
+    A = O.Ldu(om)
+    O.assemble_laplacian(om, A, np.ones(om.nFaces), lc, lk, -1)
+    csr = O.csr_from_ldu(om, A)
+    M = sp.csr_matrix((csr.vals, csr.cols, csr.rowStart), shape=(om.nCells, om.nCells))
+    M[0, 0] = M[0, 0] * 2.0  # pin the constant null space
+    M.sort_indices()
+    b = np.random.default_rng(seed).standard_normal(om.nCells)
+    return M, b - b.mean()
+
+
+@pytest.fixture(scope="module")
+def systems(ctx):
+    print("\n[case] multigrid systems: 40^3 random-conductance Laplacian, 32^3 jittered FV", flush=True)
+    return {"lap40": (laplacian3d(40, 40, 40, seed=3), None), "fv32": fv_pressure_system(32, 32, 32, 5)}
+
+
+def make(ctx, A):
+    from chuanflow_amd import native as N
+    return N, N.Csr(ctx, A.indptr.astype(np.int64), A.indices.astype(np.int64), A.data)
+
+
+@pytest.mark.parametrize("name", ["lap40", "fv32"])
+def test_hierarchy_matches_oracle(ctx, systems, name):
+    A, _ = systems[name]
+    N, G = make(ctx, A)
+    rows, nnz = G.mg_info()
+    H = MG.hierarchy(A)
+    assert rows == [m.shape[0] for m in H.A] and nnz == [m.nnz for m in H.A]
+    for l in range(len(rows)):
+        rp, cl, vv, agg = G.mg_level(l)
+        assert np.array_equal(rp, H.A[l].indptr) and np.array_equal(cl, H.A[l].indices)
+        np.testing.assert_allclose(vv, H.A[l].data, rtol=1e-13, atol=1e-13 * np.abs(H.A[l].data).max())
+        if l + 1 < len(rows):
+            assert np.array_equal(agg, H.agg[l])
+
+
+@pytest.mark.parametrize("name", ["lap40", "fv32"])
+def test_vcycle_fp32_matches_oracle(ctx, systems, name):
+    A, _ = systems[name]
+    N, G = make(ctx, A)
+    n = A.shape[0]
+    rng = np.random.default_rng(1)
+    pre, scale, post, restrict = (rng.uniform(0.4, 1.0, n).astype(np.float32),
+                                  rng.uniform(0.5, 2.5, n).astype(np.float32),
+                                  rng.uniform(0.4, 1.0, n).astype(np.float32),
+                                  rng.uniform(0.5, 1.5, n).astype(np.float32))
+    dpre, dscale, dpost, drestrict = ctx.array(pre), ctx.array(scale), ctx.array(post), ctx.array(restrict)
+    G.set_mg(coarseScale=1.7, coarseSweeps=20)
+    G.set_mg_weights(dpre, dscale, dpost, drestrict)
+    r = rng.standard_normal(n).astype(np.float32)
+    z = ctx.empty(n, np.float32)
+    G.mg_apply(ctx.array(r), z)
+    p = MG.Params(coarseScale=1.7, coarseSweeps=20)
+    ref = MG.vcycle(MG.hierarchy(A), r.astype(np.float64), p, pre.astype(np.float64), scale.astype(np.float64),
+                    post.astype(np.float64), restrict=restrict.astype(np.float64))
+    err = np.abs(z.get() - ref).max() / np.abs(ref).max()
+    assert err < 1e-4, err
+
+
+@pytest.mark.parametrize("name", ["lap40", "fv32"])
+def test_fcg_converges_to_true_fp64_residual(ctx, systems, name):
+    A, b = systems[name]
+    if b is None:
+        b = np.random.default_rng(2).standard_normal(A.shape[0])
+    N, G = make(ctx, A)
+    G.set_mg(scale=1.8, coarseScale=1.8)
+    x, rep = G.solve(b, None, 1e-8, 500, N.SOLVER_CG, N.PRECOND_LEARNED, N.FP32_IR)
+    true = np.abs(b - A @ x).max() / np.abs(b).max()
+    print(f"[{name}] fcg its {rep.iterations} outer {rep.outerIterations} true {true:.2e}", flush=True)
+    assert rep.converged and rep.precondFallbacks == 0
+    assert true < 1e-8 and abs(rep.trueResidual - true) <= 1e-3 * true + 1e-15
+    _, rj = G.solve(b, None, 1e-8, 5000, N.SOLVER_CG, N.PRECOND_JACOBI, N.FP32_IR)
+    assert rj.converged and rep.iterations * 5 < rj.iterations
+    # same iterates as the FP64 oracle's FCG up to FP32 rounding: iteration counts agree closely
+    H = MG.hierarchy(A)
+    p = MG.Params(scale=1.8, coarseScale=1.8)
+    _, ko, co = MG.fcg(A, b, lambda r: MG.vcycle(H, r, p), 1e-8, 500)
+    assert co and abs(ko - rep.iterations) <= max(2, ko // 20)
+
+
+def test_identity_network_limit_equals_jacobi_fcg(ctx, systems):
+    A, b = systems["fv32"]
+    N, G = make(ctx, A)
+    n = A.shape[0]
+    one, zero = ctx.array(np.ones(n, np.float32)), ctx.array(np.zeros(n, np.float32))
+    G.set_mg_weights(one, zero, zero)
+    _, rl = G.solve(b, None, 1e-8, 5000, N.SOLVER_CG, N.PRECOND_LEARNED, N.FP32_IR)
+    N2, J = make(ctx, A)
+    J.set_mg(maxLevels=1, omegaPre=1.0, omegaPost=0.0)
+    _, rj = J.solve(b, None, 1e-8, 5000, N.SOLVER_CG, N.PRECOND_LEARNED, N.FP32_IR)
+    # FP64 Jacobi-PCG: FCG with a fixed SPD preconditioner has PCG's iterates in exact arithmetic
+    _, rp = J.solve(b, None, 1e-8, 5000, N.SOLVER_CG, N.PRECOND_JACOBI, N.FP64)
+    print(f"identity-limit FCG {rl.iterations}, Jacobi-FCG {rj.iterations}, FP64 Jacobi-PCG {rp.iterations}",
+          flush=True)
+    assert rl.converged and rj.converged and rp.converged
+    assert rl.iterations == rj.iterations
+    assert abs(rj.iterations - rp.iterations) <= max(3, rp.iterations // 50)
+
+
+def test_nonfinite_weights_fall_back_to_jacobi_pcg(ctx, systems):
+    A, b = systems["fv32"]
+    N, G = make(ctx, A)
+    n = A.shape[0]
+    w = np.full(n, 0.7, np.float32)
+    w[n // 3] = np.nan
+    dw = ctx.array(w)
+    G.set_mg_weights(dw, None, None)
+    x, rep = G.solve(b, None, 1e-8, 5000, N.SOLVER_CG, N.PRECOND_LEARNED, N.FP32_IR)
+    true = np.abs(b - A @ x).max() / np.abs(b).max()
+    assert rep.precondFallbacks == 1 and rep.converged and true < 1e-8
+
+
+def test_new_values_on_same_pattern_rebuild_galerkin_values(ctx, systems):
+    A, b = systems["lap40"]
+    b = np.random.default_rng(4).standard_normal(A.shape[0])
+    N, G = make(ctx, A)
+    G.set_mg(scale=1.8, coarseScale=1.8)
+    A2 = A.copy()
+    A2.data = A2.data * np.where(A2.indices == np.repeat(np.arange(A.shape[0]), np.diff(A.indptr)), 1.5, 1.0)
+    G.solve(b, None, 1e-8, 500, N.SOLVER_CG, N.PRECOND_LEARNED, N.FP32_IR)
+    G.set_values(A2.data)
+    x, rep = G.solve(b, None, 1e-8, 500, N.SOLVER_CG, N.PRECOND_LEARNED, N.FP32_IR)
+    assert rep.converged and np.abs(b - A2 @ x).max() / np.abs(b).max() < 1e-8
+    _, _, vv, _ = G.mg_level(1)
+    np.testing.assert_allclose(vv, MG.hierarchy(A2).A[1].data, rtol=1e-13, atol=1e-12)
