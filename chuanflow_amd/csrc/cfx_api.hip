@@ -695,7 +695,7 @@ int cfx_mg_default_params(cfx_mg_params* p)
     if (!p) return CFX_E_INVALID;
     const MgParams d;
     *p = cfx_mg_params{d.omegaPre, d.scale, d.omegaPost, d.restrictScale, d.coarseOmega, d.coarseScale,
-                       d.coarseSweeps, d.coarsestOmega, d.maxLevels, d.coarseMax};
+                       d.coarseSweeps, d.coarsestOmega, d.maxLevels, d.coarseMax, d.kcycleDepth};
     return CFX_OK;
 }
 
@@ -717,6 +717,8 @@ void setMgParams(MgCache& mg, const cfx_mg_params* p)
     m.coarsestOmega = p->coarsestOmega;
     m.maxLevels = p->maxLevels;
     m.coarseMax = p->coarseMax;
+    need(p->kcycleDepth >= 0, "kcycleDepth < 0");
+    m.kcycleDepth = p->kcycleDepth;
     mg.valid = false;
 }
 

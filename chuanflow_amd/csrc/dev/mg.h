@@ -11,13 +11,24 @@
 //
 // V-cycle in FP32: on level l, x = Wpre b, r = b - A x (one SpMV on the
 // column-scaled values A Wpre), restrict, recurse, x += S P e, then
-// z = x + Wpost (b - A x); the coarsest level runs damped Jacobi sweeps in
-// one workgroup.  Wpre = diag(a) D^-1, Wpost = diag(c) D^-1, the
+// z = x + Wpost (b - A x).  The coarsest level (at most 128 rows) is solved
+// exactly with a dense inverse formed in LDS at setup (a singular, pure
+// Neumann coarsest operator A is inverted as A + gamma 1 1^T, which equals
+// the pseudo-inverse on zero-mean right-hand sides); a larger coarsest level
+// runs damped Jacobi sweeps in one workgroup.  Wpre = diag(a) D^-1, Wpost = diag(c) D^-1, the
 // prolongation scale S = diag(s) and the restriction scale R = diag(t)
 // (restrict P^T R r) are per-row: on the finest level they may come from the
 // learned network, on coarse levels they are constants (a = c, t = 1).  With
 // a = c and s = t the cycle is a symmetric operator; with a = 1, s = 0, c = 0
 // it is exactly z = D^-1 b.
+//
+// K-cycle (Notay's Krylov-accelerated cycle): on the coarse levels
+// 1..kcycleDepth the coarse correction is two steps of flexible CG on that
+// level, preconditioned by the cycle of that level, instead of one cycle:
+// c1 = M b, v1 = A c1, r2 = b - (c1.b / c1.v1) v1, c2 = M r2, v2 = A c2 and
+// e = k1 c1 + k2 c2 with the FCG coefficients.  It keeps the convergence of
+// the aggregation hierarchy nearly independent of its depth, and makes the
+// preconditioner nonlinear, which flexible CG allows.
 #pragma once
 
 #include "types.h"
@@ -37,9 +48,10 @@ struct MgParams
     float restrictScale = 1.0f;
     float coarseOmega = 2.0f / 3.0f;  // coarse levels: pre = post smoother weight
     float coarseScale = 1.0f;
-    int coarseSweeps = 30;            // damped Jacobi sweeps on the coarsest level
+    int coarseSweeps = 30;            // damped Jacobi sweeps on a coarsest level above 128 rows
     float coarsestOmega = 2.0f / 3.0f;
-    int maxLevels = 8, coarseMax = 2048;
+    int maxLevels = 10, coarseMax = 128;
+    int kcycleDepth = 2;              // coarse levels 1..kcycleDepth use the K-cycle (0: V-cycle)
 };
 
 struct MgLevel
@@ -48,11 +60,14 @@ struct MgLevel
     DevPattern pat;
     DevBuf val64, val32, valS32;   // FP64 Galerkin values, FP32 copy, A Wpre (FP32)
     DevBuf invD32, wPre32, wPost32, s32, t32;  // D^-1, a D^-1, c D^-1, prolongation / restriction scale
-    DevBuf b, x, r;                // FP32 work vectors
+    DevBuf b, x, r, e;             // FP32: rhs, iterate, residual, coarse-correction result
+    DevBuf c1, c2, v1, r2, part;   // K-cycle vectors and 5 x kPartMax dot partials
+    int G = 1, rbPerWg = 1;        // SpMV grid of this level (G <= kPartMax)
     // transfer to level l+1
     DevBuf agg;                    // int[n]: coarse row of each row
     DevBuf aggStart, aggMem;       // members of each coarse row (ascending)
     DevBuf gStart, gPerm;          // coarse nonzero c = sum of fine nonzeros gPerm[gStart[c]..gStart[c+1])
+    DevBuf dense;                  // coarsest level: FP32 inverse [n][n] (symmetric)
     std::vector<int> hAgg;
 };
 
@@ -60,7 +75,9 @@ class Multigrid
 {
 public:
     // Builds the hierarchy from a fine pattern (host copies of rowPtr/col).
-    Multigrid(Context& ctx, const DevPattern& fine, int maxLevels = 8, int coarseMax = 2048);
+    Multigrid(Context& ctx, const DevPattern& fine, int maxLevels = 10, int coarseMax = 128);
+    bool denseCoarsest() const { return levels() > 1 && lv_.back().n <= kDenseMax; }
+    static constexpr int kDenseMax = 128;
     const MgParams& params() const { return p_; }
     int levels() const { return static_cast<int>(lv_.size()); }
     int rows(int l) const { return lv_[l].n; }
@@ -81,12 +98,18 @@ public:
     // per-workgroup partials of z.q (grid Gs with rbPerWg row blocks each).
     void apply(const float* r, float* z, const float* q, double* partZQ, int Gs, int rbPerWg, const int* done);
 
+    static constexpr int kPartMax = 2048;
+
 private:
     Context& ctx_;
     const DevPattern& fine_;
     MgParams p_;
     std::vector<MgLevel> lv_;  // lv_[0] uses the fine pattern (pat unused)
-    void recurse(int l, const int* done);
+    // out = cycle of level l applied to in (in, out distinct from the level's work vectors)
+    void visit(int l, const float* in, float* out, const float* q, double* partZQ, int Gs, int rbPerWg,
+               const int* done);
+    // coarse correction of level l >= 1: L.e from L.b
+    void solveLevel(int l, const int* done);
 };
 
 // The multigrid preconditioner of one matrix pattern: the hierarchy (rebuilt

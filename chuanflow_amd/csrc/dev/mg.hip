@@ -234,6 +234,228 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(int n, const int* 
     for (int i = threadIdx.x; i < n; i += kCoarseThreads) x[i] = cur[i];
 }
 
+// Dense inverse of the coarsest matrix (n <= 128), FP64 Gauss-Jordan in LDS
+// by one workgroup, stored as FP32.  A pure Neumann operator (every row sum
+// below 1e-9 max|diag|) is inverted as A + gamma 1 1^T with gamma =
+// mean(diag) / n; a non-positive pivot is replaced by max|diag| (the result
+// is then an approximate inverse, still a valid preconditioner).
+constexpr int kDense = Multigrid::kDenseMax;
+__global__ __launch_bounds__(1024) void k_mg_dense_inverse(CsrView A, const double* __restrict__ val,
+                                                           float* __restrict__ inv)
+{
+    __shared__ double a[kDense * kDense];
+    __shared__ double f[kDense];
+    __shared__ double red[1024 / kWave];
+    const int n = A.n, t = threadIdx.x;
+    for (int i = t; i < n * n; i += 1024) a[i] = 0.0;
+    __syncthreads();
+    double dmax = 0.0, dsum = 0.0, rmax = 0.0;
+    for (int i = t; i < n; i += 1024)
+    {
+        double rs = 0.0;
+        for (int k = A.rowPtr[i]; k < A.rowPtr[i + 1]; ++k)
+        {
+            a[i * n + A.col[k]] = val[k];
+            rs += val[k];
+        }
+        const double d = a[i * n + i];
+        dmax = fmax(dmax, fabs(d));
+        dsum += d;
+        rmax = fmax(rmax, fabs(rs));
+    }
+    __syncthreads();
+    dmax = blockMax(dmax, red);
+    if (t == 0) f[0] = dmax;
+    __syncthreads();
+    dmax = f[0];
+    __syncthreads();
+    rmax = blockMax(rmax, red);
+    if (t == 0) f[0] = rmax;
+    __syncthreads();
+    rmax = f[0];
+    __syncthreads();
+    dsum = blockSum(dsum, red);
+    if (t == 0) f[0] = dsum;
+    __syncthreads();
+    dsum = f[0];
+    __syncthreads();
+    if (rmax <= 1e-9 * dmax)
+    {
+        const double gamma = dsum / n / n;
+        for (int i = t; i < n * n; i += 1024) a[i] += gamma;
+        __syncthreads();
+    }
+    for (int k = 0; k < n; ++k)
+    {
+        if (t < n) f[t] = a[t * n + k];
+        __syncthreads();
+        double piv = f[k];
+        if (!(piv > 1e-14 * dmax)) piv = dmax > 0 ? dmax : 1.0;
+        if (t < n) a[k * n + t] = (t == k) ? 1.0 / piv : a[k * n + t] / piv;
+        __syncthreads();
+        for (int e = t; e < n * n; e += 1024)
+        {
+            const int i = e / n, j = e % n;
+            if (i == k) continue;
+            a[e] = (j == k) ? -f[i] / piv : a[e] - f[i] * a[k * n + j];
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < n * n; e += 1024) inv[e] = static_cast<float>(a[e]);
+}
+
+// x = inv b on the coarsest level: thread i forms row i (inv is symmetric,
+// so column reads inv[j][i] are coalesced across threads)
+__global__ __launch_bounds__(kDense) void k_mg_dense_apply(int n, const float* __restrict__ inv,
+                                                           const float* __restrict__ b, float* __restrict__ x,
+                                                           const int* done)
+{
+    __shared__ float bl[kDense];
+    if (done && *done) return;
+    const int i = threadIdx.x;
+    if (i < n) bl[i] = b[i];
+    __syncthreads();
+    if (i >= n) return;
+    float s = 0.0f;
+    for (int j = 0; j < n; ++j) s += inv[j * n + i] * bl[j];
+    x[i] = s;
+}
+
+
+// ----------------------------------------------------------------- K-cycle
+// deterministic all-workgroup sum of G <= 2048 partials (blockDim B)
+__device__ __forceinline__ double redAll(const double* __restrict__ p, int G, double* lds)
+{
+    double v = 0.0;
+    for (int i = threadIdx.x; i < G; i += B) v += p[i];
+    v = blockSum(v, lds);
+    if (threadIdx.x == 0) lds[B / kWave] = v;
+    __syncthreads();
+    v = lds[B / kWave];
+    __syncthreads();
+    return v;
+}
+
+// v1 = A c1; partials c1.v1 (P0), c1.b (P1)
+__global__ __launch_bounds__(B) void k_kc_spmv1(CsrView A, int rbPerWg, const float* __restrict__ val,
+                                                const float* __restrict__ c, const float* __restrict__ b,
+                                                float* __restrict__ v, double* __restrict__ part, const int* done)
+{
+    __shared__ float lds[kSpmvLds];
+    __shared__ double red[B / kWave];
+    if (done && *done) return;
+    const int w = xcdRemap(blockIdx.x, gridDim.x);
+    double p0 = 0.0, p1 = 0.0;
+    for (int j = 0; j < rbPerWg; ++j)
+    {
+        const int blk = w * rbPerWg + j;
+        if (blk >= A.nRowBlocks) break;
+        int row;
+        float sum;
+        if (rowSum<kSpmvLds>(A, val, c, blk, lds, row, sum))
+        {
+            v[row] = sum;
+            const double ci = c[row];
+            p0 += ci * static_cast<double>(sum);
+            p1 += ci * static_cast<double>(b[row]);
+        }
+    }
+    p0 = blockSum(p0, red);
+    p1 = blockSum(p1, red);
+    if (threadIdx.x == 0)
+    {
+        part[blockIdx.x] = p0;
+        part[Multigrid::kPartMax + blockIdx.x] = p1;
+    }
+}
+
+// r2 = b - (c1.b / c1.v1) v1
+__global__ __launch_bounds__(B) void k_kc_axpy(int n, const float* __restrict__ b, const float* __restrict__ v,
+                                               float* __restrict__ r2, const double* __restrict__ part, int G,
+                                               const int* done)
+{
+    __shared__ double red[B / kWave + 1];
+    if (done && *done) return;
+    const double rho = redAll(part, G, red);
+    const double a1 = redAll(part + Multigrid::kPartMax, G, red);
+    const float coef = static_cast<float>(rho > 0.0 ? a1 / rho : 0.0);
+    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) r2[i] = b[i] - coef * v[i];
+}
+
+// v2 = A c2 (not stored); partials c2.v1 (P2), c2.v2 (P3), c2.r2 (P4)
+__global__ __launch_bounds__(B) void k_kc_spmv2(CsrView A, int rbPerWg, const float* __restrict__ val,
+                                                const float* __restrict__ c, const float* __restrict__ v1,
+                                                const float* __restrict__ r2, double* __restrict__ part,
+                                                const int* done)
+{
+    __shared__ float lds[kSpmvLds];
+    __shared__ double red[B / kWave];
+    if (done && *done) return;
+    const int w = xcdRemap(blockIdx.x, gridDim.x);
+    double p2 = 0.0, p3 = 0.0, p4 = 0.0;
+    for (int j = 0; j < rbPerWg; ++j)
+    {
+        const int blk = w * rbPerWg + j;
+        if (blk >= A.nRowBlocks) break;
+        int row;
+        float sum;
+        if (rowSum<kSpmvLds>(A, val, c, blk, lds, row, sum))
+        {
+            const double ci = c[row];
+            p2 += ci * static_cast<double>(v1[row]);
+            p3 += ci * static_cast<double>(sum);
+            p4 += ci * static_cast<double>(r2[row]);
+        }
+    }
+    p2 = blockSum(p2, red);
+    p3 = blockSum(p3, red);
+    p4 = blockSum(p4, red);
+    if (threadIdx.x == 0)
+    {
+        part[2 * Multigrid::kPartMax + blockIdx.x] = p2;
+        part[3 * Multigrid::kPartMax + blockIdx.x] = p3;
+        part[4 * Multigrid::kPartMax + blockIdx.x] = p4;
+    }
+}
+
+// e = k1 c1 + k2 c2 with the two-step FCG coefficients
+//   rho1 = c1.v1, a1 = c1.b, g = c2.v1, beta = c2.v2, a2 = c2.r2,
+//   rho2 = beta - g^2 / rho1, k1 = a1/rho1 - g a2 / (rho1 rho2), k2 = a2 / rho2
+// (one step, k2 = 0, if rho2 <= 0; e = c1 if rho1 <= 0)
+__device__ __forceinline__ void kcCoefficients(const double* part, int G, double* red, float& k1, float& k2)
+{
+    const double rho1 = redAll(part, G, red);
+    const double a1 = redAll(part + Multigrid::kPartMax, G, red);
+    const double g = redAll(part + 2 * Multigrid::kPartMax, G, red);
+    const double be = redAll(part + 3 * Multigrid::kPartMax, G, red);
+    const double a2 = redAll(part + 4 * Multigrid::kPartMax, G, red);
+    double c1 = 1.0, c2 = 0.0;
+    if (rho1 > 0.0)
+    {
+        const double rho2 = be - g * g / rho1;
+        if (rho2 > 0.0)
+        {
+            c1 = a1 / rho1 - g * a2 / (rho1 * rho2);
+            c2 = a2 / rho2;
+        }
+        else
+            c1 = a1 / rho1;
+    }
+    k1 = static_cast<float>(c1);
+    k2 = static_cast<float>(c2);
+}
+
+__global__ __launch_bounds__(B) void k_kc_combine(int n, const float* __restrict__ c1, const float* __restrict__ c2,
+                                                  float* __restrict__ e, const double* __restrict__ part, int G,
+                                                  const int* done)
+{
+    __shared__ double red[B / kWave + 1];
+    if (done && *done) return;
+    float k1, k2;
+    kcCoefficients(part, G, red, k1, k2);
+    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) e[i] = k1 * c1[i] + k2 * c2[i];
+}
+
 } // namespace
 
 Multigrid::Multigrid(Context& ctx, const DevPattern& fine, int maxLevels, int coarseMax) : ctx_(ctx), fine_(fine)
@@ -310,8 +532,19 @@ Multigrid::Multigrid(Context& ctx, const DevPattern& fine, int maxLevels, int co
         if (l > 0) L.val64.alloc(sizeof(double) * nnz);
         L.val32.alloc(sizeof(float) * nnz);
         L.valS32.alloc(sizeof(float) * nnz);
-        for (DevBuf* b : {&L.invD32, &L.wPre32, &L.wPost32, &L.s32, &L.t32, &L.b, &L.x, &L.r})
+        for (DevBuf* b : {&L.invD32, &L.wPre32, &L.wPost32, &L.s32, &L.t32, &L.b, &L.x, &L.r, &L.e})
             b->alloc(sizeof(float) * n);
+        const int nrb = l == 0 ? fine_.nRowBlocks : L.pat.nRowBlocks;
+        L.G = std::max(1, std::min(nrb, kPartMax));
+        L.rbPerWg = divUp(nrb, L.G);
+        L.G = std::max(1, divUp(nrb, L.rbPerWg));
+        if (l > 0 && l + 1 < lv_.size())
+        {
+            // K-cycle buffers on every intermediate coarse level (the depth is a parameter)
+            for (DevBuf* b : {&L.c1, &L.c2, &L.v1, &L.r2}) b->alloc(sizeof(float) * n);
+            L.part.alloc(sizeof(double) * 5 * kPartMax);
+        }
+        if (l > 0 && l + 1 == lv_.size() && L.n <= kDenseMax) L.dense.alloc(sizeof(float) * n * n);
     }
     ctx.sync();
 }
@@ -343,61 +576,72 @@ void Multigrid::setMatrix(const double* fineVal64, const float* pre, const float
                                                   fineLevel ? scale : nullptr, sC, fineLevel ? post : nullptr, cC,
                                                   fineLevel ? restr : nullptr, fineLevel ? p.restrictScale : 1.0f);
         k_scale_vals<<<divUp(L.n, B), B, 0, s>>>(A, L.val32.as<float>(), L.wPre32.as<float>(), L.valS32.as<float>());
+        if (L.dense.p) k_mg_dense_inverse<<<1, 1024, 0, s>>>(A, L.val64.as<double>(), L.dense.as<float>());
     }
     CFX_CHECK_LAUNCH();
 }
 
-void Multigrid::recurse(int l, const int* done)
+void Multigrid::solveLevel(int l, const int* done)
 {
     hipStream_t s = ctx_.stream();
     MgLevel& L = lv_[l];
     const int last = levels() - 1;
     if (l == last)
     {
-        const CsrView A = L.pat.view();
-        k_mg_coarse<<<1, kCoarseThreads, 0, s>>>(L.n, A.rowPtr, A.col, L.val32.as<float>(), L.invD32.as<float>(),
-                                                 L.b.as<float>(), L.x.as<float>(), p_.coarseSweeps, p_.coarsestOmega, done);
+        if (L.dense.p)
+            k_mg_dense_apply<<<1, kDense, 0, s>>>(L.n, L.dense.as<float>(), L.b.as<float>(), L.e.as<float>(), done);
+        else
+        {
+            const CsrView A = L.pat.view();
+            k_mg_coarse<<<1, kCoarseThreads, 0, s>>>(L.n, A.rowPtr, A.col, L.val32.as<float>(), L.invD32.as<float>(),
+                                                     L.b.as<float>(), L.e.as<float>(), p_.coarseSweeps,
+                                                     p_.coarsestOmega, done);
+        }
         return;
     }
-    // levels > 0: b in L.b, result left in L.r
+    if (l > p_.kcycleDepth)
+    {
+        visit(l, L.b.as<float>(), L.e.as<float>(), nullptr, nullptr, L.G, L.rbPerWg, done);
+        return;
+    }
     const CsrView A = L.pat.view();
-    MgLevel& C = lv_[l + 1];
-    k_mg_down<<<A.nRowBlocks, B, 0, s>>>(A, L.valS32.as<float>(), L.b.as<float>(), L.wPre32.as<float>(),
-                                         L.x.as<float>(), L.r.as<float>(), done);
-    k_mg_restrict<<<divUp(C.n, B), B, 0, s>>>(C.n, L.aggStart.as<int>(), L.aggMem.as<int>(), nullptr,
-                                              L.r.as<float>(), C.b.as<float>(), done);
-    recurse(l + 1, done);
-    const float* ec = (l + 1 == last) ? C.x.as<float>() : C.r.as<float>();
-    k_mg_prolong<<<divUp(L.n, B), B, 0, s>>>(L.n, L.agg.as<int>(), L.s32.as<float>(), ec, L.x.as<float>(), done);
-    k_mg_up<<<A.nRowBlocks, B, 0, s>>>(A, 1, L.val32.as<float>(), L.x.as<float>(), L.b.as<float>(),
-                                       L.wPost32.as<float>(), L.r.as<float>(), nullptr, nullptr, done);
+    const int Gv = std::max(1, std::min(64, divUp(L.n, 4 * B)));
+    double* part = L.part.as<double>();
+    visit(l, L.b.as<float>(), L.c1.as<float>(), nullptr, nullptr, L.G, L.rbPerWg, done);
+    k_kc_spmv1<<<L.G, B, 0, s>>>(A, L.rbPerWg, L.val32.as<float>(), L.c1.as<float>(), L.b.as<float>(),
+                                 L.v1.as<float>(), part, done);
+    k_kc_axpy<<<Gv, B, 0, s>>>(L.n, L.b.as<float>(), L.v1.as<float>(), L.r2.as<float>(), part, L.G, done);
+    visit(l, L.r2.as<float>(), L.c2.as<float>(), nullptr, nullptr, L.G, L.rbPerWg, done);
+    k_kc_spmv2<<<L.G, B, 0, s>>>(A, L.rbPerWg, L.val32.as<float>(), L.c2.as<float>(), L.v1.as<float>(),
+                                 L.r2.as<float>(), part, done);
+    k_kc_combine<<<Gv, B, 0, s>>>(L.n, L.c1.as<float>(), L.c2.as<float>(), L.e.as<float>(), part, L.G, done);
+}
+
+void Multigrid::visit(int l, const float* in, float* out, const float* q, double* partZQ, int Gs, int rbPerWg,
+                      const int* done)
+{
+    hipStream_t s = ctx_.stream();
+    MgLevel& L = lv_[l];
+    const CsrView A = l == 0 ? fine_.view() : L.pat.view();
+    k_mg_down<<<A.nRowBlocks, B, 0, s>>>(A, L.valS32.as<float>(), in, L.wPre32.as<float>(), L.x.as<float>(),
+                                         L.r.as<float>(), done);
+    if (l + 1 < levels())
+    {
+        MgLevel& C = lv_[l + 1];
+        k_mg_restrict<<<divUp(C.n, B), B, 0, s>>>(C.n, L.aggStart.as<int>(), L.aggMem.as<int>(),
+                                                  l == 0 ? L.t32.as<float>() : nullptr, L.r.as<float>(),
+                                                  C.b.as<float>(), done);
+        solveLevel(l + 1, done);
+        k_mg_prolong<<<divUp(L.n, B), B, 0, s>>>(L.n, L.agg.as<int>(), L.s32.as<float>(), C.e.as<float>(),
+                                                  L.x.as<float>(), done);
+    }
+    k_mg_up<<<Gs, B, 0, s>>>(A, rbPerWg, L.val32.as<float>(), L.x.as<float>(), in, L.wPost32.as<float>(), out, q,
+                             partZQ, done);
 }
 
 void Multigrid::apply(const float* r, float* z, const float* q, double* partZQ, int Gs, int rbPerWg, const int* done)
 {
-    hipStream_t s = ctx_.stream();
-    MgLevel& L = lv_[0];
-    const CsrView A = fine_.view();
-    if (levels() == 1)
-    {
-        // no coarse level: two Jacobi sweeps
-        k_mg_down<<<A.nRowBlocks, B, 0, s>>>(A, L.valS32.as<float>(), r, L.wPre32.as<float>(), L.x.as<float>(),
-                                             L.r.as<float>(), done);
-        k_mg_up<<<Gs, B, 0, s>>>(A, rbPerWg, L.val32.as<float>(), L.x.as<float>(), r, L.wPost32.as<float>(), z, q,
-                                 partZQ, done);
-        CFX_CHECK_LAUNCH();
-        return;
-    }
-    MgLevel& C = lv_[1];
-    k_mg_down<<<A.nRowBlocks, B, 0, s>>>(A, L.valS32.as<float>(), r, L.wPre32.as<float>(), L.x.as<float>(),
-                                         L.r.as<float>(), done);
-    k_mg_restrict<<<divUp(C.n, B), B, 0, s>>>(C.n, L.aggStart.as<int>(), L.aggMem.as<int>(), L.t32.as<float>(),
-                                              L.r.as<float>(), C.b.as<float>(), done);
-    recurse(1, done);
-    const float* ec = (levels() == 2) ? C.x.as<float>() : C.r.as<float>();
-    k_mg_prolong<<<divUp(L.n, B), B, 0, s>>>(L.n, L.agg.as<int>(), L.s32.as<float>(), ec, L.x.as<float>(), done);
-    k_mg_up<<<Gs, B, 0, s>>>(A, rbPerWg, L.val32.as<float>(), L.x.as<float>(), r, L.wPost32.as<float>(), z, q,
-                             partZQ, done);
+    visit(0, r, z, q, partZQ, Gs, rbPerWg, done);
     CFX_CHECK_LAUNCH();
 }
 

@@ -77,7 +77,8 @@ class MgParams(C.Structure):
     """cfx_mg_params: finest-level a (pre), s (scale), c (post); coarse-level constants; hierarchy caps."""
     _fields_ = [("omegaPre", C.c_float), ("scale", C.c_float), ("omegaPost", C.c_float),
                 ("restrictScale", C.c_float), ("coarseOmega", C.c_float), ("coarseScale", C.c_float), ("coarseSweeps", C.c_int),
-                ("coarsestOmega", C.c_float), ("maxLevels", C.c_int), ("coarseMax", C.c_int)]
+                ("coarsestOmega", C.c_float), ("maxLevels", C.c_int), ("coarseMax", C.c_int),
+                ("kcycleDepth", C.c_int)]
 
 
 class CfxError(RuntimeError):

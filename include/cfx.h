@@ -200,7 +200,8 @@ typedef struct cfx_mg_params {
     float coarseOmega, coarseScale;     /* coarse levels: a = c, s */
     int coarseSweeps;                   /* damped Jacobi sweeps on the coarsest level */
     float coarsestOmega;
-    int maxLevels, coarseMax;           /* hierarchy depth cap, coarsest size cap (<= 2048) */
+    int maxLevels, coarseMax;           /* hierarchy depth cap, coarsest size cap (<= 2048; <= 128: exact solve) */
+    int kcycleDepth;                    /* coarse levels 1..kcycleDepth use the K-cycle (0: V-cycle) */
 } cfx_mg_params;
 int cfx_mg_default_params(cfx_mg_params* p);
 int cfx_csr_set_mg(cfx_csr* A, const cfx_mg_params* p);

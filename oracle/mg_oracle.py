@@ -85,8 +85,12 @@ class Params:
     coarseScale: float = 1.0
     coarseSweeps: int = 30
     coarsestOmega: float = 2.0 / 3.0
-    maxLevels: int = 8
-    coarseMax: int = 2048
+    maxLevels: int = 10
+    coarseMax: int = 128
+    kcycleDepth: int = 2
+
+
+DENSE_MAX = 128
 
 
 @dataclass
@@ -96,7 +100,7 @@ class Hierarchy:
     P: list = field(default_factory=list)
 
 
-def hierarchy(A, maxLevels=8, coarseMax=2048):
+def hierarchy(A, maxLevels=10, coarseMax=128):
     A = sp.csr_matrix(A)
     A.sort_indices()
     H = Hierarchy(A=[A])
@@ -114,9 +118,21 @@ def hierarchy(A, maxLevels=8, coarseMax=2048):
     return H
 
 
+def dense_coarse_solve(Ac, b):
+    """Exact coarsest solve; a pure Neumann operator (all row sums below
+    1e-9 max|diag|) is solved as A + gamma 1 1^T, gamma = mean(diag) / n,
+    i.e. the pseudo-inverse on zero-mean right-hand sides."""
+    M = Ac.toarray()
+    n = M.shape[0]
+    d = np.diag(M)
+    if np.abs(M.sum(axis=1)).max() <= 1e-9 * np.abs(d).max():
+        M = M + d.sum() / n / n
+    return np.linalg.solve(M, b)
+
+
 def coarsest(Ac, b, sweeps, omega, exact=False):
     if exact:
-        return np.linalg.solve(Ac.toarray(), b)
+        return dense_coarse_solve(Ac, b)
     dinv = 1.0 / Ac.diagonal()
     x = np.zeros_like(b)
     for _ in range(sweeps):
@@ -124,15 +140,40 @@ def coarsest(Ac, b, sweeps, omega, exact=False):
     return x
 
 
+def solve_level(H, b, p: Params, level, exact_coarse=False):
+    """Coarse correction on level >= 1: exact / Jacobi on the coarsest
+    level, the K-cycle (two flexible-CG steps preconditioned by the level's
+    cycle) on levels 1..kcycleDepth, one cycle below."""
+    A = H.A[level]
+    if level == len(H.A) - 1:
+        return coarsest(A, b, p.coarseSweeps, p.coarsestOmega, exact_coarse or A.shape[0] <= DENSE_MAX)
+    M = lambda v: vcycle(H, v, p, exact_coarse=exact_coarse, level=level)
+    if level > p.kcycleDepth:
+        return M(b)
+    c1 = M(b)
+    v1 = A @ c1
+    rho1, a1 = c1 @ v1, c1 @ b
+    r2 = b - (a1 / rho1 if rho1 > 0 else 0.0) * v1
+    c2 = M(r2)
+    v2 = A @ c2
+    g, be, a2 = c2 @ v1, c2 @ v2, c2 @ r2
+    k1, k2 = 1.0, 0.0
+    if rho1 > 0:
+        rho2 = be - g * g / rho1
+        if rho2 > 0:
+            k1, k2 = a1 / rho1 - g * a2 / (rho1 * rho2), a2 / rho2
+        else:
+            k1 = a1 / rho1
+    return k1 * c1 + k2 * c2
+
+
 def vcycle(H, r, p: Params, pre=None, scale=None, post=None, exact_coarse=False, level=0, restrict=None):
-    """z = M r: x = Wpre b; r = b - A x; bc = P^T T r; recurse; x += S P e;
-    z = x + Wpost (b - A x).  pre/scale/post/restrict are per-row
-    finest-level a, s, c, t (None -> constant)."""
+    """z = M r: x = Wpre b; r = b - A x; bc = P^T T r; e = coarse correction
+    (solve_level); x += S P e; z = x + Wpost (b - A x).  pre/scale/post/
+    restrict are per-row finest-level a, s, c, t (None -> constant)."""
     A = H.A[level]
     b = np.asarray(r, np.float64)
     last = len(H.A) - 1
-    if level == last and level > 0:
-        return coarsest(A, b, p.coarseSweeps, p.coarsestOmega, exact_coarse)
     dinv = 1.0 / A.diagonal()
     if level == 0:
         a = p.omegaPre if pre is None else np.asarray(pre, np.float64)
@@ -148,7 +189,7 @@ def vcycle(H, r, p: Params, pre=None, scale=None, post=None, exact_coarse=False,
         res = b - A @ x
         agg = H.agg[level]
         bc = np.bincount(agg, weights=t * res, minlength=H.A[level + 1].shape[0])
-        ec = vcycle(H, bc, p, exact_coarse=exact_coarse, level=level + 1)
+        ec = solve_level(H, bc, p, level + 1, exact_coarse)
         x = x + s * ec[agg]
     return x + c * dinv * (b - A @ x)
 

@@ -16,9 +16,9 @@ from test_oracle_mg import laplacian3d
 pytestmark = pytest.mark.gpu
 
 
-def fv_pressure_system(nx, ny, nz, seed):
-    """Jittered, shuffled, RCM-renumbered FV Laplacian (gamma = 1, Neumann)
-    with one Dirichlet-pinned cell so the matrix is SPD, and a zero-mean
+def fv_pressure_system(nx, ny, nz, seed, pin=True):
+    """Jittered, shuffled, RCM-renumbered FV Laplacian (gamma = 1, Neumann),
+    by default with one pinned cell so the matrix is SPD, and a zero-mean
     N(0,1) right-hand side."""
     hm, a, om = host_and_oracle(nx, ny, nz, (0, 0, 0), (1, 1, 1), jitter=0.15, seed=seed, shuffle=seed + 1,
                                 rcm=True)
@@ -28,7 +28,8 @@ def fv_pressure_system(nx, ny, nz, seed):
     O.assemble_laplacian(om, A, np.ones(om.nFaces), lc, lk, -1)
     csr = O.csr_from_ldu(om, A)
     M = sp.csr_matrix((csr.vals, csr.cols, csr.rowStart), shape=(om.nCells, om.nCells))
-    M[0, 0] = M[0, 0] * 2.0  # pin the constant null space
+    if pin:
+        M[0, 0] = M[0, 0] * 2.0  # pin the constant null space
     M.sort_indices()
     b = np.random.default_rng(seed).standard_normal(om.nCells)
     return M, b - b.mean()
@@ -37,7 +38,8 @@ def fv_pressure_system(nx, ny, nz, seed):
 @pytest.fixture(scope="module")
 def systems(ctx):
     print("\n[case] multigrid systems: 40^3 random-conductance Laplacian, 32^3 jittered FV", flush=True)
-    return {"lap40": (laplacian3d(40, 40, 40, seed=3), None), "fv32": fv_pressure_system(32, 32, 32, 5)}
+    return {"lap40": (laplacian3d(40, 40, 40, seed=3), None), "fv32": fv_pressure_system(32, 32, 32, 5),
+            "fv24n": fv_pressure_system(24, 24, 24, 6, pin=False)}
 
 
 def make(ctx, A):
@@ -45,7 +47,7 @@ def make(ctx, A):
     return N, N.Csr(ctx, A.indptr.astype(np.int64), A.indices.astype(np.int64), A.data)
 
 
-@pytest.mark.parametrize("name", ["lap40", "fv32"])
+@pytest.mark.parametrize("name", ["lap40", "fv32", "fv24n"])
 def test_hierarchy_matches_oracle(ctx, systems, name):
     A, _ = systems[name]
     N, G = make(ctx, A)
@@ -60,8 +62,10 @@ def test_hierarchy_matches_oracle(ctx, systems, name):
             assert np.array_equal(agg, H.agg[l])
 
 
-@pytest.mark.parametrize("name", ["lap40", "fv32"])
-def test_vcycle_fp32_matches_oracle(ctx, systems, name):
+@pytest.mark.parametrize("name,maxLevels", [("lap40", 10), ("fv32", 10), ("fv32", 3), ("fv24n", 10)])
+def test_vcycle_fp32_matches_oracle(ctx, systems, name, maxLevels):
+    """Dense exact coarsest solve (<= 128 rows; pure Neumann in fv24n) and,
+    with 3 levels, Jacobi sweeps on a 253-row coarsest level."""
     A, _ = systems[name]
     N, G = make(ctx, A)
     n = A.shape[0]
@@ -71,19 +75,22 @@ def test_vcycle_fp32_matches_oracle(ctx, systems, name):
                                   rng.uniform(0.4, 1.0, n).astype(np.float32),
                                   rng.uniform(0.5, 1.5, n).astype(np.float32))
     dpre, dscale, dpost, drestrict = ctx.array(pre), ctx.array(scale), ctx.array(post), ctx.array(restrict)
-    G.set_mg(coarseScale=1.7, coarseSweeps=20)
+    G.set_mg(coarseScale=1.7, coarseSweeps=20, maxLevels=maxLevels)
     G.set_mg_weights(dpre, dscale, dpost, drestrict)
-    r = rng.standard_normal(n).astype(np.float32)
+    r = rng.standard_normal(n)
+    r = (r - r.mean()).astype(np.float32)
     z = ctx.empty(n, np.float32)
     G.mg_apply(ctx.array(r), z)
     p = MG.Params(coarseScale=1.7, coarseSweeps=20)
-    ref = MG.vcycle(MG.hierarchy(A), r.astype(np.float64), p, pre.astype(np.float64), scale.astype(np.float64),
+    H = MG.hierarchy(A, maxLevels=maxLevels)
+    assert len(H.A) == len(G.mg_info()[0])
+    ref = MG.vcycle(H, r.astype(np.float64), p, pre.astype(np.float64), scale.astype(np.float64),
                     post.astype(np.float64), restrict=restrict.astype(np.float64))
     err = np.abs(z.get() - ref).max() / np.abs(ref).max()
     assert err < 1e-4, err
 
 
-@pytest.mark.parametrize("name", ["lap40", "fv32"])
+@pytest.mark.parametrize("name", ["lap40", "fv32", "fv24n"])
 def test_fcg_converges_to_true_fp64_residual(ctx, systems, name):
     A, b = systems[name]
     if b is None:

@@ -100,12 +100,26 @@ def test_tied_weights_give_a_symmetric_cycle():
     n = A.shape[0]
     rng = np.random.default_rng(13)
     w, t = rng.uniform(0.4, 0.9, n), rng.uniform(0.8, 1.6, n)
-    p = MG.Params(coarseScale=1.5, coarseSweeps=7)
+    p = MG.Params(coarseScale=1.5, coarseSweeps=7, kcycleDepth=0)  # the K-cycle is nonlinear
     M = np.column_stack([MG.vcycle(H, e, p, w, t, w, restrict=t) for e in np.eye(n)])
     np.testing.assert_allclose(M, M.T, atol=1e-12 * np.abs(M).max())
     # untied weights break the symmetry
     M2 = np.column_stack([MG.vcycle(H, e, p, w, t, 0.5 * w, restrict=t) for e in np.eye(n)])
     assert np.abs(M2 - M2.T).max() > 1e-3 * np.abs(M2).max()
+
+
+def test_kcycle_equals_two_flexible_cg_steps_on_the_level():
+    A = laplacian3d(16, 15, 14, seed=14)
+    H = MG.hierarchy(A, coarseMax=10)
+    assert len(H.A) >= 4
+    p = MG.Params(scale=1.6, coarseScale=1.6, kcycleDepth=2)
+    b = np.random.default_rng(15).standard_normal(H.A[1].shape[0])
+    e = MG.solve_level(H, b, p, 1)
+    x, _, _ = MG.fcg(H.A[1], b, lambda r: MG.vcycle(H, r, p, level=1), tol=0.0, maxIter=2)
+    np.testing.assert_allclose(e, x, rtol=0, atol=1e-12 * np.abs(x).max())
+    # below kcycleDepth the correction is one cycle
+    p0 = MG.Params(scale=1.6, coarseScale=1.6, kcycleDepth=0)
+    np.testing.assert_array_equal(MG.solve_level(H, b, p0, 1), MG.vcycle(H, b, p0, level=1))
 
 
 def test_coarsest_sweeps_match_polynomial_closed_form():
