@@ -19,6 +19,7 @@ struct cfx_ctx
     std::unique_ptr<Context> ctx;
     std::string err;
     std::map<int, std::unique_ptr<Krylov>> krylov;  // workspace per system size
+    bool graphs = true;
 
     Krylov& solver(int n, int rowBlocks)
     {
@@ -26,6 +27,7 @@ struct cfx_ctx
         // a workspace is sized for a row-block count; grow it for a matrix
         // of the same size with more CSR-stream blocks
         if (!k || k->maxBlocks() < rowBlocks) k = std::make_unique<Krylov>(*ctx, n, std::max(rowBlocks, 1));
+        k->useGraphs = graphs;
         return *k;
     }
 };
@@ -688,6 +690,13 @@ int cfx_csr_residual_history(cfx_csr* A, double* hist, int maxLen)
         CFX_HIP(hipMemcpyAsync(hist, A->resHist.p, sizeof(double) * len, hipMemcpyDeviceToHost, A->owner->ctx->stream()));
         A->owner->ctx->sync();
     });
+}
+
+int cfx_set_graphs(cfx_ctx* c, int enable)
+{
+    if (!c) return CFX_E_INVALID;
+    c->graphs = enable != 0;
+    return CFX_OK;
 }
 
 int cfx_mg_default_params(cfx_mg_params* p)

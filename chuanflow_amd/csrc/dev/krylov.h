@@ -11,6 +11,8 @@
 
 #include "types.h"
 
+#include <vector>
+
 namespace cfx
 {
 
@@ -81,6 +83,8 @@ public:
 
     // Batch control: iterations launched between host polls.
     int minBatch = 8, maxBatch = 256;
+    // flexible CG: replay one captured iteration (true) or launch its kernels one by one
+    bool useGraphs = true;
 
 private:
     template <typename T>
@@ -96,6 +100,11 @@ private:
     DevBuf r_, p_, q_, r0_, v_, s_, sh_, t_, z_, invD_, invDb_;
     DevBuf r32_, p32_, q32_, d32_, b32_, invD32_;
     DevBuf z32_, fcgPart_;  // allocated by the first fcgMixed
+    // captured flexible-CG iteration, valid for the key (multigrid, buffers) and the stop parameters
+    hipGraphExec_t fcgExec_ = nullptr;
+    std::vector<const void*> fcgKey_;
+    double fcgTol_ = -1, fcgDelta_ = -1;
+    int fcgMaxIter_ = -1;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     double lastMs_ = 0;
 };

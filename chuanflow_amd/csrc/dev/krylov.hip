@@ -470,6 +470,7 @@ Krylov::Krylov(Context& ctx, int n, int maxRowBlocks) : ctx_(ctx), n_(n)
 
 Krylov::~Krylov()
 {
+    if (fcgExec_) (void)hipGraphExecDestroy(fcgExec_);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
 }

@@ -33,6 +33,7 @@
 
 #include "types.h"
 
+#include <cstdint>
 #include <memory>
 #include <utility>
 #include <vector>
@@ -66,6 +67,8 @@ struct MgLevel
     // transfer to level l+1
     DevBuf agg;                    // int[n]: coarse row of each row
     DevBuf aggStart, aggMem;       // members of each coarse row (ascending)
+    DevPattern rpat;               // P^T as a CSR pattern (rows = coarse rows, columns = members)
+    DevBuf rval;                   // its values: the restriction scale of each member (1 off the finest level)
     DevBuf gStart, gPerm;          // coarse nonzero c = sum of fine nonzeros gPerm[gStart[c]..gStart[c+1])
     DevBuf dense;                  // coarsest level: FP32 inverse [n][n] (symmetric)
     std::vector<int> hAgg;
@@ -79,6 +82,8 @@ public:
     bool denseCoarsest() const { return levels() > 1 && lv_.back().n <= kDenseMax; }
     static constexpr int kDenseMax = 128;
     const MgParams& params() const { return p_; }
+    // unique per constructed hierarchy (device buffers never move after construction)
+    std::uintptr_t id() const { return id_; }
     int levels() const { return static_cast<int>(lv_.size()); }
     int rows(int l) const { return lv_[l].n; }
     int nnz(int l) const { return lv_[l].nnz; }
@@ -104,6 +109,7 @@ private:
     Context& ctx_;
     const DevPattern& fine_;
     MgParams p_;
+    std::uintptr_t id_ = 0;
     std::vector<MgLevel> lv_;  // lv_[0] uses the fine pattern (pat unused)
     // out = cycle of level l applied to in (in, out distinct from the level's work vectors)
     void visit(int l, const float* in, float* out, const float* q, double* partZQ, int Gs, int rbPerWg,

@@ -3,6 +3,7 @@
 #include "mg.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <numeric>
 #include <stdexcept>
@@ -143,20 +144,24 @@ __global__ __launch_bounds__(B) void k_mg_down(CsrView A, const float* __restric
     }
 }
 
-// bc = P^T (T r): T = diag(ts) (null: identity)
-__global__ void k_mg_restrict(int nc, const int* __restrict__ aggStart, const int* __restrict__ aggMem,
-                              const float* __restrict__ ts, const float* __restrict__ r, float* __restrict__ bc,
-                              const int* done)
+// bc = P^T (T r) as a CSR-stream product over the transfer pattern
+__global__ __launch_bounds__(B) void k_mg_restrict_csr(CsrView R, const float* __restrict__ rval,
+                                                       const float* __restrict__ r, float* __restrict__ bc,
+                                                       const int* done)
 {
+    __shared__ float lds[kSpmvLds];
     if (done && *done) return;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
-    float s = 0.0f;
-    if (ts)
-        for (int t = aggStart[c]; t < aggStart[c + 1]; ++t) s += ts[aggMem[t]] * r[aggMem[t]];
-    else
-        for (int t = aggStart[c]; t < aggStart[c + 1]; ++t) s += r[aggMem[t]];
-    bc[c] = s;
+    int row;
+    float s;
+    if (rowSum<kSpmvLds>(R, rval, r, xcdRemap(blockIdx.x, gridDim.x), lds, row, s)) bc[row] = s;
+}
+
+// transfer values: restriction scale of each member (null: 1)
+__global__ void k_restrict_vals(int n, const int* __restrict__ mem, const float* __restrict__ ts,
+                                float* __restrict__ rval)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) rval[k] = ts ? ts[mem[k]] : 1.0f;
 }
 
 __global__ void k_mg_prolong(int n, const int* __restrict__ agg, const float* __restrict__ sc,
@@ -326,8 +331,14 @@ __global__ __launch_bounds__(kDense) void k_mg_dense_apply(int n, const float* _
 // deterministic all-workgroup sum of G <= 2048 partials (blockDim B)
 __device__ __forceinline__ double redAll(const double* __restrict__ p, int G, double* lds)
 {
-    double v = 0.0;
-    for (int i = threadIdx.x; i < G; i += B) v += p[i];
+    double t[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+    {
+        const int i = threadIdx.x + j * B;
+        t[j] = i < G ? p[i] : 0.0;
+    }
+    double v = ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
     v = blockSum(v, lds);
     if (threadIdx.x == 0) lds[B / kWave] = v;
     __syncthreads();
@@ -460,6 +471,8 @@ __global__ __launch_bounds__(B) void k_kc_combine(int n, const float* __restrict
 
 Multigrid::Multigrid(Context& ctx, const DevPattern& fine, int maxLevels, int coarseMax) : ctx_(ctx), fine_(fine)
 {
+    static std::atomic<std::uintptr_t> counter{0};
+    id_ = ++counter;
     hipStream_t s = ctx.stream();
     coarseMax = std::min(coarseMax, kCoarseMax);
     std::vector<int> rp = fine.hRowPtr, cl = fine.hCol;
@@ -512,6 +525,8 @@ Multigrid::Multigrid(Context& ctx, const DevPattern& fine, int maxLevels, int co
         L.agg = upload(agg, s);
         L.aggStart = upload(aStart, s);
         L.aggMem = upload(aMem, s);
+        L.rpat.build(aStart, aMem, s);
+        L.rval.alloc(sizeof(float) * n);
         L.gStart = upload(gStart, s);
         L.gPerm = upload(gPerm, s);
         L.hAgg = agg;
@@ -577,6 +592,9 @@ void Multigrid::setMatrix(const double* fineVal64, const float* pre, const float
                                                   fineLevel ? restr : nullptr, fineLevel ? p.restrictScale : 1.0f);
         k_scale_vals<<<divUp(L.n, B), B, 0, s>>>(A, L.val32.as<float>(), L.wPre32.as<float>(), L.valS32.as<float>());
         if (L.dense.p) k_mg_dense_inverse<<<1, 1024, 0, s>>>(A, L.val64.as<double>(), L.dense.as<float>());
+        if (L.rval.p)
+            k_restrict_vals<<<divUp(L.n, B), B, 0, s>>>(L.n, L.aggMem.as<int>(), l == 0 ? L.t32.as<float>() : nullptr,
+                                                        L.rval.as<float>());
     }
     CFX_CHECK_LAUNCH();
 }
@@ -628,9 +646,8 @@ void Multigrid::visit(int l, const float* in, float* out, const float* q, double
     if (l + 1 < levels())
     {
         MgLevel& C = lv_[l + 1];
-        k_mg_restrict<<<divUp(C.n, B), B, 0, s>>>(C.n, L.aggStart.as<int>(), L.aggMem.as<int>(),
-                                                  l == 0 ? L.t32.as<float>() : nullptr, L.r.as<float>(),
-                                                  C.b.as<float>(), done);
+        k_mg_restrict_csr<<<L.rpat.nRowBlocks, B, 0, s>>>(L.rpat.view(), L.rval.as<float>(), L.r.as<float>(),
+                                                          C.b.as<float>(), done);
         solveLevel(l + 1, done);
         k_mg_prolong<<<divUp(L.n, B), B, 0, s>>>(L.n, L.agg.as<int>(), L.s32.as<float>(), C.e.as<float>(),
                                                   L.x.as<float>(), done);

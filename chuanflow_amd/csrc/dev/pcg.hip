@@ -40,6 +40,7 @@ struct CgState
     double rz[2];        // r.z after iteration k (slot k & 1)
     double bnorm, initialResidual, rnorm, alpha, pAp, updAbs, beta;
     int iter, done, converged, breakdown, paused, nan;
+    int kcur, pad;       // flexible CG: iteration index of the running step (set by B, read by C)
 };
 
 __device__ __forceinline__ int xcdRemap(int h, int G)
@@ -414,6 +415,7 @@ __global__ __launch_bounds__(kB) void k_fcg_B(int n, float* __restrict__ x, floa
     const double PQ = allSum(partPQ, Gs, red);
     const double PR = allSum(partPR, Gs, red);
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (lead) st->kcur = st->iter + 1;
     if (!(PQ > 0) || !(fabs(PR) < INFINITY))
     {
         // A p.p <= 0 (not SPD) or a non-finite value from the preconditioner
@@ -450,15 +452,17 @@ __global__ __launch_bounds__(kB) void k_fcg_B(int n, float* __restrict__ x, floa
 
 // FCG C(k): stop decision on max|r_k| (k = 0: initial residual),
 // beta = -z.q / p.q, p = z + beta p.  flags as in k_pcg_C (bit0 true residual,
-// bit1 mixed).
+// bit1 mixed).  kArg < 0 takes k from the state (set by B), so that one
+// captured iteration can be replayed.
 __global__ __launch_bounds__(kB) void k_fcg_C(int n, const float* __restrict__ z, float* __restrict__ p, CgState* st,
                                               const double* __restrict__ partZQ, int nZQ,
                                               const double* __restrict__ partRM, const double* __restrict__ partBM,
-                                              int nRM, int k, double tol, int maxIter, double delta, int flags,
+                                              int nRM, int kArg, double tol, int maxIter, double delta, int flags,
                                               double* resHist)
 {
     __shared__ double red[kB / kWave];
     if (st->done) return;
+    const int k = kArg >= 0 ? kArg : st->kcur;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     const double RM = allMaxNan(partRM, nRM, red);
     const double ZQ = k == 0 ? 0.0 : allSum(partZQ, nZQ, red);
@@ -726,15 +730,43 @@ SolveReport Krylov::fcgMixed(const CsrView& A, const double* val, Multigrid& mg,
         batch = std::max(2, std::min(64, batch));
         batch = std::min(batch, maxIter - k);
         if (batch <= 0) break;
-        for (int j = 0; j < batch; ++j)
+        if (!useGraphs)
         {
-            ++k;
-            k_fcg_A<<<g.Gs, kB, 0, s>>>(A, g.rbPerWg, val, p, r, q, st, partPQ, partPR);
-            k_fcg_B<<<g.Gv, kB, 0, s>>>(n_, x32, r, p, q, st, partPQ, partPR, g.Gs, partRM);
-            mg.apply(r, z, q, partZQ, g.Gs, g.rbPerWg, done);
-            k_fcg_C<<<g.Gv, kB, 0, s>>>(n_, z, p, st, partZQ, g.Gs, partRM, partBM, g.Gv, k, tol, maxIter, delta, 2,
-                                        resHist);
+            for (int j = 0; j < batch; ++j)
+            {
+                k_fcg_A<<<g.Gs, kB, 0, s>>>(A, g.rbPerWg, val, p, r, q, st, partPQ, partPR);
+                k_fcg_B<<<g.Gv, kB, 0, s>>>(n_, x32, r, p, q, st, partPQ, partPR, g.Gs, partRM);
+                mg.apply(r, z, q, partZQ, g.Gs, g.rbPerWg, done);
+                k_fcg_C<<<g.Gv, kB, 0, s>>>(n_, z, p, st, partZQ, g.Gs, partRM, partBM, g.Gv, -1, tol, maxIter,
+                                            delta, 2, resHist);
+            }
         }
+        else
+        {
+            // one iteration (A, B, the cycle, C) as a captured graph, replayed per step
+            const std::vector<const void*> key{&mg, val, resHist, reinterpret_cast<const void*>(mg.id()), A.rowPtr};
+            if (!fcgExec_ || key != fcgKey_ || tol != fcgTol_ || maxIter != fcgMaxIter_ || delta != fcgDelta_)
+            {
+                if (fcgExec_) CFX_HIP(hipGraphExecDestroy(fcgExec_));
+                fcgExec_ = nullptr;
+                hipGraph_t graph;
+                CFX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                k_fcg_A<<<g.Gs, kB, 0, s>>>(A, g.rbPerWg, val, p, r, q, st, partPQ, partPR);
+                k_fcg_B<<<g.Gv, kB, 0, s>>>(n_, x32, r, p, q, st, partPQ, partPR, g.Gs, partRM);
+                mg.apply(r, z, q, partZQ, g.Gs, g.rbPerWg, done);
+                k_fcg_C<<<g.Gv, kB, 0, s>>>(n_, z, p, st, partZQ, g.Gs, partRM, partBM, g.Gv, -1, tol, maxIter, delta, 2,
+                                            resHist);
+                CFX_HIP(hipStreamEndCapture(s, &graph));
+                CFX_HIP(hipGraphInstantiate(&fcgExec_, graph, nullptr, nullptr, 0));
+                CFX_HIP(hipGraphDestroy(graph));
+                fcgKey_ = key;
+                fcgTol_ = tol;
+                fcgMaxIter_ = maxIter;
+                fcgDelta_ = delta;
+            }
+            for (int j = 0; j < batch; ++j) CFX_HIP(hipGraphLaunch(fcgExec_, s));
+        }
+        k += batch;
         CFX_CHECK_LAUNCH();
         pollState();
         if (r0 < 0) r0 = hs->initialResidual;

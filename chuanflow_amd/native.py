@@ -145,6 +145,7 @@ _sig("cfx_csr_solve_device", C.c_int, _vp, _vp, _vp, C.c_double, C.c_int, C.c_in
 _sig("cfx_csr_residual_history", C.c_int, _vp, _vp, C.c_int)
 _sig("cfx_csr_set_inner_tol", C.c_int, _vp, C.c_double)
 _sig("cfx_mg_default_params", C.c_int, C.POINTER(MgParams))
+_sig("cfx_set_graphs", C.c_int, _vp, C.c_int)
 _sig("cfx_csr_set_mg", C.c_int, _vp, C.POINTER(MgParams))
 _sig("cfx_csr_set_mg_weights", C.c_int, _vp, _vp, _vp, _vp, _vp)
 _sig("cfx_matrix_set_mg", C.c_int, _vp, C.POINTER(MgParams))
@@ -232,6 +233,10 @@ class Context:
 
     def sync(self):
         self.check(_L.cfx_synchronize(self.h))
+
+    def set_graphs(self, enable: bool):
+        """Flexible CG: replay one captured iteration per step (True) or launch kernel by kernel."""
+        self.check(_L.cfx_set_graphs(self.h, 1 if enable else 0))
 
     def device_info(self):
         buf = C.create_string_buffer(64)

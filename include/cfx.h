@@ -204,6 +204,9 @@ typedef struct cfx_mg_params {
     int kcycleDepth;                    /* coarse levels 1..kcycleDepth use the K-cycle (0: V-cycle) */
 } cfx_mg_params;
 int cfx_mg_default_params(cfx_mg_params* p);
+/* flexible CG replays one captured iteration per step (hipGraph, default 1)
+ * or launches its kernels one by one (0); both give identical results */
+int cfx_set_graphs(cfx_ctx* ctx, int enable);
 int cfx_csr_set_mg(cfx_csr* A, const cfx_mg_params* p);
 /* per-row finest-level a, s, c, t (device FP32 [n] each, or null for the
  * constant); the pointers are kept and read whenever the hierarchy's values

@@ -156,3 +156,18 @@ def test_new_values_on_same_pattern_rebuild_galerkin_values(ctx, systems):
     assert rep.converged and np.abs(b - A2 @ x).max() / np.abs(b).max() < 1e-8
     _, _, vv, _ = G.mg_level(1)
     np.testing.assert_allclose(vv, MG.hierarchy(A2).A[1].data, rtol=1e-13, atol=1e-12)
+
+
+def test_graph_replay_equals_eager_launches(ctx, systems):
+    A, b = systems["fv32"]
+    N, G = make(ctx, A)
+    G.set_mg(scale=1.8, coarseScale=1.8)
+    xs, reps = [], []
+    for graphs in (True, False, True):
+        ctx.set_graphs(graphs)
+        x, rep = G.solve(b, None, 1e-8, 500, N.SOLVER_CG, N.PRECOND_LEARNED, N.FP32_IR)
+        xs.append(x)
+        reps.append(rep.iterations)
+    ctx.set_graphs(True)
+    assert reps[0] == reps[1] == reps[2]
+    assert np.array_equal(xs[0], xs[1]) and np.array_equal(xs[0], xs[2])
