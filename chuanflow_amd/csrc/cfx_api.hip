@@ -6,6 +6,7 @@
 #include "dev/mesh.h"
 #include "dev/mg.h"
 #include "dev/simple.h"
+#include "dev/train.h"
 
 #include <cstring>
 #include <map>
@@ -32,6 +33,13 @@ struct cfx_ctx
     }
 };
 
+struct cfx_mpnn
+{
+    cfx_ctx* owner = nullptr;
+    std::unique_ptr<Mpnn> net;
+    std::unique_ptr<MpnnTrainer> trainer;
+};
+
 struct cfx_mesh
 {
     cfx_ctx* owner = nullptr;
@@ -45,6 +53,8 @@ struct cfx_matrix
     int ncomp = 1;
     DevBuf diag, rhs, rhsBase, upper, lower, val, val32, tmp;
     MgCache mg;
+    cfx_mpnn* net = nullptr;     // learned finest-level weights (precond LEARNED)
+    DevBuf geo, wa, wt;          // per-slot face geometry, network outputs
     Ldu ldu() const
     {
         Ldu A;
@@ -66,6 +76,9 @@ struct cfx_csr
     int lastIters = -1;
     bool val32Valid = false;
     MgCache mg;
+    cfx_mpnn* net = nullptr;     // learned finest-level weights (precond LEARNED)
+    const double* geo = nullptr; // caller's per-slot geometry (optional)
+    DevBuf wa, wt;
 };
 
 namespace
@@ -517,7 +530,24 @@ int cfx_matrix_solve(cfx_matrix* A, double* x, int solver, double tol, int maxIt
         need(precond != CFX_PRECOND_LEARNED || (solver == CFX_SOLVER_CG && precision == CFX_FP32_IR),
              "the multigrid preconditioner runs inside flexible CG in FP32 with FP64 refinement");
         FvSolveArgs a;
-        if (precond == CFX_PRECOND_LEARNED) a.mg = &A->mg;
+        if (precond == CFX_PRECOND_LEARNED)
+        {
+            a.mg = &A->mg;
+            if (A->net)
+            {
+                if (!A->geo.p)
+                {
+                    A->geo.alloc(sizeof(double) * m.pattern.nnz);
+                    meshSlotGeometry(m, A->geo.as<double>());
+                    A->wa.alloc(sizeof(float) * m.n);
+                    A->wt.alloc(sizeof(float) * m.n);
+                }
+                a.net = A->net->net.get();
+                a.geo = A->geo.as<double>();
+                a.wa = A->wa.as<float>();
+                a.wt = A->wt.as<float>();
+            }
+        }
         a.solver = solver;
         a.tol = tol;
         a.maxIter = maxIter;
@@ -650,6 +680,19 @@ int cfx_csr_solve_device(cfx_csr* A, const double* b, double* x, double tol, int
             need(solver == CFX_SOLVER_CG, "the multigrid preconditioner runs inside flexible CG");
             need(precision == CFX_FP32_IR, "the multigrid preconditioner runs in FP32 with FP64 refinement");
             Context& ctx = *A->owner->ctx;
+            if (A->net)
+            {
+                if (!A->wa.p)
+                {
+                    A->wa.alloc(sizeof(float) * A->pattern.n);
+                    A->wt.alloc(sizeof(float) * A->pattern.n);
+                }
+                learnedWeights(*A->net->net, A->pattern, A->val.as<double>(), A->geo, b, A->wa.as<float>(),
+                               A->wt.as<float>());
+                A->mg.pre = A->mg.post = A->wa.as<float>();
+                A->mg.scale = A->mg.restr = A->wt.as<float>();
+                A->mg.valid = false;
+            }
             Multigrid& mg = A->mg.ready(ctx, A->pattern, A->val.as<double>());
             s = A->owner->solver(A->pattern.n, A->pattern.nRowBlocks)
                     .fcgMixed(A->pattern.view(), A->val.as<double>(), mg, b, x, tol, maxIter, A->innerTol, nullptr);
@@ -972,4 +1015,163 @@ int cfx_piso_step(cfx_mesh* m, double nu, const cfx_schemes* schemes, const cfx_
     });
 }
 
+// ------------------------------------------------- message-passing network
+int cfx_mpnn_num_params(void) { return MpnnShape::nParams; }
+
+int cfx_mpnn_create(cfx_ctx* c, uint64_t seed, int fp16, cfx_mpnn** out)
+{
+    if (!c || !out) return CFX_E_INVALID;
+    *out = nullptr;
+    return guard(c, [&] {
+        auto* m = new cfx_mpnn;
+        m->owner = c;
+        try
+        {
+            m->net = std::make_unique<Mpnn>(*c->ctx, seed, fp16 != 0);
+            m->trainer = std::make_unique<MpnnTrainer>(*c->ctx, *m->net);
+        }
+        catch (...)
+        {
+            delete m;
+            throw;
+        }
+        *out = m;
+    });
+}
+
+int cfx_mpnn_destroy(cfx_mpnn* m)
+{
+    if (!m) return CFX_E_INVALID;
+    return guard(m->owner, [&] {
+        m->owner->ctx->sync();
+        delete m;
+    });
+}
+
+int cfx_mpnn_get_params(cfx_mpnn* m, float* host)
+{
+    if (!m || !host) return CFX_E_INVALID;
+    return guard(m->owner, [&] { m->net->getParams(host); });
+}
+
+int cfx_mpnn_set_params(cfx_mpnn* m, const float* host)
+{
+    if (!m || !host) return CFX_E_INVALID;
+    return guard(m->owner, [&] { m->net->setParams(host); });
+}
+
+int cfx_mpnn_forward_csr(cfx_mpnn* m, cfx_csr* A, const double* geo, const double* r0, float* a, float* t,
+                         int train)
+{
+    if (!m || !A || A->owner != m->owner) return CFX_E_INVALID;
+    return guard(m->owner, [&] {
+        m->net->setGraph(A->pattern.view(), A->val.as<double>(), geo, r0);
+        m->net->forward(a, t, train != 0);
+    });
+}
+
+int cfx_mpnn_backward(cfx_mpnn* m, const float* ga, const float* gt, float* grad)
+{
+    if (!m || !ga || !gt || !grad) return CFX_E_INVALID;
+    return guard(m->owner, [&] {
+        need(m->net->n() > 0, "no forward pass recorded");
+        m->net->backward(ga, gt, grad);
+    });
+}
+
+int cfx_mpnn_adam(cfx_mpnn* m, const float* grad, float lr)
+{
+    if (!m || !grad) return CFX_E_INVALID;
+    return guard(m->owner, [&] { m->net->adam(grad, lr); });
+}
+
+namespace
+{
+TrainOptions toTrain(const cfx_train_options* o, const MgParams& mg)
+{
+    TrainOptions t;
+    t.mg = mg;
+    if (o)
+    {
+        t.steps = o->steps;
+        t.samples = o->samples;
+        t.smooth = o->smooth;
+        t.lr = o->lr;
+        t.seed = o->seed;
+    }
+    need(t.steps >= 0 && t.samples >= 1 && t.smooth >= 0 && t.lr > 0, "bad training options");
+    return t;
+}
+} // namespace
+
+int cfx_mpnn_train(cfx_mpnn* m, cfx_csr* const* systems, const double* const* geo, const double* const* r0, int count,
+                   const cfx_train_options* opt, double* lossHist)
+{
+    if (!m || !systems || count <= 0) return CFX_E_INVALID;
+    return guard(m->owner, [&] {
+        std::vector<TrainSystem> sys(count);
+        for (int i = 0; i < count; ++i)
+        {
+            need(systems[i] && systems[i]->owner == m->owner, "bad training system");
+            sys[i].pat = &systems[i]->pattern;
+            sys[i].val = systems[i]->val.as<double>();
+            sys[i].geo = geo ? geo[i] : nullptr;
+            sys[i].r0 = r0 ? r0[i] : nullptr;
+        }
+        const std::vector<double> h = m->trainer->train(sys, toTrain(opt, systems[0]->mg.p));
+        if (lossHist) std::copy(h.begin(), h.end(), lossHist);
+    });
+}
+
+int cfx_mpnn_evaluate(cfx_mpnn* m, cfx_csr* A, const double* geo, const double* r0, const cfx_train_options* opt,
+                      int samples, int withNetwork, double* loss)
+{
+    if (!m || !A || !loss || samples <= 0) return CFX_E_INVALID;
+    return guard(m->owner, [&] {
+        TrainSystem s;
+        s.pat = &A->pattern;
+        s.val = A->val.as<double>();
+        s.geo = geo;
+        s.r0 = r0;
+        *loss = m->trainer->evaluate(s, toTrain(opt, A->mg.p), samples, withNetwork != 0);
+    });
+}
+
+int cfx_csr_mg_loss_grad(cfx_csr* A, const float* e, const float* a, const float* t, float* ga, float* gt,
+                         double* loss)
+{
+    if (!A || !e || !a || !t || !ga || !gt || !loss) return CFX_E_INVALID;
+    return guard(A->owner, [&] {
+        MgParams p = A->mg.p;
+        p.kcycleDepth = 0;
+        Multigrid mg(*A->owner->ctx, A->pattern, p.maxLevels, p.coarseMax);
+        mg.setMatrix(A->val.as<double>(), a, t, a, t, p);
+        *loss = mg.lossGrad(e, a, t, ga, gt);
+    });
+}
+
+int cfx_csr_set_mpnn(cfx_csr* A, cfx_mpnn* m, const double* geo)
+{
+    if (!A) return CFX_E_INVALID;
+    return guard(A->owner, [&] {
+        need(!m || m->owner == A->owner, "network of another context");
+        A->net = m;
+        A->geo = geo;
+        if (!m) A->mg.pre = A->mg.post = A->mg.scale = A->mg.restr = nullptr;
+        A->mg.valid = false;
+    });
+}
+
+int cfx_matrix_set_mpnn(cfx_matrix* A, cfx_mpnn* m)
+{
+    if (!A) return CFX_E_INVALID;
+    return guard(A->mesh->owner, [&] {
+        need(!m || m->owner == A->mesh->owner, "network of another context");
+        A->net = m;
+        if (!m) A->mg.pre = A->mg.post = A->mg.scale = A->mg.restr = nullptr;
+        A->mg.valid = false;
+    });
+}
+
 } // extern "C"
+

@@ -103,6 +103,15 @@ public:
     // per-workgroup partials of z.q (grid Gs with rbPerWg row blocks each).
     void apply(const float* r, float* z, const float* q, double* partZQ, int Gs, int rbPerWg, const int* done);
 
+    // Training objective of the finest-level weights.  Requires setMatrix
+    // with pre = post = a, scale = restr = t and kcycleDepth = 0 (so the
+    // coarse correction Q is linear and symmetric).  For an error sample e,
+    // the cycle's error e - M A e has energy L = |e - M A e|_A^2 / |e|_A^2;
+    // returns L and adds dL/da, dL/dt (per fine row) into ga, gt.
+    double lossGrad(const float* e, const float* a, const float* t, float* ga, float* gt);
+    // e <- (I - M A) e, `sweeps` times, then e / max|e| (smooth error samples)
+    void smoothSample(float* e, int sweeps);
+
     static constexpr int kPartMax = 2048;
 
 private:
@@ -116,6 +125,8 @@ private:
                const int* done);
     // coarse correction of level l >= 1: L.e from L.b
     void solveLevel(int l, const int* done);
+    DevBuf tv_[14];          // finest-level FP32 scratch of lossGrad / smoothSample
+    DevBuf tpart_;
 };
 
 // The multigrid preconditioner of one matrix pattern: the hierarchy (rebuilt

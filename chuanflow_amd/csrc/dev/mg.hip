@@ -467,6 +467,150 @@ __global__ __launch_bounds__(B) void k_kc_combine(int n, const float* __restrict
     for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) e[i] = k1 * c1[i] + k2 * c2[i];
 }
 
+
+// ------------------------------------------------------ training objective
+__global__ __launch_bounds__(B) void k_spmv_f(CsrView A, const float* __restrict__ val, const float* __restrict__ x,
+                                              float* __restrict__ y)
+{
+    __shared__ float lds[kSpmvLds];
+    int row;
+    float s;
+    if (rowSum<kSpmvLds>(A, val, x, xcdRemap(blockIdx.x, gridDim.x), lds, row, s)) y[row] = s;
+}
+
+__global__ void k_dot_part(int n, const float* __restrict__ x, const float* __restrict__ y, double* __restrict__ part)
+{
+    __shared__ double red[B / kWave];
+    double v = 0.0;
+    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B)
+        v += static_cast<double>(x[i]) * static_cast<double>(y[i]);
+    v = blockSum(v, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+__global__ void k_absmax_part(int n, const float* __restrict__ x, double* __restrict__ part)
+{
+    __shared__ double red[B / kWave];
+    double v = 0.0;
+    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) v = fmax(v, fabs(static_cast<double>(x[i])));
+    v = blockMax(v, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+// out[0] = sum (mode 0) or max (mode 1) of G <= 2048 partials
+__global__ void k_final(int G, const double* __restrict__ part, double* __restrict__ out, int mode)
+{
+    __shared__ double red[B / kWave + 1];
+    if (mode == 0)
+    {
+        const double v = redAll(part, G, red);
+        if (threadIdx.x == 0) *out = v;
+    }
+    else
+    {
+        double v = 0.0;
+        for (int i = threadIdx.x; i < G; i += B) v = fmax(v, part[i]);
+        v = blockMax(v, red);
+        if (threadIdx.x == 0) *out = v;
+    }
+}
+
+__global__ void k_lg_start(int n, const float* __restrict__ invD, const float* __restrict__ a,
+                           const float* __restrict__ b, float* __restrict__ db, float* __restrict__ x1)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    const float d = invD[i] * b[i];
+    db[i] = d;
+    x1[i] = a[i] * d;
+}
+
+__global__ void k_lg_sub(int n, const float* __restrict__ b, const float* __restrict__ ax, float* __restrict__ r)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i < n) r[i] = b[i] - ax[i];
+}
+
+__global__ void k_lg_gather(int n, const int* __restrict__ agg, const float* __restrict__ ec, float* __restrict__ u)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i < n) u[i] = ec[agg[i]];
+}
+
+__global__ void k_lg_x2(int n, const float* __restrict__ x1, const float* __restrict__ t, const float* __restrict__ u,
+                        float* __restrict__ x2)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i < n) x2[i] = x1[i] + t[i] * u[i];
+}
+
+// pd = D^-1 (b - A x2), z = x2 + a pd, eps = e - z
+__global__ void k_lg_z(int n, const float* __restrict__ b, const float* __restrict__ ax2,
+                       const float* __restrict__ invD, const float* __restrict__ a, const float* __restrict__ x2,
+                       const float* __restrict__ e, float* __restrict__ pd, float* __restrict__ eps)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    const float p = invD[i] * (b[i] - ax2[i]);
+    pd[i] = p;
+    eps[i] = e[i] - (x2[i] + a[i] * p);
+}
+
+// gz = -2 A eps / (e.A e); ga += gz pd; w1 = D^-1 (a gz)
+__global__ void k_lg_g(int n, const float* __restrict__ aeps, const double* __restrict__ eAe,
+                       const float* __restrict__ a, const float* __restrict__ invD, const float* __restrict__ pd,
+                       float* __restrict__ gz, float* __restrict__ w1, float* __restrict__ ga)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    const float g = static_cast<float>(-2.0 * aeps[i] / *eAe);
+    gz[i] = g;
+    ga[i] += g * pd[i];
+    w1[i] = invD[i] * (a[i] * g);
+}
+
+// gx2 = gz - A w1; gt += gx2 u
+__global__ void k_lg_gx2(int n, const float* __restrict__ gz, const float* __restrict__ aw1,
+                         const float* __restrict__ u, float* __restrict__ gx2, float* __restrict__ gt)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    const float g = gz[i] - aw1[i];
+    gx2[i] = g;
+    gt[i] += g * u[i];
+}
+
+// gt += r1 py; tp = t py
+__global__ void k_lg_tail(int n, const float* __restrict__ r1, const float* __restrict__ py,
+                          const float* __restrict__ t, float* __restrict__ gt, float* __restrict__ tp)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    gt[i] += r1[i] * py[i];
+    tp[i] = t[i] * py[i];
+}
+
+// ga += (gx2 - A tp) D^-1 b
+__global__ void k_lg_end(int n, const float* __restrict__ gx2, const float* __restrict__ atp,
+                         const float* __restrict__ db, float* __restrict__ ga)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i < n) ga[i] += (gx2[i] - atp[i]) * db[i];
+}
+
+__global__ void k_lg_isub(int n, float* __restrict__ e, const float* __restrict__ z)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i < n) e[i] -= z[i];
+}
+
+__global__ void k_scale_by(int n, float* __restrict__ x, const double* __restrict__ m)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    const double s = *m;
+    if (i < n && s > 0.0) x[i] = static_cast<float>(x[i] / s);
+}
+
 } // namespace
 
 Multigrid::Multigrid(Context& ctx, const DevPattern& fine, int maxLevels, int coarseMax) : ctx_(ctx), fine_(fine)
@@ -659,6 +803,88 @@ void Multigrid::visit(int l, const float* in, float* out, const float* q, double
 void Multigrid::apply(const float* r, float* z, const float* q, double* partZQ, int Gs, int rbPerWg, const int* done)
 {
     visit(0, r, z, q, partZQ, Gs, rbPerWg, done);
+    CFX_CHECK_LAUNCH();
+}
+
+double Multigrid::lossGrad(const float* e, const float* a, const float* t, float* ga, float* gt)
+{
+    if (levels() < 2) throw std::invalid_argument("lossGrad: the hierarchy has no coarse level");
+    if (p_.kcycleDepth != 0) throw std::invalid_argument("lossGrad: needs the linear cycle (kcycleDepth = 0)");
+    hipStream_t s = ctx_.stream();
+    MgLevel& L = lv_[0];
+    MgLevel& C = lv_[1];
+    const int n = L.n;
+    if (!tv_[0].p)
+    {
+        for (DevBuf& v : tv_) v.alloc(sizeof(float) * n);
+        tpart_.alloc(sizeof(double) * (2 * kPartMax + 16));
+    }
+    float *b = tv_[0].as<float>(), *db = tv_[1].as<float>(), *x1 = tv_[2].as<float>(), *ax = tv_[3].as<float>(),
+          *r1 = tv_[4].as<float>(), *u = tv_[5].as<float>(), *x2 = tv_[6].as<float>(), *pd = tv_[7].as<float>(),
+          *eps = tv_[8].as<float>(), *aeps = tv_[9].as<float>(), *gz = tv_[10].as<float>(),
+          *w1 = tv_[11].as<float>(), *gx2 = tv_[12].as<float>(), *py = tv_[13].as<float>();
+    double* part = tpart_.as<double>();
+    double* sc = part + 2 * kPartMax;  // [0] e.Ae, [1] eps.A eps
+    const CsrView A = fine_.view();
+    const float* v32 = L.val32.as<float>();
+    const float* invD = L.invD32.as<float>();
+    const int G = divUp(n, B), Gd = std::max(1, std::min(kPartMax, divUp(n, 4 * B)));
+    // forward: the cycle applied to b = A e
+    k_spmv_f<<<A.nRowBlocks, B, 0, s>>>(A, v32, e, b);
+    k_lg_start<<<G, B, 0, s>>>(n, invD, a, b, db, x1);
+    k_spmv_f<<<A.nRowBlocks, B, 0, s>>>(A, v32, x1, ax);
+    k_lg_sub<<<G, B, 0, s>>>(n, b, ax, r1);
+    k_mg_restrict_csr<<<L.rpat.nRowBlocks, B, 0, s>>>(L.rpat.view(), L.rval.as<float>(), r1, C.b.as<float>(), nullptr);
+    solveLevel(1, nullptr);
+    k_lg_gather<<<G, B, 0, s>>>(n, L.agg.as<int>(), C.e.as<float>(), u);
+    k_lg_x2<<<G, B, 0, s>>>(n, x1, t, u, x2);
+    k_spmv_f<<<A.nRowBlocks, B, 0, s>>>(A, v32, x2, ax);
+    k_lg_z<<<G, B, 0, s>>>(n, b, ax, invD, a, x2, e, pd, eps);
+    k_spmv_f<<<A.nRowBlocks, B, 0, s>>>(A, v32, eps, aeps);
+    k_dot_part<<<Gd, B, 0, s>>>(n, e, b, part);
+    k_final<<<1, B, 0, s>>>(Gd, part, sc, 0);
+    k_dot_part<<<Gd, B, 0, s>>>(n, eps, aeps, part + kPartMax);
+    k_final<<<1, B, 0, s>>>(Gd, part + kPartMax, sc + 1, 0);
+    // adjoint
+    k_lg_g<<<G, B, 0, s>>>(n, aeps, sc, a, invD, pd, gz, w1, ga);
+    k_spmv_f<<<A.nRowBlocks, B, 0, s>>>(A, v32, w1, ax);
+    k_lg_gx2<<<G, B, 0, s>>>(n, gz, ax, u, gx2, gt);
+    k_mg_restrict_csr<<<L.rpat.nRowBlocks, B, 0, s>>>(L.rpat.view(), L.rval.as<float>(), gx2, C.b.as<float>(), nullptr);
+    solveLevel(1, nullptr);
+    k_lg_gather<<<G, B, 0, s>>>(n, L.agg.as<int>(), C.e.as<float>(), py);
+    k_lg_tail<<<G, B, 0, s>>>(n, r1, py, t, gt, x1);
+    k_spmv_f<<<A.nRowBlocks, B, 0, s>>>(A, v32, x1, ax);
+    k_lg_end<<<G, B, 0, s>>>(n, gx2, ax, db, ga);
+    CFX_CHECK_LAUNCH();
+    double h[2];
+    CFX_HIP(hipMemcpyAsync(h, sc, sizeof(h), hipMemcpyDeviceToHost, s));
+    CFX_HIP(hipStreamSynchronize(s));
+    return h[1] / h[0];
+}
+
+void Multigrid::smoothSample(float* e, int sweeps)
+{
+    hipStream_t s = ctx_.stream();
+    MgLevel& L = lv_[0];
+    const int n = L.n;
+    if (!tv_[0].p)
+    {
+        for (DevBuf& v : tv_) v.alloc(sizeof(float) * n);
+        tpart_.alloc(sizeof(double) * (2 * kPartMax + 16));
+    }
+    const CsrView A = fine_.view();
+    float *b = tv_[0].as<float>(), *z = tv_[1].as<float>();
+    double* part = tpart_.as<double>();
+    const int G = divUp(n, B), Gd = std::max(1, std::min(kPartMax, divUp(n, 4 * B)));
+    for (int k = 0; k < sweeps; ++k)
+    {
+        k_spmv_f<<<A.nRowBlocks, B, 0, s>>>(A, L.val32.as<float>(), e, b);
+        visit(0, b, z, nullptr, nullptr, L.G, L.rbPerWg, nullptr);
+        k_lg_isub<<<G, B, 0, s>>>(n, e, z);
+    }
+    k_absmax_part<<<Gd, B, 0, s>>>(n, e, part);
+    k_final<<<1, B, 0, s>>>(Gd, part, part + 2 * kPartMax + 2, 1);
+    k_scale_by<<<G, B, 0, s>>>(n, e, part + 2 * kPartMax + 2);
     CFX_CHECK_LAUNCH();
 }
 

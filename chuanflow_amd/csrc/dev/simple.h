@@ -4,6 +4,7 @@
 #include "fvops.h"
 #include "krylov.h"
 #include "mg.h"
+#include "mpnn.h"
 
 namespace cfx
 {
@@ -35,6 +36,9 @@ struct FvSolveArgs
     int precision = kFP64;
     double delta = 1e-2;
     MgCache* mg = nullptr;  // non-null: flexible CG with the multigrid preconditioner (FP32 + FP64 refinement)
+    Mpnn* net = nullptr;    // non-null: finest-level weights from the network (per-slot geometry geo)
+    const double* geo = nullptr;
+    float *wa = nullptr, *wt = nullptr;
 };
 SolveReport fvSolve(DevMesh& m, Krylov& K, const Ldu& A, double* val, double* rhsBase, DevBuf& val32,
                     double* x, const FvSolveArgs& a, double* gradTmp, double* faceTmp);

@@ -390,6 +390,13 @@ SolveReport fvSolve(DevMesh& m, Krylov& K, const Ldu& A, double* val, double* rh
             }
             else if (a.mg)
             {
+                if (a.net)
+                {
+                    a.net->setGraph(view, v, a.geo, b);
+                    a.net->forward(a.wa, a.wt, false);
+                    a.mg->pre = a.mg->post = a.wa;
+                    a.mg->scale = a.mg->restr = a.wt;
+                }
                 a.mg->valid = false;  // new values every call
                 Multigrid& M = a.mg->ready(*m.ctx, m.pattern, v);
                 last = K.fcgMixed(view, v, M, b, xk, a.tol, a.maxIter, a.delta, nullptr);

@@ -81,6 +81,12 @@ class MgParams(C.Structure):
                 ("kcycleDepth", C.c_int)]
 
 
+class TrainOptions(C.Structure):
+    """cfx_train_options: Adam steps, error samples per step, smoothing cycles, learning rate, seed."""
+    _fields_ = [("steps", C.c_int), ("samples", C.c_int), ("smooth", C.c_int), ("lr", C.c_float),
+                ("seed", C.c_uint64)]
+
+
 class CfxError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"cfx error {code}: {msg}")
@@ -146,6 +152,19 @@ _sig("cfx_csr_residual_history", C.c_int, _vp, _vp, C.c_int)
 _sig("cfx_csr_set_inner_tol", C.c_int, _vp, C.c_double)
 _sig("cfx_mg_default_params", C.c_int, C.POINTER(MgParams))
 _sig("cfx_set_graphs", C.c_int, _vp, C.c_int)
+_sig("cfx_mpnn_num_params", C.c_int)
+_sig("cfx_mpnn_create", C.c_int, _vp, C.c_uint64, C.c_int, C.POINTER(_vp))
+_sig("cfx_mpnn_destroy", C.c_int, _vp)
+_sig("cfx_mpnn_get_params", C.c_int, _vp, _vp)
+_sig("cfx_mpnn_set_params", C.c_int, _vp, _vp)
+_sig("cfx_mpnn_forward_csr", C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int)
+_sig("cfx_mpnn_backward", C.c_int, _vp, _vp, _vp, _vp)
+_sig("cfx_mpnn_adam", C.c_int, _vp, _vp, C.c_float)
+_sig("cfx_mpnn_train", C.c_int, _vp, _vp, _vp, _vp, C.c_int, C.POINTER(TrainOptions), _vp)
+_sig("cfx_mpnn_evaluate", C.c_int, _vp, _vp, _vp, _vp, C.POINTER(TrainOptions), C.c_int, C.c_int, _dp)
+_sig("cfx_csr_mg_loss_grad", C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _dp)
+_sig("cfx_csr_set_mpnn", C.c_int, _vp, _vp, _vp)
+_sig("cfx_matrix_set_mpnn", C.c_int, _vp, _vp)
 _sig("cfx_csr_set_mg", C.c_int, _vp, C.POINTER(MgParams))
 _sig("cfx_csr_set_mg_weights", C.c_int, _vp, _vp, _vp, _vp, _vp)
 _sig("cfx_matrix_set_mg", C.c_int, _vp, C.POINTER(MgParams))
@@ -513,6 +532,10 @@ class Matrix:
         self.mesh.ctx.check(_L.cfx_matrix_set_mg(self.h, C.byref(p)))
         return p
 
+    def set_mpnn(self, net):
+        self._net = net
+        self.mesh.ctx.check(_L.cfx_matrix_set_mpnn(self.h, net.h if net is not None else None))
+
     def set_mg_weights(self, pre=None, scale=None, post=None, restrict=None):
         self._mgw = (pre, scale, post, restrict)
         self.mesh.ctx.check(_L.cfx_matrix_set_mg_weights(self.h, *[_p(a) if a is not None else None
@@ -656,9 +679,87 @@ class Csr:
     def mg_apply(self, r, z):
         self.ctx.check(_L.cfx_csr_mg_apply(self.h, _p(r), _p(z)))
 
+    def mg_loss_grad(self, e, a, t):
+        """(L, dL/da, dL/dt) of the cycle-energy objective for error sample e (device FP32 arrays)."""
+        ga, gt = self.ctx.zeros(self.n, np.float32), self.ctx.zeros(self.n, np.float32)
+        loss = C.c_double(0)
+        self.ctx.check(_L.cfx_csr_mg_loss_grad(self.h, _p(e), _p(a), _p(t), _p(ga), _p(gt), C.byref(loss)))
+        return loss.value, ga, gt
+
+    def set_mpnn(self, net, geo=None):
+        """Learned solves take their finest-level weights from `net` (None detaches)."""
+        self._net = (net, geo)
+        self.ctx.check(_L.cfx_csr_set_mpnn(self.h, net.h if net is not None else None,
+                                           _p(geo) if geo is not None else None))
+
 
 def mg_default_params():
     p = MgParams()
     if _L.cfx_mg_default_params(C.byref(p)):
         raise CfxError(-1, "cfx_mg_default_params")
     return p
+
+
+class Mpnn:
+    """Message-passing network giving the finest-level multigrid weights (cfx_mpnn)."""
+
+    nparams = _L.cfx_mpnn_num_params()
+
+    def __init__(self, ctx: Context, seed=1, fp16=True):
+        self.ctx = ctx
+        h = _vp()
+        ctx.check(_L.cfx_mpnn_create(ctx.h, seed, 1 if fp16 else 0, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                _L.cfx_mpnn_destroy(self.h)
+        except Exception:
+            pass
+
+    def params(self):
+        p = np.zeros(self.nparams, np.float32)
+        self.ctx.check(_L.cfx_mpnn_get_params(self.h, _p(p)))
+        return p
+
+    def set_params(self, p):
+        self.ctx.check(_L.cfx_mpnn_set_params(self.h, _p(np.ascontiguousarray(p, np.float32))))
+
+    def forward(self, csr, geo=None, r0=None, train=False):
+        a, t = self.ctx.empty(csr.n, np.float32), self.ctx.empty(csr.n, np.float32)
+        self.ctx.check(_L.cfx_mpnn_forward_csr(self.h, csr.h, _p(geo) if geo is not None else None,
+                                               _p(r0) if r0 is not None else None, _p(a), _p(t),
+                                               1 if train else 0))
+        return a, t
+
+    def backward(self, ga, gt):
+        g = self.ctx.zeros(self.nparams, np.float32)
+        self.ctx.check(_L.cfx_mpnn_backward(self.h, _p(ga), _p(gt), _p(g)))
+        return g
+
+    def adam(self, grad, lr):
+        self.ctx.check(_L.cfx_mpnn_adam(self.h, _p(grad), lr))
+
+    @staticmethod
+    def _opts(steps=100, samples=2, smooth=4, lr=1e-3, seed=1):
+        return TrainOptions(steps, samples, smooth, lr, seed)
+
+    def train(self, systems, geos=None, r0s=None, **kw):
+        o = self._opts(**kw)
+        n = len(systems)
+        S = (_vp * n)(*[s.h for s in systems])
+        G = (_vp * n)(*[_p(g) if g is not None else None for g in geos]) if geos is not None else None
+        R = (_vp * n)(*[_p(r) if r is not None else None for r in r0s]) if r0s is not None else None
+        hist = np.zeros(max(o.steps, 1))
+        self.ctx.check(_L.cfx_mpnn_train(self.h, C.cast(S, _vp), C.cast(G, _vp) if G is not None else None,
+                                         C.cast(R, _vp) if R is not None else None, n, C.byref(o), _p(hist)))
+        return hist[:o.steps]
+
+    def evaluate(self, csr, geo=None, r0=None, samples=4, with_network=True, **kw):
+        o = self._opts(**kw)
+        loss = C.c_double(0)
+        self.ctx.check(_L.cfx_mpnn_evaluate(self.h, csr.h, _p(geo) if geo is not None else None,
+                                            _p(r0) if r0 is not None else None, C.byref(o), samples,
+                                            1 if with_network else 0, C.byref(loss)))
+        return loss.value

@@ -229,6 +229,54 @@ int cfx_csr_mg_level(cfx_csr* A, int l, int* rowPtr, int* col, double* val, int*
 /* one V-cycle z = M r on device FP32 vectors */
 int cfx_csr_mg_apply(cfx_csr* A, const float* d_r, float* d_z);
 
+/* ------------------------------------------- message-passing network */
+/* The learned part of the multigrid preconditioner (no counterpart in the
+ * reference): a 3-layer message-passing network on the matrix graph (hidden
+ * width 32, MFMA with FP16 or FP32 storage and FP32 accumulation) mapping
+ * per-row inputs (normalised diagonal/off-diagonal coefficients, face
+ * non-orthogonality, |r0|) to tied finest-level weights a (pre = post
+ * smoother) and t (restriction = prolongation scale); an all-zero output
+ * layer gives a = 2/3, t = sqrt(1.8).  Trained on the GPU with the
+ * project's own forward and backward kernels against the energy of the
+ * cycle's error on smooth error samples. */
+typedef struct cfx_mpnn cfx_mpnn;
+int cfx_mpnn_num_params(void);
+int cfx_mpnn_create(cfx_ctx* ctx, uint64_t seed, int fp16, cfx_mpnn** out);
+int cfx_mpnn_destroy(cfx_mpnn* net);
+/* flat FP32 parameters (host): W_in[8][32], b_in[32], 3 x (W[96][32], b[32]),
+ * W_out[32][2], b_out[2] */
+int cfx_mpnn_get_params(cfx_mpnn* net, float* host);
+int cfx_mpnn_set_params(cfx_mpnn* net, const float* host);
+/* a, t (device FP32 [n]) for a CSR system; d_geo per slot and d_r0 per row
+ * may be null; train = 1 keeps the activations for cfx_mpnn_backward */
+int cfx_mpnn_forward_csr(cfx_mpnn* net, cfx_csr* A, const double* d_geo, const double* d_r0, float* d_a,
+                         float* d_t, int train);
+/* d_grad (device FP32 [num_params]) = dJ/dparams for dJ/da, dJ/dt of the last training forward */
+int cfx_mpnn_backward(cfx_mpnn* net, const float* d_ga, const float* d_gt, float* d_grad);
+int cfx_mpnn_adam(cfx_mpnn* net, const float* d_grad, float lr);
+typedef struct cfx_train_options {
+    int steps;        /* Adam steps, systems visited round robin */
+    int samples;      /* error samples per step */
+    int smooth;       /* cycles applied to a random vector to form a sample */
+    float lr;
+    uint64_t seed;
+} cfx_train_options;
+/* trains on count systems (the hierarchy constants of systems[0]); d_geo and
+ * d_r0 are arrays of per-system device pointers (or null); lossHist [steps] */
+int cfx_mpnn_train(cfx_mpnn* net, cfx_csr* const* systems, const double* const* d_geo,
+                   const double* const* d_r0, int count, const cfx_train_options* opt, double* lossHist);
+/* mean objective over `samples` error samples, with the network or (0) with the constants */
+int cfx_mpnn_evaluate(cfx_mpnn* net, cfx_csr* A, const double* d_geo, const double* d_r0,
+                      const cfx_train_options* opt, int samples, int withNetwork, double* loss);
+/* the objective and its gradient for one error sample: L = |e - M A e|_A^2 / |e|_A^2
+ * with tied per-row weights a, t (linear V-cycle below level 1); ga, gt accumulate */
+int cfx_csr_mg_loss_grad(cfx_csr* A, const float* d_e, const float* d_a, const float* d_t, float* d_ga,
+                         float* d_gt, double* loss);
+/* CFX_PRECOND_LEARNED solves take their finest-level weights from the
+ * network (inputs from the matrix, d_geo and the right-hand side); null detaches */
+int cfx_csr_set_mpnn(cfx_csr* A, cfx_mpnn* net, const double* d_geo);
+int cfx_matrix_set_mpnn(cfx_matrix* A, cfx_mpnn* net);
+
 /* ------------------------------------------------------------ drivers */
 /* Schemes (src/3d/include/cf/fvm/FvMatrix.h:30-38): div 0 upwind, 1 linear,
  * 2 blend; laplacian "corrected" (1) or "none" (0); grad "corrected" (1)
