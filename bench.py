@@ -95,7 +95,7 @@ def cpu_info():
     return model, os.cpu_count()
 
 
-def cpu_baseline_and_check(arrays, b, x64_gpu, xmix_gpu, gpu_iters, sample_iters, A_dev, xmg_gpu=None):
+def cpu_baseline_and_check(arrays, b, x64_gpu, xmix_gpu, gpu_iters, sample_iters, A_dev, xmg_gpu=None, xln_gpu=None):
     """Checker + CPU baseline: the oracle restates the reference; the
     reference build itself (oracle/_ref) is timed when present."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -117,6 +117,10 @@ def cpu_baseline_and_check(arrays, b, x64_gpu, xmix_gpu, gpu_iters, sample_iters
     out["mixed_true_residual"] = float(np.abs(b - O.spmv(csr, xmix_gpu)).max() / bn)
     out["mixed_vs_fp64_rel_err"] = float(np.abs((xmix_gpu - xmix_gpu.mean()) - (x64_gpu - x64_gpu.mean())).max()
                                          / np.abs(x64_gpu - x64_gpu.mean()).max())
+    if xln_gpu is not None:
+        out["learned_true_residual"] = float(np.abs(b - O.spmv(csr, xln_gpu)).max() / bn)
+        out["learned_vs_fp64_rel_err"] = float(np.abs((xln_gpu - xln_gpu.mean()) - (x64_gpu - x64_gpu.mean())).max()
+                                               / np.abs(x64_gpu - x64_gpu.mean()).max())
     if xmg_gpu is not None:
         out["mg_true_residual"] = float(np.abs(b - O.spmv(csr, xmg_gpu)).max() / bn)
         out["mg_vs_fp64_rel_err"] = float(np.abs((xmg_gpu - xmg_gpu.mean()) - (x64_gpu - x64_gpu.mean())).max()
@@ -193,6 +197,12 @@ def main():
     A.set_mg(scale=args.mg_scale, coarseScale=args.mg_scale)
     rmg = time_solve(N, ctx, A, d_x, N.FP32_IR, args.steps, args.warmup, args.tol, maxIter, N.PRECOND_LEARNED)
     xmg = d_x.get()
+    log("learned: network weights on the finest level, flexible CG (FP32 + FP64 refinement)")
+    from chuanflow_amd import learned as LN
+    net = LN.load(ctx)
+    A.set_mpnn(net)
+    rln = time_solve(N, ctx, A, d_x, N.FP32_IR, args.steps, args.warmup, args.tol, maxIter, N.PRECOND_LEARNED)
+    xln = d_x.get()
 
     # SpMV roofline: average launch time over reps on the library stream (HIP events)
     xs = ctx.array(np.random.default_rng(1).standard_normal(n))
@@ -211,19 +221,19 @@ def main():
     tmix = float(np.mean([w for w, _ in rmix]))
     it64 = r64[-1][1]["iterations"]
     tmg = float(np.mean([w for w, _ in rmg]))
-    best = min(t64, tmix, tmg)
+    tln = float(np.mean([w for w, _ in rln]))
     result = {
         "metric": METRIC,
-        "value": round(best, 3),
+        "value": round(tln, 3),
         "unit": "ms",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(best, 3),
+        "ms_per_step": round(tln, 3),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 inner + f64 refinement" if min(tmix, tmg) <= t64 else "f64",
+        "dtype": "f32 inner + f64 refinement",
         "data": "synthetic",
         "config": {
             "workload": f"pressure-Poisson {args.n}^3 jittered hex (+-15%, seed {SEED_MESH}), faces/cells shuffled "
@@ -244,7 +254,16 @@ def main():
         "mg_fcg_outer_iterations": rmg[-1][1]["outerIterations"],
         "mg_fcg_true_residual": rmg[-1][1]["trueResidual"],
         "speedup_mg_vs_plain": round(t64 / tmg, 3),
-        "learned_preconditioner": "not in this build",
+        "learned_ms": round(tln, 3),
+        "learned_iterations": rln[-1][1]["iterations"],
+        "learned_outer_iterations": rln[-1][1]["outerIterations"],
+        "learned_true_residual": rln[-1][1]["trueResidual"],
+        "learned_precond_fallbacks": rln[-1][1]["precondFallbacks"],
+        "speedup_learned_vs_plain": round(t64 / tln, 3),
+        "learned_preconditioner": "3-layer MPNN (width 32, FP16 MFMA) giving the finest-level smoother weight and "
+                                  "transfer scale of an aggregation K-cycle inside flexible CG; weights "
+                                  "chuanflow_amd/weights/mpnn_pressure.npy trained on the GPU on SIMPLE pressure "
+                                  "systems of obstacle channels",
         "spmv_fp64_us": round(ms64 * 1e3, 2),
         "spmv_fp64_GBps": round(gbs64, 1),
         "spmv_fp32_us": round(ms32 * 1e3, 2),
@@ -254,7 +273,7 @@ def main():
                      "kernel": "k_spmv<double> (CSR-stream SpMV, FP64)", "bytes_per_launch": b64},
     }
     if not args.no_cpu:
-        chk = cpu_baseline_and_check(arrays, b, x64, xmix, it64, args.cpu_sample_iters, A, xmg)
+        chk = cpu_baseline_and_check(arrays, b, x64, xmix, it64, args.cpu_sample_iters, A, xmg, xln)
         result["cpu_baseline"] = chk.pop("cpu_baseline")
         result["checks"] = chk
     print(json.dumps(result), flush=True)

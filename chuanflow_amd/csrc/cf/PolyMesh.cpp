@@ -618,4 +618,83 @@ polyMesh BlockMesh::createTets(const Vec3& mn, const Vec3& mx, label nx, label n
                     std::move(patches));
 }
 
+polyMesh removeCells(const polyMesh& m, const std::vector<char>& keep, const std::string& name,
+                     const std::string& type)
+{
+    const label nC = m.nCells(), nI = m.nInternalFaces(), nF = m.nFaces();
+    if (static_cast<label>(keep.size()) != nC) throw Error("removeCells: keep mask size mismatch");
+    std::vector<label> newId(nC, -1);
+    label k = 0;
+    for (label c = 0; c < nC; ++c)
+        if (keep[c]) newId[c] = k++;
+    if (k == 0) throw Error("removeCells: no cell left");
+    const auto& fs = m.faceStart();
+    const auto& fv = m.faceVerts();
+    std::vector<label> ofs{0}, ofv, ow, nb;
+    auto addFace = [&](label f, bool reverse, label owner) {
+        const label a = fs[f], b = fs[f + 1];
+        if (!reverse)
+            for (label q = a; q < b; ++q) ofv.push_back(fv[q]);
+        else
+        {
+            ofv.push_back(fv[a]);  // same first vertex, opposite winding
+            for (label q = b - 1; q > a; --q) ofv.push_back(fv[q]);
+        }
+        ofs.push_back(static_cast<label>(ofv.size()));
+        ow.push_back(owner);
+    };
+    std::vector<label> cut;  // internal faces with one kept cell
+    for (label f = 0; f < nI; ++f)
+    {
+        const label o = m.owner()[f], n = m.neighbour()[f];
+        if (keep[o] && keep[n])
+        {
+            addFace(f, false, newId[o]);
+            nb.push_back(newId[n]);
+        }
+        else if (keep[o] || keep[n])
+            cut.push_back(f);
+    }
+    std::vector<polyMesh::Patch> patches;
+    label start = static_cast<label>(nb.size());
+    for (const auto& p : m.patches())
+    {
+        polyMesh::Patch q{p.name, p.type, start, 0};
+        for (label f = p.start; f < p.start + p.size; ++f)
+            if (keep[m.owner()[f]])
+            {
+                addFace(f, false, newId[m.owner()[f]]);
+                ++q.size;
+            }
+        start += q.size;
+        patches.push_back(q);
+    }
+    polyMesh::Patch q{name, type, start, static_cast<label>(cut.size())};
+    for (label f : cut)
+    {
+        const label o = m.owner()[f], n = m.neighbour()[f];
+        if (keep[o])
+            addFace(f, false, newId[o]);
+        else
+            addFace(f, true, newId[n]);
+    }
+    patches.push_back(q);
+    (void)nF;
+    return polyMesh(m.points(), std::move(ofs), std::move(ofv), std::move(ow), std::move(nb), std::move(patches));
+}
+
+polyMesh obstacleChannel(const Vec3& mn, const Vec3& mx, label nx, label ny, label nz, double jitter,
+                         std::uint64_t seed, double cx, double cy, double radius)
+{
+    const polyMesh box = BlockMesh::create(mn, mx, nx, ny, nz, {}, jitter, seed);
+    const auto& C = box.cellCentres();
+    std::vector<char> keep(box.nCells(), 1);
+    for (label c = 0; c < box.nCells(); ++c)
+    {
+        const double dx = C[3 * c] - cx, dy = C[3 * c + 1] - cy;
+        if (dx * dx + dy * dy < radius * radius) keep[c] = 0;
+    }
+    return removeCells(box, keep, "obstacle", "wall");
+}
+
 } // namespace cf

@@ -139,6 +139,20 @@ struct BlockMesh
                                const Sides& sides = {}, double jitter = 0.0, std::uint64_t seed = 0);
 };
 
+/// The sub-mesh of the cells with keep[c] != 0: internal faces between two
+/// kept cells stay internal (same order), boundary faces of kept cells stay
+/// in their patches, and faces between a kept and a removed cell become a new
+/// patch (name, type) appended last, oriented out of the kept cell.  Cells
+/// are renumbered compactly in their old order; points are kept.
+polyMesh removeCells(const polyMesh& m, const std::vector<char>& keep, const std::string& name,
+                     const std::string& type);
+
+/// Channel block with a cylindrical obstacle: BlockMesh::create (seeded
+/// jitter) minus the cells whose centre lies within `radius` of the axis
+/// parallel to z through (cx, cy); the cut faces form patch "obstacle" (wall).
+polyMesh obstacleChannel(const Vec3& min, const Vec3& max, label nx, label ny, label nz, double jitter,
+                         std::uint64_t seed, double cx, double cy, double radius);
+
 /// Seeded interior-vertex jitter of a structured (nx+1)(ny+1)(nz+1) lattice:
 /// x += (2u-1)*amp*h per component, u the top 53 bits of successive
 /// MT19937-64 outputs, interior points in lattice order (i fastest).

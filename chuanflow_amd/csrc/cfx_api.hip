@@ -40,11 +40,20 @@ struct cfx_mpnn
     std::unique_ptr<MpnnTrainer> trainer;
 };
 
+struct cfx_mpnn;
+
 struct cfx_mesh
 {
     cfx_ctx* owner = nullptr;
     std::unique_ptr<DevMesh> m;
     DevBuf faceTmp, gradTmp, cellTmp, result;
+    // pressure solves of the drivers with pPrecision CFX_FP32_MG
+    MgCache pmg;
+    cfx_mpnn* net = nullptr;
+    DevBuf geo, wa, wt;
+    PressureDump dump;
+    PressureSolver* pressureSolver();
+    PressureSolver ps;
 };
 
 struct cfx_matrix
@@ -177,6 +186,28 @@ SolveReport solveOn(cfx_ctx* c, const CsrView& A, const double* val, DevBuf& val
 }
 
 } // namespace
+
+PressureSolver* cfx_mesh::pressureSolver()
+{
+    DevMesh& dm = *m;
+    ps.mg = &pmg;
+    ps.net = nullptr;
+    if (net)
+    {
+        if (!geo.p)
+        {
+            geo.alloc(sizeof(double) * dm.pattern.nnz);
+            meshSlotGeometry(dm, geo.as<double>());
+            wa.alloc(sizeof(float) * dm.n);
+            wt.alloc(sizeof(float) * dm.n);
+        }
+        ps.net = net->net.get();
+        ps.geo = geo.as<double>();
+        ps.wa = wa.as<float>();
+        ps.wt = wt.as<float>();
+    }
+    return &ps;
+}
 
 extern "C" {
 
@@ -956,7 +987,8 @@ int cfx_simple_solve(cfx_mesh* m, double nu, const cfx_schemes* schemes, const c
     if (!m || !ctl) return CFX_E_INVALID;
     return guard(m->owner, [&] {
         need(m->m->nB == 0 || (Utype && Uvalue && ptype && pvalue), "null boundary conditions");
-        need(ctl->pPrecision == CFX_FP64 || ctl->pPrecision == CFX_FP32_IR, "unknown pressure precision");
+        need(ctl->pPrecision == CFX_FP64 || ctl->pPrecision == CFX_FP32_IR || ctl->pPrecision == CFX_FP32_MG,
+             "unknown pressure precision");
         SimpleControls c;
         c.maxIterations = ctl->maxIterations;
         c.continuityTol = ctl->continuityTol;
@@ -970,6 +1002,8 @@ int cfx_simple_solve(cfx_mesh* m, double nu, const cfx_schemes* schemes, const c
         c.pRelaxStart = ctl->pRelaxStart;
         for (int k = 0; k < 3; ++k) c.bodyForce[k] = ctl->bodyForce[k];
         c.pPrecision = ctl->pPrecision;
+        c.pSolver = ctl->pPrecision == CFX_FP32_MG ? m->pressureSolver() : nullptr;
+        c.dump = &m->dump;
         const SimpleResult r = simpleSolve(*m->m, m->owner->solver(m->m->n, m->m->pattern.nRowBlocks), nu,
                                            toSchemes(schemes), c, Utype, Uvalue, ptype, pvalue, U, p, phi, hist);
         if (result)
@@ -991,7 +1025,8 @@ int cfx_piso_step(cfx_mesh* m, double nu, const cfx_schemes* schemes, const cfx_
     if (!m || !ctl) return CFX_E_INVALID;
     return guard(m->owner, [&] {
         need(m->m->nB == 0 || (Utype && Uvalue && ptype && pvalue), "null boundary conditions");
-        need(ctl->pPrecision == CFX_FP64 || ctl->pPrecision == CFX_FP32_IR, "unknown pressure precision");
+        need(ctl->pPrecision == CFX_FP64 || ctl->pPrecision == CFX_FP32_IR || ctl->pPrecision == CFX_FP32_MG,
+             "unknown pressure precision");
         PisoControls c;
         c.dt = ctl->dt;
         c.correctors = ctl->correctors;
@@ -1001,6 +1036,8 @@ int cfx_piso_step(cfx_mesh* m, double nu, const cfx_schemes* schemes, const cfx_
         c.nonOrthCorrectors = ctl->nonOrthCorrectors;
         for (int k = 0; k < 3; ++k) c.bodyForce[k] = ctl->bodyForce[k];
         c.pPrecision = ctl->pPrecision;
+        c.pSolver = ctl->pPrecision == CFX_FP32_MG ? m->pressureSolver() : nullptr;
+        c.dump = &m->dump;
         const PisoResult r = pisoStep(*m->m, m->owner->solver(m->m->n, m->m->pattern.nRowBlocks), nu,
                                       toSchemes(schemes), c, Utype, Uvalue, ptype, pvalue, U, p, phi);
         if (result)
@@ -1013,6 +1050,28 @@ int cfx_piso_step(cfx_mesh* m, double nu, const cfx_schemes* schemes, const cfx_
             result->ms = r.ms;
         }
     });
+}
+
+int cfx_mesh_set_mg(cfx_mesh* m, const cfx_mg_params* p)
+{
+    if (!m || !p) return CFX_E_INVALID;
+    return guard(m->owner, [&] { setMgParams(m->pmg, p); });
+}
+
+int cfx_mesh_set_pressure_dump(cfx_mesh* m, double* val, double* rhs, int every, int max)
+{
+    if (!m) return CFX_E_INVALID;
+    return guard(m->owner, [&] {
+        need(every <= 0 || (val && rhs && max >= 0), "dump buffers missing");
+        m->dump = PressureDump{val, rhs, every, max, 0, 0};
+    });
+}
+
+int cfx_mesh_pressure_dump_count(cfx_mesh* m, int* count)
+{
+    if (!m || !count) return CFX_E_INVALID;
+    *count = m->dump.count;
+    return CFX_OK;
 }
 
 // ------------------------------------------------- message-passing network
@@ -1159,6 +1218,15 @@ int cfx_csr_set_mpnn(cfx_csr* A, cfx_mpnn* m, const double* geo)
         A->geo = geo;
         if (!m) A->mg.pre = A->mg.post = A->mg.scale = A->mg.restr = nullptr;
         A->mg.valid = false;
+    });
+}
+
+int cfx_mesh_set_mpnn(cfx_mesh* M, cfx_mpnn* net)
+{
+    if (!M) return CFX_E_INVALID;
+    return guard(M->owner, [&] {
+        need(!net || net->owner == M->owner, "network of another context");
+        M->net = net;
     });
 }
 

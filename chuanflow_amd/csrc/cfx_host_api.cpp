@@ -66,6 +66,27 @@ int cfx_hmesh_box(int64_t nx, int64_t ny, int64_t nz, const double mn[3], const 
     });
 }
 
+int cfx_hmesh_obstacle_channel(int64_t nx, int64_t ny, int64_t nz, const double mn[3], const double mx[3],
+                               double jitter, uint64_t seed, double cx, double cy, double radius, cfx_hmesh** out)
+{
+    return guardHost([&] {
+        if (!out || !mn || !mx) throw std::invalid_argument("null argument");
+        if (!(radius > 0)) throw std::invalid_argument("radius must be positive");
+        const cf::Vec3 a{mn[0], mn[1], mn[2]}, b{mx[0], mx[1], mx[2]};
+        auto* h = new cfx_hmesh;
+        try
+        {
+            h->mesh = cf::obstacleChannel(a, b, nx, ny, nz, jitter, seed, cx, cy, radius);
+        }
+        catch (...)
+        {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
 int cfx_hmesh_from_arrays(int64_t nPoints, const double* points, int64_t nFaces, const int64_t* faceStart,
                           const int64_t* faceVerts, int64_t nInternal, const int64_t* owner,
                           const int64_t* neighbour, int64_t nPatches, const int64_t* patchStart,

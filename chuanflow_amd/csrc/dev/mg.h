@@ -105,9 +105,12 @@ public:
 
     // Training objective of the finest-level weights.  Requires setMatrix
     // with pre = post = a, scale = restr = t and kcycleDepth = 0 (so the
-    // coarse correction Q is linear and symmetric).  For an error sample e,
-    // the cycle's error e - M A e has energy L = |e - M A e|_A^2 / |e|_A^2;
-    // returns L and adds dL/da, dL/dt (per fine row) into ga, gt.
+    // coarse correction Q is linear and symmetric).  For an error sample e
+    // and z = M A e, L = 1 - (z.A e)^2 / ((e.A e)(z.A z)) = min over alpha of
+    // |e - alpha z|_A^2 / |e|_A^2: the error left by one preconditioned step
+    // with the best step length (invariant to the scale of M, as the Krylov
+    // iteration is).  Returns L and adds dL/da, dL/dt (per fine row) into
+    // ga, gt.
     double lossGrad(const float* e, const float* a, const float* t, float* ga, float* gt);
     // e <- (I - M A) e, `sweeps` times, then e / max|e| (smooth error samples)
     void smoothSample(float* e, int sweeps);

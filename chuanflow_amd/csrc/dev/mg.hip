@@ -556,14 +556,22 @@ __global__ void k_lg_z(int n, const float* __restrict__ b, const float* __restri
     eps[i] = e[i] - (x2[i] + a[i] * p);
 }
 
-// gz = -2 A eps / (e.A e); ga += gz pd; w1 = D^-1 (a gz)
-__global__ void k_lg_g(int n, const float* __restrict__ aeps, const double* __restrict__ eAe,
-                       const float* __restrict__ a, const float* __restrict__ invD, const float* __restrict__ pd,
-                       float* __restrict__ gz, float* __restrict__ w1, float* __restrict__ ga)
+// L = 1 - p^2 / (c q) with z = e - eps, c = e.b, p = z.b = c - d, q = z.Az =
+// c - 2 d + f (d = eps.b, f = eps.A eps, A symmetric):
+// gz = dL/dz = (2 p / (c q)) (-b + (p / q) A z), A z = b - A eps;
+// then ga += gz pd; w1 = D^-1 (a gz)
+__global__ void k_lg_g(int n, const float* __restrict__ b, const float* __restrict__ aeps,
+                       const double* __restrict__ sc, const float* __restrict__ a, const float* __restrict__ invD,
+                       const float* __restrict__ pd, float* __restrict__ gz, float* __restrict__ w1,
+                       float* __restrict__ ga)
 {
     const int i = blockIdx.x * B + threadIdx.x;
     if (i >= n) return;
-    const float g = static_cast<float>(-2.0 * aeps[i] / *eAe);
+    const double c = sc[0], d = sc[1], f = sc[2];
+    const double p = c - d, q = c - 2.0 * d + f;
+    const double k1 = 2.0 * p / (c * q), k2 = p / q;
+    const double bi = b[i];
+    const float g = static_cast<float>(k1 * (-bi + k2 * (bi - static_cast<double>(aeps[i]))));
     gz[i] = g;
     ga[i] += g * pd[i];
     w1[i] = invD[i] * (a[i] * g);
@@ -824,7 +832,7 @@ double Multigrid::lossGrad(const float* e, const float* a, const float* t, float
           *eps = tv_[8].as<float>(), *aeps = tv_[9].as<float>(), *gz = tv_[10].as<float>(),
           *w1 = tv_[11].as<float>(), *gx2 = tv_[12].as<float>(), *py = tv_[13].as<float>();
     double* part = tpart_.as<double>();
-    double* sc = part + 2 * kPartMax;  // [0] e.Ae, [1] eps.A eps
+    double* sc = part + 2 * kPartMax;  // [0] e.A e, [1] eps.A e, [2] eps.A eps
     const CsrView A = fine_.view();
     const float* v32 = L.val32.as<float>();
     const float* invD = L.invD32.as<float>();
@@ -843,10 +851,12 @@ double Multigrid::lossGrad(const float* e, const float* a, const float* t, float
     k_spmv_f<<<A.nRowBlocks, B, 0, s>>>(A, v32, eps, aeps);
     k_dot_part<<<Gd, B, 0, s>>>(n, e, b, part);
     k_final<<<1, B, 0, s>>>(Gd, part, sc, 0);
-    k_dot_part<<<Gd, B, 0, s>>>(n, eps, aeps, part + kPartMax);
+    k_dot_part<<<Gd, B, 0, s>>>(n, eps, b, part + kPartMax);
     k_final<<<1, B, 0, s>>>(Gd, part + kPartMax, sc + 1, 0);
+    k_dot_part<<<Gd, B, 0, s>>>(n, eps, aeps, part);
+    k_final<<<1, B, 0, s>>>(Gd, part, sc + 2, 0);
     // adjoint
-    k_lg_g<<<G, B, 0, s>>>(n, aeps, sc, a, invD, pd, gz, w1, ga);
+    k_lg_g<<<G, B, 0, s>>>(n, b, aeps, sc, a, invD, pd, gz, w1, ga);
     k_spmv_f<<<A.nRowBlocks, B, 0, s>>>(A, v32, w1, ax);
     k_lg_gx2<<<G, B, 0, s>>>(n, gz, ax, u, gx2, gt);
     k_mg_restrict_csr<<<L.rpat.nRowBlocks, B, 0, s>>>(L.rpat.view(), L.rval.as<float>(), gx2, C.b.as<float>(), nullptr);
@@ -856,10 +866,11 @@ double Multigrid::lossGrad(const float* e, const float* a, const float* t, float
     k_spmv_f<<<A.nRowBlocks, B, 0, s>>>(A, v32, x1, ax);
     k_lg_end<<<G, B, 0, s>>>(n, gx2, ax, db, ga);
     CFX_CHECK_LAUNCH();
-    double h[2];
+    double h[3];
     CFX_HIP(hipMemcpyAsync(h, sc, sizeof(h), hipMemcpyDeviceToHost, s));
     CFX_HIP(hipStreamSynchronize(s));
-    return h[1] / h[0];
+    const double pz = h[0] - h[1], qz = h[0] - 2.0 * h[1] + h[2];
+    return 1.0 - pz * pz / (h[0] * qz);
 }
 
 void Multigrid::smoothSample(float* e, int sweeps)

@@ -49,6 +49,27 @@ SolveReport fvSolve(DevMesh& m, Krylov& K, const Ldu& A, double* val, double* rh
 // axis aligned (the reference throws there).
 bool lineariseBc(DevMesh& m, int ncomp, const int* dType, const double* dValue, double* linc, double* link);
 
+// Copies of the assembled pressure systems (CSR values and right-hand side
+// of the last corrector pass) into caller buffers [max][nnz] / [max][n],
+// every `every` pressure solves (SIMPLE iterations, PISO correctors).
+struct PressureDump
+{
+    double* val = nullptr;
+    double* rhs = nullptr;
+    int every = 0, max = 0, count = 0, seen = 0;
+};
+
+// pPrecision == kFP32MG: flexible CG with the multigrid preconditioner and,
+// when `net` is set, the network's finest-level weights (geometry per slot).
+struct PressureSolver
+{
+    MgCache* mg = nullptr;
+    Mpnn* net = nullptr;
+    const double* geo = nullptr;
+    float *wa = nullptr, *wt = nullptr;
+};
+constexpr int kFP32MG = 2;
+
 struct SimpleControls
 {
     int maxIterations = 200;
@@ -57,7 +78,9 @@ struct SimpleControls
     int linearMaxIter = 2000, nonOrthCorrectors = 2;
     double relaxU = 0.7, relaxP = 0.3, pRelaxStart = 0;
     double bodyForce[3] = {0, 0, 0};
-    int pPrecision = kFP64;  // pressure solve: FP64 or FP32 inner + FP64 refinement
+    int pPrecision = kFP64;  // pressure solve: FP64, FP32 inner + FP64 refinement, or kFP32MG
+    PressureSolver* pSolver = nullptr;
+    PressureDump* dump = nullptr;
 };
 
 struct SimpleResult
@@ -85,6 +108,8 @@ struct PisoControls
     int linearMaxIter = 2000, nonOrthCorrectors = 2;
     double bodyForce[3] = {0, 0, 0};
     int pPrecision = kFP64;
+    PressureSolver* pSolver = nullptr;
+    PressureDump* dump = nullptr;
 };
 
 struct PisoResult

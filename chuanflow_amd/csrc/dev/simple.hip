@@ -361,6 +361,31 @@ bool lineariseBc(DevMesh& m, int ncomp, const int* dType, const double* dValue, 
     return h == 0;
 }
 
+// pressure-solve options of the drivers
+static void setPressureSolver(FvSolveArgs& pa, int precision, const PressureSolver* ps)
+{
+    pa.precision = precision == kFP32MG ? kFP32IR : precision;
+    if (precision != kFP32MG) return;
+    if (!ps || !ps->mg) throw std::invalid_argument("multigrid pressure solve without a multigrid state");
+    pa.mg = ps->mg;
+    pa.net = ps->net;
+    pa.geo = ps->geo;
+    pa.wa = ps->wa;
+    pa.wt = ps->wt;
+}
+
+static void dumpPressure(DevMesh& m, const Ldu& PE, const double* pval, PressureDump* d)
+{
+    if (!d || d->every <= 0) return;
+    const int k = d->seen++;
+    if (k % d->every != 0 || d->count >= d->max) return;
+    hipStream_t s = m.ctx->stream();
+    const std::size_t nnz = m.pattern.nnz, n = m.n;
+    CFX_HIP(hipMemcpyAsync(d->val + d->count * nnz, pval, sizeof(double) * nnz, hipMemcpyDeviceToDevice, s));
+    CFX_HIP(hipMemcpyAsync(d->rhs + d->count * n, PE.rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    ++d->count;
+}
+
 SolveReport fvSolve(DevMesh& m, Krylov& K, const Ldu& A, double* val, double* rhsBase, DevBuf& val32, double* x,
                     const FvSolveArgs& a, double* gradTmp, double* faceTmp)
 {
@@ -489,9 +514,10 @@ SimpleResult simpleSolve(DevMesh& m, Krylov& K, double nu, const Schemes& sch, c
         pa.link = w.plk.as<double>();
         pa.sign = -1;
         pa.gradCorrected = sch.gradCorrected;
-        pa.precision = c.pPrecision;
+        setPressureSolver(pa, c.pPrecision, c.pSolver);
         fvSolve(m, K, PE, w.pval.as<double>(), w.pbase.as<double>(), w.val32, w.pNew.as<double>(), pa,
                 w.gradTmp.as<double>(), w.faceTmp.as<double>());
+        dumpPressure(m, PE, w.pval.as<double>(), c.dump);
         // correct
         gradGauss(m, w.pNew.as<double>(), w.plc.as<double>(), w.plk.as<double>(), 1, 0, sch.gradCorrected, true,
                   gradPFull, w.faceTmp.as<double>());
@@ -625,9 +651,10 @@ PisoResult pisoStep(DevMesh& m, Krylov& K, double nu, const Schemes& sch, const 
         pa.linc = w.plc.as<double>();
         pa.link = w.plk.as<double>();
         pa.gradCorrected = sch.gradCorrected;
-        pa.precision = c.pPrecision;
+        setPressureSolver(pa, c.pPrecision, c.pSolver);
         fvSolve(m, K, PE, w.pval.as<double>(), w.pbase.as<double>(), w.val32, p, pa, w.gradTmp.as<double>(),
                 w.faceTmp.as<double>());
+        dumpPressure(m, PE, w.pval.as<double>(), c.dump);
         k_maxdiff<<<gridCap(n), B, 0, s>>>(n, p, w.pNew.as<double>(), sc, kPDelta, part);
         gradGauss(m, p, w.plc.as<double>(), w.plk.as<double>(), 1, 0, sch.gradCorrected, true, gradP,
                   w.faceTmp.as<double>());

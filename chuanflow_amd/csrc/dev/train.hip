@@ -2,6 +2,7 @@
 #include "train.h"
 
 #include <algorithm>
+#include <cmath>
 
 namespace cfx
 {
@@ -127,6 +128,12 @@ std::vector<double> MpnnTrainer::train(const std::vector<TrainSystem>& systems, 
         for (int q = 0; q < opt.samples; ++q)
             loss += sampleLoss(mg, n, opt.seed * 1000003ull + static_cast<std::uint64_t>(step) * 131ull + q,
                                opt.smooth, true);
+        if (!std::isfinite(loss))
+        {
+            // a degenerate sample (or system): no update from this step
+            hist.push_back(loss);
+            continue;
+        }
         const float inv = 1.0f / static_cast<float>(opt.samples);
         k_scale<<<divUp(n, B), B, 0, s>>>(n, ga_.as<float>(), inv);
         k_scale<<<divUp(n, B), B, 0, s>>>(n, gt_.as<float>(), inv);

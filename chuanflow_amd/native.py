@@ -23,7 +23,7 @@ _L = C.CDLL(LIB_PATH)
 
 CFX_OK, CFX_E_INVALID, CFX_E_HIP, CFX_E_STATE, CFX_E_NUMERIC, CFX_E_INTERNAL = range(6)
 PRECOND_JACOBI, PRECOND_DIC, PRECOND_LEARNED = 0, 1, 2
-FP64, FP32_IR = 0, 1
+FP64, FP32_IR, FP32_MG = 0, 1, 2
 SOLVER_CG, SOLVER_BICGSTAB = 0, 1
 
 _vp = C.c_void_p
@@ -165,6 +165,10 @@ _sig("cfx_mpnn_evaluate", C.c_int, _vp, _vp, _vp, _vp, C.POINTER(TrainOptions), 
 _sig("cfx_csr_mg_loss_grad", C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _dp)
 _sig("cfx_csr_set_mpnn", C.c_int, _vp, _vp, _vp)
 _sig("cfx_matrix_set_mpnn", C.c_int, _vp, _vp)
+_sig("cfx_mesh_set_mpnn", C.c_int, _vp, _vp)
+_sig("cfx_mesh_set_mg", C.c_int, _vp, C.POINTER(MgParams))
+_sig("cfx_mesh_set_pressure_dump", C.c_int, _vp, _vp, _vp, C.c_int, C.c_int)
+_sig("cfx_mesh_pressure_dump_count", C.c_int, _vp, C.POINTER(C.c_int))
 _sig("cfx_csr_set_mg", C.c_int, _vp, C.POINTER(MgParams))
 _sig("cfx_csr_set_mg_weights", C.c_int, _vp, _vp, _vp, _vp, _vp)
 _sig("cfx_matrix_set_mg", C.c_int, _vp, C.POINTER(MgParams))
@@ -181,6 +185,8 @@ _sig("cfx_csr_time_spmv", C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, _dp)
 _sig("cfx_matrix_time_spmv", C.c_int, _vp, C.c_int, _vp, _vp, C.c_int, C.c_int, _dp)
 # host mesh
 _sig("cfx_hmesh_box", C.c_int, _i64, _i64, _i64, _vp, _vp, C.c_double, C.c_uint64, C.c_int, C.POINTER(_vp))
+_sig("cfx_hmesh_obstacle_channel", C.c_int, _i64, _i64, _i64, _vp, _vp, C.c_double, C.c_uint64, C.c_double,
+     C.c_double, C.c_double, C.POINTER(_vp))
 _sig("cfx_hmesh_from_arrays", C.c_int, _i64, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp,
      C.POINTER(_vp))
 _sig("cfx_hmesh_free", C.c_int, _vp)
@@ -331,6 +337,15 @@ class HostMesh:
         return cls(h)
 
     @classmethod
+    def obstacle_channel(cls, nx, ny, nz, mn, mx, jitter, seed, cx, cy, radius):
+        """Channel box minus a z-parallel cylinder of cells (7th patch "obstacle")."""
+        h = _vp()
+        _host_check(_L.cfx_hmesh_obstacle_channel(nx, ny, nz, _p(np.asarray(mn, np.float64)),
+                                                  _p(np.asarray(mx, np.float64)), jitter, seed, cx, cy, radius,
+                                                  C.byref(h)))
+        return cls(h)
+
+    @classmethod
     def from_arrays(cls, points, faceStart, faceVerts, owner, neighbour, patchStart, patchSize):
         h = _vp()
         pts = np.ascontiguousarray(points, np.float64)
@@ -412,6 +427,30 @@ class Mesh:
                                      len(self.patchStart), _p(self.patchStart), _p(self.patchSize), C.byref(h)))
         self.h = h
         self.V = g["V"]
+
+    def set_mg(self, **kw):
+        """Multigrid constants of the drivers' CFX_FP32_MG pressure solves."""
+        p = mg_default_params()
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise KeyError(k)
+            setattr(p, k, v)
+        self.ctx.check(_L.cfx_mesh_set_mg(self.h, C.byref(p)))
+
+    def set_mpnn(self, net):
+        self._net = net
+        self.ctx.check(_L.cfx_mesh_set_mpnn(self.h, net.h if net is not None else None))
+
+    def set_pressure_dump(self, val, rhs, every, max_count):
+        """Copy every `every`-th pressure system into device arrays val [max][nnz], rhs [max][n]."""
+        self._dump = (val, rhs)
+        self.ctx.check(_L.cfx_mesh_set_pressure_dump(self.h, _p(val) if val is not None else None,
+                                                     _p(rhs) if rhs is not None else None, every, max_count))
+
+    def pressure_dump_count(self):
+        c = C.c_int(0)
+        self.ctx.check(_L.cfx_mesh_pressure_dump_count(self.h, C.byref(c)))
+        return c.value
 
     @classmethod
     def from_host(cls, ctx, hm: HostMesh):

@@ -55,7 +55,7 @@ typedef struct cfx_report {
 } cfx_report;
 
 enum { CFX_PRECOND_JACOBI = 0, CFX_PRECOND_DIC = 1, CFX_PRECOND_LEARNED = 2 };
-enum { CFX_FP64 = 0, CFX_FP32_IR = 1 };
+enum { CFX_FP64 = 0, CFX_FP32_IR = 1, CFX_FP32_MG = 2 /* drivers' pressure solve: multigrid FCG */ };
 enum { CFX_SOLVER_CG = 0, CFX_SOLVER_BICGSTAB = 1 };
 
 /* ------------------------------------------------------------ context */
@@ -213,6 +213,13 @@ int cfx_csr_set_mg(cfx_csr* A, const cfx_mg_params* p);
  * are set up (after cfx_csr_set_values or this call) */
 int cfx_csr_set_mg_weights(cfx_csr* A, const float* d_pre, const float* d_scale, const float* d_post,
                            const float* d_restrict);
+/* the drivers' pressure solves with pPrecision CFX_FP32_MG use the mesh's
+ * multigrid constants and, when attached, its network; every `every`-th
+ * pressure system (CSR values, rhs) is copied into d_val [max][nnz],
+ * d_rhs [max][n] (training data; every <= 0 stops) */
+int cfx_mesh_set_mg(cfx_mesh* m, const cfx_mg_params* p);
+int cfx_mesh_set_pressure_dump(cfx_mesh* m, double* d_val, double* d_rhs, int every, int max);
+int cfx_mesh_pressure_dump_count(cfx_mesh* m, int* count);
 /* the same for an fvMatrix: cfx_matrix_solve with precond
  * CFX_PRECOND_LEARNED (CG, CFX_FP32_IR) rebuilds the Galerkin values from
  * the assembled values at each call */
@@ -237,8 +244,8 @@ int cfx_csr_mg_apply(cfx_csr* A, const float* d_r, float* d_z);
  * non-orthogonality, |r0|) to tied finest-level weights a (pre = post
  * smoother) and t (restriction = prolongation scale); an all-zero output
  * layer gives a = 2/3, t = sqrt(1.8).  Trained on the GPU with the
- * project's own forward and backward kernels against the energy of the
- * cycle's error on smooth error samples. */
+ * project's own forward and backward kernels against the A-norm error left
+ * by one preconditioned step (optimal step length) on smooth error samples. */
 typedef struct cfx_mpnn cfx_mpnn;
 int cfx_mpnn_num_params(void);
 int cfx_mpnn_create(cfx_ctx* ctx, uint64_t seed, int fp16, cfx_mpnn** out);
@@ -268,14 +275,16 @@ int cfx_mpnn_train(cfx_mpnn* net, cfx_csr* const* systems, const double* const* 
 /* mean objective over `samples` error samples, with the network or (0) with the constants */
 int cfx_mpnn_evaluate(cfx_mpnn* net, cfx_csr* A, const double* d_geo, const double* d_r0,
                       const cfx_train_options* opt, int samples, int withNetwork, double* loss);
-/* the objective and its gradient for one error sample: L = |e - M A e|_A^2 / |e|_A^2
- * with tied per-row weights a, t (linear V-cycle below level 1); ga, gt accumulate */
+/* the objective and its gradient for one error sample: with z = M A e,
+ * L = 1 - (z.A e)^2 / ((e.A e)(z.A z)) = min_alpha |e - alpha z|_A^2 / |e|_A^2,
+ * tied per-row weights a, t (linear V-cycle below level 1); ga, gt accumulate */
 int cfx_csr_mg_loss_grad(cfx_csr* A, const float* d_e, const float* d_a, const float* d_t, float* d_ga,
                          float* d_gt, double* loss);
 /* CFX_PRECOND_LEARNED solves take their finest-level weights from the
  * network (inputs from the matrix, d_geo and the right-hand side); null detaches */
 int cfx_csr_set_mpnn(cfx_csr* A, cfx_mpnn* net, const double* d_geo);
 int cfx_matrix_set_mpnn(cfx_matrix* A, cfx_mpnn* net);
+int cfx_mesh_set_mpnn(cfx_mesh* m, cfx_mpnn* net);
 
 /* ------------------------------------------------------------ drivers */
 /* Schemes (src/3d/include/cf/fvm/FvMatrix.h:30-38): div 0 upwind, 1 linear,

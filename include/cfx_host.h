@@ -27,6 +27,12 @@ int cfx_hmesh_box(int64_t nx, int64_t ny, int64_t nz, const double mn[3], const 
                   double jitter, uint64_t seed, int tets, cfx_hmesh** out);
 /* polyMesh(points, faces, owner, neighbour, patches) (PolyMesh.h:29-33) with
  * faces as CSR (faceStart[nFaces+1], faceVerts). */
+/* channel block (BlockMesh patches xmin xmax ymin ymax zmin zmax, seeded
+ * jitter) minus the cells whose centre is within `radius` of the z-parallel
+ * axis through (cx, cy); the cut faces form a 7th patch "obstacle" (wall).
+ * No counterpart in the reference (its blockMesh makes boxes only). */
+int cfx_hmesh_obstacle_channel(int64_t nx, int64_t ny, int64_t nz, const double mn[3], const double mx[3],
+                               double jitter, uint64_t seed, double cx, double cy, double radius, cfx_hmesh** out);
 int cfx_hmesh_from_arrays(int64_t nPoints, const double* points, int64_t nFaces,
                           const int64_t* faceStart, const int64_t* faceVerts, int64_t nInternal,
                           const int64_t* owner, const int64_t* neighbour, int64_t nPatches,

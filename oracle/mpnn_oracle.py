@@ -1,6 +1,7 @@
 """Message-passing network and multigrid training objective: a CPU FP64
 restatement (numpy + scipy) of the network's forward and backward pass and
-of the cycle-energy objective with its adjoint.
+of the multigrid training objective (the A-norm error left by one
+preconditioned step with the optimal step length) with its adjoint.
 
 TEST INFRASTRUCTURE ONLY.  Importable from tests/, from
 ``__graft_entry__.smoke()`` and from bench.py's cpu_baseline leg, as the
@@ -149,14 +150,17 @@ def backward(params, cache, ga, gt):
 
 
 def objective(H_, e, a, t, p: MG.Params):
-    """L = |e - M A e|_A^2 / |e|_A^2 for the cycle with tied finest-level
-    weights (pre = post = a, restriction = prolongation = t) and the linear
-    V-cycle below level 1."""
+    """L = 1 - (z.A e)^2 / ((e.A e)(z.A z)), z = M A e, for the cycle with
+    tied finest-level weights (pre = post = a, restriction = prolongation =
+    t) and the linear V-cycle below level 1: the A-norm error left by one
+    preconditioned step with the optimal step length, min_alpha
+    |e - alpha z|_A^2 / |e|_A^2."""
     q = MG.Params(**{**p.__dict__, "kcycleDepth": 0})
     A = H_.A[0]
     b = A @ e
     z = MG.vcycle(H_, b, q, pre=a, scale=t, post=a, restrict=t)
-    eps = e - z
+    alpha = (z @ b) / (z @ (A @ z))
+    eps = e - alpha * z
     return (eps @ (A @ eps)) / (e @ b)
 
 
@@ -176,11 +180,11 @@ def objective_grad(H_, e, a, t, p: MG.Params):
     u = Q(t * r1)
     x2 = x1 + t * u
     pd = dinv * (b - A @ x2)
-    eps = e - (x2 + a * pd)
-    aeps = A @ eps
-    eAe = e @ b
-    Lval = (eps @ aeps) / eAe
-    gz = -2.0 * aeps / eAe
+    z = x2 + a * pd
+    Az = A @ z
+    c, pz, qz = e @ b, z @ b, z @ Az
+    Lval = 1.0 - pz * pz / (c * qz)
+    gz = (2.0 * pz / (c * qz)) * (-b + (pz / qz) * Az)
     ga = gz * pd
     gx2 = gz - A @ (dinv * (a * gz))
     gt = gx2 * u

@@ -1,0 +1,71 @@
+"""The learned preconditioner end to end: pressure systems from the GPU SIMPLE
+loop, on-GPU training, and the ablation of the trained network against the
+same flexible-CG multigrid solve without it.
+
+Held-out systems (seeds not used for the committed weights): the committed
+network lowers the training objective (the A-norm error left by one
+preconditioned step) against the constants it starts from, and never needs
+more flexible-CG iterations to the 1e-8 FP64 residual than those constants
+(fewer in total), nor than the textbook constants; every solve's true FP64
+residual meets the tolerance."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def heldout(ctx):
+    from chuanflow_amd import learned as LN
+    print("\n[case] held-out SIMPLE pressure systems, 2 obstacle channels 48x24x24", flush=True)
+    return LN.training_set(ctx, cases=2, size=(48, 24, 24), seed=77, per_case=2, iterations=30)
+
+
+def test_simple_dumps_pressure_systems(ctx, heldout):
+    assert len(heldout) == 4
+    for rp, cl, v, rhs, geo in heldout:
+        n = len(rp) - 1
+        diag = v[[rp[i] + np.searchsorted(cl[rp[i]:rp[i + 1]], i) for i in range(0, n, 97)]]
+        assert np.all(diag > 0) and np.all(np.isfinite(v)) and np.all(np.isfinite(rhs))
+        assert geo.min() >= 0 and geo.max() < 0.2  # 1 - cos(Sf, d) of a +-15% jittered mesh
+
+
+def test_training_on_gpu_lowers_the_objective(ctx, heldout):
+    from chuanflow_amd import learned as LN
+    print("[case] 150 training steps on two held-out systems", flush=True)
+    net, hist = LN.train(ctx, heldout[:2], steps=150, lr=1e-3, seed=5)
+    assert np.all(np.isfinite(hist))
+    assert hist[-20:].mean() < 0.8 * hist[:20].mean(), (hist[:20].mean(), hist[-20:].mean())
+
+
+def test_ablation_trained_network_vs_constants(ctx, heldout):
+    from chuanflow_amd import learned as LN
+    from chuanflow_amd import native as N
+    net = LN.load(ctx)
+    csrs, geos, rhss = LN.to_device(ctx, heldout, LN.TUNED_MG)
+    tot = {"textbook": 0, "tuned": 0, "learned": 0}
+    for G, g, r, (rp, cl, v, rhs, geo) in zip(csrs, geos, rhss, heldout):
+        Lc = net.evaluate(G, g, r, samples=8, with_network=False, seed=11)
+        Ln = net.evaluate(G, g, r, samples=8, with_network=True, seed=11)
+        b = rhs - rhs.mean()
+        A_rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+        its = {}
+        for name, mg, use in (("textbook", dict(scale=1.0, coarseScale=1.0), False), ("tuned", LN.TUNED_MG, False),
+                              ("learned", LN.TUNED_MG, True)):
+            G.set_mg(**mg)
+            G.set_mpnn(net if use else None, g if use else None)
+            x, rep = G.solve(b, None, 1e-8, 500, N.SOLVER_CG, N.PRECOND_LEARNED, N.FP32_IR)
+            Ax = np.bincount(A_rows, weights=v * x[cl], minlength=len(b))
+            assert rep.converged and rep.precondFallbacks == 0
+            assert np.abs(b - Ax).max() / np.abs(b).max() < 1e-8
+            its[name] = rep.iterations
+            tot[name] += rep.iterations
+        G.set_mpnn(None)
+        print(f"objective constants {Lc:.4f} network {Ln:.4f}; iterations {its}", flush=True)
+        assert Ln < Lc
+        assert its["learned"] <= its["tuned"] and its["learned"] <= its["textbook"]
+    assert tot["learned"] < tot["tuned"]

@@ -121,3 +121,31 @@ def test_from_arrays_roundtrip_and_bad_input_rejected():
     with pytest.raises(N.CfxError):  # a patch that covers an internal face
         N.HostMesh.from_arrays(a["points"], a["faceStart"], a["faceVerts"], a["owner"], a["neighbour"],
                                a["patchStart"] - 1, a["patchSize"])
+
+
+def test_obstacle_channel_invariants():
+    from chuanflow_amd import native as N
+    mn, mx, (nx, ny, nz) = (0, 0, 0), (1.0, 0.5, 0.1), (40, 20, 4)
+    cx, cy, R = 0.3, 0.25, 0.08
+    box = N.HostMesh.box(nx, ny, nz, mn, mx, 0.1, 5)
+    hm = N.HostMesh.obstacle_channel(nx, ny, nz, mn, mx, 0.1, 5, cx, cy, R)
+    a, g = hm.arrays(), hm.geometry()
+    gb = box.geometry()
+    inside = (gb["C"][:, 0] - cx) ** 2 + (gb["C"][:, 1] - cy) ** 2 < R * R
+    assert a["nCells"] == box.arrays()["nCells"] - inside.sum() and inside.sum() > 0
+    # kept cells keep their volumes and centres (same order; a reversed cut
+    # face changes the summation order of its cell only)
+    np.testing.assert_allclose(g["V"], gb["V"][~inside], rtol=1e-13)
+    np.testing.assert_allclose(g["C"], gb["C"][~inside], rtol=1e-13)
+    # every cell is closed: the outward face areas sum to zero
+    nI, nF, n = len(a["neighbour"]), len(a["owner"]), a["nCells"]
+    S = np.zeros((n, 3))
+    np.add.at(S, a["owner"], g["Sf"])
+    np.add.at(S, a["neighbour"], -g["Sf"][:nI])
+    assert np.abs(S).max() < 1e-14
+    # the 7th patch faces the cylinder axis: normals point out of the fluid, into the obstacle
+    ps, sz = a["patchStart"], a["patchSize"]
+    assert len(ps) == 7 and ps[6] + sz[6] == nF and sz[6] > 0
+    f = np.arange(ps[6], ps[6] + sz[6])
+    toward = np.column_stack([cx - g["Cf"][f, 0], cy - g["Cf"][f, 1]])
+    assert np.all((g["Sf"][f, :2] * toward).sum(axis=1) > 0)
