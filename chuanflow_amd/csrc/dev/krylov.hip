@@ -122,6 +122,71 @@ __global__ __launch_bounds__(kSpmvBlock) void k_spmv(CsrView A, const T* __restr
     if (spmvRowSum(A, val, x, xcdRemap(blockIdx.x, gridDim.x), lds, row, s)) y[row] = s;
 }
 
+// True residual of a finished solve, two kernels and no atomics: per-workgroup
+// partials of max|r|, max|b|, sum r^2, sum b^2 and NaN flags over rbPerWg
+// row blocks, then one workgroup reduces them into st->sum0 (max|r|/max|b|)
+// and st->rho (|r|2 / |b|2).
+__global__ __launch_bounds__(kSpmvBlock) void k_true_resid_part(CsrView A, int rbPerWg, const double* __restrict__ val,
+                                                                const double* __restrict__ x,
+                                                                const double* __restrict__ b, double* part,
+                                                                int* ipart)
+{
+    __shared__ double lds[kSpmvLds];
+    __shared__ double red[kSpmvBlock / kWave];
+    __shared__ int ired[kSpmvBlock / kWave];
+    const int G = gridDim.x;
+    const int w = xcdRemap(blockIdx.x, G);
+    double rm = 0, bm = 0, r2 = 0, b2 = 0;
+    int nan = 0;
+    for (int j = 0; j < rbPerWg; ++j)
+    {
+        const int blk = w * rbPerWg + j;
+        if (blk >= A.nRowBlocks) break;
+        int row;
+        double s;
+        if (spmvRowSum(A, val, x, blk, lds, row, s))
+        {
+            const double bd = b[row], rd = bd - s;
+            rm = fmax(rm, fabs(rd));
+            bm = fmax(bm, fabs(bd));
+            r2 += rd * rd;
+            b2 += bd * bd;
+            nan |= (rd != rd ? 1 : 0) | (bd != bd ? 2 : 0);
+        }
+    }
+    rm = blockMax(rm, red);
+    bm = blockMax(bm, red);
+    r2 = blockSum(r2, red);
+    b2 = blockSum(b2, red);
+    nan = blockOr(nan, ired);
+    if (threadIdx.x == 0)
+    {
+        part[blockIdx.x] = rm;
+        part[G + blockIdx.x] = bm;
+        part[2 * G + blockIdx.x] = r2;
+        part[3 * G + blockIdx.x] = b2;
+        ipart[blockIdx.x] = nan;
+    }
+}
+
+__global__ __launch_bounds__(kSpmvBlock) void k_true_resid_final(int G, const double* part, const int* ipart,
+                                                                 SolveState* st)
+{
+    __shared__ double red[kSpmvBlock / kWave];
+    __shared__ int ired[kSpmvBlock / kWave];
+    const double RM = maxPartials(part, G, red);
+    const double BM = maxPartials(part + G, G, red);
+    const double R2 = sumPartials(part + 2 * G, G, red);
+    const double B2 = sumPartials(part + 3 * G, G, red);
+    const int NAN_ = orPartials(ipart, G, ired);
+    if (threadIdx.x == 0)
+    {
+        const double binf = (NAN_ & 2) ? NAN : BM;
+        st->sum0 = ((NAN_ & 1) ? NAN : RM) / (binf < 1e-300 ? 1e-300 : binf);
+        st->rho = sqrt(R2) / (sqrt(B2) > 1e-300 ? sqrt(B2) : 1e-300);
+    }
+}
+
 // r = b - A x with reductions max|r|, max|b| (NaN-flagged), sum r^2, sum b^2.
 // mode 0: initial residual of a solve (sets initialResidual, bnorm and the
 //         early-convergence exit, solveCG :167-180);
@@ -516,8 +581,11 @@ void Krylov::trueResidual(const CsrView& A, const double* val, const double* b, 
     SolveState* st = state_.as<SolveState>();
     double* part = partials_.as<double>();
     int* ipart = reinterpret_cast<int*>(part + 4 * maxBlocks_);
-    k_residual<double><<<A.nRowBlocks, kSpmvBlock, 0, s>>>(A, val, x, b, r0_.as<double>(), st, part,
-                                                           ipart, 2);
+    int G = std::min(A.nRowBlocks, kVecGridMax);
+    const int rbPerWg = divUp(A.nRowBlocks, G);
+    G = std::max(1, divUp(A.nRowBlocks, rbPerWg));
+    k_true_resid_part<<<G, kSpmvBlock, 0, s>>>(A, rbPerWg, val, x, b, part, ipart);
+    k_true_resid_final<<<1, kSpmvBlock, 0, s>>>(G, part, ipart, st);
     CFX_CHECK_LAUNCH();
     poll();
     infRel = ctx_.hostState()[0].sum0;

@@ -69,6 +69,7 @@ private:
     CsrView A_{};
     DevBuf w32_, wT_, m_, v_;     // FP32 master weights, storage-type copy, Adam moments
     DevBuf x0_, ew_, eg_, scal_;   // node features [n][8] FP32, edge weights per slot, scalars
+    DevBuf dabs_;                  // guarded |a_ii|
     DevBuf h_[2], xs_[MpnnShape::L + 1], mask_[MpnnShape::L + 1], o_;
     DevBuf dh_[2], dmg_, part_;
     int adamStep_ = 0;

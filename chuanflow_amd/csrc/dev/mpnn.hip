@@ -28,10 +28,20 @@ __global__ void k_absmax(int n, const double* __restrict__ v, double* __restrict
     if (threadIdx.x == 0) part[blockIdx.x] = m;
 }
 
+// |a_ii| guarded away from zero (max(|a_ii|, 1e-300))
+__global__ void k_diag_abs(CsrView A, const double* __restrict__ val, double* __restrict__ d)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= A.n) return;
+    const double v = A.diagSlot[i] >= 0 ? fabs(val[A.diagSlot[i]]) : 0.0;
+    d[i] = v > 1e-300 ? v : 1e-300;
+}
+
 // node features x0 [n][8] and edge features ew, eg per CSR slot (0 on the diagonal)
-__global__ void k_features(CsrView A, const double* __restrict__ val, const double* __restrict__ geo,
-                           const double* __restrict__ r0, const double* __restrict__ partMax, int Gm,
-                           float* __restrict__ x0, float* __restrict__ ew, float* __restrict__ eg)
+__global__ void k_features(CsrView A, const double* __restrict__ val, const double* __restrict__ dabs,
+                           const double* __restrict__ geo, const double* __restrict__ r0,
+                           const double* __restrict__ partMax, int Gm, float* __restrict__ x0,
+                           float* __restrict__ ew, float* __restrict__ eg)
 {
     __shared__ double red[B / kWave];
     __shared__ double rmaxS;
@@ -44,8 +54,7 @@ __global__ void k_features(CsrView A, const double* __restrict__ val, const doub
     const double rmax = rmaxS;
     const int i = blockIdx.x * B + threadIdx.x;
     if (i >= A.n) return;
-    const double di = A.diagSlot[i] >= 0 ? fabs(val[A.diagSlot[i]]) : 0.0;
-    const double dd = di > 1e-300 ? di : 1e-300;
+    const double dd = dabs[i];
     double off = 0.0, rs = 0.0, wmax = 0.0, wmin = 0.0, gsum = 0.0;
     int deg = 0;
     for (int k = A.rowPtr[i]; k < A.rowPtr[i + 1]; ++k)
@@ -59,9 +68,7 @@ __global__ void k_features(CsrView A, const double* __restrict__ val, const doub
             eg[k] = 0.0f;
             continue;
         }
-        const double dj0 = A.diagSlot[j] >= 0 ? fabs(val[A.diagSlot[j]]) : 0.0;
-        const double dj = dj0 > 1e-300 ? dj0 : 1e-300;
-        const double w = fabs(v) / sqrt(dd * dj);
+        const double w = fabs(v) / sqrt(dd * dabs[j]);
         const double g = geo ? geo[k] : 0.0;
         ew[k] = static_cast<float>(w);
         eg[k] = static_cast<float>(g);
@@ -80,6 +87,35 @@ __global__ void k_features(CsrView A, const double* __restrict__ val, const doub
     x[5] = static_cast<float>(deg > 0 ? gsum / deg : 0.0);
     x[6] = static_cast<float>((r0 && rmax > 0.0) ? fabs(r0[i]) / rmax : 0.0);
     x[7] = 1.0f;
+}
+
+// 16 consecutive feature values (16-byte aligned) as FP32, in 16-byte loads
+__device__ __forceinline__ void load16(const _Float16* p, float* v)
+{
+    const uint4* q = reinterpret_cast<const uint4*>(p);
+    const uint4 a = q[0], b = q[1];
+    const _Float16* ha = reinterpret_cast<const _Float16*>(&a);
+    const _Float16* hb = reinterpret_cast<const _Float16*>(&b);
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+    {
+        v[c] = static_cast<float>(ha[c]);
+        v[8 + c] = static_cast<float>(hb[c]);
+    }
+}
+
+__device__ __forceinline__ void load16(const float* p, float* v)
+{
+    const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+    {
+        const float4 f = q[c];
+        v[4 * c] = f.x;
+        v[4 * c + 1] = f.y;
+        v[4 * c + 2] = f.z;
+        v[4 * c + 3] = f.w;
+    }
 }
 
 // ----------------------------------------------------------------- forward
@@ -109,21 +145,19 @@ __global__ __launch_bounds__(B) void k_layer_fwd(int n, CsrView A, const float* 
         for (int c = 0; c < 16; ++c) self[c] = m[c] = g[c] = 0.0f;
         if (i < n)
         {
-            const T* hi = hin + static_cast<long long>(i) * S::H + f0;
-#pragma unroll
-            for (int c = 0; c < 16; ++c) self[c] = static_cast<float>(hi[c]);
+            load16(hin + static_cast<long long>(i) * S::H + f0, self);
             for (int k = A.rowPtr[i]; k < A.rowPtr[i + 1]; ++k)
             {
                 const int j = A.col[k];
                 if (j == i) continue;
                 const float w = ew[k], gg = eg[k];
-                const T* hj = hin + static_cast<long long>(j) * S::H + f0;
+                float v[16];
+                load16(hin + static_cast<long long>(j) * S::H + f0, v);
 #pragma unroll
                 for (int c = 0; c < 16; ++c)
                 {
-                    const float v = static_cast<float>(hj[c]);
-                    m[c] += w * v;
-                    g[c] += gg * v;
+                    m[c] += w * v[c];
+                    g[c] += gg * v[c];
                 }
             }
         }
@@ -455,6 +489,7 @@ void Mpnn::reserve(int n, int nnz)
     if (n <= cap_ && ew_.bytes >= sizeof(float) * static_cast<std::size_t>(nnz)) return;
     const std::size_t N = static_cast<std::size_t>(n), ts = fp16_ ? 2 : 4;
     x0_.alloc(sizeof(float) * N * S::FIN);
+    dabs_.alloc(sizeof(double) * N);
     ew_.alloc(sizeof(float) * nnz);
     eg_.alloc(sizeof(float) * nnz);
     for (auto& b : h_) b.alloc(ts * N * S::H);
@@ -476,7 +511,9 @@ void Mpnn::setGraph(const CsrView& A, const double* val, const double* geo, cons
     const int Gm = std::max(1, std::min(256, divUp(A.n, B)));
     double* pm = scal_.as<double>();
     if (r0) k_absmax<<<Gm, B, 0, s>>>(A.n, r0, pm);
-    k_features<<<divUp(A.n, B), B, 0, s>>>(A, val, geo, r0, pm, Gm, x0_.as<float>(), ew_.as<float>(), eg_.as<float>());
+    k_diag_abs<<<divUp(A.n, B), B, 0, s>>>(A, val, dabs_.as<double>());
+    k_features<<<divUp(A.n, B), B, 0, s>>>(A, val, dabs_.as<double>(), geo, r0, pm, Gm, x0_.as<float>(),
+                                           ew_.as<float>(), eg_.as<float>());
     CFX_CHECK_LAUNCH();
 }
 
