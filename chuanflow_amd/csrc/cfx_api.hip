@@ -1,6 +1,7 @@
 // cfx_api.hip -- the C ABI (include/cfx.h) over the device objects.
 #include "../../include/cfx.h"
 
+#include "dev/dic.h"
 #include "dev/fvops.h"
 #include "dev/krylov.h"
 #include "dev/mesh.h"
@@ -88,6 +89,18 @@ struct cfx_csr
     cfx_mpnn* net = nullptr;     // learned finest-level weights (precond LEARNED)
     const double* geo = nullptr; // caller's per-slot geometry (optional)
     DevBuf wa, wt;
+    std::unique_ptr<Dic> dic;    // precond DIC
+    bool dicValid = false;
+    Dic& dicReady()
+    {
+        if (!dic) dic = std::make_unique<Dic>(*owner->ctx, pattern);
+        if (!dicValid)
+        {
+            dic->setMatrix(val.as<double>());
+            dicValid = true;
+        }
+        return *dic;
+    }
 };
 
 namespace
@@ -680,6 +693,7 @@ int cfx_csr_set_values(cfx_csr* A, const double* vals)
                                A->owner->ctx->stream()));
         A->val32Valid = false;
         A->mg.valid = false;
+        A->dicValid = false;
         A->owner->ctx->sync();
     });
 }
@@ -706,7 +720,15 @@ int cfx_csr_solve_device(cfx_csr* A, const double* b, double* x, double tol, int
             hist = A->resHist.as<double>();
         }
         SolveReport s;
-        if (precond == CFX_PRECOND_LEARNED)
+        if (precond == CFX_PRECOND_DIC)
+        {
+            need(solver == CFX_SOLVER_CG, "DIC runs inside (flexible) CG");
+            need(precision == CFX_FP32_IR, "DIC runs in FP32 with FP64 refinement");
+            s = A->owner->solver(A->pattern.n, A->pattern.nRowBlocks)
+                    .fcgMixed(A->pattern.view(), A->val.as<double>(), A->dicReady(), b, x, tol, maxIter, A->innerTol,
+                              nullptr);
+        }
+        else if (precond == CFX_PRECOND_LEARNED)
         {
             need(solver == CFX_SOLVER_CG, "the multigrid preconditioner runs inside flexible CG");
             need(precision == CFX_FP32_IR, "the multigrid preconditioner runs in FP32 with FP64 refinement");
@@ -881,6 +903,30 @@ int cfx_csr_mg_apply(cfx_csr* A, const float* r, float* z)
         Multigrid& mg = A->mg.ready(ctx, A->pattern, A->val.as<double>());
         mg.apply(r, z, nullptr, nullptr, A->pattern.nRowBlocks, 1, nullptr);
         ctx.sync();
+    });
+}
+
+int cfx_csr_dic_info(cfx_csr* A, int* levels, double* diag)
+{
+    if (!A) return CFX_E_INVALID;
+    return guard(A->owner, [&] {
+        Dic& d = A->dicReady();
+        if (levels) *levels = d.levels();
+        if (diag)
+        {
+            CFX_HIP(hipMemcpyAsync(diag, d.diag64(), sizeof(double) * A->pattern.n, hipMemcpyDeviceToHost,
+                                   A->owner->ctx->stream()));
+            A->owner->ctx->sync();
+        }
+    });
+}
+
+int cfx_csr_dic_apply(cfx_csr* A, const float* r, float* z)
+{
+    if (!A) return CFX_E_INVALID;
+    return guard(A->owner, [&] {
+        A->dicReady().apply(r, z, nullptr, nullptr, 1, 1, nullptr);
+        A->owner->ctx->sync();
     });
 }
 

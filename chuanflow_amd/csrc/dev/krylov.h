@@ -16,7 +16,6 @@
 namespace cfx
 {
 
-class Multigrid;
 
 struct SolveReport
 {
@@ -71,7 +70,7 @@ public:
     // of `val` (Multigrid::setMatrix).  On a breakdown or a non-finite value
     // the solve continues from the last FP64 iterate with cgMixed (Jacobi)
     // and reports precondFallbacks = 1.
-    SolveReport fcgMixed(const CsrView& A, const double* val, Multigrid& mg, const double* b, double* x,
+    SolveReport fcgMixed(const CsrView& A, const double* val, Preconditioner& mg, const double* b, double* x,
                          double tol, int maxIter, double delta, double* resHist = nullptr);
 
     // True residual ||b - A x|| in FP64: returns {inf-norm rel, 2-norm rel}.

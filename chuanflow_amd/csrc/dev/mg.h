@@ -74,7 +74,7 @@ struct MgLevel
     std::vector<int> hAgg;
 };
 
-class Multigrid
+class Multigrid : public Preconditioner
 {
 public:
     // Builds the hierarchy from a fine pattern (host copies of rowPtr/col).
@@ -83,7 +83,10 @@ public:
     static constexpr int kDenseMax = 128;
     const MgParams& params() const { return p_; }
     // unique per constructed hierarchy (device buffers never move after construction)
-    std::uintptr_t id() const { return id_; }
+    std::uintptr_t id() const override { return id_; }
+    int rows() const override { return lv_[0].n; }
+    const float* val32() const override { return lv_[0].val32.as<float>(); }
+    const float* invD32() const override { return lv_[0].invD32.as<float>(); }
     int levels() const { return static_cast<int>(lv_.size()); }
     int rows(int l) const { return lv_[l].n; }
     int nnz(int l) const { return lv_[l].nnz; }
@@ -101,7 +104,8 @@ public:
                    const float* restr, const MgParams& p);
     // z = V-cycle(r) on the finest level.  When partZQ is non-null also writes
     // per-workgroup partials of z.q (grid Gs with rbPerWg row blocks each).
-    void apply(const float* r, float* z, const float* q, double* partZQ, int Gs, int rbPerWg, const int* done);
+    void apply(const float* r, float* z, const float* q, double* partZQ, int Gs, int rbPerWg,
+               const int* done) override;
 
     // Training objective of the finest-level weights.  Requires setMatrix
     // with pre = post = a, scale = restr = t and kcycleDepth = 0 (so the

@@ -680,14 +680,14 @@ SolveReport Krylov::cgMixed(const CsrView& A, const double* val, const float* va
 }
 
 
-SolveReport Krylov::fcgMixed(const CsrView& A, const double* val, Multigrid& mg, const double* b, double* x,
+SolveReport Krylov::fcgMixed(const CsrView& A, const double* val, Preconditioner& mg, const double* b, double* x,
                              double tol, int maxIter, double delta, double* resHist)
 {
     hipStream_t s = ctx_.stream();
     CFX_HIP(hipEventRecord(ev0_, s));
     const PcgLaunch g = pcgGrid(A, n_);
     if (g.Gs > kRedMax || g.Gv > kRedMax) throw std::invalid_argument("fcg: grid exceeds the partial buffer");
-    if (mg.rows(0) != A.n) throw std::invalid_argument("fcg: multigrid hierarchy does not match the matrix");
+    if (mg.rows() != A.n) throw std::invalid_argument("fcg: preconditioner does not match the matrix");
     if (!fcgPart_.p)
     {
         fcgPart_.alloc(sizeof(double) * 5 * kRedMax);
@@ -702,8 +702,8 @@ SolveReport Krylov::fcgMixed(const CsrView& A, const double* val, Multigrid& mg,
     CFX_HIP(hipMemsetAsync(d32_.p, 0, sizeof(float) * n_, s));
     float *p = p32_.as<float>(), *q = q32_.as<float>(), *r = r32_.as<float>(), *z = z32_.as<float>(),
           *x32 = d32_.as<float>();
-    const float* val32 = mg.fineVal32();
-    const float* invD32 = mg.fineInvD32();
+    const float* val32 = mg.val32();
+    const float* invD32 = mg.invD32();
     const int* done = &st->done;
 
     k_pcg_residual<float><<<g.Gs, kB, 0, s>>>(A, g.rbPerWg, val, x, b, nullptr, r, invD32, partRZ, partRM, partBM);

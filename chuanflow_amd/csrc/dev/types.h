@@ -17,6 +17,7 @@
 #include <memory>
 #include <string>
 #include <unordered_map>
+#include <cstdint>
 #include <vector>
 
 namespace cfx
@@ -157,6 +158,22 @@ private:
     hipStream_t stream_ = nullptr;
     DevBuf partials_, ipartials_, tickets_;
     SolveState* hostState_ = nullptr;  // pinned
+};
+
+// A preconditioner of flexible CG in FP32: z = M r on device vectors and,
+// when partZQ is non-null, Gs per-workgroup partials of z.q (Gs workgroups
+// of rbPerWg row blocks, the grid of the CG kernels).  `done` (device, may
+// be null) skips the work once the iteration has stopped.
+class Preconditioner
+{
+public:
+    virtual ~Preconditioner() = default;
+    virtual int rows() const = 0;
+    virtual std::uintptr_t id() const = 0;        // unique per set of device buffers
+    virtual const float* val32() const = 0;       // FP32 copy of the matrix values
+    virtual const float* invD32() const = 0;      // FP32 inverse diagonal
+    virtual void apply(const float* r, float* z, const float* q, double* partZQ, int Gs, int rbPerWg,
+                       const int* done) = 0;
 };
 
 } // namespace cfx

@@ -176,6 +176,8 @@ _sig("cfx_matrix_set_mg_weights", C.c_int, _vp, _vp, _vp, _vp, _vp)
 _sig("cfx_csr_mg_info", C.c_int, _vp, C.POINTER(C.c_int), _vp, _vp, C.c_int)
 _sig("cfx_csr_mg_level", C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp)
 _sig("cfx_csr_mg_apply", C.c_int, _vp, _vp, _vp)
+_sig("cfx_csr_dic_info", C.c_int, _vp, C.POINTER(C.c_int), _vp)
+_sig("cfx_csr_dic_apply", C.c_int, _vp, _vp, _vp)
 _sig("cfx_linearise", C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp)
 _sig("cfx_simple_solve", C.c_int, _vp, C.c_double, C.POINTER(Schemes), C.POINTER(SimpleControls), _vp, _vp, _vp, _vp,
      _vp, _vp, _vp, C.POINTER(SimpleResult), _vp)
@@ -717,6 +719,16 @@ class Csr:
 
     def mg_apply(self, r, z):
         self.ctx.check(_L.cfx_csr_mg_apply(self.h, _p(r), _p(z)))
+
+    def dic_info(self):
+        """(level count, FP64 DIC factor D) of the current values."""
+        lv = C.c_int(0)
+        d = np.zeros(self.n)
+        self.ctx.check(_L.cfx_csr_dic_info(self.h, C.byref(lv), _p(d)))
+        return lv.value, d
+
+    def dic_apply(self, r, z):
+        self.ctx.check(_L.cfx_csr_dic_apply(self.h, _p(r), _p(z)))
 
     def mg_loss_grad(self, e, a, t):
         """(L, dL/da, dL/dt) of the cycle-energy objective for error sample e (device FP32 arrays)."""

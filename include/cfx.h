@@ -229,6 +229,14 @@ int cfx_matrix_set_mg_weights(cfx_matrix* A, const float* d_pre, const float* d_
 /* hierarchy sizes: rows[l], nnz[l] for l < min(maxOut, levels) */
 int cfx_csr_mg_info(cfx_csr* A, int* levels, int64_t* rows, int64_t* nnz, int maxOut);
 
+/* precond = CFX_PRECOND_DIC (solver CG, precision CFX_FP32_IR): flexible CG
+ * preconditioned by diagonal incomplete Cholesky, D_ii = a_ii - sum_{j<i}
+ * a_ij^2 / D_jj (OpenFOAM's DIC; no counterpart in the reference), with
+ * level-scheduled substitutions (one kernel per level of the ordering's
+ * dependency graph).  cfx_csr_dic_info gives the level count and the FP64
+ * factor D (host [n]); cfx_csr_dic_apply is z = M^-1 r on FP32 device vectors. */
+int cfx_csr_dic_info(cfx_csr* A, int* levels, double* diag);
+int cfx_csr_dic_apply(cfx_csr* A, const float* d_r, float* d_z);
 /* level l of the hierarchy (host outputs, each may be null): CSR rowPtr
  * [rows+1], col [nnz], FP64 Galerkin values [nnz], and agg [rows] = the
  * level-(l+1) row of each row (not written on the coarsest level) */
