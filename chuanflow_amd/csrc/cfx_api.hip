@@ -1,6 +1,7 @@
 // cfx_api.hip -- the C ABI (include/cfx.h) over the device objects.
 #include "../../include/cfx.h"
 
+#include "dev/batch.h"
 #include "dev/dic.h"
 #include "dev/fvops.h"
 #include "dev/krylov.h"
@@ -22,6 +23,7 @@ struct cfx_ctx
     std::string err;
     std::map<int, std::unique_ptr<Krylov>> krylov;  // workspace per system size
     bool graphs = true;
+    std::unique_ptr<BatchedPcg> batch;
 
     Krylov& solver(int n, int rowBlocks)
     {
@@ -1284,6 +1286,83 @@ int cfx_matrix_set_mpnn(cfx_matrix* A, cfx_mpnn* m)
         A->net = m;
         if (!m) A->mg.pre = A->mg.post = A->mg.scale = A->mg.restr = nullptr;
         A->mg.valid = false;
+    });
+}
+
+// ------------------------------------------------------- batched solves
+namespace
+{
+std::vector<BatchSystem> batchSystems(cfx_ctx* c, int count, cfx_csr* const* systems, const double* const* b,
+                                      double* const* x)
+{
+    need(count > 0 && systems && x, "bad batch");
+    std::vector<BatchSystem> v(count);
+    for (int k = 0; k < count; ++k)
+    {
+        need(systems[k] && systems[k]->owner == c, "batch system of another context");
+        v[k].A = systems[k]->pattern.view();
+        v[k].val = systems[k]->val.as<double>();
+        v[k].b = b ? b[k] : nullptr;
+        v[k].x = x[k];
+    }
+    return v;
+}
+} // namespace
+
+int cfx_batch_solve(cfx_ctx* c, int count, cfx_csr* const* systems, const double* const* b, double* const* x,
+                    double tol, int maxIter, const cfx_batch_schedule* sched, cfx_report* reps, double* finishMs,
+                    cfx_batch_stats* stats)
+{
+    if (!c) return CFX_E_INVALID;
+    return guard(c, [&] {
+        need(b != nullptr, "null right-hand sides");
+        need(maxIter >= 0, "maxIter < 0");
+        if (!c->batch) c->batch = std::make_unique<BatchedPcg>(*c->ctx);
+        const std::vector<BatchSystem> sys = batchSystems(c, count, systems, b, x);
+        BatchSchedule sc;
+        if (sched)
+        {
+            sc.vecElems = sched->vecElems;
+            sc.wgCap = sched->wgCap;
+            sc.graph = sched->graph;
+            sc.pollEvery = sched->pollEvery;
+        }
+        else
+            sc = c->batch->autotune(sys, 20, nullptr);
+        const BatchResult r = c->batch->solve(sys, tol, maxIter, sc);
+        for (int k = 0; k < count; ++k)
+        {
+            if (reps) toReport(r.reports[k], reps + k);
+            if (finishMs) finishMs[k] = r.finishMs[k];
+        }
+        if (stats)
+        {
+            stats->ms = r.ms;
+            stats->maxIterations = r.maxIterations;
+            stats->used = cfx_batch_schedule{sc.vecElems, sc.wgCap, sc.graph, sc.pollEvery};
+        }
+    });
+}
+
+int cfx_batch_autotune(cfx_ctx* c, int count, cfx_csr* const* systems, double* const* x, int iters,
+                       cfx_batch_schedule* best, cfx_batch_schedule* cand, double* candMs, int maxCand)
+{
+    if (!c || !best) return CFX_E_INVALID;
+    return guard(c, [&] {
+        need(iters > 0, "iters must be positive");
+        if (!c->batch) c->batch = std::make_unique<BatchedPcg>(*c->ctx);
+        std::vector<BatchSystem> sys = batchSystems(c, count, systems, nullptr, x);
+        // the timing runs need a right-hand side: the system's own x serves (any vector does)
+        for (auto& s : sys) s.b = s.x;
+        std::vector<std::pair<BatchSchedule, double>> log;
+        const BatchSchedule b = c->batch->autotune(sys, iters, &log);
+        *best = cfx_batch_schedule{b.vecElems, b.wgCap, b.graph, b.pollEvery};
+        for (int k = 0; k < std::min<int>(maxCand, static_cast<int>(log.size())); ++k)
+        {
+            if (cand) cand[k] = cfx_batch_schedule{log[k].first.vecElems, log[k].first.wgCap, log[k].first.graph,
+                                                   log[k].first.pollEvery};
+            if (candMs) candMs[k] = log[k].second;
+        }
     });
 }
 

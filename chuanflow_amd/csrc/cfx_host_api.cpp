@@ -1,7 +1,11 @@
 // cfx_host_api.cpp -- C ABI of the host mesh layer (include/cfx_host.h).
 #include "../../include/cfx_host.h"
 
+#include "../../include/cfx.h"
+#include "cf/DetailBlock.h"
 #include "cf/PolyMesh.h"
+
+#include <memory>
 
 #include <cstring>
 #include <string>
@@ -9,6 +13,11 @@
 struct cfx_hmesh
 {
     cf::polyMesh mesh;
+};
+
+struct cfx_detail
+{
+    std::unique_ptr<cf::DetailBlock> blk;
 };
 
 namespace
@@ -65,6 +74,109 @@ int cfx_hmesh_box(int64_t nx, int64_t ny, int64_t nz, const double mn[3], const 
         *out = h;
     });
 }
+
+int cfx_hmesh_channel(int64_t nx, int64_t ny, int64_t nz, const double mn[3], const double mx[3], double jitter,
+                      uint64_t seed, cfx_hmesh** out)
+{
+    return guardHost([&] {
+        if (!out || !mn || !mx) throw std::invalid_argument("null argument");
+        const cf::Vec3 a{mn[0], mn[1], mn[2]}, b{mx[0], mx[1], mx[2]};
+        cf::BlockSides sides;
+        sides.xmin = {"inlet", "inlet"};
+        sides.xmax = {"outlet", "outlet"};
+        sides.ymin = {"ymin", "wall"};
+        sides.ymax = {"ymax", "wall"};
+        sides.zmin = {"zmin", "wall"};
+        sides.zmax = {"zmax", "wall"};
+        auto* h = new cfx_hmesh;
+        try
+        {
+            h->mesh = cf::BlockMesh::create(a, b, nx, ny, nz, sides, jitter, seed);
+        }
+        catch (...)
+        {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int cfx_detail_create(cfx_ctx* ctx, const cfx_hmesh* mesh, double nu, double rho, const char* shape,
+                      cfx_detail** out)
+{
+    return guardHost([&] {
+        if (!ctx || !mesh || !shape || !out) throw std::invalid_argument("null argument");
+        auto* d = new cfx_detail;
+        try
+        {
+            d->blk = std::make_unique<cf::DetailBlock>(ctx, mesh->mesh, nu, rho, shape);
+        }
+        catch (...)
+        {
+            delete d;
+            throw;
+        }
+        *out = d;
+    });
+}
+
+int cfx_detail_destroy(cfx_detail* d)
+{
+    delete d;
+    return 0;
+}
+
+int cfx_detail_loss_at(cfx_detail* d, double q, const cfx_simple_controls* ctl, cfx_detail_measure* m)
+{
+    return guardHost([&] {
+        if (!d || !ctl) throw std::invalid_argument("null argument");
+        d->blk->lossAt(q, *ctl);
+        if (m)
+        {
+            const cf::BlockMeasure& b = d->blk->lastMeasure();
+            *m = cfx_detail_measure{b.cells, b.sweeps, b.converged, b.imposedQ, b.inletArea, b.plugSpeed,
+                                    b.dpPa, b.fluxIn, b.fluxOut};
+        }
+    });
+}
+
+int cfx_detail_info(const cfx_detail* d, int* axis, int* flowSign, double* inletArea, int64_t* nSections,
+                    int64_t* nInletFaces)
+{
+    return guardHost([&] {
+        if (!d) throw std::invalid_argument("null argument");
+        if (axis) *axis = d->blk->axis();
+        if (flowSign) *flowSign = d->blk->flowSign();
+        if (inletArea) *inletArea = d->blk->inletArea();
+        if (nSections) *nSections = static_cast<int64_t>(d->blk->nSections());
+        if (nInletFaces) *nInletFaces = static_cast<int64_t>(d->blk->inletFaces().size());
+    });
+}
+
+int cfx_detail_inlet(const cfx_detail* d, int64_t* faces, double* weights)
+{
+    return guardHost([&] {
+        if (!d) throw std::invalid_argument("null argument");
+        const auto& f = d->blk->inletFaces();
+        const auto& w = d->blk->inletWeights();
+        for (std::size_t j = 0; j < f.size(); ++j)
+        {
+            if (faces) faces[j] = f[j];
+            if (weights) weights[j] = w[j];
+        }
+    });
+}
+
+int cfx_detail_fields(const cfx_detail* d, double* U, double* p, double* phi)
+{
+    return guardHost([&] {
+        if (!d) throw std::invalid_argument("null argument");
+        d->blk->fields(U, p, phi);
+    });
+}
+
+cfx_mesh* cfx_detail_mesh(const cfx_detail* d) { return d ? d->blk->deviceMesh() : nullptr; }
 
 int cfx_hmesh_obstacle_channel(int64_t nx, int64_t ny, int64_t nz, const double mn[3], const double mx[3],
                                double jitter, uint64_t seed, double cx, double cy, double radius, cfx_hmesh** out)

@@ -21,6 +21,7 @@
 // FP64 and replaces the FP32 residual, and the iteration resumes with the
 // same search direction.  Convergence is only ever declared on a true FP64
 // residual.
+#include "batch.h"
 #include "krylov.h"
 #include "mg.h"
 
@@ -96,15 +97,15 @@ __device__ __forceinline__ double allMaxNan(const double* __restrict__ p, int G,
 // r = b - A x (FP64); rT = (T) r; partials: max|r| (NaN -> -1), max|b|,
 // rT.(invD rT).  Also used after a reliable update.
 template <typename T>
-__global__ __launch_bounds__(kB) void k_pcg_residual(CsrView A, int rbPerWg, const double* __restrict__ val,
+__device__ __forceinline__ void body_pcg_residual(CsrView A, int rbPerWg, const double* __restrict__ val,
                                                      const double* __restrict__ x, const double* __restrict__ b,
                                                      double* __restrict__ r64, T* __restrict__ rT,
                                                      const T* __restrict__ invD, double* __restrict__ partRZ,
-                                                     double* __restrict__ partRM, double* __restrict__ partBM)
+                                                     double* __restrict__ partRM, double* __restrict__ partBM, int bid, int G)
 {
     __shared__ double lds[kSpmvLds];
     __shared__ double red[kB / kWave];
-    const int w = xcdRemap(blockIdx.x, gridDim.x);
+    const int w = xcdRemap(bid, G);
     double rm = 0, bm = 0, rz = 0;
     int nan = 0;
     for (int j = 0; j < rbPerWg; ++j)
@@ -148,22 +149,31 @@ __global__ __launch_bounds__(kB) void k_pcg_residual(CsrView A, int rbPerWg, con
     nan = blockOr(nan, ired);
     if (threadIdx.x == 0)
     {
-        partRZ[blockIdx.x] = rz;
-        partRM[blockIdx.x] = (nan & 1) ? -1.0 : rm;
-        partBM[blockIdx.x] = (nan & 2) ? -1.0 : bm;
+        partRZ[bid] = rz;
+        partRM[bid] = (nan & 1) ? -1.0 : rm;
+        partBM[bid] = (nan & 2) ? -1.0 : bm;
     }
+}
+template <typename T>
+__global__ __launch_bounds__(kB) void k_pcg_residual(CsrView A, int rbPerWg, const double* __restrict__ val,
+                                                     const double* __restrict__ x, const double* __restrict__ b,
+                                                     double* __restrict__ r64, T* __restrict__ rT,
+                                                     const T* __restrict__ invD, double* __restrict__ partRZ,
+                                                     double* __restrict__ partRM, double* __restrict__ partBM)
+{
+    body_pcg_residual<T>(A, rbPerWg, val, x, b, r64, rT, invD, partRZ, partRM, partBM, blockIdx.x, gridDim.x);
 }
 
 // A(k): q = A p (CSR-stream, LDS-staged), partial p.q.
 template <typename T>
-__global__ __launch_bounds__(kB) void k_pcg_A(CsrView A, int rbPerWg, const T* __restrict__ val,
+__device__ __forceinline__ void body_pcg_A(CsrView A, int rbPerWg, const T* __restrict__ val,
                                               const T* __restrict__ p, T* __restrict__ q, const CgState* st,
-                                              double* __restrict__ partPQ)
+                                              double* __restrict__ partPQ, int bid, int G)
 {
     __shared__ T lds[kSpmvLds];
     __shared__ double red[kB / kWave];
     if (st->done) return;
-    const int w = xcdRemap(blockIdx.x, gridDim.x);
+    const int w = xcdRemap(bid, G);
     double pq = 0;
     for (int j = 0; j < rbPerWg; ++j)
     {
@@ -192,17 +202,24 @@ __global__ __launch_bounds__(kB) void k_pcg_A(CsrView A, int rbPerWg, const T* _
         }
     }
     pq = blockSum(pq, red);
-    if (threadIdx.x == 0) partPQ[blockIdx.x] = pq;
+    if (threadIdx.x == 0) partPQ[bid] = pq;
+}
+template <typename T>
+__global__ __launch_bounds__(kB) void k_pcg_A(CsrView A, int rbPerWg, const T* __restrict__ val,
+                                              const T* __restrict__ p, T* __restrict__ q, const CgState* st,
+                                              double* __restrict__ partPQ)
+{
+    body_pcg_A<T>(A, rbPerWg, val, p, q, st, partPQ, blockIdx.x, gridDim.x);
 }
 
 // B(k): reduce p.q, alpha = rz / p.q; x += alpha p, r -= alpha q;
 // partials r.(invD r), max|r|   (solveCG :189-211)
 template <typename T>
-__global__ __launch_bounds__(kB) void k_pcg_B(int n, const T* __restrict__ invD, T* __restrict__ x,
+__device__ __forceinline__ void body_pcg_B(int n, const T* __restrict__ invD, T* __restrict__ x,
                                               T* __restrict__ r, const T* __restrict__ p,
                                               const T* __restrict__ q, CgState* st,
                                               const double* __restrict__ partPQ, int Gs, double* __restrict__ partRZ,
-                                              double* __restrict__ partRM, int k)
+                                              double* __restrict__ partRM, int k, int bid, int G)
 {
     __shared__ double red[kB / kWave];
     __shared__ int ired[kB / kWave];
@@ -212,15 +229,15 @@ __global__ __launch_bounds__(kB) void k_pcg_B(int n, const T* __restrict__ invD,
     if (PAP <= 0 || fabs(rz) < 1e-300)
     {
         // matrix not sufficiently SPD (solveCG breaks here, Matrix.cpp:191-194)
-        if (blockIdx.x == 0 && threadIdx.x == 0) { st->breakdown = 1; st->done = 1; st->pAp = PAP; }
+        if (bid == 0 && threadIdx.x == 0) { st->breakdown = 1; st->done = 1; st->pAp = PAP; }
         return;
     }
     const double alphaD = rz / PAP;
-    if (blockIdx.x == 0 && threadIdx.x == 0) { st->alpha = alphaD; st->pAp = PAP; }
+    if (bid == 0 && threadIdx.x == 0) { st->alpha = alphaD; st->pAp = PAP; }
     const T alpha = static_cast<T>(alphaD);
     double rzp = 0, rm = 0;
     int nan = 0;
-    for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB)
+    for (int i = bid * kB + threadIdx.x; i < n; i += G * kB)
     {
         x[i] += alpha * p[i];
         const T ri = r[i] - alpha * q[i];
@@ -235,9 +252,18 @@ __global__ __launch_bounds__(kB) void k_pcg_B(int n, const T* __restrict__ invD,
     nan = blockOr(nan, ired);
     if (threadIdx.x == 0)
     {
-        partRZ[blockIdx.x] = rzp;
-        partRM[blockIdx.x] = nan ? -1.0 : rm;
+        partRZ[bid] = rzp;
+        partRM[bid] = nan ? -1.0 : rm;
     }
+}
+template <typename T>
+__global__ __launch_bounds__(kB) void k_pcg_B(int n, const T* __restrict__ invD, T* __restrict__ x,
+                                              T* __restrict__ r, const T* __restrict__ p,
+                                              const T* __restrict__ q, CgState* st,
+                                              const double* __restrict__ partPQ, int Gs, double* __restrict__ partRZ,
+                                              double* __restrict__ partRM, int k)
+{
+    body_pcg_B<T>(n, invD, x, r, p, q, st, partPQ, Gs, partRZ, partRM, k, blockIdx.x, gridDim.x);
 }
 
 // C(k): stop decision on the residual of iteration k (k = 0: the initial
@@ -245,15 +271,15 @@ __global__ __launch_bounds__(kB) void k_pcg_B(int n, const T* __restrict__ invD,
 // flags bit0: the partials hold a TRUE residual (after a residual kernel);
 //       bit1: mixed precision (pause instead of converging on sloppy residuals)
 template <typename T>
-__global__ __launch_bounds__(kB) void k_pcg_C(int n, const T* __restrict__ invD, const T* __restrict__ r,
+__device__ __forceinline__ void body_pcg_C(int n, const T* __restrict__ invD, const T* __restrict__ r,
                                               T* __restrict__ p, CgState* st, const double* __restrict__ partRZ,
                                               const double* __restrict__ partRM, const double* __restrict__ partBM,
                                               int nPart, int k, double tol, int maxIter, double delta, int flags,
-                                              double* resHist)
+                                              double* resHist, int bid, int G)
 {
     __shared__ double red[kB / kWave];
     if (st->done) return;
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const bool lead = bid == 0 && threadIdx.x == 0;
     const double RZ = allSum(partRZ, nPart, red);
     const double RM = allMaxNan(partRM, nPart, red);
     const bool trueRes = flags & 1, mixed = flags & 2;
@@ -314,11 +340,20 @@ __global__ __launch_bounds__(kB) void k_pcg_C(int n, const T* __restrict__ invD,
         st->beta = beta;
     }
     const T tb = static_cast<T>(beta);
-    for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB)
+    for (int i = bid * kB + threadIdx.x; i < n; i += G * kB)
     {
         const T z = invD[i] * r[i];
         p[i] = k == 0 ? z : z + tb * p[i];
     }
+}
+template <typename T>
+__global__ __launch_bounds__(kB) void k_pcg_C(int n, const T* __restrict__ invD, const T* __restrict__ r,
+                                              T* __restrict__ p, CgState* st, const double* __restrict__ partRZ,
+                                              const double* __restrict__ partRM, const double* __restrict__ partBM,
+                                              int nPart, int k, double tol, int maxIter, double delta, int flags,
+                                              double* resHist)
+{
+    body_pcg_C<T>(n, invD, r, p, st, partRZ, partRM, partBM, nPart, k, tol, maxIter, delta, flags, resHist, blockIdx.x, gridDim.x);
 }
 
 // reliable update: x64 += x32, x32 = 0
@@ -531,6 +566,69 @@ __global__ __launch_bounds__(kB) void k_fcg_C(int n, const float* __restrict__ z
     const float tb = static_cast<float>(beta);
     for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB)
         p[i] = k == 0 ? z[i] : z[i] + tb * p[i];
+}
+
+
+// ------------------------------------------------------ batched Jacobi-PCG
+// Per system: its matrix, vectors, state and partial arrays; workgroup
+// tables map a launch's workgroup to (system, local index) for the SpMV
+// grid (Gs per system) and for the vector grid (Gv per system).
+struct BatchSysDev
+{
+    CsrView A;
+    const double* val;
+    const double* invD;
+    const double* b;
+    double *x, *r, *p, *q;
+    CgState* st;
+    double* part;  // RZ, RM, BM, PQ: kRedMax each
+    int n, Gs, rbPerWg, Gv;
+};
+
+__global__ __launch_bounds__(kB) void k_bpcg_residual(const BatchSysDev* __restrict__ S, const int2* __restrict__ tab)
+{
+    const int2 t = tab[blockIdx.x];
+    const BatchSysDev s = S[t.x];
+    body_pcg_residual<double>(s.A, s.rbPerWg, s.val, s.x, s.b, nullptr, s.r, s.invD, s.part, s.part + kRedMax,
+                              s.part + 2 * kRedMax, t.y, s.Gs);
+}
+
+__global__ __launch_bounds__(kB) void k_bpcg_C0(const BatchSysDev* __restrict__ S, const int2* __restrict__ tab,
+                                                double tol, int maxIter)
+{
+    const int2 t = tab[blockIdx.x];
+    const BatchSysDev s = S[t.x];
+    body_pcg_C<double>(s.n, s.invD, s.r, s.p, s.st, s.part, s.part + kRedMax, s.part + 2 * kRedMax, s.Gs, 0, tol,
+                       maxIter, 0.0, 0, nullptr, t.y, s.Gv);
+}
+
+__global__ __launch_bounds__(kB) void k_bpcg_A(const BatchSysDev* __restrict__ S, const int2* __restrict__ tab)
+{
+    const int2 t = tab[blockIdx.x];
+    const BatchSysDev s = S[t.x];
+    body_pcg_A<double>(s.A, s.rbPerWg, s.val, s.p, s.q, s.st, s.part + 3 * kRedMax, t.y, s.Gs);
+}
+
+// iteration index from the system's state: C(k-1) recorded iter = k-1
+__global__ __launch_bounds__(kB) void k_bpcg_B(const BatchSysDev* __restrict__ S, const int2* __restrict__ tab)
+{
+    const int2 t = tab[blockIdx.x];
+    const BatchSysDev s = S[t.x];
+    if (s.st->done) return;
+    const int k = s.st->iter + 1;
+    if (t.y == 0 && threadIdx.x == 0) s.st->kcur = k;
+    body_pcg_B<double>(s.n, s.invD, s.x, s.r, s.p, s.q, s.st, s.part + 3 * kRedMax, s.Gs, s.part, s.part + kRedMax, k,
+                       t.y, s.Gv);
+}
+
+__global__ __launch_bounds__(kB) void k_bpcg_C(const BatchSysDev* __restrict__ S, const int2* __restrict__ tab,
+                                               double tol, int maxIter)
+{
+    const int2 t = tab[blockIdx.x];
+    const BatchSysDev s = S[t.x];
+    if (s.st->done) return;
+    body_pcg_C<double>(s.n, s.invD, s.r, s.p, s.st, s.part, s.part + kRedMax, s.part + 2 * kRedMax, s.Gv,
+                       s.st->kcur, tol, maxIter, 0.0, 0, nullptr, t.y, s.Gv);
 }
 
 } // namespace
@@ -830,6 +928,200 @@ SolveReport Krylov::fcgMixed(const CsrView& A, const double* val, Preconditioner
     CFX_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
     lastMs_ = rep.ms = ms;
     return rep;
+}
+
+// ------------------------------------------------------ batched Jacobi-PCG
+BatchedPcg::BatchedPcg(Context& ctx) : ctx_(ctx)
+{
+    CFX_HIP(hipEventCreate(&e0_));
+    CFX_HIP(hipEventCreate(&e1_));
+}
+
+BatchedPcg::~BatchedPcg()
+{
+    if (e0_) (void)hipEventDestroy(e0_);
+    if (e1_) (void)hipEventDestroy(e1_);
+}
+
+void BatchedPcg::reserve(std::size_t n, std::size_t K)
+{
+    if (n > capN_)
+    {
+        for (DevBuf* b : {&r_, &p_, &q_, &invD_, &xs_}) b->alloc(sizeof(double) * n);
+        capN_ = n;
+    }
+    if (K > capK_)
+    {
+        part_.alloc(sizeof(double) * 4 * kRedMax * K);
+        state_.alloc(sizeof(CgState) * K);
+        sys_.alloc(sizeof(BatchSysDev) * K);
+        capK_ = K;
+    }
+}
+
+BatchResult BatchedPcg::solve(const std::vector<BatchSystem>& sys, double tol, int maxIter, const BatchSchedule& sc)
+{
+    hipStream_t s = ctx_.stream();
+    const std::size_t K = sys.size();
+    if (K == 0) throw std::invalid_argument("batch: no systems");
+    if (sc.vecElems < 1 || sc.wgCap < 1 || sc.wgCap > kRedMax || sc.pollEvery < 1)
+        throw std::invalid_argument("batch: bad schedule");
+    std::size_t N = 0;
+    for (const BatchSystem& b : sys) N += static_cast<std::size_t>(b.A.n);
+    reserve(N, K);
+    std::vector<BatchSysDev> hs(K);
+    std::vector<int2> tabS, tabV;
+    std::size_t off = 0;
+    CgState* st = state_.as<CgState>();
+    for (std::size_t k = 0; k < K; ++k)
+    {
+        const BatchSystem& b = sys[k];
+        BatchSysDev& d = hs[k];
+        d.A = b.A;
+        d.val = b.val;
+        d.b = b.b;
+        d.x = b.x;
+        d.invD = invD_.as<double>() + off;
+        d.r = r_.as<double>() + off;
+        d.p = p_.as<double>() + off;
+        d.q = q_.as<double>() + off;
+        d.st = st + k;
+        d.part = part_.as<double>() + 4 * kRedMax * k;
+        d.n = b.A.n;
+        int Gs = std::max(1, std::min(b.A.nRowBlocks, sc.wgCap));
+        d.rbPerWg = divUp(b.A.nRowBlocks, Gs);
+        d.Gs = std::max(1, divUp(b.A.nRowBlocks, d.rbPerWg));
+        d.Gv = std::max(1, std::min(kRedMax, divUp(b.A.n, sc.vecElems * kB)));
+        for (int j = 0; j < d.Gs; ++j) tabS.push_back(make_int2(static_cast<int>(k), j));
+        for (int j = 0; j < d.Gv; ++j) tabV.push_back(make_int2(static_cast<int>(k), j));
+        computeInvD(ctx_, b.A, b.val, const_cast<double*>(d.invD), false);
+        off += static_cast<std::size_t>(b.A.n);
+    }
+    CFX_HIP(hipMemcpyAsync(sys_.p, hs.data(), sizeof(BatchSysDev) * K, hipMemcpyHostToDevice, s));
+    tabS_ = upload(tabS, s);
+    tabV_ = upload(tabV, s);
+    CFX_HIP(hipMemsetAsync(st, 0, sizeof(CgState) * K, s));
+    const BatchSysDev* S = sys_.as<BatchSysDev>();
+    const int GS = static_cast<int>(tabS.size()), GV = static_cast<int>(tabV.size());
+    const int2* TS = tabS_.as<int2>();
+    const int2* TV = tabV_.as<int2>();
+    std::vector<CgState> h(K);
+    BatchResult res;
+    res.finishMs.assign(K, -1.0);
+    CFX_HIP(hipEventRecord(e0_, s));
+    k_bpcg_residual<<<GS, kB, 0, s>>>(S, TS);
+    k_bpcg_C0<<<GV, kB, 0, s>>>(S, TV, tol, maxIter);
+    CFX_CHECK_LAUNCH();
+    hipGraphExec_t exec = nullptr;
+    if (sc.graph)
+    {
+        hipGraph_t g;
+        CFX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        k_bpcg_A<<<GS, kB, 0, s>>>(S, TS);
+        k_bpcg_B<<<GV, kB, 0, s>>>(S, TV);
+        k_bpcg_C<<<GV, kB, 0, s>>>(S, TV, tol, maxIter);
+        CFX_HIP(hipStreamEndCapture(s, &g));
+        CFX_HIP(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+        CFX_HIP(hipGraphDestroy(g));
+    }
+    int k = 0;
+    std::size_t left = K;
+    while (left > 0 && k < maxIter)
+    {
+        const int batch = std::min(sc.pollEvery, maxIter - k);
+        for (int j = 0; j < batch; ++j)
+        {
+            if (exec)
+                CFX_HIP(hipGraphLaunch(exec, s));
+            else
+            {
+                k_bpcg_A<<<GS, kB, 0, s>>>(S, TS);
+                k_bpcg_B<<<GV, kB, 0, s>>>(S, TV);
+                k_bpcg_C<<<GV, kB, 0, s>>>(S, TV, tol, maxIter);
+            }
+        }
+        k += batch;
+        CFX_CHECK_LAUNCH();
+        CFX_HIP(hipMemcpyAsync(h.data(), st, sizeof(CgState) * K, hipMemcpyDeviceToHost, s));
+        CFX_HIP(hipEventRecord(e1_, s));
+        CFX_HIP(hipEventSynchronize(e1_));
+        float ms = 0;
+        CFX_HIP(hipEventElapsedTime(&ms, e0_, e1_));
+        left = 0;
+        for (std::size_t q = 0; q < K; ++q)
+        {
+            if (h[q].done && res.finishMs[q] < 0) res.finishMs[q] = ms;
+            if (!h[q].done) ++left;
+        }
+    }
+    if (exec) CFX_HIP(hipGraphExecDestroy(exec));
+    CFX_HIP(hipMemcpyAsync(h.data(), st, sizeof(CgState) * K, hipMemcpyDeviceToHost, s));
+    CFX_HIP(hipEventRecord(e1_, s));
+    CFX_HIP(hipEventSynchronize(e1_));
+    float ms = 0;
+    CFX_HIP(hipEventElapsedTime(&ms, e0_, e1_));
+    res.ms = ms;
+    res.reports.resize(K);
+    for (std::size_t q = 0; q < K; ++q)
+    {
+        SolveReport& r = res.reports[q];
+        r.iterations = h[q].iter;
+        r.initialResidual = h[q].initialResidual;
+        r.finalResidual = h[q].iter > 0 ? h[q].rnorm : h[q].initialResidual;
+        r.converged = h[q].converged;
+        r.breakdown = h[q].breakdown;
+        if (res.finishMs[q] < 0) res.finishMs[q] = ms;
+        r.ms = res.finishMs[q];
+        res.maxIterations = std::max(res.maxIterations, r.iterations);
+    }
+    return res;
+}
+
+BatchSchedule BatchedPcg::autotune(const std::vector<BatchSystem>& sys, int iters,
+                                   std::vector<std::pair<BatchSchedule, double>>* log)
+{
+    std::vector<int> key;
+    for (const BatchSystem& b : sys) key.push_back(b.A.n);
+    key.push_back(iters);
+    auto it = tuned_.find(key);
+    if (it != tuned_.end() && !log) return it->second;
+    hipStream_t s = ctx_.stream();
+    std::size_t N = 0;
+    for (const BatchSystem& b : sys) N += static_cast<std::size_t>(b.A.n);
+    reserve(N, sys.size());
+    // time on scratch copies of x so the caller's initial guesses are kept
+    std::vector<BatchSystem> tmp = sys;
+    std::size_t off = 0;
+    for (BatchSystem& b : tmp)
+    {
+        b.x = xs_.as<double>() + off;
+        off += static_cast<std::size_t>(b.A.n);
+    }
+    BatchSchedule best;
+    double bestMs = 1e300;
+    for (int graph : {1, 0})
+        for (int ve : {2, 4, 8})
+            for (int cap : {512, 1024, 2048})
+            {
+                BatchSchedule c;
+                c.vecElems = ve;
+                c.wgCap = cap;
+                c.graph = graph;
+                c.pollEvery = iters;
+                for (std::size_t q = 0; q < sys.size(); ++q)
+                    CFX_HIP(hipMemcpyAsync(tmp[q].x, sys[q].x, sizeof(double) * sys[q].A.n, hipMemcpyDeviceToDevice, s));
+                // tol 0: no system stops early, every candidate runs `iters` iterations
+                const BatchResult r = solve(tmp, 0.0, iters, c);
+                if (log) log->push_back({c, r.ms});
+                if (r.ms < bestMs)
+                {
+                    bestMs = r.ms;
+                    best = c;
+                }
+            }
+    best.pollEvery = 16;
+    tuned_[key] = best;
+    return best;
 }
 
 template SolveReport Krylov::pcgRun<double>(const CsrView&, const double*, const double*, const double*,

@@ -87,6 +87,17 @@ class TrainOptions(C.Structure):
                 ("seed", C.c_uint64)]
 
 
+class BatchSchedule(C.Structure):
+    _fields_ = [("vecElems", C.c_int), ("wgCap", C.c_int), ("graph", C.c_int), ("pollEvery", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("ms", C.c_double), ("maxIterations", C.c_int), ("used", BatchSchedule)]
+
+
 class CfxError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"cfx error {code}: {msg}")
@@ -152,6 +163,9 @@ _sig("cfx_csr_residual_history", C.c_int, _vp, _vp, C.c_int)
 _sig("cfx_csr_set_inner_tol", C.c_int, _vp, C.c_double)
 _sig("cfx_mg_default_params", C.c_int, C.POINTER(MgParams))
 _sig("cfx_set_graphs", C.c_int, _vp, C.c_int)
+_sig("cfx_batch_solve", C.c_int, _vp, C.c_int, _vp, _vp, _vp, C.c_double, C.c_int, _vp, _vp, _vp,
+     C.POINTER(BatchStats))
+_sig("cfx_batch_autotune", C.c_int, _vp, C.c_int, _vp, _vp, C.c_int, C.POINTER(BatchSchedule), _vp, _vp, C.c_int)
 _sig("cfx_mpnn_num_params", C.c_int)
 _sig("cfx_mpnn_create", C.c_int, _vp, C.c_uint64, C.c_int, C.POINTER(_vp))
 _sig("cfx_mpnn_destroy", C.c_int, _vp)
@@ -187,6 +201,15 @@ _sig("cfx_csr_time_spmv", C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, _dp)
 _sig("cfx_matrix_time_spmv", C.c_int, _vp, C.c_int, _vp, _vp, C.c_int, C.c_int, _dp)
 # host mesh
 _sig("cfx_hmesh_box", C.c_int, _i64, _i64, _i64, _vp, _vp, C.c_double, C.c_uint64, C.c_int, C.POINTER(_vp))
+_sig("cfx_hmesh_channel", C.c_int, _i64, _i64, _i64, _vp, _vp, C.c_double, C.c_uint64, C.POINTER(_vp))
+_sig("cfx_detail_create", C.c_int, _vp, _vp, C.c_double, C.c_double, C.c_char_p, C.POINTER(_vp))
+_sig("cfx_detail_destroy", C.c_int, _vp)
+_sig("cfx_detail_loss_at", C.c_int, _vp, C.c_double, _vp, _vp)
+_sig("cfx_detail_info", C.c_int, _vp, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp, C.POINTER(_i64),
+     C.POINTER(_i64))
+_sig("cfx_detail_inlet", C.c_int, _vp, _vp, _vp)
+_sig("cfx_detail_fields", C.c_int, _vp, _vp, _vp, _vp)
+_sig("cfx_detail_mesh", _vp, _vp)
 _sig("cfx_hmesh_obstacle_channel", C.c_int, _i64, _i64, _i64, _vp, _vp, C.c_double, C.c_uint64, C.c_double,
      C.c_double, C.c_double, C.POINTER(_vp))
 _sig("cfx_hmesh_from_arrays", C.c_int, _i64, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp,
@@ -339,6 +362,14 @@ class HostMesh:
         return cls(h)
 
     @classmethod
+    def channel(cls, nx, ny, nz, mn, mx, jitter=0.0, seed=0):
+        """Channel block with patches inlet (x min), outlet (x max) and four walls."""
+        h = _vp()
+        _host_check(_L.cfx_hmesh_channel(nx, ny, nz, _p(np.asarray(mn, np.float64)), _p(np.asarray(mx, np.float64)),
+                                         jitter, seed, C.byref(h)))
+        return cls(h)
+
+    @classmethod
     def obstacle_channel(cls, nx, ny, nz, mn, mx, jitter, seed, cx, cy, radius):
         """Channel box minus a z-parallel cylinder of cells (7th patch "obstacle")."""
         h = _vp()
@@ -430,6 +461,15 @@ class Mesh:
         self.h = h
         self.V = g["V"]
 
+    @classmethod
+    def borrowed(cls, ctx, handle, n_cells, n_faces, n_internal):
+        """A view of a device mesh owned elsewhere (not destroyed by this object)."""
+        m = cls.__new__(cls)
+        m.ctx, m.h, m._borrowed = ctx, handle, True
+        m.nCells, m.nFaces, m.nInternal = n_cells, n_faces, n_internal
+        m.nB = n_faces - n_internal
+        return m
+
     def set_mg(self, **kw):
         """Multigrid constants of the drivers' CFX_FP32_MG pressure solves."""
         p = mg_default_params()
@@ -460,7 +500,7 @@ class Mesh:
 
     def __del__(self):
         try:
-            if self.h and self.ctx.h:
+            if self.h and self.ctx.h and not getattr(self, "_borrowed", False):
                 _L.cfx_mesh_destroy(self.h)
         except Exception:
             pass
@@ -814,3 +854,90 @@ class Mpnn:
                                             _p(r0) if r0 is not None else None, C.byref(o), samples,
                                             1 if with_network else 0, C.byref(loss)))
         return loss.value
+
+
+def _ptr_array(items):
+    arr = (_vp * len(items))(*[_p(a) if not isinstance(a, _vp) else a for a in items])
+    return arr
+
+
+def batch_solve(ctx, systems, bs, xs, tol=1e-8, maxIter=5000, schedule=None):
+    """Jacobi-PCG on many CSR systems in one batched launch sequence.
+    Returns (reports, finishMs, stats)."""
+    K = len(systems)
+    S = (_vp * K)(*[s.h for s in systems])
+    B = _ptr_array(bs)
+    X = _ptr_array(xs)
+    reps = (Report * K)()
+    fin = np.zeros(K)
+    st = BatchStats()
+    sched = C.byref(schedule) if schedule is not None else None
+    ctx.check(_L.cfx_batch_solve(ctx.h, K, C.cast(S, _vp), C.cast(B, _vp), C.cast(X, _vp), tol, maxIter, sched,
+                                 C.cast(reps, _vp), _p(fin), C.byref(st)))
+    return list(reps), fin, st
+
+
+def batch_autotune(ctx, systems, xs, iters=20):
+    K = len(systems)
+    S = (_vp * K)(*[s.h for s in systems])
+    X = _ptr_array(xs)
+    best = BatchSchedule()
+    cand = (BatchSchedule * 32)()
+    ms = np.zeros(32)
+    ctx.check(_L.cfx_batch_autotune(ctx.h, K, C.cast(S, _vp), C.cast(X, _vp), iters, C.byref(best),
+                                    C.cast(cand, _vp), _p(ms), 32))
+    n = int((ms > 0).sum())
+    return best, [(cand[i].as_dict(), ms[i]) for i in range(n)]
+
+
+class DetailMeasure(C.Structure):
+    _fields_ = [("cells", C.c_int64), ("sweeps", C.c_int), ("converged", C.c_int), ("imposedQ", C.c_double),
+                ("inletArea", C.c_double), ("plugSpeed", C.c_double), ("dpPa", C.c_double), ("fluxIn", C.c_double),
+                ("fluxOut", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Detail:
+    """A 3D detail region (DetailBlock): inlet profile, GPU SIMPLE, section-mean pressure drop."""
+
+    def __init__(self, ctx: Context, hm: HostMesh, nu=1e-3, rho=1.0, shape="flat"):
+        self.ctx, self.hm = ctx, hm
+        h = _vp()
+        _host_check(_L.cfx_detail_create(ctx.h, hm.h, nu, rho, shape.encode(), C.byref(h)))
+        self.h = h
+        a = hm.arrays()
+        self.nCells, self.nFaces, self.nInternal = int(a["nCells"]), len(a["owner"]), len(a["neighbour"])
+
+    def __del__(self):
+        try:
+            if self.h:
+                _L.cfx_detail_destroy(self.h)
+        except Exception:
+            pass
+
+    def loss_at(self, q, ctl: "SimpleControls"):
+        m = DetailMeasure()
+        _host_check(_L.cfx_detail_loss_at(self.h, q, C.byref(ctl), C.byref(m)))
+        return m.dpPa, m.as_dict()
+
+    def info(self):
+        ax, fs, area, ns, ni = C.c_int(0), C.c_int(0), C.c_double(0), C.c_int64(0), C.c_int64(0)
+        _host_check(_L.cfx_detail_info(self.h, C.byref(ax), C.byref(fs), C.byref(area), C.byref(ns), C.byref(ni)))
+        return {"axis": ax.value, "flowSign": fs.value, "inletArea": area.value, "sections": ns.value,
+                "inletFaces": ni.value}
+
+    def inlet(self):
+        k = self.info()["inletFaces"]
+        f, w = np.zeros(k, np.int64), np.zeros(k)
+        _host_check(_L.cfx_detail_inlet(self.h, _p(f), _p(w)))
+        return f, w
+
+    def fields(self):
+        U, p, phi = np.zeros(3 * self.nCells), np.zeros(self.nCells), np.zeros(self.nFaces)
+        _host_check(_L.cfx_detail_fields(self.h, _p(U), _p(p), _p(phi)))
+        return U.reshape(3, self.nCells), p, phi
+
+    def mesh(self):
+        return Mesh.borrowed(self.ctx, _L.cfx_detail_mesh(self.h), self.nCells, self.nFaces, self.nInternal)

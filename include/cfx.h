@@ -294,6 +294,37 @@ int cfx_csr_set_mpnn(cfx_csr* A, cfx_mpnn* net, const double* d_geo);
 int cfx_matrix_set_mpnn(cfx_matrix* A, cfx_mpnn* net);
 int cfx_mesh_set_mpnn(cfx_mesh* m, cfx_mpnn* net);
 
+/* ----------------------------------------------------- batched solves */
+/* K independent systems (the pressure systems of many detail regions)
+ * solved by Jacobi-PCG together: each of the three PCG kernels is one
+ * launch over all systems, every system keeps its own scalars and stop
+ * test (solveCG's, Matrix.cpp:153-222), and the iteration is replayed as a
+ * captured hipGraph.  With the default schedule {4, 2048, 1, 16} each
+ * system's result is bit for bit that of cfx_csr_solve_device (FP64
+ * Jacobi-PCG); sched = null picks the schedule with the autotuner.  The
+ * reference solves its detail blocks one after another
+ * (coupledNetwork.cpp:1461-1475); it has no batched solver. */
+typedef struct cfx_batch_schedule {
+    int vecElems;   /* elements per thread of the vector kernels */
+    int wgCap;      /* SpMV workgroups per system at most (<= 2048) */
+    int graph;      /* 1: replay the captured iteration */
+    int pollEvery;  /* iterations between host polls */
+} cfx_batch_schedule;
+typedef struct cfx_batch_stats {
+    double ms;              /* device time of the whole batch */
+    int maxIterations;
+    cfx_batch_schedule used;
+} cfx_batch_stats;
+/* d_b[k], d_x[k]: device FP64 vectors of system k (x = initial guess and result);
+ * reps[k] and finishMs[k] (device time at which system k's stop was seen) may be null */
+int cfx_batch_solve(cfx_ctx* ctx, int count, cfx_csr* const* systems, const double* const* d_b,
+                    double* const* d_x, double tol, int maxIter, const cfx_batch_schedule* sched,
+                    cfx_report* reps, double* finishMs, cfx_batch_stats* stats);
+/* times the candidate schedules for `iters` iterations each (the x are restored)
+ * and returns the fastest; cand/candMs (may be null) receive up to maxCand timings */
+int cfx_batch_autotune(cfx_ctx* ctx, int count, cfx_csr* const* systems, double* const* d_x, int iters,
+                       cfx_batch_schedule* best, cfx_batch_schedule* cand, double* candMs, int maxCand);
+
 /* ------------------------------------------------------------ drivers */
 /* Schemes (src/3d/include/cf/fvm/FvMatrix.h:30-38): div 0 upwind, 1 linear,
  * 2 blend; laplacian "corrected" (1) or "none" (0); grad "corrected" (1)

@@ -11,6 +11,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "cfx.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -33,6 +35,40 @@ int cfx_hmesh_box(int64_t nx, int64_t ny, int64_t nz, const double mn[3], const 
  * No counterpart in the reference (its blockMesh makes boxes only). */
 int cfx_hmesh_obstacle_channel(int64_t nx, int64_t ny, int64_t nz, const double mn[3], const double mx[3],
                                double jitter, uint64_t seed, double cx, double cy, double radius, cfx_hmesh** out);
+/* channel block for detail regions: patches "inlet" (x min, type inlet),
+ * "outlet" (x max, type outlet) and four walls */
+int cfx_hmesh_channel(int64_t nx, int64_t ny, int64_t nz, const double mn[3], const double mx[3],
+                      double jitter, uint64_t seed, cfx_hmesh** out);
+
+/* ---------------------------------------------------- detail regions */
+/* DetailBlock (src/coupling/include/cf/coupling/CoupledNetwork.h,
+ * coupledNetwork.cpp:644-926, 1080-1200), laminar incompressible: inlet and
+ * outlet are the patches typed inlet / outlet; the inlet profile is "flat",
+ * "parabolic" (prod(1 - eta^2) over the wall-bounded transverse axes) or
+ * "power1over7", normalised to the section area; cfx_detail_loss_at imposes
+ * rate q, runs the GPU SIMPLE (warm-started from the block's fields) and
+ * returns the drop of the volume-weighted section-mean pressure (Pa) between
+ * the first and the last cross-section. */
+typedef struct cfx_detail cfx_detail;
+typedef struct cfx_detail_measure {
+    int64_t cells;
+    int sweeps, converged;
+    double imposedQ, inletArea, plugSpeed;
+    double dpPa, fluxIn, fluxOut;
+} cfx_detail_measure;
+int cfx_detail_create(cfx_ctx* ctx, const cfx_hmesh* mesh, double nu, double rho, const char* shape,
+                      cfx_detail** out);
+int cfx_detail_destroy(cfx_detail* d);
+int cfx_detail_loss_at(cfx_detail* d, double q, const cfx_simple_controls* ctl, cfx_detail_measure* m);
+int cfx_detail_info(const cfx_detail* d, int* axis, int* flowSign, double* inletArea, int64_t* nSections,
+                    int64_t* nInletFaces);
+/* inlet faces and their profile weights (flat: none) */
+int cfx_detail_inlet(const cfx_detail* d, int64_t* faces, double* weights);
+/* host copies: U [3][n] SoA, p [n], phi [nFaces] (each may be null) */
+int cfx_detail_fields(const cfx_detail* d, double* U, double* p, double* phi);
+/* the block's device mesh (pressure dumps, multigrid settings) */
+cfx_mesh* cfx_detail_mesh(const cfx_detail* d);
+
 int cfx_hmesh_from_arrays(int64_t nPoints, const double* points, int64_t nFaces,
                           const int64_t* faceStart, const int64_t* faceVerts, int64_t nInternal,
                           const int64_t* owner, const int64_t* neighbour, int64_t nPatches,

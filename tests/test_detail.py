@@ -1,0 +1,77 @@
+"""Detail regions (the coupling's DetailBlock) against the detail oracle:
+CPU pins of the oracle and, on the GPU, the block's profile, boundary
+conditions and pressure drop against the oracle's restatement driven by
+the oracle's SIMPLE."""
+import numpy as np
+import pytest
+
+import detail_oracle as DO
+import oracle as O
+from helpers import oracle_mesh_from_arrays
+
+TYPES = ["inlet", "outlet", "wall", "wall", "wall", "wall"]
+
+
+def channel(nx, ny, nz, L=(1.0, 0.2, 0.2), jitter=0.0, seed=1):
+    from chuanflow_amd import native as N
+    hm = N.HostMesh.channel(nx, ny, nz, (0, 0, 0), L, jitter, seed)
+    return hm, hm.arrays(), hm.geometry()
+
+
+def test_oracle_profile_and_sections_closed_forms():
+    hm, a, g = channel(8, 6, 5)
+    pts = np.asarray(a["points"]).reshape(-1, 3)
+    pin, pout, area, axis, fs = DO.flow_axis(g["Sf"], a["patchStart"], a["patchSize"], TYPES)
+    assert (pin, pout, axis, fs) == (0, 1, 0, 1) and abs(area - 0.04) < 1e-15
+    f, w = DO.inlet_profile(pts, g["Sf"], g["Cf"], g["magSf"], a["patchStart"], a["patchSize"], TYPES, axis, pin,
+                            area, "parabolic")
+    a_f = g["magSf"][f]
+    assert abs((w * a_f).sum() - a_f.sum()) < 1e-14 * a_f.sum()
+    # on a uniform grid the weights are prod(1 - eta^2) up to one factor
+    eta_y = (g["Cf"][f, 1] - 0.1) / 0.1
+    eta_z = (g["Cf"][f, 2] - 0.1) / 0.1
+    shape = (1 - eta_y ** 2) * (1 - eta_z ** 2)
+    np.testing.assert_allclose(w / shape, (w / shape)[0], rtol=1e-13)
+    assert len(DO.inlet_profile(pts, g["Sf"], g["Cf"], g["magSf"], a["patchStart"], a["patchSize"], TYPES, axis,
+                                pin, area, "flat")[0]) == 0
+    secs = DO.sections(g["C"], pts, axis, fs)
+    assert len(secs) == 8 and all(len(s) == 30 for s in secs)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape", ["flat", "parabolic"])
+def test_detail_block_matches_oracle(ctx, shape):
+    from chuanflow_amd import native as N
+    print(f"\n[case] detail block 24x6x6 jittered, {shape}", flush=True)
+    hm, a, g = channel(24, 6, 6, jitter=0.1, seed=4)
+    pts = np.asarray(a["points"]).reshape(-1, 3)
+    blk = N.Detail(ctx, hm, nu=1e-3, rho=1000.0, shape=shape)
+    info = blk.info()
+    pin, pout, area, axis, fs = DO.flow_axis(g["Sf"], a["patchStart"], a["patchSize"], TYPES)
+    assert info["axis"] == axis and info["flowSign"] == fs and info["inletArea"] == area
+    fo, wo = DO.inlet_profile(pts, g["Sf"], g["Cf"], g["magSf"], a["patchStart"], a["patchSize"], TYPES, axis, pin,
+                              area, shape)
+    f, w = blk.inlet()
+    assert np.array_equal(f, fo) and np.allclose(w, wo, rtol=1e-14, atol=0)
+    secs = DO.sections(g["C"], pts, axis, fs)
+    assert info["sections"] == len(secs)
+    q = 0.02 * area
+    kw = dict(maxIterations=15, continuityTol=0, pcorrTol=0, momentumTol=0, linearTol=1e-10, linearMaxIter=5000,
+              nonOrthCorrectors=2, relaxU=0.9, relaxP=0.03, pRelaxStart=0.02)
+    dp, m = blk.loss_at(q, N.SimpleControls(**kw))
+    # the oracle: same boundary conditions, the oracle's SIMPLE from zero fields, the same measurement
+    om = oracle_mesh_from_arrays(a)
+    Ut, Uv, pt, pv = DO.boundary(q, om.nInternal, a["patchStart"], a["patchSize"], TYPES, axis, fs, area, pin, pout,
+                                 fo, wo)
+    U, p, phi = np.zeros((om.nCells, 3)), np.zeros(om.nCells), np.zeros(om.nFaces)
+    lc, lk = O.linearise(om, 3, Ut, Uv)
+    O.correct_boundary_flux(om, U, lc, lk, phi)
+    O.simple(om, U, p, phi, 1e-3, O.Schemes(0, 1, 1, 0.9), O.SimpleControls(**kw), (Ut, Uv), (pt, pv))
+    dpo, fino, fouto = DO.measure(p, phi, g["V"], secs, 1000.0, a["patchStart"], a["patchSize"], pin, pout)
+    print(f"dp gpu {dp:.9e} oracle {dpo:.9e}; flux in {m['fluxIn']:.6e} (q {q:.6e}) out {m['fluxOut']:.6e}",
+          flush=True)
+    assert dp > 0 and abs(dp - dpo) <= 1e-6 * abs(dpo)
+    # on a jittered mesh SIMPLE moves the inlet flux slightly off q, as the oracle (= reference) does
+    assert abs(m["fluxIn"] - fino) <= 1e-9 * q and abs(m["fluxOut"] - fouto) <= 1e-6 * q
+    Ug, pg, phig = blk.fields()
+    assert np.abs(pg - p).max() <= 1e-6 * np.abs(p).max()
