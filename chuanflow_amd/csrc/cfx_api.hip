@@ -1344,16 +1344,14 @@ int cfx_batch_solve(cfx_ctx* c, int count, cfx_csr* const* systems, const double
     });
 }
 
-int cfx_batch_autotune(cfx_ctx* c, int count, cfx_csr* const* systems, double* const* x, int iters,
-                       cfx_batch_schedule* best, cfx_batch_schedule* cand, double* candMs, int maxCand)
+int cfx_batch_autotune(cfx_ctx* c, int count, cfx_csr* const* systems, const double* const* b, double* const* x,
+                       int iters, cfx_batch_schedule* best, cfx_batch_schedule* cand, double* candMs, int maxCand)
 {
-    if (!c || !best) return CFX_E_INVALID;
+    if (!c || !best || !b) return CFX_E_INVALID;
     return guard(c, [&] {
         need(iters > 0, "iters must be positive");
         if (!c->batch) c->batch = std::make_unique<BatchedPcg>(*c->ctx);
-        std::vector<BatchSystem> sys = batchSystems(c, count, systems, nullptr, x);
-        // the timing runs need a right-hand side: the system's own x serves (any vector does)
-        for (auto& s : sys) s.b = s.x;
+        std::vector<BatchSystem> sys = batchSystems(c, count, systems, b, x);
         std::vector<std::pair<BatchSchedule, double>> log;
         const BatchSchedule b = c->batch->autotune(sys, iters, &log);
         *best = cfx_batch_schedule{b.vecElems, b.wgCap, b.graph, b.pollEvery};

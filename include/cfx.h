@@ -320,10 +320,12 @@ typedef struct cfx_batch_stats {
 int cfx_batch_solve(cfx_ctx* ctx, int count, cfx_csr* const* systems, const double* const* d_b,
                     double* const* d_x, double tol, int maxIter, const cfx_batch_schedule* sched,
                     cfx_report* reps, double* finishMs, cfx_batch_stats* stats);
-/* times the candidate schedules for `iters` iterations each (the x are restored)
+/* times the candidate schedules for `iters` iterations each from the given
+ * b and x (no stop test; the x are left as they were)
  * and returns the fastest; cand/candMs (may be null) receive up to maxCand timings */
-int cfx_batch_autotune(cfx_ctx* ctx, int count, cfx_csr* const* systems, double* const* d_x, int iters,
-                       cfx_batch_schedule* best, cfx_batch_schedule* cand, double* candMs, int maxCand);
+int cfx_batch_autotune(cfx_ctx* ctx, int count, cfx_csr* const* systems, const double* const* d_b,
+                       double* const* d_x, int iters, cfx_batch_schedule* best, cfx_batch_schedule* cand,
+                       double* candMs, int maxCand);
 
 /* ------------------------------------------------------------ drivers */
 /* Schemes (src/3d/include/cf/fvm/FvMatrix.h:30-38): div 0 upwind, 1 linear,

@@ -44,7 +44,7 @@ def test_autotuned_schedule_agrees(ctx, blocks):
     from chuanflow_amd import detail_batch as DB
     from chuanflow_amd import native as N
     xs = [ctx.zeros(b["n"]) for b in blocks]
-    best, table = N.batch_autotune(ctx, [b["csr"] for b in blocks], xs, iters=10)
+    best, table = N.batch_autotune(ctx, [b["csr"] for b in blocks], [b["b"] for b in blocks], xs, iters=10)
     print(f"autotuned {best.as_dict()} over {len(table)} candidates", flush=True)
     assert len(table) == 18 and min(ms for _, ms in table) > 0
     x1, r1 = DB.solve_one_by_one(ctx, blocks, tol=1e-12)
