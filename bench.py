@@ -220,6 +220,96 @@ def bench_c5(args):
     print(json.dumps(result), flush=True)
 
 
+def cavity_bcs(patch_start, patch_size, n_internal, n_boundary, lid=1.0):
+    """Lid-driven cavity: the y-max patch (4th of the box) moves at (lid, 0, 0),
+    the other walls are at rest; p zeroGradient everywhere."""
+    pid = np.empty(n_boundary, np.int64)
+    for p, (s, n) in enumerate(zip(patch_start, patch_size)):
+        pid[s - n_internal:s - n_internal + n] = p
+    Ut = np.zeros(n_boundary, np.int32)
+    Uv = np.zeros((n_boundary, 3))
+    Uv[pid == 3, 0] = lid
+    return (Ut, Uv), (np.full(n_boundary, 2, np.int32), np.zeros((n_boundary, 1)))
+
+
+def kinetic_energy(V, U3):
+    return 0.5 * float((V * (U3 ** 2).sum(axis=0)).sum())
+
+
+def bench_c3(args):
+    """Transient lid-driven cavity, Re 1000, on Kuhn-split jittered tets: PISO per-step time and
+    kinetic energy; the kinetic-energy drift against the FP64 CPU oracle on a small instance."""
+    from chuanflow_amd import learned as LN
+    from chuanflow_amd import native as N
+    ctx = N.Context(0)
+    prec = {"fp64": N.FP64, "fp32ir": N.FP32_IR, "mg": N.FP32_MG}[args.c3_pressure]
+
+    def setup(n, learned):
+        hm = N.HostMesh.box(n, n, n, (0, 0, 0), (1, 1, 1), 0.10, SEED_MESH, True)
+        hm.renumber_rcm()
+        arrays, geom = hm.arrays(), hm.geometry()
+        dm = N.Mesh(ctx, arrays, geom)
+        Ubc, pbc = cavity_bcs(arrays["patchStart"], arrays["patchSize"], len(arrays["neighbour"]), dm.nB)
+        dU, dp, dphi = ctx.zeros((3, dm.nCells)), ctx.zeros(dm.nCells), ctx.zeros(dm.nFaces)
+        dm.set_mg(scale=args.mg_scale, coarseScale=args.mg_scale)
+        if learned:
+            dm.set_mpnn(LN.load(ctx))
+        return hm, arrays, geom, dm, Ubc, pbc, dU, dp, dphi
+
+    kw = dict(dt=2e-3, correctors=2, continuityTol=1e-10, linearTol=args.tol, linearMaxIter=20000,
+              nonOrthCorrectors=1)
+    sch = N.Schemes(0, 1, 1, 0.9)
+    t0 = time.perf_counter()
+    hm, arrays, geom, dm, Ubc, pbc, dU, dp, dphi = setup(args.c3_n, prec == N.FP32_MG)
+    V = np.asarray(geom["V"])
+    log(f"cavity: {dm.nCells} tets, mesh {time.perf_counter() - t0:.1f}s, pressure solve {args.c3_pressure}")
+    # the closed cavity's pressure is a singular Neumann system: both the GPU and the oracle project its
+    # right-hand side to zero sum (the reference PISO does not, and diverges on these meshes)
+    ctl = N.PisoControls(**kw, pPrecision=prec, projectNeumann=1)
+    times, ke, dev = [], [], []
+    for step in range(args.c3_steps):
+        ctx.sync()
+        t1 = time.perf_counter()
+        r = dm.piso_step(1e-3, sch, ctl, Ubc, pbc, dU, dp, dphi)
+        ctx.sync()
+        times.append((time.perf_counter() - t1) * 1e3)
+        dev.append(r.ms)
+        if step % 10 == 9 or step == args.c3_steps - 1:
+            ke.append((step + 1, kinetic_energy(V, dU.get())))
+            log(f"step {step + 1}: {times[-1]:.1f} ms, KE {ke[-1][1]:.6e}, continuity {r.continuityError:.2e}")
+    # kinetic-energy drift against the FP64 CPU oracle: the same case on a small mesh, a step prefix
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O  # noqa: E402  (checker only)
+    hs, a_s, g_s, ds, Ubs, pbs, sU, sp_, sphi = setup(args.c3_check_n, prec == N.FP32_MG)
+    om = O.Mesh(a_s["points"], a_s["faceStart"], a_s["faceVerts"], a_s["owner"], a_s["neighbour"], a_s["patchStart"],
+                a_s["patchSize"], a_s["nCells"]).compute_geometry()
+    Uo, po, phio = np.zeros((om.nCells, 3)), np.zeros(om.nCells), np.zeros(om.nFaces)
+    drift = []
+    tc = time.perf_counter()
+    Vs = np.asarray(g_s["V"])
+    for step in range(args.c3_check_steps):
+        ds.piso_step(1e-3, sch, ctl, Ubs, pbs, sU, sp_, sphi)
+        O.piso_step(om, Uo, po, phio, 1e-3, O.Schemes(0, 1, 1, 0.9), O.PisoControls(**kw, projectNeumann=1), Ubs, pbs)
+        kg, ko = kinetic_energy(Vs, sU.get()), kinetic_energy(Vs, Uo.T)
+        drift.append({"step": step + 1, "ke_gpu": kg, "ke_oracle": ko, "rel": abs(kg - ko) / ko})
+    cpu_ms = (time.perf_counter() - tc) * 1e3 / max(1, args.c3_check_steps)
+    mean_ms = float(np.mean(times[1:] if len(times) > 1 else times))
+    result = {
+        "metric": "PISO step time, lid-driven cavity Re 1000 on Kuhn tets; kinetic-energy drift vs FP64 CPU oracle",
+        "value": round(mean_ms, 3), "unit": "ms", "n_gpus": 1, "steps": args.c3_steps, "warmup": 1,
+        "ms_per_step": round(mean_ms, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 assembly and momentum; pressure " + args.c3_pressure, "data": "synthetic",
+        "config": {"workload": f"{args.c3_n}^3 hex unit box, +-10% jitter (seed {SEED_MESH}), 6 Kuhn tets per hex, "
+                               f"RCM; lid U=(1,0,0), nu 1e-3, PISO 2 correctors, dt 2e-3", "cells": dm.nCells,
+                   "parallelism": "single GPU"},
+        "step_ms_first": round(times[0], 3), "step_ms_mean": round(mean_ms, 3),
+        "device_ms_mean": round(float(np.mean(dev)), 3), "kinetic_energy": ke,
+        "ke_check": {"cells": ds.nCells, "steps": args.c3_check_steps, "max_rel_drift": max(d["rel"] for d in drift),
+                     "per_step": drift, "oracle_step_ms": round(cpu_ms, 1)},
+    }
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -231,8 +321,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--mg-scale", type=float, default=1.8)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c5"],
-                    help="c2: the 128^3 pressure system (the headline metric); c5: batched detail regions")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c5"],
+                    help="c2: the 128^3 pressure system (the headline metric); c3: PISO cavity on tets; "
+                         "c5: batched detail regions")
+    ap.add_argument("--c3-n", type=int, default=100)
+    ap.add_argument("--c3-steps", type=int, default=500)
+    ap.add_argument("--c3-pressure", default="mg", choices=["fp64", "fp32ir", "mg"])
+    ap.add_argument("--c3-check-n", type=int, default=12)
+    ap.add_argument("--c3-check-steps", type=int, default=5)
     ap.add_argument("--blocks", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -242,6 +338,8 @@ def main():
 
     if args.workload == "c5":
         return bench_c5(args)
+    if args.workload == "c3":
+        return bench_c3(args)
     from chuanflow_amd import native as N
     ctx = N.Context(0)
     arch, cus = ctx.device_info()

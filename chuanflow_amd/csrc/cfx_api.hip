@@ -1083,6 +1083,7 @@ int cfx_piso_step(cfx_mesh* m, double nu, const cfx_schemes* schemes, const cfx_
         c.linearMaxIter = ctl->linearMaxIter;
         c.nonOrthCorrectors = ctl->nonOrthCorrectors;
         for (int k = 0; k < 3; ++k) c.bodyForce[k] = ctl->bodyForce[k];
+        c.projectNeumann = ctl->projectNeumann != 0;
         c.pPrecision = ctl->pPrecision;
         c.pSolver = ctl->pPrecision == CFX_FP32_MG ? m->pressureSolver() : nullptr;
         c.dump = &m->dump;

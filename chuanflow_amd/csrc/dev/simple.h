@@ -39,6 +39,7 @@ struct FvSolveArgs
     Mpnn* net = nullptr;    // non-null: finest-level weights from the network (per-slot geometry geo)
     const double* geo = nullptr;
     float *wa = nullptr, *wt = nullptr;
+    bool projectMean = false;  // subtract the mean of every pass' rhs (singular Neumann systems)
 };
 SolveReport fvSolve(DevMesh& m, Krylov& K, const Ldu& A, double* val, double* rhsBase, DevBuf& val32,
                     double* x, const FvSolveArgs& a, double* gradTmp, double* faceTmp);
@@ -110,6 +111,7 @@ struct PisoControls
     int pPrecision = kFP64;
     PressureSolver* pSolver = nullptr;
     PressureDump* dump = nullptr;
+    bool projectNeumann = false;  // zero-sum pressure rhs when no pressure face is fixed
 };
 
 struct PisoResult

@@ -361,6 +361,40 @@ bool lineariseBc(DevMesh& m, int ncomp, const int* dType, const double* dValue, 
     return h == 0;
 }
 
+// rhs -= mean(rhs): per-workgroup partial sums, then every workgroup reduces
+// them (fixed order) and subtracts the mean from its elements
+__global__ void k_rhs_sum(int n, const double* __restrict__ v, double* __restrict__ part)
+{
+    __shared__ double red[B / kWave];
+    double s = 0.0;
+    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) s += v[i];
+    s = blockSum(s, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void k_rhs_sub_mean(int n, double* __restrict__ v, const double* __restrict__ part, int G)
+{
+    __shared__ double red[B / kWave];
+    __shared__ double meanS;
+    double s = 0.0;
+    for (int g = threadIdx.x; g < G; g += B) s += part[g];
+    s = blockSum(s, red);
+    if (threadIdx.x == 0) meanS = s / n;
+    __syncthreads();
+    const double mean = meanS;
+    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) v[i] -= mean;
+}
+
+static void projectZeroMean(DevMesh& m, double* v)
+{
+    hipStream_t s = m.ctx->stream();
+    const int G = std::max(1, std::min(256, divUp(m.n, 4 * B)));
+    double* part = m.ctx->partials(3);
+    k_rhs_sum<<<G, B, 0, s>>>(m.n, v, part);
+    k_rhs_sub_mean<<<G, B, 0, s>>>(m.n, v, part, G);
+    CFX_CHECK_LAUNCH();
+}
+
 // pressure-solve options of the drivers
 static void setPressureSolver(FvSolveArgs& pa, int precision, const PressureSolver* ps)
 {
@@ -404,6 +438,8 @@ SolveReport fvSolve(DevMesh& m, Krylov& K, const Ldu& A, double* val, double* rh
             for (int k = 0; k < A.ncomp; ++k)
                 nonOrthCorrect(m, a.gammaFace, x + static_cast<long long>(k) * n, a.linc, a.link, A.ncomp, k,
                                a.gradCorrected, a.sign, A.rhs + static_cast<long long>(k) * n, gradTmp, faceTmp);
+        if (a.projectMean)
+            for (int k = 0; k < A.ncomp; ++k) projectZeroMean(m, A.rhs + static_cast<long long>(k) * n);
         for (int k = 0; k < A.ncomp; ++k)
         {
             const double* v = val + static_cast<long long>(k) * nnz;
@@ -651,6 +687,7 @@ PisoResult pisoStep(DevMesh& m, Krylov& K, double nu, const Schemes& sch, const 
         pa.linc = w.plc.as<double>();
         pa.link = w.plk.as<double>();
         pa.gradCorrected = sch.gradCorrected;
+        pa.projectMean = c.projectNeumann && needReference;
         setPressureSolver(pa, c.pPrecision, c.pSolver);
         fvSolve(m, K, PE, w.pval.as<double>(), w.pbase.as<double>(), w.val32, p, pa, w.gradTmp.as<double>(),
                 w.faceTmp.as<double>());

@@ -64,7 +64,7 @@ class SimpleResult(C.Structure):
 class PisoControls(C.Structure):
     _fields_ = [("dt", C.c_double), ("correctors", C.c_int), ("continuityTol", C.c_double),
                 ("linearTol", C.c_double), ("linearMaxIter", C.c_int), ("nonOrthCorrectors", C.c_int),
-                ("bodyForce", C.c_double * 3), ("pPrecision", C.c_int)]
+                ("bodyForce", C.c_double * 3), ("pPrecision", C.c_int), ("projectNeumann", C.c_int)]
 
 
 class PisoResult(C.Structure):

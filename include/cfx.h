@@ -366,6 +366,12 @@ typedef struct cfx_piso_controls {
     int linearMaxIter, nonOrthCorrectors;
     double bodyForce[3];
     int pPrecision;
+    /* not in the reference: with no fixedValue pressure face (a closed
+     * cavity), project each pass' pressure right-hand side to zero sum, the
+     * compatibility condition of the singular Neumann system; without it the
+     * non-orthogonal correction of a jittered or tet mesh makes the system
+     * inconsistent and the reference's PISO diverges */
+    int projectNeumann;
 } cfx_piso_controls;
 
 typedef struct cfx_piso_result {

@@ -242,6 +242,10 @@ typedef struct cfo_piso_controls {
     double continuityTol, linearTol;
     int linearMaxIter, nonOrthCorrectors;
     double bodyForce[3];
+    /* not in the reference: with no fixedValue pressure face, project each
+     * pass' pressure right-hand side to zero sum (the compatibility
+     * condition of the singular Neumann system); 0 = the reference */
+    int projectNeumann;
 } cfo_piso_controls;
 
 typedef struct cfo_piso_result {

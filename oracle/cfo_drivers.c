@@ -36,10 +36,25 @@ typedef struct corrector {
 /* fvMatrix::solve (fvMatrix.cpp:73-129).  x [ncomp][n] in/out; rhsOut
  * (optional, [ncomp][n]) receives rhs_ as it stands after the solve (the
  * base rhs plus the final pass' corrections), which explicitOperator reads. */
+static cfo_report fvmatrix_solve_p(const cfo_mesh* m, const cfo_ldu* A, double* x,
+                                   const corrector* corr, double tol, int maxIter,
+                                   int nonOrth, int preferCG, double* rhsOut,
+                                   cfo_csr* csrOut, int projectMean);
+
 static cfo_report fvmatrix_solve(const cfo_mesh* m, const cfo_ldu* A, double* x,
                                  const corrector* corr, double tol, int maxIter,
                                  int nonOrth, int preferCG, double* rhsOut,
                                  cfo_csr* csrOut /* [ncomp], may be NULL */)
+{
+    return fvmatrix_solve_p(m, A, x, corr, tol, maxIter, nonOrth, preferCG, rhsOut, csrOut, 0);
+}
+
+/* projectMean: subtract the mean of each pass' right-hand side (singular
+ * Neumann systems) */
+static cfo_report fvmatrix_solve_p(const cfo_mesh* m, const cfo_ldu* A, double* x,
+                                   const corrector* corr, double tol, int maxIter,
+                                   int nonOrth, int preferCG, double* rhsOut,
+                                   cfo_csr* csrOut, int projectMean)
 {
     const int64_t n = m->nCells;
     const int nc = A->ncomp;
@@ -63,6 +78,13 @@ static cfo_report fvmatrix_solve(const cfo_mesh* m, const cfo_ldu* A, double* x,
                 cfo_nonorth_correct(m, corr->gammaFace, x + (int64_t)c * n, corr->lin_c,
                                     corr->lin_k, nc, c, corr->gradCorrected, corr->sign,
                                     rhs + (int64_t)c * n);
+        if (projectMean)
+            for (int c = 0; c < nc; ++c) {
+                double sum = 0;
+                for (int64_t i = 0; i < n; ++i) sum += rhs[(int64_t)c * n + i];
+                const double mean = sum / (double)n;
+                for (int64_t i = 0; i < n; ++i) rhs[(int64_t)c * n + i] -= mean;
+            }
         for (int c = 0; c < nc; ++c) {
             if (preferCG) last = cfo_solve_cg(&mats[c], rhs + (int64_t)c * n, x + (int64_t)c * n, tol, maxIter, NULL);
             else last = cfo_solve_bicgstab(&mats[c], rhs + (int64_t)c * n, x + (int64_t)c * n, tol, maxIter);
@@ -293,6 +315,9 @@ cfo_piso_result cfo_piso_step(const cfo_mesh* m, double* U, double* p,
     lin_set Ul = make_lin(m, 3, Ubc.type, Ubc.value, &rc);
     lin_set pl = make_lin(m, 1, pbc.type, pbc.value, &rc);
     lin_set Hl = hbya_lin(m, Ubc);
+    int needRefP = 1;  /* no fixedValue pressure face: a singular Neumann system */
+    for (int64_t b = 0; b < nF - nI; ++b)
+        if (pbc.type[b] == 0) needRefP = 0;
 
     double* gradP = dalloc(3 * n);
     double* gradPn = dalloc(3 * n);
@@ -370,8 +395,8 @@ cfo_piso_result cfo_piso_step(const cfo_mesh* m, double* U, double* p,
         for (int64_t c = 0; c < n; ++c) PE.rhs[c] += -divPhi[c];
         memcpy(pBefore, p, sizeof(double) * (size_t)n);
         corrector pc = {s->laplacianCorrected, rAUf, pl.c, pl.k, -1, s->gradCorrected};
-        fvmatrix_solve(m, &PE, p, &pc, ctl->linearTol, ctl->linearMaxIter,
-                       ctl->nonOrthCorrectors, 1, NULL, NULL);
+        fvmatrix_solve_p(m, &PE, p, &pc, ctl->linearTol, ctl->linearMaxIter,
+                         ctl->nonOrthCorrectors, 1, NULL, NULL, ctl->projectNeumann && needRefP);
         ldu_free(&PE);
         double pDeltaMax = 0;
         for (int64_t c = 0; c < n; ++c) pDeltaMax = SMAX(pDeltaMax, fabs(p[c] - pBefore[c]));
