@@ -30,6 +30,9 @@ sys.path.insert(0, REPO)
 
 METRIC = "p-solve time to 1e-8 FP64 residual: learned+mixed PCG vs plain PCG; SpMV GB/s"
 HBM_PEAK_GBPS = 8000.0
+# HBM bytes per FP64 SpMV launch on the 128^3 system from rocprofv3 PMC (profiles/spmv_128cube_hbm_traffic.md):
+# FETCH_SIZE 100,831.1 KiB / 0.50005 (the copy-kernel calibration) + WRITE_SIZE 16,384 KiB
+SPMV_FP64_TRAFFIC_128 = 223257689
 SEED_MESH, SEED_SHUFFLE, SEED_RHS = 128, 7, 2024
 
 
@@ -251,21 +254,25 @@ def bench_c3(args):
         dm = N.Mesh(ctx, arrays, geom)
         Ubc, pbc = cavity_bcs(arrays["patchStart"], arrays["patchSize"], len(arrays["neighbour"]), dm.nB)
         dU, dp, dphi = ctx.zeros((3, dm.nCells)), ctx.zeros(dm.nCells), ctx.zeros(dm.nFaces)
-        dm.set_mg(scale=args.mg_scale, coarseScale=args.mg_scale)
+        dm.set_mg(scale=args.mg_scale, coarseScale=args.mg_scale, kcycleDepth=args.kcycle_depth,
+                  fusion=args.mg_fusion)
         if learned:
             dm.set_mpnn(LN.load(ctx))
         return hm, arrays, geom, dm, Ubc, pbc, dU, dp, dphi
 
     kw = dict(dt=2e-3, correctors=2, continuityTol=1e-10, linearTol=args.tol, linearMaxIter=20000,
               nonOrthCorrectors=1)
-    sch = N.Schemes(0, 1, 1, 0.9)
+    # laplacian "uncorrected" by default: on Kuhn tets the reference's explicit non-orthogonal correction is a
+    # diverging fixed-point iteration (continuity error x4 every 5 steps, worse with more correctors, at any dt)
+    corrected = 1 if args.c3_laplacian == "corrected" else 0
+    sch = N.Schemes(0, corrected, 1, 0.9)
     t0 = time.perf_counter()
     hm, arrays, geom, dm, Ubc, pbc, dU, dp, dphi = setup(args.c3_n, prec == N.FP32_MG)
     V = np.asarray(geom["V"])
     log(f"cavity: {dm.nCells} tets, mesh {time.perf_counter() - t0:.1f}s, pressure solve {args.c3_pressure}")
-    # the closed cavity's pressure is a singular Neumann system: both the GPU and the oracle project its
-    # right-hand side to zero sum (the reference PISO does not, and diverges on these meshes)
-    ctl = N.PisoControls(**kw, pPrecision=prec, projectNeumann=1)
+    # the closed cavity's pressure is a singular Neumann system; --c3-project makes both the GPU and the oracle
+    # project its right-hand side to zero sum (the reference PISO does not)
+    ctl = N.PisoControls(**kw, pPrecision=prec, projectNeumann=args.c3_project)
     times, ke, dev = [], [], []
     for step in range(args.c3_steps):
         ctx.sync()
@@ -289,7 +296,8 @@ def bench_c3(args):
     Vs = np.asarray(g_s["V"])
     for step in range(args.c3_check_steps):
         ds.piso_step(1e-3, sch, ctl, Ubs, pbs, sU, sp_, sphi)
-        O.piso_step(om, Uo, po, phio, 1e-3, O.Schemes(0, 1, 1, 0.9), O.PisoControls(**kw, projectNeumann=1), Ubs, pbs)
+        O.piso_step(om, Uo, po, phio, 1e-3, O.Schemes(0, corrected, 1, 0.9),
+                    O.PisoControls(**kw, projectNeumann=args.c3_project), Ubs, pbs)
         kg, ko = kinetic_energy(Vs, sU.get()), kinetic_energy(Vs, Uo.T)
         drift.append({"step": step + 1, "ke_gpu": kg, "ke_oracle": ko, "rel": abs(kg - ko) / ko})
     cpu_ms = (time.perf_counter() - tc) * 1e3 / max(1, args.c3_check_steps)
@@ -300,7 +308,8 @@ def bench_c3(args):
         "ms_per_step": round(mean_ms, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64 assembly and momentum; pressure " + args.c3_pressure, "data": "synthetic",
         "config": {"workload": f"{args.c3_n}^3 hex unit box, +-10% jitter (seed {SEED_MESH}), 6 Kuhn tets per hex, "
-                               f"RCM; lid U=(1,0,0), nu 1e-3, PISO 2 correctors, dt 2e-3", "cells": dm.nCells,
+                               f"RCM; lid U=(1,0,0), nu 1e-3, PISO 2 correctors, dt 2e-3, laplacian "
+                               f"{args.c3_laplacian}, rhs projection {args.c3_project}", "cells": dm.nCells,
                    "parallelism": "single GPU"},
         "step_ms_first": round(times[0], 3), "step_ms_mean": round(mean_ms, 3),
         "device_ms_mean": round(float(np.mean(dev)), 3), "kinetic_energy": ke,
@@ -308,6 +317,34 @@ def bench_c3(args):
                      "per_step": drift, "oracle_step_ms": round(cpu_ms, 1)},
     }
     print(json.dumps(result), flush=True)
+
+
+def bench_spmv(args):
+    """SpMV alone on the 128^3 pressure system (FP64 and FP32 values, args.spmv_reps launches each) and a
+    device-to-device copy of a known byte count: the workload the HBM traffic counters are read on
+    (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE; the copy's counters give the scale to bytes)."""
+    from chuanflow_amd import native as N
+    ctx = N.Context(0)
+    hm, arrays, geom, dm, A, b, bw = build_system(N, ctx, args.n)
+    n, nnz = dm.nCells, dm.sizes()["nnz"]
+    xs = ctx.array(np.random.default_rng(1).standard_normal(n))
+    ys = ctx.empty(n)
+    ms64 = A.time_spmv(0, xs, ys, args.spmv_reps, N.FP64)
+    xs32 = ctx.array(np.random.default_rng(1).standard_normal(n).astype(np.float32))
+    ys32 = ctx.empty(n, np.float32)
+    ms32 = A.time_spmv(0, xs32, ys32, args.spmv_reps, N.FP32_IR)
+    copy_bytes = 256 << 20
+    src, dst = ctx.zeros(copy_bytes // 8), ctx.empty(copy_bytes // 8)
+    for _ in range(args.spmv_reps):
+        dst.copy_from(src)
+    ctx.sync()
+    b64, b32 = canonical_spmv_bytes(n, nnz, 8), canonical_spmv_bytes(n, nnz, 4)
+    print(json.dumps({"workload": "spmv", "cells": n, "nnz": nnz, "reps": args.spmv_reps,
+                      "spmv_fp64_us": round(ms64 * 1e3, 2), "spmv_fp64_bytes": b64,
+                      "spmv_fp64_GBps": round(b64 / (ms64 * 1e-3) / 1e9, 1),
+                      "spmv_fp32_us": round(ms32 * 1e3, 2), "spmv_fp32_bytes": b32,
+                      "spmv_fp32_GBps": round(b32 / (ms32 * 1e-3) / 1e9, 1),
+                      "copy_bytes_read": copy_bytes, "copy_bytes_written": copy_bytes}), flush=True)
 
 
 def main():
@@ -321,14 +358,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--mg-scale", type=float, default=1.8)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c5"],
+    ap.add_argument("--kcycle-depth", type=int, default=2)
+    ap.add_argument("--mg-fusion", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c5", "spmv"],
                     help="c2: the 128^3 pressure system (the headline metric); c3: PISO cavity on tets; "
-                         "c5: batched detail regions")
+                         "c5: batched detail regions; spmv: SpMV and a copy of known size (traffic counters)")
     ap.add_argument("--c3-n", type=int, default=100)
     ap.add_argument("--c3-steps", type=int, default=500)
     ap.add_argument("--c3-pressure", default="mg", choices=["fp64", "fp32ir", "mg"])
     ap.add_argument("--c3-check-n", type=int, default=12)
-    ap.add_argument("--c3-check-steps", type=int, default=5)
+    ap.add_argument("--c3-check-steps", type=int, default=20)
+    ap.add_argument("--c3-laplacian", default="uncorrected", choices=["uncorrected", "corrected"])
+    ap.add_argument("--c3-project", type=int, default=0, choices=[0, 1])
     ap.add_argument("--blocks", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -340,6 +381,8 @@ def main():
         return bench_c5(args)
     if args.workload == "c3":
         return bench_c3(args)
+    if args.workload == "spmv":
+        return bench_spmv(args)
     from chuanflow_amd import native as N
     ctx = N.Context(0)
     arch, cus = ctx.device_info()
@@ -356,7 +399,7 @@ def main():
     rmix = time_solve(N, ctx, A, d_x, N.FP32_IR, args.steps, args.warmup, args.tol, maxIter)
     xmix = d_x.get()
     log("multigrid-preconditioned flexible CG (FP32 + FP64 refinement), tuned constants")
-    A.set_mg(scale=args.mg_scale, coarseScale=args.mg_scale)
+    A.set_mg(scale=args.mg_scale, coarseScale=args.mg_scale, kcycleDepth=args.kcycle_depth, fusion=args.mg_fusion)
     rmg = time_solve(N, ctx, A, d_x, N.FP32_IR, args.steps, args.warmup, args.tol, maxIter, N.PRECOND_LEARNED)
     xmg = d_x.get()
     log("learned: network weights on the finest level, flexible CG (FP32 + FP64 refinement)")
@@ -431,7 +474,8 @@ def main():
         "spmv_fp32_us": round(ms32 * 1e3, 2),
         "spmv_fp32_GBps": round(gbs32, 1),
         "roofline": {"bound": "hbm", "achieved": round(gbs64, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(gbs64 / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "frac": round(gbs64 / HBM_PEAK_GBPS, 4),
+                     "traffic": SPMV_FP64_TRAFFIC_128 if args.n == 128 else None,
                      "kernel": "k_spmv<double> (CSR-stream SpMV, FP64)", "bytes_per_launch": b64},
     }
     if not args.no_cpu:

@@ -802,7 +802,7 @@ int cfx_mg_default_params(cfx_mg_params* p)
     if (!p) return CFX_E_INVALID;
     const MgParams d;
     *p = cfx_mg_params{d.omegaPre, d.scale, d.omegaPost, d.restrictScale, d.coarseOmega, d.coarseScale,
-                       d.coarseSweeps, d.coarsestOmega, d.maxLevels, d.coarseMax, d.kcycleDepth};
+                       d.coarseSweeps, d.coarsestOmega, d.maxLevels, d.coarseMax, d.kcycleDepth, d.fusion};
     return CFX_OK;
 }
 
@@ -826,6 +826,7 @@ void setMgParams(MgCache& mg, const cfx_mg_params* p)
     m.coarseMax = p->coarseMax;
     need(p->kcycleDepth >= 0, "kcycleDepth < 0");
     m.kcycleDepth = p->kcycleDepth;
+    m.fusion = p->fusion;
     mg.valid = false;
 }
 
@@ -875,6 +876,15 @@ int cfx_csr_mg_info(cfx_csr* A, int* levels, int64_t* rows, int64_t* nnz, int ma
             if (rows) rows[l] = mg.rows(l);
             if (nnz) nnz[l] = mg.nnz(l);
         }
+    });
+}
+
+int cfx_csr_mg_tail_level(cfx_csr* A, int* level)
+{
+    if (!A || !level) return CFX_E_INVALID;
+    return guard(A->owner, [&] {
+        Multigrid& mg = A->mg.ready(*A->owner->ctx, A->pattern, A->val.as<double>());
+        *level = mg.tailLevel();
     });
 }
 

@@ -53,6 +53,30 @@ struct MgParams
     float coarsestOmega = 2.0f / 3.0f;
     int maxLevels = 10, coarseMax = 128;
     int kcycleDepth = 2;              // coarse levels 1..kcycleDepth use the K-cycle (0: V-cycle)
+    int fusion = 3;                   // bit 0: the small coarse levels' cycle in one workgroup;
+                                      // bit 1: prolongation (and K-cycle combination) fused into the
+                                      // post-smoothing of coarse levels of at most 65536 rows
+};
+
+// One level of the single-workgroup coarse tail (device pointers).
+struct TailLevel
+{
+    int n = 0;
+    const int *rowPtr = nullptr, *col = nullptr;
+    const float *valS = nullptr, *val = nullptr, *wPre = nullptr, *wPost = nullptr, *sc = nullptr, *invD = nullptr;
+    const int* agg = nullptr;                       // coarse row of each row (to the next level)
+    const int *rRowPtr = nullptr, *rCol = nullptr;  // P^T pattern of that transfer
+    const float* rval = nullptr;
+    const float* dense = nullptr;                   // coarsest: FP32 inverse, or null (Jacobi sweeps)
+};
+constexpr int kTailLevels = 4;
+constexpr int kTailLds = 12288;  // LDS floats of the tail: 3 per row of every tail level
+struct TailArgs
+{
+    int count = 0;    // levels in the tail, the coarsest included
+    int sweeps = 0;   // coarsest Jacobi sweeps and weight (no dense inverse)
+    float omega = 0.0f;
+    TailLevel L[kTailLevels];
 };
 
 struct MgLevel
@@ -82,8 +106,11 @@ public:
     bool denseCoarsest() const { return levels() > 1 && lv_.back().n <= kDenseMax; }
     static constexpr int kDenseMax = 128;
     const MgParams& params() const { return p_; }
-    // unique per constructed hierarchy (device buffers never move after construction)
-    std::uintptr_t id() const override { return id_; }
+    // unique per constructed hierarchy (device buffers never move after construction) and per
+    // launch structure (K-cycle depth, fusion, coarsest sweeps): a captured cycle is valid while it holds
+    std::uintptr_t id() const override { return (id_ << 16) | (structVersion_ & 0xffffu); }
+    // first level whose cycle (with all coarser levels) runs in one workgroup; -1: none
+    int tailLevel() const { return tailFrom_; }
     int rows() const override { return lv_[0].n; }
     const float* val32() const override { return lv_[0].val32.as<float>(); }
     const float* invD32() const override { return lv_[0].invD32.as<float>(); }
@@ -126,12 +153,19 @@ private:
     const DevPattern& fine_;
     MgParams p_;
     std::uintptr_t id_ = 0;
+    std::uintptr_t structVersion_ = 0;
+    int tailFrom_ = -1;
     std::vector<MgLevel> lv_;  // lv_[0] uses the fine pattern (pat unused)
     // out = cycle of level l applied to in (in, out distinct from the level's work vectors)
     void visit(int l, const float* in, float* out, const float* q, double* partZQ, int Gs, int rbPerWg,
                const int* done);
-    // coarse correction of level l >= 1: L.e from L.b
-    void solveLevel(int l, const int* done);
+    // coarse correction of level l >= 1: L.e from L.b.  With deferCombine a
+    // K-cycle level leaves e = k1 c1 + k2 c2 unformed (its partials hold the
+    // coefficients) and returns true
+    bool solveLevel(int l, const int* done, bool deferCombine = false);
+    // the same for l >= tailLevel(), below the K-cycle levels: one single-workgroup launch
+    bool useTail(int l) const;
+    void launchTail(int l, const int* done);
     DevBuf tv_[14];          // finest-level FP32 scratch of lossGrad / smoothSample
     DevBuf tpart_;
 };

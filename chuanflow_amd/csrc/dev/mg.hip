@@ -16,6 +16,8 @@ namespace
 constexpr int B = 256;
 constexpr int kCoarseThreads = 1024;
 constexpr int kCoarseMax = 2048;
+constexpr int kTailThreads = 1024;
+constexpr int kFuseRowsMax = 65536;  // largest level whose prolongation is fused into post-smoothing
 
 __device__ __forceinline__ int xcdRemap(int h, int G)
 {
@@ -205,6 +207,53 @@ __global__ __launch_bounds__(B) void k_mg_up(CsrView A, int rbPerWg, const float
     }
 }
 
+// sum over k in [k0, k1) of val[k] * x[col[k]], added in slot order; the
+// global loads go out 8 at a time (x is in LDS), so a row costs one or two
+// memory round trips instead of one per nonzero
+__device__ __forceinline__ float rowDot8(const float* __restrict__ val, const int* __restrict__ col,
+                                         const float* x, int k0, int k1)
+{
+    float s = 0.0f;
+    for (int k = k0; k < k1; k += 8)
+    {
+        float v[8];
+        int c[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+        {
+            const bool ok = k + u < k1;
+            v[u] = ok ? val[k + u] : 0.0f;
+            c[u] = ok ? col[k + u] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+        {
+            const float xv = x[c[u]];
+            if (k + u < k1) s += v[u] * xv;
+        }
+    }
+    return s;
+}
+
+// sum over j < n of m[j * n + i] * x[j] in column order, 8 loads at a time
+__device__ __forceinline__ float colDot8(const float* __restrict__ m, const float* x, int n, int i)
+{
+    float s = 0.0f;
+    for (int j = 0; j < n; j += 8)
+    {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = j + u < n ? m[(j + u) * n + i] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+        {
+            const float xv = x[j + u < n ? j + u : 0];
+            if (j + u < n) s += v[u] * xv;
+        }
+    }
+    return s;
+}
+
 // coarsest level: damped Jacobi sweeps from zero in one workgroup (LDS vectors)
 __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(int n, const int* __restrict__ rowPtr,
                                                               const int* __restrict__ col,
@@ -227,8 +276,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(int n, const int* 
     {
         for (int i = threadIdx.x; i < n; i += kCoarseThreads)
         {
-            float s = 0.0f;
-            for (int k = rowPtr[i]; k < rowPtr[i + 1]; ++k) s += val[k] * cur[col[k]];
+            const float s = rowDot8(val, col, cur, rowPtr[i], rowPtr[i + 1]);
             nxt[i] = cur[i] + omega * invD[i] * (bl[i] - s);
         }
         __syncthreads();
@@ -321,11 +369,98 @@ __global__ __launch_bounds__(kDense) void k_mg_dense_apply(int n, const float* _
     if (i < n) bl[i] = b[i];
     __syncthreads();
     if (i >= n) return;
-    float s = 0.0f;
-    for (int j = 0; j < n; ++j) s += inv[j * n + i] * bl[j];
-    x[i] = s;
+    x[i] = colDot8(inv, bl, n, i);
 }
 
+// ------------------------------------------------------------ coarse tail
+// The cycle of levels lt..last in ONE workgroup, vectors in LDS: the same
+// operations in the same order as k_mg_down, k_mg_restrict_csr, the coarsest
+// solve, k_mg_prolong and k_mg_up (row sums in slot order, member sums in
+// ascending member order, dense products in column order), so the result is
+// bit for bit that of the separate kernels; only the launches go.
+__global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const float* __restrict__ bIn,
+                                                          float* __restrict__ eOut, const int* done)
+{
+    __shared__ float v[kTailLds];
+    if (done && *done) return;
+    const int t = threadIdx.x;
+    const int last = a.count - 1;
+    // level m: b at off[m], x at +n, r (later z) at +2n; coarsest: b, e (and the Jacobi ping-pong buffer)
+    int off[kTailLevels];
+    {
+        int o = 0;
+        for (int m = 0; m < a.count; ++m)
+        {
+            off[m] = o;
+            o += 3 * a.L[m].n;
+        }
+    }
+    for (int i = t; i < a.L[0].n; i += kTailThreads) v[off[0] + i] = bIn[i];
+    __syncthreads();
+    for (int m = 0; m < last; ++m)
+    {
+        const TailLevel& L = a.L[m];
+        float *b = v + off[m], *x = b + L.n, *r = x + L.n;
+        for (int i = t; i < L.n; i += kTailThreads)
+        {
+            const float s = rowDot8(L.valS, L.col, b, L.rowPtr[i], L.rowPtr[i + 1]);
+            const float bi = b[i];
+            x[i] = L.wPre[i] * bi;
+            r[i] = bi - s;
+        }
+        __syncthreads();
+        float* bc = v + off[m + 1];
+        for (int c = t; c < a.L[m + 1].n; c += kTailThreads)
+            bc[c] = rowDot8(L.rval, L.rCol, r, L.rRowPtr[c], L.rRowPtr[c + 1]);
+        __syncthreads();
+    }
+    const float* ec;
+    {
+        const TailLevel& C = a.L[last];
+        float *b = v + off[last], *e = b + C.n, *w = e + C.n;
+        if (C.dense)
+        {
+            for (int i = t; i < C.n; i += kTailThreads) e[i] = colDot8(C.dense, b, C.n, i);
+            __syncthreads();
+            ec = e;
+        }
+        else
+        {
+            // damped Jacobi sweeps from zero (k_mg_coarse)
+            for (int i = t; i < C.n; i += kTailThreads) e[i] = 0.0f;
+            __syncthreads();
+            float *cur = e, *nxt = w;
+            for (int it = 0; it < a.sweeps; ++it)
+            {
+                for (int i = t; i < C.n; i += kTailThreads)
+                {
+                    const float s = rowDot8(C.val, C.col, cur, C.rowPtr[i], C.rowPtr[i + 1]);
+                    nxt[i] = cur[i] + a.omega * C.invD[i] * (b[i] - s);
+                }
+                __syncthreads();
+                float* tmp = cur;
+                cur = nxt;
+                nxt = tmp;
+            }
+            ec = cur;
+        }
+    }
+    for (int m = last - 1; m >= 0; --m)
+    {
+        const TailLevel& L = a.L[m];
+        float *b = v + off[m], *x = b + L.n;
+        float* z = m == 0 ? eOut : x + L.n;
+        for (int i = t; i < L.n; i += kTailThreads) x[i] += L.sc[i] * ec[L.agg[i]];
+        __syncthreads();
+        for (int i = t; i < L.n; i += kTailThreads)
+        {
+            const float s = rowDot8(L.val, L.col, x, L.rowPtr[i], L.rowPtr[i + 1]);
+            z[i] = x[i] + L.wPost[i] * (b[i] - s);
+        }
+        __syncthreads();
+        ec = z;
+    }
+}
 
 // ----------------------------------------------------------------- K-cycle
 // deterministic all-workgroup sum of G <= 2048 partials (blockDim B)
@@ -465,6 +600,55 @@ __global__ __launch_bounds__(B) void k_kc_combine(int n, const float* __restrict
     float k1, k2;
     kcCoefficients(part, G, red, k1, k2);
     for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) e[i] = k1 * c1[i] + k2 * c2[i];
+}
+
+// Prolongation fused into the post-smoothing of a coarse level: with
+// xp = x + S P e, z = xp + W (b - A xp), xp formed where it is read.  When the
+// coarse correction was a K-cycle its combination e = k1 c1 + k2 c2 is formed
+// here too (coefficients from its partials, as k_kc_combine).  Each value is
+// the same float expression as in k_mg_prolong / k_kc_combine, so z is bit
+// for bit that of the separate kernels.
+__global__ __launch_bounds__(B) void k_mg_prolong_up(CsrView A, int rbPerWg, const float* __restrict__ val32,
+                                                     const float* __restrict__ x, const float* __restrict__ b,
+                                                     const float* __restrict__ wD, const int* __restrict__ agg,
+                                                     const float* __restrict__ sc, const float* __restrict__ ec,
+                                                     const float* __restrict__ c1, const float* __restrict__ c2,
+                                                     const double* __restrict__ kcPart, int kcG,
+                                                     float* __restrict__ z, const int* done)
+{
+    __shared__ float lds[kSpmvLds];
+    __shared__ double red[B / kWave + 1];
+    if (done && *done) return;
+    float k1 = 0.0f, k2 = 0.0f;
+    if (kcPart) kcCoefficients(kcPart, kcG, red, k1, k2);
+    auto xp = [&](int j) {
+        const int a = agg[j];
+        const float e = kcPart ? k1 * c1[a] + k2 * c2[a] : ec[a];
+        return x[j] + sc[j] * e;
+    };
+    const int w = xcdRemap(blockIdx.x, gridDim.x);
+    for (int jb = 0; jb < rbPerWg; ++jb)
+    {
+        const int blk = w * rbPerWg + jb;
+        if (blk >= A.nRowBlocks) break;
+        const int r0 = A.rowBlocks[blk], r1 = A.rowBlocks[blk + 1];
+        const int k0 = A.rowPtr[r0], nnz = A.rowPtr[r1] - k0;
+        const int row = r0 + static_cast<int>(threadIdx.x);
+        float s = 0.0f;
+        if (nnz <= kSpmvLds)
+        {
+            for (int k = threadIdx.x; k < nnz; k += B) lds[k] = val32[k0 + k] * xp(A.col[k0 + k]);
+            __syncthreads();
+            if (row < r1)
+                for (int k = A.rowPtr[row] - k0, e = A.rowPtr[row + 1] - k0; k < e; ++k) s += lds[k];
+            __syncthreads();
+        }
+        else if (row < r1)
+        {
+            for (int k = A.rowPtr[row]; k < A.rowPtr[row + 1]; ++k) s += val32[k] * xp(A.col[k]);
+        }
+        if (row < r1) z[row] = xp(row) + wD[row] * (b[row] - s);
+    }
 }
 
 
@@ -692,6 +876,14 @@ Multigrid::Multigrid(Context& ctx, const DevPattern& fine, int maxLevels, int co
     }
     if (lv_.size() > 1 && lv_.back().n > kCoarseMax)
         throw std::invalid_argument("multigrid: coarsest level exceeds 2048 rows (raise maxLevels)");
+    // the coarse tail: the first level from which every level fits the tail's LDS
+    const int last = static_cast<int>(lv_.size()) - 1;
+    for (int l = 1; l < last && tailFrom_ < 0; ++l)
+    {
+        long long rows = 0;
+        for (int m = l; m <= last; ++m) rows += lv_[m].n;
+        if (last - l + 1 <= kTailLevels && 3 * rows <= kTailLds) tailFrom_ = l;
+    }
     for (std::size_t l = 0; l < lv_.size(); ++l)
     {
         MgLevel& L = lv_[l];
@@ -720,6 +912,9 @@ void Multigrid::setMatrix(const double* fineVal64, const float* pre, const float
                           const float* restr, const MgParams& p)
 {
     hipStream_t s = ctx_.stream();
+    if (p.kcycleDepth != p_.kcycleDepth || p.fusion != p_.fusion || p.coarseSweeps != p_.coarseSweeps ||
+        p.coarsestOmega != p_.coarsestOmega)
+        ++structVersion_;
     p_ = p;
     for (std::size_t l = 0; l < lv_.size(); ++l)
     {
@@ -751,11 +946,56 @@ void Multigrid::setMatrix(const double* fineVal64, const float* pre, const float
     CFX_CHECK_LAUNCH();
 }
 
-void Multigrid::solveLevel(int l, const int* done)
+bool Multigrid::useTail(int l) const
+{
+    return (p_.fusion & 1) && tailFrom_ > 0 && l >= tailFrom_ && l > p_.kcycleDepth && l < levels() - 1;
+}
+
+void Multigrid::launchTail(int l, const int* done)
+{
+    const int last = levels() - 1;
+    TailArgs a;
+    a.count = last - l + 1;
+    a.sweeps = p_.coarseSweeps;
+    a.omega = p_.coarsestOmega;
+    for (int m = 0; m < a.count; ++m)
+    {
+        const MgLevel& L = lv_[l + m];
+        const CsrView A = L.pat.view();
+        TailLevel& T = a.L[m];
+        T.n = L.n;
+        T.rowPtr = A.rowPtr;
+        T.col = A.col;
+        T.valS = L.valS32.as<float>();
+        T.val = L.val32.as<float>();
+        T.wPre = L.wPre32.as<float>();
+        T.wPost = L.wPost32.as<float>();
+        T.sc = L.s32.as<float>();
+        T.invD = L.invD32.as<float>();
+        if (l + m < last)
+        {
+            const CsrView R = L.rpat.view();
+            T.agg = L.agg.as<int>();
+            T.rRowPtr = R.rowPtr;
+            T.rCol = R.col;
+            T.rval = L.rval.as<float>();
+        }
+        else
+            T.dense = L.dense.p ? L.dense.as<float>() : nullptr;
+    }
+    k_mg_tail<<<1, kTailThreads, 0, ctx_.stream()>>>(a, lv_[l].b.as<float>(), lv_[l].e.as<float>(), done);
+}
+
+bool Multigrid::solveLevel(int l, const int* done, bool deferCombine)
 {
     hipStream_t s = ctx_.stream();
     MgLevel& L = lv_[l];
     const int last = levels() - 1;
+    if (useTail(l))
+    {
+        launchTail(l, done);
+        return false;
+    }
     if (l == last)
     {
         if (L.dense.p)
@@ -767,12 +1007,12 @@ void Multigrid::solveLevel(int l, const int* done)
                                                      L.b.as<float>(), L.e.as<float>(), p_.coarseSweeps,
                                                      p_.coarsestOmega, done);
         }
-        return;
+        return false;
     }
     if (l > p_.kcycleDepth)
     {
         visit(l, L.b.as<float>(), L.e.as<float>(), nullptr, nullptr, L.G, L.rbPerWg, done);
-        return;
+        return false;
     }
     const CsrView A = L.pat.view();
     const int Gv = std::max(1, std::min(64, divUp(L.n, 4 * B)));
@@ -784,7 +1024,9 @@ void Multigrid::solveLevel(int l, const int* done)
     visit(l, L.r2.as<float>(), L.c2.as<float>(), nullptr, nullptr, L.G, L.rbPerWg, done);
     k_kc_spmv2<<<L.G, B, 0, s>>>(A, L.rbPerWg, L.val32.as<float>(), L.c2.as<float>(), L.v1.as<float>(),
                                  L.r2.as<float>(), part, done);
+    if (deferCombine) return true;  // the caller forms e = k1 c1 + k2 c2 where it reads it
     k_kc_combine<<<Gv, B, 0, s>>>(L.n, L.c1.as<float>(), L.c2.as<float>(), L.e.as<float>(), part, L.G, done);
+    return false;
 }
 
 void Multigrid::visit(int l, const float* in, float* out, const float* q, double* partZQ, int Gs, int rbPerWg,
@@ -800,7 +1042,18 @@ void Multigrid::visit(int l, const float* in, float* out, const float* q, double
         MgLevel& C = lv_[l + 1];
         k_mg_restrict_csr<<<L.rpat.nRowBlocks, B, 0, s>>>(L.rpat.view(), L.rval.as<float>(), L.r.as<float>(),
                                                           C.b.as<float>(), done);
-        solveLevel(l + 1, done);
+        // fused only where the level is small: on a large level forming S P e per nonzero costs more
+        // gathers than the separate prolongation saves
+        const bool fuse = (p_.fusion & 2) && l >= 1 && L.n <= kFuseRowsMax;
+        const bool deferred = solveLevel(l + 1, done, fuse);
+        if (fuse)
+        {
+            k_mg_prolong_up<<<Gs, B, 0, s>>>(A, rbPerWg, L.val32.as<float>(), L.x.as<float>(), in,
+                                             L.wPost32.as<float>(), L.agg.as<int>(), L.s32.as<float>(),
+                                             C.e.as<float>(), C.c1.as<float>(), C.c2.as<float>(),
+                                             deferred ? C.part.as<double>() : nullptr, C.G, out, done);
+            return;
+        }
         k_mg_prolong<<<divUp(L.n, B), B, 0, s>>>(L.n, L.agg.as<int>(), L.s32.as<float>(), C.e.as<float>(),
                                                   L.x.as<float>(), done);
     }

@@ -37,11 +37,17 @@ __global__ void k_diag_abs(CsrView A, const double* __restrict__ val, double* __
     d[i] = v > 1e-300 ? v : 1e-300;
 }
 
-// node features x0 [n][8] and edge features ew, eg per CSR slot (0 on the diagonal)
-__global__ void k_features(CsrView A, const double* __restrict__ val, const double* __restrict__ dabs,
-                           const double* __restrict__ geo, const double* __restrict__ r0,
-                           const double* __restrict__ partMax, int Gm, float* __restrict__ x0,
-                           float* __restrict__ ew, float* __restrict__ eg)
+// node features x0 [n][8] and edge features ew, eg per CSR slot (0 on the
+// diagonal): 8 lanes per row, lane q taking slots q, q+8, ... of the row (a
+// wavefront streams 8 consecutive rows), row reductions over the 8 lanes,
+// lane c writing feature c
+constexpr int kFeatLanes = 8;
+static_assert(S::FIN == kFeatLanes, "one lane per node feature");
+__global__ __launch_bounds__(B) void k_features(CsrView A, const double* __restrict__ val,
+                                                const double* __restrict__ dabs, const double* __restrict__ geo,
+                                                const double* __restrict__ r0, const double* __restrict__ partMax,
+                                                int Gm, float* __restrict__ x0, float* __restrict__ ew,
+                                                float* __restrict__ eg)
 {
     __shared__ double red[B / kWave];
     __shared__ double rmaxS;
@@ -52,12 +58,13 @@ __global__ void k_features(CsrView A, const double* __restrict__ val, const doub
     if (threadIdx.x == 0) rmaxS = m;
     __syncthreads();
     const double rmax = rmaxS;
-    const int i = blockIdx.x * B + threadIdx.x;
-    if (i >= A.n) return;
+    const int i = blockIdx.x * (B / kFeatLanes) + static_cast<int>(threadIdx.x) / kFeatLanes;
+    const int q = threadIdx.x % kFeatLanes;
+    if (i >= A.n) return;  // all 8 lanes of a row leave together
     const double dd = dabs[i];
-    double off = 0.0, rs = 0.0, wmax = 0.0, wmin = 0.0, gsum = 0.0;
+    double off = 0.0, rs = 0.0, wmax = 0.0, wmin = HUGE_VAL, gsum = 0.0;
     int deg = 0;
-    for (int k = A.rowPtr[i]; k < A.rowPtr[i + 1]; ++k)
+    for (int k = A.rowPtr[i] + q; k < A.rowPtr[i + 1]; k += kFeatLanes)
     {
         const int j = A.col[k];
         const double v = val[k];
@@ -73,20 +80,34 @@ __global__ void k_features(CsrView A, const double* __restrict__ val, const doub
         ew[k] = static_cast<float>(w);
         eg[k] = static_cast<float>(g);
         off += fabs(v);
-        wmax = deg == 0 ? w : fmax(wmax, w);
-        wmin = deg == 0 ? w : fmin(wmin, w);
+        wmax = fmax(wmax, w);
+        wmin = fmin(wmin, w);
         gsum += g;
         ++deg;
     }
-    float* x = x0 + static_cast<long long>(i) * S::FIN;
-    x[0] = static_cast<float>(off / dd);
-    x[1] = static_cast<float>(rs / dd);
-    x[2] = static_cast<float>(deg / 8.0);
-    x[3] = static_cast<float>(wmax);
-    x[4] = static_cast<float>(wmin);
-    x[5] = static_cast<float>(deg > 0 ? gsum / deg : 0.0);
-    x[6] = static_cast<float>((r0 && rmax > 0.0) ? fabs(r0[i]) / rmax : 0.0);
-    x[7] = 1.0f;
+#pragma unroll
+    for (int o = kFeatLanes / 2; o > 0; o >>= 1)
+    {
+        off += __shfl_xor(off, o, kFeatLanes);
+        rs += __shfl_xor(rs, o, kFeatLanes);
+        gsum += __shfl_xor(gsum, o, kFeatLanes);
+        wmax = fmax(wmax, __shfl_xor(wmax, o, kFeatLanes));
+        wmin = fmin(wmin, __shfl_xor(wmin, o, kFeatLanes));
+        deg += __shfl_xor(deg, o, kFeatLanes);
+    }
+    double f;
+    switch (q)
+    {
+    case 0: f = off / dd; break;
+    case 1: f = rs / dd; break;
+    case 2: f = deg / 8.0; break;
+    case 3: f = wmax; break;
+    case 4: f = deg > 0 ? wmin : 0.0; break;
+    case 5: f = deg > 0 ? gsum / deg : 0.0; break;
+    case 6: f = (r0 && rmax > 0.0) ? fabs(r0[i]) / rmax : 0.0; break;
+    default: f = 1.0; break;
+    }
+    x0[static_cast<long long>(i) * S::FIN + q] = static_cast<float>(f);
 }
 
 // 16 consecutive feature values (16-byte aligned) as FP32, in 16-byte loads
@@ -512,7 +533,7 @@ void Mpnn::setGraph(const CsrView& A, const double* val, const double* geo, cons
     double* pm = scal_.as<double>();
     if (r0) k_absmax<<<Gm, B, 0, s>>>(A.n, r0, pm);
     k_diag_abs<<<divUp(A.n, B), B, 0, s>>>(A, val, dabs_.as<double>());
-    k_features<<<divUp(A.n, B), B, 0, s>>>(A, val, dabs_.as<double>(), geo, r0, pm, Gm, x0_.as<float>(),
+    k_features<<<divUp(A.n, B / kFeatLanes), B, 0, s>>>(A, val, dabs_.as<double>(), geo, r0, pm, Gm, x0_.as<float>(),
                                            ew_.as<float>(), eg_.as<float>());
     CFX_CHECK_LAUNCH();
 }

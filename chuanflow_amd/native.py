@@ -78,7 +78,7 @@ class MgParams(C.Structure):
     _fields_ = [("omegaPre", C.c_float), ("scale", C.c_float), ("omegaPost", C.c_float),
                 ("restrictScale", C.c_float), ("coarseOmega", C.c_float), ("coarseScale", C.c_float), ("coarseSweeps", C.c_int),
                 ("coarsestOmega", C.c_float), ("maxLevels", C.c_int), ("coarseMax", C.c_int),
-                ("kcycleDepth", C.c_int)]
+                ("kcycleDepth", C.c_int), ("fusion", C.c_int)]
 
 
 class TrainOptions(C.Structure):
@@ -189,6 +189,7 @@ _sig("cfx_csr_set_mg_weights", C.c_int, _vp, _vp, _vp, _vp, _vp)
 _sig("cfx_matrix_set_mg", C.c_int, _vp, C.POINTER(MgParams))
 _sig("cfx_matrix_set_mg_weights", C.c_int, _vp, _vp, _vp, _vp, _vp)
 _sig("cfx_csr_mg_info", C.c_int, _vp, C.POINTER(C.c_int), _vp, _vp, C.c_int)
+_sig("cfx_csr_mg_tail_level", C.c_int, _vp, C.POINTER(C.c_int))
 _sig("cfx_csr_mg_level", C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp)
 _sig("cfx_csr_mg_apply", C.c_int, _vp, _vp, _vp)
 _sig("cfx_csr_dic_info", C.c_int, _vp, C.POINTER(C.c_int), _vp)
@@ -746,6 +747,11 @@ class Csr:
         nnz = np.zeros(16, np.int64)
         self.ctx.check(_L.cfx_csr_mg_info(self.h, C.byref(lv), _p(rows), _p(nnz), 16))
         return rows[:lv.value].tolist(), nnz[:lv.value].tolist()
+
+    def mg_tail_level(self):
+        lv = C.c_int(0)
+        self.ctx.check(_L.cfx_csr_mg_tail_level(self.h, C.byref(lv)))
+        return lv.value
 
     def mg_level(self, l):
         """(rowPtr, col, FP64 values, agg or None) of hierarchy level l."""

@@ -202,6 +202,11 @@ typedef struct cfx_mg_params {
     float coarsestOmega;
     int maxLevels, coarseMax;           /* hierarchy depth cap, coarsest size cap (<= 2048; <= 128: exact solve) */
     int kcycleDepth;                    /* coarse levels 1..kcycleDepth use the K-cycle (0: V-cycle) */
+    int fusion;                         /* bit 0: the cycle of the small coarsest levels (below the K-cycle
+                                           levels) in one workgroup launch; bit 1: prolongation and K-cycle
+                                           combination fused into the post-smoothing of coarse levels up to
+                                           65536 rows (default 3); results
+                                           are bit for bit those of fusion 0 */
 } cfx_mg_params;
 int cfx_mg_default_params(cfx_mg_params* p);
 /* flexible CG replays one captured iteration per step (hipGraph, default 1)
@@ -228,6 +233,9 @@ int cfx_matrix_set_mg_weights(cfx_matrix* A, const float* d_pre, const float* d_
                               const float* d_restrict);
 /* hierarchy sizes: rows[l], nnz[l] for l < min(maxOut, levels) */
 int cfx_csr_mg_info(cfx_csr* A, int* levels, int64_t* rows, int64_t* nnz, int maxOut);
+/* first level whose cycle, with every coarser level, fits one workgroup (cfx_mg_params.fusion bit 0
+ * runs it as one launch below the K-cycle levels); -1: none */
+int cfx_csr_mg_tail_level(cfx_csr* A, int* level);
 
 /* precond = CFX_PRECOND_DIC (solver CG, precision CFX_FP32_IR): flexible CG
  * preconditioned by diagonal incomplete Cholesky, D_ii = a_ii - sum_{j<i}

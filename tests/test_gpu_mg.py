@@ -171,3 +171,52 @@ def test_graph_replay_equals_eager_launches(ctx, systems):
     ctx.set_graphs(True)
     assert reps[0] == reps[1] == reps[2]
     assert np.array_equal(xs[0], xs[1]) and np.array_equal(xs[0], xs[2])
+
+
+@pytest.mark.parametrize("case", ["lap40-k1", "fv24n-k0", "lap30-jacobi", "fv32-k2"])
+def test_fused_kernels_equal_separate_launches(ctx, systems, case):
+    """Launch fusion changes no bit: the small coarsest levels' cycle in one
+    workgroup (fusion bit 0; dense and Jacobi-swept coarsest level) and the
+    prolongation with the K-cycle combination folded into coarse
+    post-smoothing (bit 1) give the cycle of the separate kernels exactly,
+    and the same flexible-CG iterates."""
+    if case == "lap40-k1":
+        A, b = systems["lap40"][0], None
+        kw = dict(kcycleDepth=1)
+    elif case == "fv24n-k0":
+        A, b = systems["fv24n"]
+        kw = dict(kcycleDepth=0)
+    elif case == "lap30-jacobi":
+        A, b = laplacian3d(30, 30, 30, seed=8), None
+        kw = dict(kcycleDepth=0, coarseMax=1000, coarseSweeps=12)
+    else:
+        A, b = systems["fv32"]
+        kw = dict(kcycleDepth=2)
+    n = A.shape[0]
+    if b is None:
+        b = np.random.default_rng(9).standard_normal(n)
+    N, G = make(ctx, A)
+    G.set_mg(scale=1.8, coarseScale=1.8, **kw)
+    rows, _ = G.mg_info()
+    tail = G.mg_tail_level()
+    print(f"[{case}] levels {rows}, tail from level {tail}", flush=True)
+    if case != "fv32-k2":
+        assert 1 <= tail < len(rows) - 1 and tail > kw["kcycleDepth"]
+    else:
+        assert len(rows) >= 4  # a K-cycle level below level 1: its combination is deferred
+    if case == "lap30-jacobi":
+        assert rows[-1] > 128  # coarsest level swept, not inverted
+    r = ctx.array(np.random.default_rng(3).standard_normal(n).astype(np.float32))
+    zs, xs, its = [], [], []
+    for fusion in (0, 1, 2, 3):
+        G.set_mg(scale=1.8, coarseScale=1.8, fusion=fusion, **kw)
+        z = ctx.empty(n, np.float32)
+        G.mg_apply(r, z)
+        zs.append(z.get())
+        x, rep = G.solve(b, None, 1e-8, 500, N.SOLVER_CG, N.PRECOND_LEARNED, N.FP32_IR)
+        assert rep.converged
+        xs.append(x)
+        its.append(rep.iterations)
+    for k in (1, 2, 3):
+        assert np.array_equal(zs[0], zs[k]), k
+        assert its[0] == its[k] and np.array_equal(xs[0], xs[k]), k

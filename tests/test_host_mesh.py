@@ -149,3 +149,20 @@ def test_obstacle_channel_invariants():
     f = np.arange(ps[6], ps[6] + sz[6])
     toward = np.column_stack([cx - g["Cf"][f, 0], cy - g["Cf"][f, 1]])
     assert np.all((g["Sf"][f, :2] * toward).sum(axis=1) > 0)
+
+
+def test_oracle_piso_cavity_on_tets_uncorrected_closes_continuity():
+    """With the laplacian "uncorrected" the corrected face flux is exactly the
+    flux of the pressure equation just solved, so after every PISO step of the
+    closed Kuhn-tet cavity the continuity error is at the solver tolerance (no
+    right-hand-side projection needed)."""
+    import bench
+    hm, a, om = host_and_oracle(6, 6, 6, jitter=0.10, seed=3, tets=True, rcm=True)
+    Ubc, pbc = bench.cavity_bcs(a["patchStart"], a["patchSize"], len(a["neighbour"]), om.nB)
+    U, p, phi = np.zeros((om.nCells, 3)), np.zeros(om.nCells), np.zeros(om.nFaces)
+    ctl = O.PisoControls(dt=2e-3, correctors=2, continuityTol=1e-10, linearTol=1e-12, linearMaxIter=5000,
+                         nonOrthCorrectors=1)
+    for step in range(8):
+        r = O.piso_step(om, U, p, phi, 1e-3, O.Schemes(0, 0, 1, 0.9), ctl, Ubc, pbc)
+        assert r.continuityError < 1e-9, (step, r.continuityError)
+    assert 0 < np.abs(U).max() < 1.0
