@@ -7,9 +7,21 @@ Workload (one "step" = one pressure solve from x0 = 0):
   cell size (MT19937-64, seed 128), faces and cells shuffled (seed 7) and then
   RCM-renumbered; Laplacian coefficients (Sf.d)/|d|^2 (gamma = 1), all
   boundaries zeroGradient (pure Neumann); rhs N(0,1) (seed 2024) projected to
-  zero mean.  Solved to ||r||inf / ||b||inf < 1e-8 (the reference stop test)
-  by plain FP64 Jacobi-PCG and by FP32-inner Jacobi-PCG with FP64 iterative
-  refinement (convergence decided on the true FP64 residual).
+  zero mean.  Solved to ||r||inf / ||b||inf < 1e-8 (the reference stop test,
+  decided on true FP64 residuals) by
+    - plain FP64 Jacobi-PCG (the reference's solveCG on the GPU),
+    - FP32-inner Jacobi-PCG with FP64 reliable updates,
+    - flexible CG with the aggregation multigrid cycle (FP32, FP64 refinement),
+      tuned constants,
+    - the same with the message-passing network's per-row finest-level weights
+      (network inference included in the time): `value`.
+  The explicit non-orthogonal corrector is not part of the timed system: its
+  boundary-face terms do not sum to zero, so with it the pure Neumann system
+  has no solution and no solver reaches 1e-8 (the reference's included).
+
+Other workloads: --workload c3 (PISO lid-driven cavity on 6M Kuhn tets),
+c5 (64 batched detail regions), spmv (SpMV and a copy of known size, for the
+HBM traffic counters).
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """

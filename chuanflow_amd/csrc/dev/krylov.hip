@@ -5,10 +5,11 @@
 //   spmv_dot : q = A p  (LDS-staged CSR-stream)        + partial p.q
 //   update   : x += a p, r -= a q, z = invD r          + partial r.z, max|r|
 //   direction: p = invD r + b p
-// Each reduction finishes in the last workgroup to arrive (ticket), which
-// writes the new scalars into the device-resident SolveState; the host only
-// polls `done` between batches of iterations, sized from the observed
-// convergence rate.
+// Reductions leave per-workgroup partials (grids capped at 2048 workgroups);
+// a one-workgroup kernel, or the consumer kernel itself, sums them in a fixed
+// order and writes the new scalars into the device-resident SolveState (no
+// atomics, deterministic); the host only polls `done` between batches of
+// iterations, sized from the observed convergence rate.
 #include "krylov.h"
 
 #include <algorithm>
@@ -187,39 +188,43 @@ __global__ __launch_bounds__(kSpmvBlock) void k_true_resid_final(int G, const do
     }
 }
 
-// r = b - A x with reductions max|r|, max|b| (NaN-flagged), sum r^2, sum b^2.
+// r = b - A x over rbPerWg row blocks per workgroup, with per-workgroup
+// partials of max|r|, max|b| (NaN-flagged), sum r^2, sum b^2; k_residual_final
+// (one workgroup) reduces them in a fixed order and sets the state.
 // mode 0: initial residual of a solve (sets initialResidual, bnorm and the
 //         early-convergence exit, solveCG :167-180);
 // mode 1: refinement residual (sets rnorm; converged if rnorm/bnorm < tol);
 // mode 2: reductions only (sum0 = rnorm, rho = ||r||2 / ||b||2).
 template <typename T>
-__global__ __launch_bounds__(kSpmvBlock) void k_residual(CsrView A, const T* __restrict__ val,
-                                                         const T* __restrict__ x,
-                                                         const T* __restrict__ b, T* __restrict__ r,
-                                                         SolveState* st, double* part, int* ipart,
-                                                         int mode)
+__global__ __launch_bounds__(kSpmvBlock) void k_residual(CsrView A, int rbPerWg, const T* __restrict__ val,
+                                                         const T* __restrict__ x, const T* __restrict__ b,
+                                                         T* __restrict__ r, double* part, int* ipart)
 {
     __shared__ T lds[kSpmvLds];
     __shared__ double red[kSpmvBlock / kWave];
     __shared__ int ired[kSpmvBlock / kWave];
-    __shared__ int flag;
     const int G = gridDim.x;
-    const int blk = xcdRemap(blockIdx.x, G);
-    int row;
-    T s;
+    const int w = xcdRemap(blockIdx.x, G);
     double rm = 0, bm = 0, r2 = 0, b2 = 0;
     int nan = 0;
-    if (spmvRowSum(A, val, x, blk, lds, row, s))
+    for (int j = 0; j < rbPerWg; ++j)
     {
-        const T bb = b[row];
-        const T rr = bb - s;
-        r[row] = rr;
-        const double rd = rr, bd = bb;
-        rm = fabs(rd);
-        bm = fabs(bd);
-        r2 = rd * rd;
-        b2 = bd * bd;
-        nan = (rd != rd ? 1 : 0) | (bd != bd ? 2 : 0);
+        const int blk = w * rbPerWg + j;
+        if (blk >= A.nRowBlocks) break;
+        int row;
+        T s;
+        if (spmvRowSum(A, val, x, blk, lds, row, s))
+        {
+            const T bb = b[row];
+            const T rr = bb - s;
+            r[row] = rr;
+            const double rd = rr, bd = bb;
+            rm = fmax(rm, fabs(rd));
+            bm = fmax(bm, fabs(bd));
+            r2 += rd * rd;
+            b2 += bd * bd;
+            nan |= (rd != rd ? 1 : 0) | (bd != bd ? 2 : 0);
+        }
     }
     rm = blockMax(rm, red);
     bm = blockMax(bm, red);
@@ -228,13 +233,19 @@ __global__ __launch_bounds__(kSpmvBlock) void k_residual(CsrView A, const T* __r
     nan = blockOr(nan, ired);
     if (threadIdx.x == 0)
     {
-        part[blk] = rm;
-        part[G + blk] = bm;
-        part[2 * G + blk] = r2;
-        part[3 * G + blk] = b2;
-        ipart[blk] = nan;
+        part[w] = rm;
+        part[G + w] = bm;
+        part[2 * G + w] = r2;
+        part[3 * G + w] = b2;
+        ipart[w] = nan;
     }
-    if (!lastBlockDone(&st->ticket[0], &flag)) return;
+}
+
+__global__ __launch_bounds__(kSpmvBlock) void k_residual_final(int G, const double* part, const int* ipart,
+                                                               SolveState* st, int mode)
+{
+    __shared__ double red[kSpmvBlock / kWave];
+    __shared__ int ired[kSpmvBlock / kWave];
     const double RM = maxPartials(part, G, red);
     const double BM = maxPartials(part + G, G, red);
     const double R2 = sumPartials(part + 2 * G, G, red);
@@ -281,11 +292,10 @@ __global__ __launch_bounds__(kSpmvBlock) void k_residual(CsrView A, const T* __r
 // r0 = r, p = v = 0, rho/beta of iteration 1 (solveBiCGStab :253-266)
 __global__ __launch_bounds__(kVecBlock) void k_bicg_init(int n, const double* __restrict__ r,
                                                          double* __restrict__ r0, double* __restrict__ p,
-                                                         double* __restrict__ v, SolveState* st,
+                                                         double* __restrict__ v, const SolveState* st,
                                                          double* part)
 {
     __shared__ double red[kVecBlock / kWave];
-    __shared__ int flag;
     if (st->done) return;
     double s = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -298,8 +308,13 @@ __global__ __launch_bounds__(kVecBlock) void k_bicg_init(int n, const double* __
     }
     s = blockSum(s, red);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
-    if (!lastBlockDone(&st->ticket[0], &flag)) return;
-    const double rhoNew = sumPartials(part, gridDim.x, red);
+}
+
+__global__ __launch_bounds__(kVecBlock) void k_bicg_init_final(int G, const double* part, SolveState* st)
+{
+    __shared__ double red[kVecBlock / kWave];
+    if (st->done) return;
+    const double rhoNew = sumPartials(part, G, red);
     if (threadIdx.x == 0)
     {
         st->rho = 1.0;
@@ -337,31 +352,42 @@ __global__ __launch_bounds__(kVecBlock) void k_bicg_p(int n, const double* __res
 // y = A x, then a dot with `w` (r0.v) or two dots (t.t, t.s) in the epilogue.
 // which == 0: v = A z, denom = r0.v, alpha = rho/denom   (:273-277)
 // which == 1: t = A sh, tt = t.t, ts = t.s, omega = ts/tt (:290-300)
-__global__ __launch_bounds__(kSpmvBlock) void k_bicg_spmv(CsrView A, const double* __restrict__ val,
+// (per-workgroup partials over rbPerWg row blocks; k_bicg_dot_final reduces them)
+__global__ __launch_bounds__(kSpmvBlock) void k_bicg_spmv(CsrView A, int rbPerWg, const double* __restrict__ val,
                                                           const double* __restrict__ x,
                                                           double* __restrict__ y,
                                                           const double* __restrict__ w,
-                                                          SolveState* st, double* part, int which)
+                                                          const SolveState* st, double* part, int which)
 {
     __shared__ double lds[kSpmvLds];
     __shared__ double red[kSpmvBlock / kWave];
-    __shared__ int flag;
     if (st->done) return;
     const int G = gridDim.x;
-    const int blk = xcdRemap(blockIdx.x, G);
-    int row;
-    double s;
+    const int wg = xcdRemap(blockIdx.x, G);
     double d0 = 0, d1 = 0;
-    if (spmvRowSum(A, val, x, blk, lds, row, s))
+    for (int j = 0; j < rbPerWg; ++j)
     {
-        y[row] = s;
-        if (which == 0) d0 = w[row] * s;
-        else { d0 = s * s; d1 = s * w[row]; }
+        const int blk = wg * rbPerWg + j;
+        if (blk >= A.nRowBlocks) break;
+        int row;
+        double s;
+        if (spmvRowSum(A, val, x, blk, lds, row, s))
+        {
+            y[row] = s;
+            if (which == 0) d0 += w[row] * s;
+            else { d0 += s * s; d1 += s * w[row]; }
+        }
     }
     d0 = blockSum(d0, red);
     d1 = blockSum(d1, red);
-    if (threadIdx.x == 0) { part[blk] = d0; part[G + blk] = d1; }
-    if (!lastBlockDone(&st->ticket[0], &flag)) return;
+    if (threadIdx.x == 0) { part[wg] = d0; part[G + wg] = d1; }
+}
+
+__global__ __launch_bounds__(kSpmvBlock) void k_bicg_dot_final(int G, const double* part, SolveState* st,
+                                                               int which)
+{
+    __shared__ double red[kSpmvBlock / kWave];
+    if (st->done) return;
     const double D0 = sumPartials(part, G, red);
     const double D1 = sumPartials(part + G, G, red);
     if (threadIdx.x == 0)
@@ -408,12 +434,11 @@ __global__ __launch_bounds__(kVecBlock) void k_bicg_r(int n, double* __restrict_
                                                       const double* __restrict__ s,
                                                       const double* __restrict__ t,
                                                       const double* __restrict__ r0,
-                                                      double* __restrict__ r, SolveState* st,
+                                                      double* __restrict__ r, const SolveState* st,
                                                       double* part, int* ipart)
 {
     __shared__ double red[kVecBlock / kWave];
     __shared__ int ired[kVecBlock / kWave];
-    __shared__ int flag;
     if (st->done) return;
     const bool tsmall = st->breakdown == 2;
     const double omega = st->omega;
@@ -441,7 +466,15 @@ __global__ __launch_bounds__(kVecBlock) void k_bicg_r(int n, double* __restrict_
     rr0 = blockSum(rr0, red);
     nan = blockOr(nan, ired);
     if (threadIdx.x == 0) { part[blockIdx.x] = rm; part[G + blockIdx.x] = rr0; ipart[blockIdx.x] = nan; }
-    if (!lastBlockDone(&st->ticket[0], &flag)) return;
+}
+
+__global__ __launch_bounds__(kVecBlock) void k_bicg_r_final(int G, const double* part, const int* ipart,
+                                                            SolveState* st)
+{
+    __shared__ double red[kVecBlock / kWave];
+    __shared__ int ired[kVecBlock / kWave];
+    if (st->done) return;
+    const bool tsmall = st->breakdown == 2;
     const double RM = maxPartials(part, G, red);
     const double RR0 = sumPartials(part + G, G, red);
     const int NAN_ = orPartials(ipart, G, ired);
@@ -606,11 +639,16 @@ SolveReport Krylov::bicgstab(const CsrView& A, const double* val, const double* 
     double* part = partials_.as<double>();
     int* ipart = reinterpret_cast<int*>(part + 4 * maxBlocks_);
     const int Gv = vecGridFor(n_);
+    int Gs = std::min(A.nRowBlocks, kVecGridMax);
+    const int rbPerWg = divUp(A.nRowBlocks, Gs);
+    Gs = std::max(1, divUp(A.nRowBlocks, rbPerWg));
     double *r = r_.as<double>(), *r0 = r0_.as<double>(), *p = p_.as<double>(), *v = v_.as<double>();
     double *z = z_.as<double>(), *sv = s_.as<double>(), *sh = sh_.as<double>(), *t = t_.as<double>();
     const double* invD = invDb_.as<double>();
-    k_residual<double><<<A.nRowBlocks, kSpmvBlock, 0, s>>>(A, val, x, b, r, st, part, ipart, 0);
+    k_residual<double><<<Gs, kSpmvBlock, 0, s>>>(A, rbPerWg, val, x, b, r, part, ipart);
+    k_residual_final<<<1, kSpmvBlock, 0, s>>>(Gs, part, ipart, st, 0);
     k_bicg_init<<<Gv, kVecBlock, 0, s>>>(n_, r, r0, p, v, st, part);
+    k_bicg_init_final<<<1, kVecBlock, 0, s>>>(Gv, part, st);
     CFX_CHECK_LAUNCH();
     poll();
     SolveState* hs = ctx_.hostState() + 1;
@@ -622,10 +660,13 @@ SolveReport Krylov::bicgstab(const CsrView& A, const double* val, const double* 
         for (int k = 0; k < batch; ++k)
         {
             k_bicg_p<<<Gv, kVecBlock, 0, s>>>(n_, invD, r, p, v, z, st);
-            k_bicg_spmv<<<A.nRowBlocks, kSpmvBlock, 0, s>>>(A, val, z, v, r0, st, part, 0);
+            k_bicg_spmv<<<Gs, kSpmvBlock, 0, s>>>(A, rbPerWg, val, z, v, r0, st, part, 0);
+            k_bicg_dot_final<<<1, kSpmvBlock, 0, s>>>(Gs, part, st, 0);
             k_bicg_s<<<Gv, kVecBlock, 0, s>>>(n_, invD, x, z, r, v, sv, sh, st);
-            k_bicg_spmv<<<A.nRowBlocks, kSpmvBlock, 0, s>>>(A, val, sh, t, sv, st, part, 1);
+            k_bicg_spmv<<<Gs, kSpmvBlock, 0, s>>>(A, rbPerWg, val, sh, t, sv, st, part, 1);
+            k_bicg_dot_final<<<1, kSpmvBlock, 0, s>>>(Gs, part, st, 1);
             k_bicg_r<<<Gv, kVecBlock, 0, s>>>(n_, x, sh, sv, t, r0, r, st, part, ipart);
+            k_bicg_r_final<<<1, kVecBlock, 0, s>>>(Gv, part, ipart, st);
         }
         CFX_CHECK_LAUNCH();
         launched += batch;

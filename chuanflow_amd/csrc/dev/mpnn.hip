@@ -167,18 +167,35 @@ __global__ __launch_bounds__(B) void k_layer_fwd(int n, CsrView A, const float* 
         if (i < n)
         {
             load16(hin + static_cast<long long>(i) * S::H + f0, self);
-            for (int k = A.rowPtr[i]; k < A.rowPtr[i + 1]; ++k)
+            // two neighbours per step: both rows' loads are in flight before either is used; the
+            // sums still take the neighbours in slot order
+            const int kEnd = A.rowPtr[i + 1];
+            for (int k = A.rowPtr[i]; k < kEnd; k += 2)
             {
-                const int j = A.col[k];
-                if (j == i) continue;
-                const float w = ew[k], gg = eg[k];
-                float v[16];
-                load16(hin + static_cast<long long>(j) * S::H + f0, v);
-#pragma unroll
-                for (int c = 0; c < 16; ++c)
+                const bool two = k + 1 < kEnd;
+                const int j0 = A.col[k], j1 = two ? A.col[k + 1] : j0;
+                const float w0 = ew[k], g0 = eg[k];
+                const float w1 = two ? ew[k + 1] : 0.0f, g1 = two ? eg[k + 1] : 0.0f;
+                float v0[16], v1[16];
+                load16(hin + static_cast<long long>(j0) * S::H + f0, v0);
+                load16(hin + static_cast<long long>(j1) * S::H + f0, v1);
+                if (j0 != i)
                 {
-                    m[c] += w * v[c];
-                    g[c] += gg * v[c];
+#pragma unroll
+                    for (int c = 0; c < 16; ++c)
+                    {
+                        m[c] += w0 * v0[c];
+                        g[c] += g0 * v0[c];
+                    }
+                }
+                if (two && j1 != i)
+                {
+#pragma unroll
+                    for (int c = 0; c < 16; ++c)
+                    {
+                        m[c] += w1 * v1[c];
+                        g[c] += g1 * v1[c];
+                    }
                 }
             }
         }
