@@ -700,6 +700,18 @@ int cfx_csr_set_values(cfx_csr* A, const double* vals)
     });
 }
 
+int cfx_csr_set_values_device(cfx_csr* A, const double* d_vals)
+{
+    if (!A || !d_vals) return CFX_E_INVALID;
+    return guard(A->owner, [&] {
+        CFX_HIP(hipMemcpyAsync(A->val.p, d_vals, sizeof(double) * A->pattern.nnz, hipMemcpyDeviceToDevice,
+                               A->owner->ctx->stream()));
+        A->val32Valid = false;
+        A->mg.valid = false;
+        A->dicValid = false;
+    });
+}
+
 int cfx_csr_spmv(cfx_csr* A, const double* x, double* y)
 {
     if (!A) return CFX_E_INVALID;

@@ -13,6 +13,7 @@
 #include "krylov.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 
 namespace cfx
@@ -43,6 +44,8 @@ std::vector<int> buildRowBlocks(const std::vector<int>& rowPtr)
 
 void DevPattern::build(std::vector<int> rowPtrH, std::vector<int> colH, hipStream_t s)
 {
+    static std::atomic<std::uint64_t> counter{0};
+    uid = ++counter;
     n = static_cast<int>(rowPtrH.size()) - 1;
     nnz = rowPtrH.back();
     std::vector<int> diag(static_cast<std::size_t>(n), -1);

@@ -48,7 +48,7 @@ private:
     Context& ctx_;
     Mpnn& net_;
     std::vector<std::unique_ptr<Multigrid>> mg_;
-    std::vector<const DevPattern*> mgPat_;
+    std::vector<std::uint64_t> mgPat_;  // DevPattern::uid each hierarchy was built for
     DevBuf a_, t_, ga_, gt_, e_, grad_;
     std::size_t cap_ = 0;
     Multigrid& hierarchy(std::size_t i, const TrainSystem& s, const MgParams& p);

@@ -85,12 +85,13 @@ Multigrid& MpnnTrainer::hierarchy(std::size_t i, const TrainSystem& s, const MgP
     if (mg_.size() <= i)
     {
         mg_.resize(i + 1);
-        mgPat_.resize(i + 1, nullptr);
+        mgPat_.resize(i + 1, 0);
     }
-    if (!mg_[i] || mgPat_[i] != s.pat || mg_[i]->params().maxLevels != p.maxLevels)
+    if (!mg_[i] || mgPat_[i] != s.pat->uid || mg_[i]->params().maxLevels != p.maxLevels)
     {
+        mg_[i].reset();
         mg_[i] = std::make_unique<Multigrid>(ctx_, *s.pat, p.maxLevels, p.coarseMax);
-        mgPat_[i] = s.pat;
+        mgPat_[i] = s.pat->uid;
     }
     return *mg_[i];
 }

@@ -93,6 +93,7 @@ struct CsrView
 struct DevPattern
 {
     int n = 0, nnz = 0, nRowBlocks = 0;
+    std::uint64_t uid = 0;  // unique per build (caches keyed on a pattern compare this, not the address)
     DevBuf rowPtr, col, diagSlot, rowBlocks;
     std::vector<int> hRowPtr, hCol;
     CsrView view() const

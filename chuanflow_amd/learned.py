@@ -109,6 +109,146 @@ def training_set(ctx, cases=8, size=(48, 24, 24), seed=1, per_case=3, iterations
     return out
 
 
+def channel_run_device(ctx, hm, Uin, iterations, every=1, max_dumps=None, nu=1e-3, pressure=None, driver="simple",
+                       h=None):
+    """Runs the GPU SIMPLE loop (`iterations` outer iterations) or the PISO
+    transient from rest (`iterations` pressure solves: iterations / 2 steps of
+    2 correctors, dt = 0.25 h / Uin) on a channel mesh, copying every
+    `every`-th pressure system into device memory (values [max][nnz], rhs
+    [max][n]).  Returns a dict with the device mesh, the dump arrays and count,
+    the pattern, the per-slot geometry and the driver result (`diverged`)."""
+    arrays, geom = hm.arrays(), hm.geometry()
+    dm = N.Mesh(ctx, arrays, geom)
+    n, nnz = dm.nCells, dm.sizes()["nnz"]
+    Ubc, pbc = channel_bcs(arrays["patchStart"], arrays["patchSize"], len(arrays["neighbour"]), dm.nB, Uin)
+    dU, dp, dphi = ctx.zeros((3, n)), ctx.zeros(n), ctx.zeros(dm.nFaces)
+    lc, lk = dm.linearise(3, *Ubc)
+    dm.correct_boundary_flux(dU, lc, lk, dphi)
+    max_dumps = max_dumps or (iterations + every - 1) // every
+    dval, drhs = ctx.empty((max_dumps, nnz)), ctx.empty((max_dumps, n))
+    dm.set_pressure_dump(dval, drhs, every, max_dumps)
+    if pressure == N.FP32_MG:
+        dm.set_mg(**TUNED_MG)
+    prec = N.FP32_IR if pressure is None else pressure
+    if driver == "simple":
+        ctl = N.SimpleControls(maxIterations=iterations, continuityTol=0, pcorrTol=0, momentumTol=0,
+                               linearTol=1e-6, linearMaxIter=3000, nonOrthCorrectors=2, relaxU=0.9, relaxP=0.03,
+                               pRelaxStart=0.02, pPrecision=prec)
+        res = dm.simple(nu, N.Schemes(0, 1, 1, 0.9), ctl, Ubc, pbc, dU, dp, dphi)
+        diverged = bool(res.diverged)
+    else:
+        ctl = N.PisoControls(dt=0.25 * h / Uin, correctors=2, continuityTol=0, linearTol=1e-6, linearMaxIter=3000,
+                             nonOrthCorrectors=1, pPrecision=prec)
+        diverged = False
+        for _ in range((iterations + 1) // 2):
+            res = dm.piso_step(nu, N.Schemes(0, 1, 1, 0.9), ctl, Ubc, pbc, dU, dp, dphi)
+            if not np.isfinite(res.continuityError):
+                diverged = True
+                break
+    count = dm.pressure_dump_count()
+    dm.set_pressure_dump(None, None, 0, 0)
+    rp, cl = dm.pattern()
+    rp, cl = rp.astype(np.int64), cl.astype(np.int64)
+    return dict(dm=dm, dval=dval, drhs=drhs, count=count, n=n, nnz=nnz, rp=rp, cl=cl,
+                geo=slot_geometry(arrays, geom, rp, cl), res=res, diverged=diverged)
+
+
+def _fcg_iterations(G, b, net, dgeo, maxIter):
+    G.set_mpnn(net, dgeo if net is not None else None)
+    _, rep = G.solve(b, None, 1e-8, maxIter, N.SOLVER_CG, N.PRECOND_LEARNED, N.FP32_IR)
+    G.set_mpnn(None)
+    return rep.iterations if rep.converged and rep.precondFallbacks == 0 else None
+
+
+def stream_train(ctx, net, channels=20, size=(128, 128, 128), length=(1.0, 1.0, 1.0), per_channel=100,
+                 steps_per_system=2, lr=1e-4, seed=11, samples=2, smooth=4, log=None, driver="piso"):
+    """Trains `net` on pressure systems streamed from the GPU flow solver: per
+    seeded obstacle channel (radius U[0.05, 0.15], inlet speed U[0.005, 0.05])
+    the transient from rest (PISO, or SIMPLE iterations) leaves `per_channel`
+    pressure systems in device memory, and the network takes
+    `steps_per_system` Adam steps on each before the next channel runs.  No
+    system is kept beyond its channel.
+
+    The error samples alternate between smooth ones (`smooth` cycles applied
+    to a random vector: the error flexible CG leaves) and raw random vectors
+    (which keep the smoother weights where high frequencies are damped: a
+    smoother tuned on smooth error alone drifts to weights that amplify them).
+    After each channel the network must solve that channel's last system
+    within the iterations of the tuned constants (K-cycle, FP64 residual
+    1e-8); otherwise its parameters return to those after the previous
+    channel.  Returns (loss history, systems used, channels rejected)."""
+    rng = np.random.default_rng(seed)
+    hist, used, rejected = [], 0, 0
+    keep = net.params()
+    for c in range(channels):
+        hm, radius = channel_mesh(*size, seed=seed * 1000 + c, length=length)
+        Uin = rng.uniform(0.005, 0.05)
+        run = channel_run_device(ctx, hm, Uin, per_channel, pressure=N.FP32_MG, driver=driver,
+                                 h=length[0] / size[0])
+        if run["diverged"]:
+            if log:
+                log(f"channel {c}: {driver} diverged, skipped")
+            continue
+        G = N.Csr(ctx, run["rp"], run["cl"], np.zeros(run["nnz"]))
+        G.set_mg(**TUNED_MG)
+        dgeo = ctx.array(run["geo"])
+        before = len(hist)
+        for k in range(run["count"]):
+            G.set_values_device(run["dval"].offset(k * run["nnz"]))
+            h = net.train([G], [dgeo], [run["drhs"].offset(k * run["n"])], steps=steps_per_system,
+                          samples=samples, smooth=smooth if k % 2 == 0 else 0, lr=lr,
+                          seed=seed * 100003 + used)
+            hist.extend(h.tolist())
+            used += 1
+        b = run["drhs"].get()[run["count"] - 1]
+        base = _fcg_iterations(G, b, None, None, 1000)
+        got = _fcg_iterations(G, b, net, dgeo, 1000 if base is None else base)
+        ok = got is not None and base is not None and got <= base
+        if ok:
+            keep = net.params()
+        else:
+            net.set_params(keep)
+            rejected += 1
+        if log:
+            seg = np.asarray(hist[before:])
+            log(f"channel {c}: radius {radius:.3f}, Uin {Uin:.4f}, {run['count']} systems, "
+                f"loss {seg[:10].mean():.4f} -> {seg[-10:].mean():.4f}; last system {got} vs {base} iterations"
+                f"{'' if ok else ' -> rejected, parameters of the previous channel restored'}")
+        del G, run
+    return np.asarray(hist), used, rejected
+
+
+def heldout_iterations(ctx, nets, channels=2, size=(128, 128, 128), length=(1.0, 1.0, 1.0), seed=77,
+                       iterations=61, every=30, tol=1e-8, driver="piso"):
+    """Flexible-CG iterations to tol (FP32 + FP64 refinement) on held-out
+    channel pressure systems with the tuned constants and with each network
+    of `nets` (name -> Mpnn); one dict per system."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for c in range(channels):
+        hm, radius = channel_mesh(*size, seed=seed * 1000 + c, length=length)
+        Uin = rng.uniform(0.005, 0.05)
+        run = channel_run_device(ctx, hm, Uin, iterations, every=every, pressure=N.FP32_MG, driver=driver,
+                                 h=length[0] / size[0])
+        if run["diverged"]:
+            continue
+        G = N.Csr(ctx, run["rp"], run["cl"], np.zeros(run["nnz"]))
+        dgeo = ctx.array(run["geo"])
+        for k in range(run["count"]):
+            G.set_values_device(run["dval"].offset(k * run["nnz"]))
+            b = run["drhs"].get()[k]
+            row = {"channel": c, "system": k, "cells": run["n"]}
+            for name, net in [("tuned", None)] + list(nets.items()):
+                G.set_mg(**TUNED_MG)
+                G.set_mpnn(net, dgeo if net is not None else None)
+                _, rep = G.solve(b, None, tol, 2000, N.SOLVER_CG, N.PRECOND_LEARNED, N.FP32_IR)
+                row[name] = rep.iterations
+                row[name + "_converged"] = bool(rep.converged)
+            G.set_mpnn(None)
+            out.append(row)
+    return out
+
+
 def to_device(ctx, systems, mg=None):
     """Csr objects (with multigrid constants) and device geometry / rhs."""
     csrs, geos, rhss = [], [], []

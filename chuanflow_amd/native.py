@@ -154,6 +154,7 @@ _sig("cfx_matrix_spmv", C.c_int, _vp, C.c_int, _vp, _vp)
 _sig("cfx_csr_create", C.c_int, _vp, _i64, _vp, _vp, _vp, C.POINTER(_vp))
 _sig("cfx_csr_destroy", C.c_int, _vp)
 _sig("cfx_csr_set_values", C.c_int, _vp, _vp)
+_sig("cfx_csr_set_values_device", C.c_int, _vp, _vp)
 _sig("cfx_csr_spmv", C.c_int, _vp, _vp, _vp)
 _sig("cfx_csr_solve", C.c_int, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
      C.POINTER(Report))
@@ -690,6 +691,10 @@ class Csr:
                 _L.cfx_csr_destroy(self.h)
         except Exception:
             pass
+
+    def set_values_device(self, d_vals):
+        """Values from a device array or raw device pointer (nnz FP64)."""
+        self.ctx.check(_L.cfx_csr_set_values_device(self.h, _p(d_vals)))
 
     def set_values(self, vals):
         self.ctx.check(_L.cfx_csr_set_values(self.h, _p(np.ascontiguousarray(vals, np.float64))))

@@ -165,6 +165,9 @@ int cfx_csr_create(cfx_ctx* ctx, int64_t n, const int64_t* rowStart, const int64
 int cfx_csr_destroy(cfx_csr* A);
 /* replace the values (same pattern) */
 int cfx_csr_set_values(cfx_csr* A, const double* vals);
+/* the same from a device array (stream-ordered, no host synchronisation); no counterpart in the
+ * reference, whose matrices live in host memory */
+int cfx_csr_set_values_device(cfx_csr* A, const double* d_vals);
 /* SparseMatrix::product (Matrix.h:47, Matrix.cpp:119-133) */
 int cfx_csr_spmv(cfx_csr* A, const double* d_x, double* d_y);
 /* solveCG / solveBiCGStab / solveWithBackend (Matrix.h:67-89): host b, x
