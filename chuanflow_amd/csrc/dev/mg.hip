@@ -66,18 +66,25 @@ int aggregate(const std::vector<int>& rp, const std::vector<int>& col, std::vect
 }
 
 // ------------------------------------------------------------------ kernels
+// coarse nonzero c = sum of its fine nonzeros gPerm[gStart[c] .. gStart[c+1]): L lanes per coarse
+// nonzero (lane q takes every L-th fine slot), then a fixed butterfly over the L lanes, so the result
+// does not depend on the launch
+template <int L>
 __global__ void k_galerkin(int nnzC, const int* __restrict__ gStart, const int* __restrict__ gPerm,
                            const double* __restrict__ fine, double* __restrict__ coarse)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nnzC) return;
+    const long long g = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int c = static_cast<int>(g / L), q = static_cast<int>(g % L);
+    if (c >= nnzC) return;  // the L lanes of one coarse nonzero leave together
     double s = 0.0;
-    for (int t = gStart[c]; t < gStart[c + 1]; ++t) s += fine[gPerm[t]];
-    coarse[c] = s;
+    for (int t = gStart[c] + q; t < gStart[c + 1]; t += L) s += fine[gPerm[t]];
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, L);
+    if (q == 0) coarse[c] = s;
 }
 
-// per row: FP32 values, D^-1, Wpre = a D^-1, Wpost = c D^-1, prolongation scale
-__global__ void k_level_setup(CsrView A, const double* __restrict__ val64, float* __restrict__ val32,
+// per row: D^-1, Wpre = a D^-1, Wpost = c D^-1, prolongation and restriction scale
+__global__ void k_level_setup(CsrView A, const double* __restrict__ val64,
                               float* __restrict__ invD32, float* __restrict__ wPre, float* __restrict__ wPost,
                               float* __restrict__ s32, float* __restrict__ t32, const float* __restrict__ aIn,
                               float aC, const float* __restrict__ sIn, float sC, const float* __restrict__ cIn,
@@ -85,7 +92,6 @@ __global__ void k_level_setup(CsrView A, const double* __restrict__ val64, float
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.n) return;
-    for (int k = A.rowPtr[i]; k < A.rowPtr[i + 1]; ++k) val32[k] = static_cast<float>(val64[k]);
     const double d = A.diagSlot[i] >= 0 ? val64[A.diagSlot[i]] : 0.0;
     const float inv = static_cast<float>(fabs(d) > 1e-300 ? 1.0 / d : 1.0);
     invD32[i] = inv;
@@ -95,13 +101,18 @@ __global__ void k_level_setup(CsrView A, const double* __restrict__ val64, float
     t32[i] = tIn ? tIn[i] : tC;
 }
 
-// A Wpre: column-scaled values so that r = b - A (Wpre b) is one SpMV over b
-__global__ void k_scale_vals(CsrView A, const float* __restrict__ val32, const float* __restrict__ wD32,
-                             float* __restrict__ valS)
+// per nonzero (flat over the slots): the FP32 copy and A Wpre, the column-scaled values with which
+// r = b - A (Wpre b) is one SpMV over b
+__global__ void k_scale_vals(CsrView A, const double* __restrict__ val64, float* __restrict__ val32,
+                             const float* __restrict__ wD32, float* __restrict__ valS)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.n) return;
-    for (int k = A.rowPtr[i]; k < A.rowPtr[i + 1]; ++k) valS[k] = val32[k] * wD32[A.col[k]];
+    for (long long k = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; k < A.nnz;
+         k += static_cast<long long>(gridDim.x) * blockDim.x)
+    {
+        const float v = static_cast<float>(val64[k]);
+        val32[k] = v;
+        valS[k] = v * wD32[A.col[k]];
+    }
 }
 
 // CSR-stream row sums of y = M v for the rows of block blk (M = valM)
@@ -925,19 +936,30 @@ void Multigrid::setMatrix(const double* fineVal64, const float* pre, const float
         {
             const MgLevel& F = lv_[l - 1];
             const double* fv = l == 1 ? fineVal64 : F.val64.as<double>();
-            k_galerkin<<<divUp(L.nnz, B), B, 0, s>>>(L.nnz, F.gStart.as<int>(), F.gPerm.as<int>(), fv,
-                                                     L.val64.as<double>());
+            // lanes per coarse nonzero from the mean number of fine nonzeros it sums
+            const double per = static_cast<double>(F.nnz) / std::max(1, L.nnz);
+            if (per >= 48.0)
+                k_galerkin<64><<<divUp(64LL * L.nnz, B), B, 0, s>>>(L.nnz, F.gStart.as<int>(), F.gPerm.as<int>(),
+                                                                   fv, L.val64.as<double>());
+            else if (per >= 3.0)
+                k_galerkin<8><<<divUp(8LL * L.nnz, B), B, 0, s>>>(L.nnz, F.gStart.as<int>(), F.gPerm.as<int>(),
+                                                                 fv, L.val64.as<double>());
+            else
+                k_galerkin<1><<<divUp(L.nnz, B), B, 0, s>>>(L.nnz, F.gStart.as<int>(), F.gPerm.as<int>(), fv,
+                                                           L.val64.as<double>());
         }
         const bool fineLevel = l == 0;
         const float aC = fineLevel ? p.omegaPre : p.coarseOmega;
         const float cC = fineLevel ? p.omegaPost : p.coarseOmega;
         const float sC = fineLevel ? p.scale : p.coarseScale;
-        k_level_setup<<<divUp(L.n, B), B, 0, s>>>(A, v64, L.val32.as<float>(), L.invD32.as<float>(),
+        k_level_setup<<<divUp(L.n, B), B, 0, s>>>(A, v64, L.invD32.as<float>(),
                                                   L.wPre32.as<float>(), L.wPost32.as<float>(), L.s32.as<float>(),
                                                   L.t32.as<float>(), fineLevel ? pre : nullptr, aC,
                                                   fineLevel ? scale : nullptr, sC, fineLevel ? post : nullptr, cC,
                                                   fineLevel ? restr : nullptr, fineLevel ? p.restrictScale : 1.0f);
-        k_scale_vals<<<divUp(L.n, B), B, 0, s>>>(A, L.val32.as<float>(), L.wPre32.as<float>(), L.valS32.as<float>());
+        k_scale_vals<<<std::max(1, std::min(8192, divUp(L.nnz, B))), B, 0, s>>>(A, v64, L.val32.as<float>(),
+                                                                               L.wPre32.as<float>(),
+                                                                               L.valS32.as<float>());
         if (L.dense.p) k_mg_dense_inverse<<<1, 1024, 0, s>>>(A, L.val64.as<double>(), L.dense.as<float>());
         if (L.rval.p)
             k_restrict_vals<<<divUp(L.n, B), B, 0, s>>>(L.n, L.aggMem.as<int>(), l == 0 ? L.t32.as<float>() : nullptr,

@@ -19,6 +19,7 @@ from . import native as N
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_WEIGHTS = os.path.join(HERE, "weights", "mpnn_pressure.npy")
+STAGE1_WEIGHTS = os.path.join(HERE, "weights", "mpnn_pressure_stage1.npy")
 
 
 def channel_bcs(patch_start, patch_size, n_internal, n_boundary, Uin):
@@ -161,7 +162,8 @@ def _fcg_iterations(G, b, net, dgeo, maxIter):
 
 
 def stream_train(ctx, net, channels=20, size=(128, 128, 128), length=(1.0, 1.0, 1.0), per_channel=100,
-                 steps_per_system=2, lr=1e-4, seed=11, samples=2, smooth=4, log=None, driver="piso"):
+                 steps_per_system=2, lr=1e-4, seed=11, samples=2, smooth=4, log=None, driver="piso",
+                 validate_every=10):
     """Trains `net` on pressure systems streamed from the GPU flow solver: per
     seeded obstacle channel (radius U[0.05, 0.15], inlet speed U[0.005, 0.05])
     the transient from rest (PISO, or SIMPLE iterations) leaves `per_channel`
@@ -173,12 +175,13 @@ def stream_train(ctx, net, channels=20, size=(128, 128, 128), length=(1.0, 1.0, 
     to a random vector: the error flexible CG leaves) and raw random vectors
     (which keep the smoother weights where high frequencies are damped: a
     smoother tuned on smooth error alone drifts to weights that amplify them).
-    After each channel the network must solve that channel's last system
-    within the iterations of the tuned constants (K-cycle, FP64 residual
-    1e-8); otherwise its parameters return to those after the previous
-    channel.  Returns (loss history, systems used, channels rejected)."""
+    Every `validate_every` systems the updated network must solve the
+    current system (K-cycle, FP64 residual 1e-8) in no more iterations than
+    the tuned constants and than the last accepted parameters; otherwise
+    those parameters are restored.  Returns (loss history, systems used,
+    updates rejected, updates accepted)."""
     rng = np.random.default_rng(seed)
-    hist, used, rejected = [], 0, 0
+    hist, used, rejected, accepted = [], 0, 0, 0
     keep = net.params()
     for c in range(channels):
         hm, radius = channel_mesh(*size, seed=seed * 1000 + c, length=length)
@@ -200,22 +203,27 @@ def stream_train(ctx, net, channels=20, size=(128, 128, 128), length=(1.0, 1.0, 
                           seed=seed * 100003 + used)
             hist.extend(h.tolist())
             used += 1
-        b = run["drhs"].get()[run["count"] - 1]
-        base = _fcg_iterations(G, b, None, None, 1000)
-        got = _fcg_iterations(G, b, net, dgeo, 1000 if base is None else base)
-        ok = got is not None and base is not None and got <= base
-        if ok:
-            keep = net.params()
-        else:
-            net.set_params(keep)
-            rejected += 1
+            if (k + 1) % validate_every == 0 or k + 1 == run["count"]:
+                b = run["drhs"].get()[k]
+                base = _fcg_iterations(G, b, None, None, 1000)
+                got = _fcg_iterations(G, b, net, dgeo, 1000 if base is None else base)
+                cand = net.params()
+                net.set_params(keep)
+                prev = _fcg_iterations(G, b, net, dgeo, 1000 if base is None else base)
+                ok = got is not None and base is not None and got <= base and (prev is None or got <= prev)
+                if ok:
+                    keep = cand
+                    net.set_params(cand)
+                    accepted += 1
+                else:
+                    rejected += 1
         if log:
             seg = np.asarray(hist[before:])
             log(f"channel {c}: radius {radius:.3f}, Uin {Uin:.4f}, {run['count']} systems, "
-                f"loss {seg[:10].mean():.4f} -> {seg[-10:].mean():.4f}; last system {got} vs {base} iterations"
-                f"{'' if ok else ' -> rejected, parameters of the previous channel restored'}")
+                f"loss {seg[:10].mean():.4f} -> {seg[-10:].mean():.4f}; updates accepted {accepted}, "
+                f"rejected {rejected} so far")
         del G, run
-    return np.asarray(hist), used, rejected
+    return np.asarray(hist), used, rejected, accepted
 
 
 def heldout_iterations(ctx, nets, channels=2, size=(128, 128, 128), length=(1.0, 1.0, 1.0), seed=77,
