@@ -284,7 +284,8 @@ def bench_c3(args):
     log(f"cavity: {dm.nCells} tets, mesh {time.perf_counter() - t0:.1f}s, pressure solve {args.c3_pressure}")
     # the closed cavity's pressure is a singular Neumann system; --c3-project makes both the GPU and the oracle
     # project its right-hand side to zero sum (the reference PISO does not)
-    ctl = N.PisoControls(**kw, pPrecision=prec, projectNeumann=args.c3_project)
+    uprec = N.FP32_IR if args.c3_momentum == "fp32ir" else N.FP64
+    ctl = N.PisoControls(**kw, pPrecision=prec, projectNeumann=args.c3_project, uPrecision=uprec)
     times, ke, dev = [], [], []
     for step in range(args.c3_steps):
         ctx.sync()
@@ -318,7 +319,7 @@ def bench_c3(args):
         "metric": "PISO step time, lid-driven cavity Re 1000 on Kuhn tets; kinetic-energy drift vs FP64 CPU oracle",
         "value": round(mean_ms, 3), "unit": "ms", "n_gpus": 1, "steps": args.c3_steps, "warmup": 1,
         "ms_per_step": round(mean_ms, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 assembly and momentum; pressure " + args.c3_pressure, "data": "synthetic",
+        "dtype": f"f64 assembly; momentum {args.c3_momentum}; pressure {args.c3_pressure}", "data": "synthetic",
         "config": {"workload": f"{args.c3_n}^3 hex unit box, +-10% jitter (seed {SEED_MESH}), 6 Kuhn tets per hex, "
                                f"RCM; lid U=(1,0,0), nu 1e-3, PISO 2 correctors, dt 2e-3, laplacian "
                                f"{args.c3_laplacian}, rhs projection {args.c3_project}", "cells": dm.nCells,
@@ -382,6 +383,8 @@ def main():
     ap.add_argument("--c3-check-steps", type=int, default=20)
     ap.add_argument("--c3-laplacian", default="uncorrected", choices=["uncorrected", "corrected"])
     ap.add_argument("--c3-project", type=int, default=0, choices=[0, 1])
+    ap.add_argument("--c3-momentum", default="fp32ir", choices=["fp32ir", "fp64"],
+                    help="momentum BiCGStab: FP32 with FP64 refinement, or FP64")
     ap.add_argument("--blocks", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:

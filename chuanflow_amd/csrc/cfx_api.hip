@@ -186,10 +186,8 @@ SolveReport solveOn(cfx_ctx* c, const CsrView& A, const double* val, DevBuf& val
     need(precond == CFX_PRECOND_JACOBI, "preconditioner not available in this build");
     Krylov& K = c->solver(A.n, A.nRowBlocks);
     if (solver == CFX_SOLVER_BICGSTAB)
-    {
-        need(precision == CFX_FP64, "BiCGStab runs in FP64");
-        return K.bicgstab(A, val, b, x, tol, maxIter);
-    }
+        return precision == CFX_FP64 ? K.bicgstab(A, val, b, x, tol, maxIter)
+                                     : K.bicgstabMixed(A, val, b, x, tol, maxIter);
     if (precision == CFX_FP64) return K.cg(A, val, b, x, tol, maxIter, resHist);
     if (!val32Valid)
     {
@@ -570,7 +568,7 @@ int cfx_matrix_solve(cfx_matrix* A, double* x, int solver, double tol, int maxIt
     return guard(c, [&] {
         DevMesh& m = *A->mesh->m;
         need(solver == CFX_SOLVER_CG || solver == CFX_SOLVER_BICGSTAB, "unknown solver");
-        need(precision == CFX_FP64 || (precision == CFX_FP32_IR && solver == CFX_SOLVER_CG), "unknown precision");
+        need(precision == CFX_FP64 || precision == CFX_FP32_IR, "unknown precision");
         need(precond == CFX_PRECOND_JACOBI || precond == CFX_PRECOND_LEARNED,
              "preconditioner not available in this build");
         need(precond != CFX_PRECOND_LEARNED || (solver == CFX_SOLVER_CG && precision == CFX_FP32_IR),
@@ -1072,6 +1070,8 @@ int cfx_simple_solve(cfx_mesh* m, double nu, const cfx_schemes* schemes, const c
         c.pRelaxStart = ctl->pRelaxStart;
         for (int k = 0; k < 3; ++k) c.bodyForce[k] = ctl->bodyForce[k];
         c.pPrecision = ctl->pPrecision;
+        need(ctl->uPrecision == CFX_FP64 || ctl->uPrecision == CFX_FP32_IR, "uPrecision must be FP64 or FP32_IR");
+        c.uPrecision = ctl->uPrecision;
         c.pSolver = ctl->pPrecision == CFX_FP32_MG ? m->pressureSolver() : nullptr;
         c.dump = &m->dump;
         const SimpleResult r = simpleSolve(*m->m, m->owner->solver(m->m->n, m->m->pattern.nRowBlocks), nu,
@@ -1107,6 +1107,8 @@ int cfx_piso_step(cfx_mesh* m, double nu, const cfx_schemes* schemes, const cfx_
         for (int k = 0; k < 3; ++k) c.bodyForce[k] = ctl->bodyForce[k];
         c.projectNeumann = ctl->projectNeumann != 0;
         c.pPrecision = ctl->pPrecision;
+        need(ctl->uPrecision == CFX_FP64 || ctl->uPrecision == CFX_FP32_IR, "uPrecision must be FP64 or FP32_IR");
+        c.uPrecision = ctl->uPrecision;
         c.pSolver = ctl->pPrecision == CFX_FP32_MG ? m->pressureSolver() : nullptr;
         c.dump = &m->dump;
         const PisoResult r = pisoStep(*m->m, m->owner->solver(m->m->n, m->m->pattern.nRowBlocks), nu,

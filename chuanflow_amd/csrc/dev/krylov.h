@@ -55,6 +55,16 @@ public:
     SolveReport bicgstab(const CsrView& A, const double* val, const double* b, double* x, double tol,
                          int maxIter);
 
+    // Jacobi-BiCGStab in FP32 with FP64 iterative refinement: r = b - A x in
+    // FP64, the FP32 solve of A d = r from d = 0 to `innerTol` relative (or to
+    // the overall tolerance when that is closer), x += d; the reference's stop
+    // test is applied to the true FP64 residual only.  When a correction no
+    // longer halves the FP64 residual the solve finishes in FP64
+    // (precondFallbacks = 1).  iterations counts the FP32 (and FP64) BiCGStab
+    // iterations, outerIterations the corrections.
+    SolveReport bicgstabMixed(const CsrView& A, const double* val, const double* b, double* x, double tol,
+                              int maxIter, double innerTol = 1e-4);
+
     // FP32-inner Jacobi-PCG with FP64 iterative refinement by reliable
     // updates: the Krylov recurrence runs in FP32 (matrix `val32`) and
     // accumulates a correction; each time its residual has fallen by `delta`
@@ -87,6 +97,9 @@ public:
 
 private:
     template <typename T>
+    SolveReport bicgRun(const CsrView& A, const T* val, const T* invD, const T* b, T* x, double tol, int maxIter,
+                        T* const* work);
+    template <typename T>
     SolveReport pcgRun(const CsrView& A, const double* val64, const T* valT, const T* invD,
                        const double* b, double* x64, T* xT, T* r, double tol, int maxIter,
                        double* resHist, bool mixed, double delta);
@@ -99,6 +112,7 @@ private:
     DevBuf r_, p_, q_, r0_, v_, s_, sh_, t_, z_, invD_, invDb_;
     DevBuf r32_, p32_, q32_, d32_, b32_, invD32_;
     DevBuf z32_, fcgPart_;  // allocated by the first fcgMixed
+    DevBuf bf_[6], val32b_;  // FP32 BiCGStab vectors and matrix, allocated by the first bicgstabMixed
     // captured flexible-CG iteration, valid for the key (multigrid, buffers) and the stop parameters
     hipGraphExec_t fcgExec_ = nullptr;
     std::vector<const void*> fcgKey_;

@@ -293,9 +293,9 @@ __global__ __launch_bounds__(kSpmvBlock) void k_residual_final(int G, const doub
 
 // ---------------------------------------------------------------- BiCGStab
 // r0 = r, p = v = 0, rho/beta of iteration 1 (solveBiCGStab :253-266)
-__global__ __launch_bounds__(kVecBlock) void k_bicg_init(int n, const double* __restrict__ r,
-                                                         double* __restrict__ r0, double* __restrict__ p,
-                                                         double* __restrict__ v, const SolveState* st,
+template <typename T>
+__global__ __launch_bounds__(kVecBlock) void k_bicg_init(int n, const T* __restrict__ r, T* __restrict__ r0,
+                                                         T* __restrict__ p, T* __restrict__ v, const SolveState* st,
                                                          double* part)
 {
     __shared__ double red[kVecBlock / kWave];
@@ -303,11 +303,11 @@ __global__ __launch_bounds__(kVecBlock) void k_bicg_init(int n, const double* __
     double s = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
-        const double ri = r[i];
+        const T ri = r[i];
         r0[i] = ri;
-        p[i] = 0.0;
-        v[i] = 0.0;
-        s += ri * ri;
+        p[i] = T(0);
+        v[i] = T(0);
+        s += static_cast<double>(ri) * static_cast<double>(ri);
     }
     s = blockSum(s, red);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
@@ -336,17 +336,16 @@ __global__ __launch_bounds__(kVecBlock) void k_bicg_init_final(int G, const doub
 }
 
 // p = r + beta (p - omega v); z = p invD   (:267-272)
-__global__ __launch_bounds__(kVecBlock) void k_bicg_p(int n, const double* __restrict__ invD,
-                                                      const double* __restrict__ r,
-                                                      double* __restrict__ p,
-                                                      const double* __restrict__ v,
-                                                      double* __restrict__ z, const SolveState* st)
+template <typename T>
+__global__ __launch_bounds__(kVecBlock) void k_bicg_p(int n, const T* __restrict__ invD, const T* __restrict__ r,
+                                                      T* __restrict__ p, const T* __restrict__ v,
+                                                      T* __restrict__ z, const SolveState* st)
 {
     if (st->done) return;
-    const double beta = st->beta, omega = st->omega;
+    const T beta = static_cast<T>(st->beta), omega = static_cast<T>(st->omega);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
-        const double pi = r[i] + beta * (p[i] - omega * v[i]);
+        const T pi = r[i] + beta * (p[i] - omega * v[i]);
         p[i] = pi;
         z[i] = pi * invD[i];
     }
@@ -356,13 +355,13 @@ __global__ __launch_bounds__(kVecBlock) void k_bicg_p(int n, const double* __res
 // which == 0: v = A z, denom = r0.v, alpha = rho/denom   (:273-277)
 // which == 1: t = A sh, tt = t.t, ts = t.s, omega = ts/tt (:290-300)
 // (per-workgroup partials over rbPerWg row blocks; k_bicg_dot_final reduces them)
-__global__ __launch_bounds__(kSpmvBlock) void k_bicg_spmv(CsrView A, int rbPerWg, const double* __restrict__ val,
-                                                          const double* __restrict__ x,
-                                                          double* __restrict__ y,
-                                                          const double* __restrict__ w,
-                                                          const SolveState* st, double* part, int which)
+template <typename T>
+__global__ __launch_bounds__(kSpmvBlock) void k_bicg_spmv(CsrView A, int rbPerWg, const T* __restrict__ val,
+                                                          const T* __restrict__ x, T* __restrict__ y,
+                                                          const T* __restrict__ w, const SolveState* st,
+                                                          double* part, int which)
 {
-    __shared__ double lds[kSpmvLds];
+    __shared__ T lds[kSpmvLds];
     __shared__ double red[kSpmvBlock / kWave];
     if (st->done) return;
     const int G = gridDim.x;
@@ -373,12 +372,13 @@ __global__ __launch_bounds__(kSpmvBlock) void k_bicg_spmv(CsrView A, int rbPerWg
         const int blk = wg * rbPerWg + j;
         if (blk >= A.nRowBlocks) break;
         int row;
-        double s;
+        T s;
         if (spmvRowSum(A, val, x, blk, lds, row, s))
         {
             y[row] = s;
-            if (which == 0) d0 += w[row] * s;
-            else { d0 += s * s; d1 += s * w[row]; }
+            const double sd = s, wd = w[row];
+            if (which == 0) d0 += wd * sd;
+            else { d0 += sd * sd; d1 += sd * wd; }
         }
     }
     d0 = blockSum(d0, red);
@@ -412,44 +412,40 @@ __global__ __launch_bounds__(kSpmvBlock) void k_bicg_dot_final(int G, const doub
 }
 
 // x += alpha z; s = r - alpha v; sh = s invD   (:278-289)
-__global__ __launch_bounds__(kVecBlock) void k_bicg_s(int n, const double* __restrict__ invD,
-                                                      double* __restrict__ x,
-                                                      const double* __restrict__ z,
-                                                      const double* __restrict__ r,
-                                                      const double* __restrict__ v,
-                                                      double* __restrict__ s, double* __restrict__ sh,
-                                                      const SolveState* st)
+template <typename T>
+__global__ __launch_bounds__(kVecBlock) void k_bicg_s(int n, const T* __restrict__ invD, T* __restrict__ x,
+                                                      const T* __restrict__ z, const T* __restrict__ r,
+                                                      const T* __restrict__ v, T* __restrict__ s,
+                                                      T* __restrict__ sh, const SolveState* st)
 {
     if (st->done) return;
-    const double alpha = st->alpha;
+    const T alpha = static_cast<T>(st->alpha);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
         x[i] += alpha * z[i];
-        const double si = r[i] - alpha * v[i];
+        const T si = r[i] - alpha * v[i];
         s[i] = si;
         sh[i] = si * invD[i];
     }
 }
 
 // x += omega sh; r = s - omega t; ||r||inf; next rho = r0.r   (:301-314, :263-266)
-__global__ __launch_bounds__(kVecBlock) void k_bicg_r(int n, double* __restrict__ x,
-                                                      const double* __restrict__ sh,
-                                                      const double* __restrict__ s,
-                                                      const double* __restrict__ t,
-                                                      const double* __restrict__ r0,
-                                                      double* __restrict__ r, const SolveState* st,
-                                                      double* part, int* ipart)
+template <typename T>
+__global__ __launch_bounds__(kVecBlock) void k_bicg_r(int n, T* __restrict__ x, const T* __restrict__ sh,
+                                                      const T* __restrict__ s, const T* __restrict__ t,
+                                                      const T* __restrict__ r0, T* __restrict__ r,
+                                                      const SolveState* st, double* part, int* ipart)
 {
     __shared__ double red[kVecBlock / kWave];
     __shared__ int ired[kVecBlock / kWave];
     if (st->done) return;
     const bool tsmall = st->breakdown == 2;
-    const double omega = st->omega;
+    const T omega = static_cast<T>(st->omega);
     double rm = 0, rr0 = 0;
     int nan = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
-        double ri;
+        T ri;
         if (tsmall)
         {
             ri = s[i];
@@ -460,9 +456,10 @@ __global__ __launch_bounds__(kVecBlock) void k_bicg_r(int n, double* __restrict_
             ri = s[i] - omega * t[i];
         }
         r[i] = ri;
-        rm = fmax(rm, fabs(ri));
-        nan |= (ri != ri);
-        rr0 += r0[i] * ri;
+        const double rd = ri;
+        rm = fmax(rm, fabs(rd));
+        nan |= (rd != rd);
+        rr0 += static_cast<double>(r0[i]) * rd;
     }
     const int G = gridDim.x;
     rm = blockMax(rm, red);
@@ -537,6 +534,13 @@ __global__ void k_fill(T* __restrict__ p, T v, long long n)
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x)
         p[i] = v;
+}
+
+// x += d (FP32 correction folded into the FP64 iterate)
+__global__ void k_add_float(int n, const float* __restrict__ d, double* __restrict__ x)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        x[i] += static_cast<double>(d[i]);
 }
 
 int vecGridFor(long long n) { return std::max(1, std::min(kVecGridMax, divUp(n, 4LL * kVecBlock))); }
@@ -628,12 +632,12 @@ void Krylov::trueResidual(const CsrView& A, const double* val, const double* b, 
     l2Rel = ctx_.hostState()[0].rho;
 }
 
-SolveReport Krylov::bicgstab(const CsrView& A, const double* val, const double* b, double* x,
-                             double tol, int maxIter)
+template <typename T>
+SolveReport Krylov::bicgRun(const CsrView& A, const T* val, const T* invD, const T* b, T* x, double tol, int maxIter,
+                            T* const* w)
 {
+    // w: r, r0, p, v, z, s, sh, t
     hipStream_t s = ctx_.stream();
-    CFX_HIP(hipEventRecord(ev0_, s));
-    computeInvD(ctx_, A, val, invDb_.as<double>(), true);
     SolveState h{};
     h.tol = tol;
     h.maxIter = maxIter;
@@ -645,12 +649,10 @@ SolveReport Krylov::bicgstab(const CsrView& A, const double* val, const double* 
     int Gs = std::min(A.nRowBlocks, kVecGridMax);
     const int rbPerWg = divUp(A.nRowBlocks, Gs);
     Gs = std::max(1, divUp(A.nRowBlocks, rbPerWg));
-    double *r = r_.as<double>(), *r0 = r0_.as<double>(), *p = p_.as<double>(), *v = v_.as<double>();
-    double *z = z_.as<double>(), *sv = s_.as<double>(), *sh = sh_.as<double>(), *t = t_.as<double>();
-    const double* invD = invDb_.as<double>();
-    k_residual<double><<<Gs, kSpmvBlock, 0, s>>>(A, rbPerWg, val, x, b, r, part, ipart);
+    T *r = w[0], *r0 = w[1], *p = w[2], *v = w[3], *z = w[4], *sv = w[5], *sh = w[6], *t = w[7];
+    k_residual<T><<<Gs, kSpmvBlock, 0, s>>>(A, rbPerWg, val, x, b, r, part, ipart);
     k_residual_final<<<1, kSpmvBlock, 0, s>>>(Gs, part, ipart, st, 0);
-    k_bicg_init<<<Gv, kVecBlock, 0, s>>>(n_, r, r0, p, v, st, part);
+    k_bicg_init<T><<<Gv, kVecBlock, 0, s>>>(n_, r, r0, p, v, st, part);
     k_bicg_init_final<<<1, kVecBlock, 0, s>>>(Gv, part, st);
     CFX_CHECK_LAUNCH();
     poll();
@@ -662,13 +664,13 @@ SolveReport Krylov::bicgstab(const CsrView& A, const double* val, const double* 
         const int batch = nextBatch(hs->iter, rInit, hs->rnorm, tol * bnorm, maxIter);
         for (int k = 0; k < batch; ++k)
         {
-            k_bicg_p<<<Gv, kVecBlock, 0, s>>>(n_, invD, r, p, v, z, st);
-            k_bicg_spmv<<<Gs, kSpmvBlock, 0, s>>>(A, rbPerWg, val, z, v, r0, st, part, 0);
+            k_bicg_p<T><<<Gv, kVecBlock, 0, s>>>(n_, invD, r, p, v, z, st);
+            k_bicg_spmv<T><<<Gs, kSpmvBlock, 0, s>>>(A, rbPerWg, val, z, v, r0, st, part, 0);
             k_bicg_dot_final<<<1, kSpmvBlock, 0, s>>>(Gs, part, st, 0);
-            k_bicg_s<<<Gv, kVecBlock, 0, s>>>(n_, invD, x, z, r, v, sv, sh, st);
-            k_bicg_spmv<<<Gs, kSpmvBlock, 0, s>>>(A, rbPerWg, val, sh, t, sv, st, part, 1);
+            k_bicg_s<T><<<Gv, kVecBlock, 0, s>>>(n_, invD, x, z, r, v, sv, sh, st);
+            k_bicg_spmv<T><<<Gs, kSpmvBlock, 0, s>>>(A, rbPerWg, val, sh, t, sv, st, part, 1);
             k_bicg_dot_final<<<1, kSpmvBlock, 0, s>>>(Gs, part, st, 1);
-            k_bicg_r<<<Gv, kVecBlock, 0, s>>>(n_, x, sh, sv, t, r0, r, st, part, ipart);
+            k_bicg_r<T><<<Gv, kVecBlock, 0, s>>>(n_, x, sh, sv, t, r0, r, st, part, ipart);
             k_bicg_r_final<<<1, kVecBlock, 0, s>>>(Gv, part, ipart, st);
         }
         CFX_CHECK_LAUNCH();
@@ -680,6 +682,94 @@ SolveReport Krylov::bicgstab(const CsrView& A, const double* val, const double* 
     rep.initialResidual = hs->initialResidual;
     rep.finalResidual = hs->iter > 0 ? hs->rnorm : hs->initialResidual;
     rep.converged = hs->converged;
+    return rep;
+}
+
+SolveReport Krylov::bicgstab(const CsrView& A, const double* val, const double* b, double* x, double tol,
+                             int maxIter)
+{
+    hipStream_t s = ctx_.stream();
+    CFX_HIP(hipEventRecord(ev0_, s));
+    computeInvD(ctx_, A, val, invDb_.as<double>(), true);
+    double* w[8] = {r_.as<double>(), r0_.as<double>(), p_.as<double>(), v_.as<double>(),
+                    z_.as<double>(), s_.as<double>(), sh_.as<double>(), t_.as<double>()};
+    SolveReport rep = bicgRun<double>(A, val, invDb_.as<double>(), b, x, tol, maxIter, w);
+    CFX_HIP(hipEventRecord(ev1_, s));
+    CFX_HIP(hipEventSynchronize(ev1_));
+    float ms = 0;
+    CFX_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+    lastMs_ = rep.ms = ms;
+    return rep;
+}
+
+SolveReport Krylov::bicgstabMixed(const CsrView& A, const double* val, const double* b, double* x, double tol,
+                                  int maxIter, double innerTol)
+{
+    hipStream_t s = ctx_.stream();
+    CFX_HIP(hipEventRecord(ev0_, s));
+    const std::size_t f = sizeof(float) * static_cast<std::size_t>(n_);
+    if (!bf_[0].p)
+        for (DevBuf& v : bf_) v.alloc(f);
+    if (!val32b_.p || val32b_.bytes < sizeof(float) * static_cast<std::size_t>(A.nnz))
+        val32b_.alloc(sizeof(float) * static_cast<std::size_t>(A.nnz));
+    computeInvD(ctx_, A, val, invDb_.as<double>(), true);
+    castToFloat(ctx_, invDb_.as<double>(), invD32_.as<float>(), n_);
+    castToFloat(ctx_, val, val32b_.as<float>(), A.nnz);
+    float* w[8] = {r32_.as<float>(), bf_[0].as<float>(), p32_.as<float>(), bf_[1].as<float>(),
+                   bf_[2].as<float>(), bf_[3].as<float>(), bf_[4].as<float>(), bf_[5].as<float>()};
+    float *d = d32_.as<float>(), *b32 = b32_.as<float>();
+    double* r64 = r_.as<double>();
+    double* part = partials_.as<double>();
+    int* ipart = reinterpret_cast<int*>(part + 4 * maxBlocks_);
+    int Gs = std::min(A.nRowBlocks, kVecGridMax);
+    const int rbPerWg = divUp(A.nRowBlocks, Gs);
+    Gs = std::max(1, divUp(A.nRowBlocks, rbPerWg));
+    const int Gv = vecGridFor(n_);
+    SolveState* st0 = state_.as<SolveState>();
+    SolveState h{};
+    h.tol = tol;
+    CFX_HIP(hipMemcpyAsync(st0, &h, offsetof(SolveState, ticket), hipMemcpyHostToDevice, s));
+    SolveReport rep;
+    int its = 0, outer = 0;
+    double last = INFINITY;
+    for (;;)
+    {
+        // true FP64 residual r = b - A x and the reference's stop test on it
+        k_residual<double><<<Gs, kSpmvBlock, 0, s>>>(A, rbPerWg, val, x, b, r64, part, ipart);
+        k_residual_final<<<1, kSpmvBlock, 0, s>>>(Gs, part, ipart, st0, outer == 0 ? 0 : 1);
+        CFX_CHECK_LAUNCH();
+        poll();
+        const SolveState& o = ctx_.hostState()[0];
+        if (outer == 0) rep.initialResidual = o.initialResidual;
+        rep.finalResidual = o.rnorm;
+        rep.converged = o.converged;
+        if (o.converged || !std::isfinite(o.rnorm) || its >= maxIter) break;
+        if (outer > 0 && !(o.rnorm < 0.5 * last))
+        {
+            // the FP32 correction no longer reduces the FP64 residual: finish in FP64
+            double* w64[8] = {r_.as<double>(), r0_.as<double>(), p_.as<double>(), v_.as<double>(),
+                              z_.as<double>(), s_.as<double>(), sh_.as<double>(), t_.as<double>()};
+            const SolveReport f64 = bicgRun<double>(A, val, invDb_.as<double>(), b, x, tol, maxIter - its, w64);
+            its += f64.iterations;
+            rep.finalResidual = f64.finalResidual;
+            rep.converged = f64.converged;
+            rep.precondFallbacks = 1;
+            break;
+        }
+        last = o.rnorm;
+        // FP32 BiCGStab on A d = r from d = 0, to innerTol relative (or to the overall tolerance)
+        castToFloat(ctx_, r64, b32, n_);
+        k_fill<float><<<Gv, kVecBlock, 0, s>>>(d, 0.0f, n_);
+        const double inner = std::max(innerTol, 0.5 * tol * o.bnorm / o.rnorm);
+        const SolveReport in = bicgRun<float>(A, val32b_.as<float>(), invD32_.as<float>(), b32, d, inner,
+                                              maxIter - its, w);
+        its += in.iterations;
+        k_add_float<<<Gv, kVecBlock, 0, s>>>(n_, d, x);
+        ++outer;
+        if (in.iterations == 0) break;  // nothing more the FP32 solve can do (breakdown at its start)
+    }
+    rep.iterations = its;
+    rep.outerIterations = outer;
     CFX_HIP(hipEventRecord(ev1_, s));
     CFX_HIP(hipEventSynchronize(ev1_));
     float ms = 0;

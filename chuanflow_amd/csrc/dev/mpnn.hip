@@ -179,13 +179,14 @@ __global__ __launch_bounds__(B) void k_layer_fwd(int n, CsrView A, const float* 
                 float v0[16], v1[16];
                 load16(hin + static_cast<long long>(j0) * S::H + f0, v0);
                 load16(hin + static_cast<long long>(j1) * S::H + f0, v1);
+                // fused multiply-adds (with FP16 rows they become v_fma_mix_f32, the conversion folded in)
                 if (j0 != i)
                 {
 #pragma unroll
                     for (int c = 0; c < 16; ++c)
                     {
-                        m[c] += w0 * v0[c];
-                        g[c] += g0 * v0[c];
+                        m[c] = fmaf(w0, v0[c], m[c]);
+                        g[c] = fmaf(g0, v0[c], g[c]);
                     }
                 }
                 if (two && j1 != i)
@@ -193,8 +194,8 @@ __global__ __launch_bounds__(B) void k_layer_fwd(int n, CsrView A, const float* 
 #pragma unroll
                     for (int c = 0; c < 16; ++c)
                     {
-                        m[c] += w1 * v1[c];
-                        g[c] += g1 * v1[c];
+                        m[c] = fmaf(w1, v1[c], m[c]);
+                        g[c] = fmaf(g1, v1[c], g[c]);
                     }
                 }
             }

@@ -80,6 +80,7 @@ struct SimpleControls
     double relaxU = 0.7, relaxP = 0.3, pRelaxStart = 0;
     double bodyForce[3] = {0, 0, 0};
     int pPrecision = kFP64;  // pressure solve: FP64, FP32 inner + FP64 refinement, or kFP32MG
+    int uPrecision = kFP64;  // momentum solve: FP64 or FP32 BiCGStab with FP64 refinement
     PressureSolver* pSolver = nullptr;
     PressureDump* dump = nullptr;
 };
@@ -109,6 +110,7 @@ struct PisoControls
     int linearMaxIter = 2000, nonOrthCorrectors = 2;
     double bodyForce[3] = {0, 0, 0};
     int pPrecision = kFP64;
+    int uPrecision = kFP64;
     PressureSolver* pSolver = nullptr;
     PressureDump* dump = nullptr;
     bool projectNeumann = false;  // zero-sum pressure rhs when no pressure face is fixed

@@ -447,7 +447,8 @@ SolveReport fvSolve(DevMesh& m, Krylov& K, const Ldu& A, double* val, double* rh
             double* xk = x + static_cast<long long>(k) * n;
             if (a.solver == 1)
             {
-                last = K.bicgstab(view, v, b, xk, a.tol, a.maxIter);
+                last = a.precision == kFP32IR ? K.bicgstabMixed(view, v, b, xk, a.tol, a.maxIter)
+                                              : K.bicgstab(view, v, b, xk, a.tol, a.maxIter);
             }
             else if (a.mg)
             {
@@ -513,6 +514,7 @@ SimpleResult simpleSolve(DevMesh& m, Krylov& K, double nu, const Schemes& sch, c
         k_zero<<<gridCap(3LL * n), B, 0, s>>>(w.xU.as<double>(), 3LL * n);
         FvSolveArgs ua;
         ua.solver = 1;
+        ua.precision = c.uPrecision;
         ua.tol = c.linearTol;
         ua.maxIter = c.linearMaxIter;
         ua.nonOrth = c.nonOrthCorrectors;
@@ -640,6 +642,7 @@ PisoResult pisoStep(DevMesh& m, Krylov& K, double nu, const Schemes& sch, const 
     k_zero<<<gridCap(3LL * n), B, 0, s>>>(w.xU.as<double>(), 3LL * n);
     FvSolveArgs ua;
     ua.solver = 1;
+    ua.precision = c.uPrecision;
     ua.tol = c.linearTol;
     ua.maxIter = c.linearMaxIter;
     ua.nonOrth = c.nonOrthCorrectors;

@@ -52,7 +52,8 @@ class SimpleControls(C.Structure):
     _fields_ = [("maxIterations", C.c_int), ("continuityTol", C.c_double), ("pcorrTol", C.c_double),
                 ("momentumTol", C.c_double), ("linearTol", C.c_double), ("linearMaxIter", C.c_int),
                 ("nonOrthCorrectors", C.c_int), ("relaxU", C.c_double), ("relaxP", C.c_double),
-                ("pRelaxStart", C.c_double), ("bodyForce", C.c_double * 3), ("pPrecision", C.c_int)]
+                ("pRelaxStart", C.c_double), ("bodyForce", C.c_double * 3), ("pPrecision", C.c_int),
+                ("uPrecision", C.c_int)]
 
 
 class SimpleResult(C.Structure):
@@ -64,7 +65,8 @@ class SimpleResult(C.Structure):
 class PisoControls(C.Structure):
     _fields_ = [("dt", C.c_double), ("correctors", C.c_int), ("continuityTol", C.c_double),
                 ("linearTol", C.c_double), ("linearMaxIter", C.c_int), ("nonOrthCorrectors", C.c_int),
-                ("bodyForce", C.c_double * 3), ("pPrecision", C.c_int), ("projectNeumann", C.c_int)]
+                ("bodyForce", C.c_double * 3), ("pPrecision", C.c_int), ("projectNeumann", C.c_int),
+                ("uPrecision", C.c_int)]
 
 
 class PisoResult(C.Structure):

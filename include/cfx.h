@@ -171,7 +171,10 @@ int cfx_csr_set_values_device(cfx_csr* A, const double* d_vals);
 /* SparseMatrix::product (Matrix.h:47, Matrix.cpp:119-133) */
 int cfx_csr_spmv(cfx_csr* A, const double* d_x, double* d_y);
 /* solveCG / solveBiCGStab / solveWithBackend (Matrix.h:67-89): host b, x
- * (x = initial guess and result), tol/maxIter as the reference. */
+ * (x = initial guess and result), tol/maxIter as the reference.  precision
+ * CFX_FP32_IR (an extension): CG with FP32 Krylov vectors and FP64 reliable
+ * updates, or BiCGStab in FP32 with FP64 iterative refinement; both decide
+ * convergence on the true FP64 residual with the reference's stop test. */
 int cfx_csr_solve(cfx_csr* A, const double* b, double* x, double tol, int maxIter, int solver,
                   int precond, int precision, cfx_report* rep);
 /* the same with device-resident b and x */
@@ -359,6 +362,9 @@ typedef struct cfx_simple_controls {
     double relaxU, relaxP, pRelaxStart;
     double bodyForce[3];
     int pPrecision;
+    /* momentum solve: CFX_FP64 (the reference's BiCGStab) or CFX_FP32_IR (FP32 BiCGStab with FP64
+     * iterative refinement; stop test on the true FP64 residual); not in the reference */
+    int uPrecision;
 } cfx_simple_controls;
 
 /* SimpleResult (SimpleSolver.h:66-72) + divergence flag and device time */
@@ -383,6 +389,7 @@ typedef struct cfx_piso_controls {
      * non-orthogonal correction of a jittered or tet mesh makes the system
      * inconsistent and the reference's PISO diverges */
     int projectNeumann;
+    int uPrecision; /* momentum solve, as in cfx_simple_controls */
 } cfx_piso_controls;
 
 typedef struct cfx_piso_result {

@@ -87,6 +87,28 @@ def test_c1_channel_200_iterations_within_1e6_of_oracle(ctx, pprec):
         assert e < 1e-6, name
 
 
+def test_c1_channel_fp32_momentum_and_pressure_within_1e6_of_oracle(ctx):
+    """C1 with both linear solves in FP32 and FP64 refinement (momentum
+    BiCGStab and pressure PCG decide convergence on true FP64 residuals):
+    after 200 SIMPLE iterations the fields stay within 1e-6 of the FP64
+    oracle."""
+    N, om, dm, Ubc, pbc = c1_case(ctx)
+    print("\n[case] C1 40x20x20 channel, 200 SIMPLE iterations, FP32 momentum and pressure", flush=True)
+    U, p, phi = initial_fields(om, Ubc)
+    dU, dp, dphi = ctx.array(soa(U)), ctx.array(p), ctx.array(phi)
+    kw = dict(maxIterations=200, continuityTol=0, pcorrTol=0, momentumTol=0, linearTol=1e-8, linearMaxIter=2000,
+              nonOrthCorrectors=2, relaxU=0.9, relaxP=0.03, pRelaxStart=0.02)
+    r = dm.simple(1e-3, N.Schemes(0, 1, 1, 0.9), N.SimpleControls(**kw, pPrecision=1, uPrecision=1), Ubc, pbc,
+                  dU, dp, dphi)
+    ro = O.simple(om, U, p, phi, 1e-3, O.Schemes(0, 1, 1, 0.9), O.SimpleControls(**kw), Ubc, pbc)
+    print(f"GPU {r.ms:.0f} ms for {r.iterations} iterations")
+    assert r.iterations == ro.iterations == 200 and not r.diverged
+    for a, b, name in ((aos(dU.get()), U, "U"), (dp.get(), p, "p"), (dphi.get(), phi, "phi")):
+        e = rel(a, b)
+        print(f"  {name}: max rel diff {e:.2e}")
+        assert e < 1e-6, name
+
+
 def test_c1_literal_controls_reproduce_reference_divergence(ctx):
     """U 0.7 / p 0.3 without pRelaxStart diverges on the reference (iterations
     4-8); the GPU run must diverge as well, within two iterations of it."""

@@ -95,6 +95,33 @@ def test_bicgstab_matches_oracle(pmesh):
     assert np.abs(xg - xo).max() <= 1e-9 * np.abs(xo).max()
 
 
+def test_bicgstab_fp32_with_fp64_refinement(pmesh):
+    """BiCGStab in FP32 with FP64 iterative refinement on a momentum system
+    (ddt + upwind convection + laplacian, non-symmetric): the true FP64
+    residual meets the tolerance, and the solution matches the oracle's FP64
+    BiCGStab to the tolerance's scale."""
+    N, ctx, om, dm = pmesh
+    rng = np.random.default_rng(17)
+    Ut, Uv, *_ = channel_bcs(om)
+    lc, lk = O.linearise(om, 3, Ut, Uv)
+    A = O.Ldu(om, 3)
+    O.assemble_ddt(om, A, rng.standard_normal((om.nCells, 3)), 5e-3)
+    O.assemble_div(om, A, rng.standard_normal(om.nFaces) * 1e-2, lc, lk)
+    O.assemble_laplacian(om, A, O.harmonic_gamma_uniform(om, 1e-3), lc, lk, -1)
+    csr = O.csr_from_ldu(om, A, 0)
+    b = A.rhs[0]
+    G = N.Csr(ctx, csr.rowStart, csr.cols, csr.vals)
+    for tol in (1e-8, 1e-12):
+        xg, rg = G.solve(b, None, tol, 2000, solver=N.SOLVER_BICGSTAB, precision=N.FP32_IR)
+        true = np.abs(b - O.spmv(csr, xg)).max() / np.abs(b).max()
+        print(f"tol {tol:g}: {rg.iterations} FP32 iterations in {rg.outerIterations} corrections, "
+              f"true residual {true:.2e}, fallbacks {rg.precondFallbacks}", flush=True)
+        assert rg.converged and true < tol
+        assert rg.outerIterations >= 1
+        xo, ro = O.solve_bicgstab(csr, b, None, 1e-14, 5000)
+        assert np.abs(xg - xo).max() <= 100 * tol * np.abs(xo).max()
+
+
 def test_fvmatrix_pass_loop_matches_oracle(pmesh):
     """fvMatrix::solve with nonOrth=2: 3 passes, rhs restored and corrected
     from the current solution each pass (fvMatrix.cpp:86-121)."""
