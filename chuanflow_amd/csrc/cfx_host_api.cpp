@@ -3,6 +3,7 @@
 
 #include "../../include/cfx.h"
 #include "cf/DetailBlock.h"
+#include "cf/MeshIO.h"
 #include "cf/PolyMesh.h"
 
 #include <memory>
@@ -313,6 +314,53 @@ int cfx_hmesh_bandwidth(const cfx_hmesh* h, int64_t* bw)
     return guardHost([&] {
         if (!h || !bw) throw std::invalid_argument("null argument");
         *bw = h->mesh.bandwidth();
+    });
+}
+
+
+// ------------------------------------------------------------ mesh files
+int cfx_hmesh_read_openfoam(const char* dir, const char* patchTypes, cfx_hmesh** out)
+{
+    return guardHost([&] {
+        if (!dir || !out) throw std::invalid_argument("null argument");
+        *out = new cfx_hmesh{cf::readOpenFoam(dir, cf::parsePatchTypes(patchTypes ? patchTypes : ""))};
+    });
+}
+
+int cfx_hmesh_read_gmsh(const char* path, const char* patchTypes, cfx_hmesh** out)
+{
+    return guardHost([&] {
+        if (!path || !out) throw std::invalid_argument("null argument");
+        *out = new cfx_hmesh{cf::readGmsh(path, cf::parsePatchTypes(patchTypes ? patchTypes : ""))};
+    });
+}
+
+int cfx_hmesh_patch(const cfx_hmesh* h, int64_t i, char* name, int64_t nameCap, char* type, int64_t typeCap)
+{
+    return guardHost([&] {
+        if (!h) throw std::invalid_argument("null mesh");
+        if (i < 0 || i >= h->mesh.nPatches()) throw std::invalid_argument("patch index out of range");
+        const auto& p = h->mesh.patches()[static_cast<std::size_t>(i)];
+        auto copy = [](const std::string& s, char* dst, int64_t cap) {
+            if (!dst) return;
+            if (cap <= static_cast<int64_t>(s.size())) throw std::invalid_argument("patch string buffer too small");
+            std::memcpy(dst, s.c_str(), s.size() + 1);
+        };
+        copy(p.name, name, nameCap);
+        copy(p.type, type, typeCap);
+    });
+}
+
+int cfx_vtu_write(const cfx_hmesh* h, const char* path, int nFields, const char* const* names,
+                  const int* components, const double* const* data)
+{
+    return guardHost([&] {
+        if (!h || !path) throw std::invalid_argument("null argument");
+        if (nFields < 0 || (nFields > 0 && (!names || !components || !data)))
+            throw std::invalid_argument("bad field arrays");
+        std::vector<cf::VtuField> f;
+        for (int k = 0; k < nFields; ++k) f.push_back({names[k] ? names[k] : "", components[k], data[k]});
+        cf::writeVtu(path, h->mesh, f);
     });
 }
 

@@ -227,6 +227,10 @@ _sig("cfx_hmesh_shuffle", C.c_int, _vp, C.c_uint64)
 _sig("cfx_hmesh_renumber_rcm", C.c_int, _vp, _vp)
 _sig("cfx_hmesh_bandwidth", C.c_int, _vp, C.POINTER(_i64))
 _sig("cfx_hmesh_last_error", C.c_char_p)
+_sig("cfx_hmesh_read_openfoam", C.c_int, C.c_char_p, C.c_char_p, C.POINTER(_vp))
+_sig("cfx_hmesh_read_gmsh", C.c_int, C.c_char_p, C.c_char_p, C.POINTER(_vp))
+_sig("cfx_hmesh_patch", C.c_int, _vp, _i64, C.c_char_p, _i64, C.c_char_p, _i64)
+_sig("cfx_vtu_write", C.c_int, _vp, C.c_char_p, C.c_int, _vp, _vp, _vp)
 
 
 def lib():
@@ -353,6 +357,12 @@ class DeviceArray:
 
 
 # ------------------------------------------------------------ host mesh
+def _types_spec(patch_types):
+    if not patch_types:
+        return b""
+    return ";".join(f"{k}={v}" for k, v in patch_types.items()).encode()
+
+
 class HostMesh:
     """The library's host polyMesh (BlockMesh, jitter, Kuhn tets, RCM)."""
 
@@ -382,6 +392,39 @@ class HostMesh:
                                                   _p(np.asarray(mx, np.float64)), jitter, seed, cx, cy, radius,
                                                   C.byref(h)))
         return cls(h)
+
+    @classmethod
+    def read_openfoam(cls, directory, patch_types=None):
+        """OpenFOAM ASCII polyMesh directory (or case root); patch_types: {name: type} overrides."""
+        h = _vp()
+        _host_check(_L.cfx_hmesh_read_openfoam(str(directory).encode(), _types_spec(patch_types), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def read_gmsh(cls, path, patch_types=None):
+        """Gmsh ASCII .msh 2.2 / 4.1; patch_types: {name: type} (default wall)."""
+        h = _vp()
+        _host_check(_L.cfx_hmesh_read_gmsh(str(path).encode(), _types_spec(patch_types), C.byref(h)))
+        return cls(h)
+
+    def patches(self):
+        """[(name, type)] in patch order."""
+        out = []
+        for i in range(self.sizes()["nPatches"]):
+            name, typ = C.create_string_buffer(512), C.create_string_buffer(512)
+            _host_check(_L.cfx_hmesh_patch(self.h, i, name, 512, typ, 512))
+            out.append((name.value.decode(), typ.value.decode()))
+        return out
+
+    def write_vtu(self, path, fields=None):
+        """fields: {name: array [nCells] or [nCells, 3]} written as cell data."""
+        fields = fields or {}
+        keep = [np.ascontiguousarray(v, np.float64) for v in fields.values()]
+        names = (C.c_char_p * max(1, len(keep)))(*[k.encode() for k in fields.keys()])
+        comps = (C.c_int * max(1, len(keep)))(*[1 if a.ndim == 1 else a.shape[1] for a in keep])
+        data = (_vp * max(1, len(keep)))(*[a.ctypes.data_as(_vp) for a in keep])
+        _host_check(_L.cfx_vtu_write(self.h, str(path).encode(), len(keep), C.cast(names, _vp), C.cast(comps, _vp),
+                                     C.cast(data, _vp)))
 
     @classmethod
     def from_arrays(cls, points, faceStart, faceVerts, owner, neighbour, patchStart, patchSize):

@@ -69,6 +69,22 @@ int cfx_detail_fields(const cfx_detail* d, double* U, double* p, double* phi);
 /* the block's device mesh (pressure dumps, multigrid settings) */
 cfx_mesh* cfx_detail_mesh(const cfx_detail* d);
 
+/* OpenFoamMesh::fromDirectory (OpenFoamMesh.h:40-43): the ASCII polyMesh files of `dir` (or of
+ * dir/constant/polyMesh) as written; patch types wall/patch/"67" -> wall, empty/symmetryPlane ->
+ * symmetry, others verbatim.  patchTypes: "name=type;name=type" overrides (null or "" for none). */
+int cfx_hmesh_read_openfoam(const char* dir, const char* patchTypes, cfx_hmesh** out);
+/* GmshMesh::fromFile (GmshMesh.h:30-31): ASCII .msh 2.2 / 4.1, first-order tets, hexes, prisms;
+ * faces from the cell connectivity (internal faces first, boundary faces grouped by physical
+ * surface or bounding-box plane); patch types from patchTypes, default "wall". */
+int cfx_hmesh_read_gmsh(const char* path, const char* patchTypes, cfx_hmesh** out);
+/* name and type of patch i (nul-terminated; buffers of the given capacity, each may be null) */
+int cfx_hmesh_patch(const cfx_hmesh* h, int64_t i, char* name, int64_t nameCap, char* type, int64_t typeCap);
+/* VtkWriter (VtkWriter.h:11-24): ASCII .vtu, every cell a polyhedron (type 42, with its face stream
+ * and face offsets), plus nFields cell fields: names[k], components[k] (1 or 3; vectors AoS [n][3]),
+ * data[k] (host) */
+int cfx_vtu_write(const cfx_hmesh* h, const char* path, int nFields, const char* const* names,
+                  const int* components, const double* const* data);
+
 int cfx_hmesh_from_arrays(int64_t nPoints, const double* points, int64_t nFaces,
                           const int64_t* faceStart, const int64_t* faceVerts, int64_t nInternal,
                           const int64_t* owner, const int64_t* neighbour, int64_t nPatches,

@@ -10,6 +10,7 @@
 // 2 zeroGradient, 3 symmetry, 4 wall (vector only).
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -17,10 +18,13 @@
 #include "cf/fvm/BlockMesh.h"
 #include "cf/fvm/Fields.h"
 #include "cf/fvm/FvMatrix.h"
+#include "cf/fvm/GmshMesh.h"
+#include "cf/fvm/OpenFoamMesh.h"
 #include "cf/fvm/PisoSolver.h"
 #include "cf/fvm/PolyMesh.h"
 #include "cf/fvm/PressureCorrection.h"
 #include "cf/fvm/SimpleSolver.h"
+#include "cf/fvm/VtkWriter.h"
 
 using namespace cf;
 
@@ -97,6 +101,114 @@ void* cfr_mesh_new(int64_t nPoints, const double* pts, int64_t nFaces,
 }
 
 void cfr_mesh_free(void* m) { delete static_cast<polyMesh*>(m); }
+
+// The reference's mesh readers: kind 0 OpenFoamMesh::fromDirectory, 1 GmshMesh::fromFile, with
+// patch types "name=type;..." (may be empty).  Returns the mesh handle (null on error, message in
+// cfr_read_error).
+static std::string g_readError;
+const char* cfr_read_error() { return g_readError.c_str(); }
+
+void* cfr_read_mesh(int kind, const char* path, const char* patchTypes)
+{
+    try
+    {
+        std::map<std::string, std::string> types;
+        std::string spec(patchTypes ? patchTypes : "");
+        std::size_t i = 0;
+        while (i < spec.size())
+        {
+            std::size_t j = spec.find(';', i);
+            if (j == std::string::npos) j = spec.size();
+            const std::string item = spec.substr(i, j - i);
+            const std::size_t eq = item.find('=');
+            if (eq != std::string::npos) types[item.substr(0, eq)] = item.substr(eq + 1);
+            i = j + 1;
+        }
+        polyMesh m = kind == 0 ? OpenFoamMesh::fromDirectory(path, types) : GmshMesh::fromFile(path, types);
+        return new polyMesh(std::move(m));
+    }
+    catch (const std::exception& e)
+    {
+        g_readError = e.what();
+        return nullptr;
+    }
+}
+
+// sizes: nPoints, nFaces, nInternal, nCells, nPatches, total face vertices
+void cfr_mesh_sizes(void* mh, int64_t* s)
+{
+    const polyMesh& m = *static_cast<polyMesh*>(mh);
+    int64_t fv = 0;
+    for (const auto& f : m.faces()) fv += static_cast<int64_t>(f.size());
+    s[0] = m.nPoints();
+    s[1] = m.nFaces();
+    s[2] = m.nInternalFaces();
+    s[3] = m.nCells();
+    s[4] = m.nPatches();
+    s[5] = fv;
+}
+
+// arrays (caller-sized from cfr_mesh_sizes); patch names and types as "name=type;..." into buf
+int cfr_mesh_arrays(void* mh, double* points, int64_t* faceStart, int64_t* faceVerts, int64_t* owner,
+                    int64_t* nbr, int64_t* pStart, int64_t* pSize, char* buf, int64_t cap)
+{
+    const polyMesh& m = *static_cast<polyMesh*>(mh);
+    for (label i = 0; i < m.nPoints(); ++i)
+    {
+        points[3 * i] = m.points()[i].x;
+        points[3 * i + 1] = m.points()[i].y;
+        points[3 * i + 2] = m.points()[i].z;
+    }
+    int64_t k = 0;
+    faceStart[0] = 0;
+    for (label f = 0; f < m.nFaces(); ++f)
+    {
+        for (const label v : m.faces()[f]) faceVerts[k++] = v;
+        faceStart[f + 1] = k;
+        owner[f] = m.owner()[f];
+        if (f < m.nInternalFaces()) nbr[f] = m.neighbour()[f];
+    }
+    std::string names;
+    for (label p = 0; p < m.nPatches(); ++p)
+    {
+        pStart[p] = m.patches()[p].start;
+        pSize[p] = m.patches()[p].size;
+        names += m.patches()[p].name + "=" + m.patches()[p].type + ";";
+    }
+    if (static_cast<int64_t>(names.size()) + 1 > cap) return -1;
+    std::memcpy(buf, names.c_str(), names.size() + 1);
+    return 0;
+}
+
+// VtkWriter with one cell scalar and one cell vector (AoS [n][3]); null skips a field
+int cfr_write_vtu(void* mh, const char* path, const char* sname, const double* s, const char* vname,
+                  const double* v)
+{
+    try
+    {
+        const polyMesh& m = *static_cast<polyMesh*>(mh);
+        Field<scalar> sf(static_cast<std::size_t>(m.nCells()));
+        Field<Vector> vf(static_cast<std::size_t>(m.nCells()));
+        VtkWriter w(m);
+        if (s)
+        {
+            for (label c = 0; c < m.nCells(); ++c) sf[c] = s[c];
+            w.addCellScalar(sname, sf);
+        }
+        if (v)
+        {
+            for (label c = 0; c < m.nCells(); ++c) vf[c] = {v[3 * c], v[3 * c + 1], v[3 * c + 2]};
+            w.addCellVector(vname, vf);
+        }
+        w.write(path);
+        return 0;
+    }
+    catch (const std::exception& e)
+    {
+        g_readError = e.what();
+        return -1;
+    }
+}
 
 int64_t cfr_mesh_ncells(void* m) { return static_cast<polyMesh*>(m)->nCells(); }
 
