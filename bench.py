@@ -183,7 +183,7 @@ def bench_c5(args):
     ctx = N.Context(0)
     t0 = time.perf_counter()
     specs = DB.block_specs(count=args.blocks, seed=5)
-    blocks = DB.build_blocks(ctx, specs, simple_iterations=3, log=log)
+    blocks = DB.build_blocks(ctx, specs, simple_iterations=3, log=log, own_context=True)
     cells = sum(b["n"] for b in blocks)
     log(f"{len(blocks)} blocks, {cells} cells, built in {time.perf_counter() - t0:.1f}s")
     xs = [ctx.zeros(b["n"]) for b in blocks]
@@ -210,6 +210,13 @@ def bench_c5(args):
             bdef.append((t4 - t3) * 1e3)
         log(f"step {i}: one by one {(t2 - t1) * 1e3:.1f} ms (device {sum(r.ms for r in r1):.1f}), batched tuned "
             f"{(t3 - t2) * 1e3:.1f} ms (device {st.ms:.1f}), batched default {(t4 - t3) * 1e3:.1f} ms")
+    # the coupling loop's pass over its blocks (lossAt: 3 warm-started GPU SIMPLE iterations each), one
+    # block after another and on concurrent streams (one context per block, host threads)
+    passes = {}
+    for threads in (1, 4, 8):
+        ms, dps = DB.coupling_pass(blocks, 3, threads)
+        passes[threads] = ms
+        log(f"coupling pass, {threads} thread(s): {ms:.1f} ms, dp[0] {dps[0]:.6g} Pa")
     bitwise = all(np.array_equal(a.get(), b.get()) for a, b in zip(x1, xd))
     conv = all(r.converged for r in rb) and all(r.converged for r in r1)
     tb, to = float(np.mean(bat)), float(np.mean(one))
@@ -230,6 +237,10 @@ def bench_c5(args):
         "speedup_batched_vs_one_by_one": round(to / tb, 3), "autotune_s": round(tune_s, 3),
         "schedule": best.as_dict(), "candidates": [{"schedule": c, "ms": round(m, 4)} for c, m in table],
         "max_iterations": st.maxIterations, "all_converged": conv, "default_schedule_bitwise_equal": bitwise,
+        "coupling_pass_ms": {str(k): round(v, 1) for k, v in passes.items()},
+        "coupling_pass": "lossAt of every block (3 warm-started GPU SIMPLE iterations each, relaxU 0.9, relaxP 0.01; "
+                         "momentum BiCGStab, pressure Jacobi-PCG FP32+FP64 refinement to 1e-6), serial vs host "
+                         "threads on one stream per block",
         "per_block": per_block,
     }
     print(json.dumps(result), flush=True)

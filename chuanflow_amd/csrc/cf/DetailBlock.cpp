@@ -2,6 +2,8 @@
 // conditions, GPU SIMPLE and section measurement.
 #include "DetailBlock.h"
 
+#include <initializer_list>
+
 #include <algorithm>
 #include <cmath>
 #include <limits>
@@ -87,11 +89,36 @@ DetailBlock::DetailBlock(cfx_ctx* ctx, const polyMesh& mesh, double nu, double r
     check(cfx_zero(ctx_, dU_, sizeof(double) * 3 * n));
     check(cfx_zero(ctx_, dp_, sizeof(double) * n));
     check(cfx_zero(ctx_, dphi_, sizeof(double) * std::max<label>(nF, 1)));
+
+    // what measure() reads, resident on the device
+    auto upload = [&](const void* src, std::size_t bytes) {
+        void* d = nullptr;
+        check(cfx_malloc(ctx_, std::max<std::size_t>(bytes, 8), &d));
+        if (bytes) check(cfx_memcpy_htod(ctx_, d, src, bytes));
+        return d;
+    };
+    dV_ = static_cast<double*>(upload(mesh_.cellVolumes().data(), sizeof(double) * n));
+    const auto& s0 = sections_.front();
+    const auto& s1 = sections_.back();
+    std::vector<int64_t> cellIdx(s0.begin(), s0.end());
+    cellIdx.insert(cellIdx.end(), s1.begin(), s1.end());
+    const int64_t cellSeg[3] = {0, static_cast<int64_t>(s0.size()), static_cast<int64_t>(cellIdx.size())};
+    dCellIdx_ = static_cast<int64_t*>(upload(cellIdx.data(), sizeof(int64_t) * cellIdx.size()));
+    dCellSeg_ = static_cast<int64_t*>(upload(cellSeg, sizeof(cellSeg)));
+    const auto& pin = mesh_.patches()[static_cast<std::size_t>(inletPatch_)];
+    const auto& pout = mesh_.patches()[static_cast<std::size_t>(outletPatch_)];
+    std::vector<int64_t> faceIdx;
+    for (label f = pin.start; f < pin.start + pin.size; ++f) faceIdx.push_back(f);
+    for (label f = pout.start; f < pout.start + pout.size; ++f) faceIdx.push_back(f);
+    const int64_t faceSeg[3] = {0, static_cast<int64_t>(pin.size), static_cast<int64_t>(faceIdx.size())};
+    dFaceIdx_ = static_cast<int64_t*>(upload(faceIdx.data(), sizeof(int64_t) * faceIdx.size()));
+    dFaceSeg_ = static_cast<int64_t*>(upload(faceSeg, sizeof(faceSeg)));
 }
 
 DetailBlock::~DetailBlock()
 {
-    for (double* q : {dU_, dp_, dphi_, dlc_, dlk_})
+    for (void* q : std::initializer_list<void*>{dU_, dp_, dphi_, dlc_, dlk_, dV_, dCellSeg_, dCellIdx_, dFaceSeg_,
+                                                 dFaceIdx_})
         if (q) (void)cfx_free(ctx_, q);
     if (dm_) (void)cfx_mesh_destroy(dm_);
 }
@@ -259,38 +286,24 @@ double DetailBlock::lossAt(double q, const cfx_simple_controls& ctl)
     return measure_.dpPa;
 }
 
-// section-mean pressures and boundary fluxes (coupledNetwork.cpp:1154-1200)
+// section-mean pressures and boundary fluxes (coupledNetwork.cpp:1154-1200),
+// reduced on the device: the fields never leave it
 void DetailBlock::measure(const cfx_simple_result& r, double q)
 {
-    const label n = mesh_.nCells(), nF = mesh_.nFaces();
-    std::vector<double> p(static_cast<std::size_t>(n)), phi(static_cast<std::size_t>(nF));
-    check(cfx_memcpy_dtoh(ctx_, p.data(), dp_, sizeof(double) * n));
-    check(cfx_memcpy_dtoh(ctx_, phi.data(), dphi_, sizeof(double) * nF));
     measure_ = BlockMeasure{};
-    measure_.cells = n;
+    measure_.cells = mesh_.nCells();
     measure_.sweeps = r.iterations;
     measure_.converged = r.converged;
     measure_.imposedQ = q;
     measure_.inletArea = inletArea_;
     measure_.plugSpeed = q / inletArea_;
-    const auto& V = mesh_.cellVolumes();
-    auto sectionMean = [&](const std::vector<label>& cells) {
-        double num = 0, den = 0;
-        for (label c : cells)
-        {
-            num += p[static_cast<std::size_t>(c)] * V[static_cast<std::size_t>(c)];
-            den += V[static_cast<std::size_t>(c)];
-        }
-        return rho_ * (den > 0 ? num / den : 0.0);
-    };
-    measure_.dpPa = sectionMean(sections_.front()) - sectionMean(sections_.back());
-    double fin = 0, fout = 0;
-    const auto& pin = mesh_.patches()[static_cast<std::size_t>(inletPatch_)];
-    const auto& pout = mesh_.patches()[static_cast<std::size_t>(outletPatch_)];
-    for (label f = pin.start; f < pin.start + pin.size; ++f) fin += phi[static_cast<std::size_t>(f)];
-    for (label f = pout.start; f < pout.start + pout.size; ++f) fout += phi[static_cast<std::size_t>(f)];
-    measure_.fluxIn = -fin;
-    measure_.fluxOut = fout;
+    double sec[4], flux[4];
+    check(cfx_segment_sums(ctx_, 2, dCellSeg_, dCellIdx_, dp_, dV_, sec));
+    check(cfx_segment_sums(ctx_, 2, dFaceSeg_, dFaceIdx_, dphi_, nullptr, flux));
+    auto mean = [&](int s) { return rho_ * (sec[2 * s + 1] > 0 ? sec[2 * s] / sec[2 * s + 1] : 0.0); };
+    measure_.dpPa = mean(0) - mean(1);
+    measure_.fluxIn = -flux[0];
+    measure_.fluxOut = flux[2];
 }
 
 void DetailBlock::fields(double* U, double* p, double* phi) const

@@ -45,6 +45,7 @@ public:
     int axis() const { return axis_; }
     int flowSign() const { return flowSign_; }
     double inletArea() const { return inletArea_; }
+    cfx_ctx* context() const { return ctx_; }
     std::size_t nSections() const { return sections_.size(); }
     const std::vector<label>& inletFaces() const { return inletFaces_; }
     const std::vector<double>& inletWeights() const { return inletWeights_; }
@@ -65,6 +66,10 @@ private:
     std::vector<double> inletWeights_;
     cfx_mesh* dm_ = nullptr;
     double *dU_ = nullptr, *dp_ = nullptr, *dphi_ = nullptr, *dlc_ = nullptr, *dlk_ = nullptr;
+    // measurement on the device: cell volumes; segments {first section, last section} of cells and
+    // {inlet, outlet} of faces, as start offsets [3] and indices
+    double* dV_ = nullptr;
+    int64_t *dCellSeg_ = nullptr, *dCellIdx_ = nullptr, *dFaceSeg_ = nullptr, *dFaceIdx_ = nullptr;
     BlockMeasure measure_;
 
     Vec3 patchArea(label patch) const;

@@ -158,6 +158,32 @@ void need(bool ok, const char* what)
     if (!ok) throw std::invalid_argument(what);
 }
 
+// segment sums out[s] = (sum x[i] w[i], sum w[i]) over i = idx[start[s] .. start[s+1]) (w null: 1),
+// one workgroup per segment, fixed-order block reduction
+__global__ __launch_bounds__(256) void k_segment_sums(const long long* __restrict__ start,
+                                                      const long long* __restrict__ idx,
+                                                      const double* __restrict__ x, const double* __restrict__ w,
+                                                      double* __restrict__ out)
+{
+    __shared__ double red[256 / kWave];
+    const int sgm = blockIdx.x;
+    double num = 0.0, den = 0.0;
+    for (long long k = start[sgm] + threadIdx.x; k < start[sgm + 1]; k += 256)
+    {
+        const long long i = idx[k];
+        const double wi = w ? w[i] : 1.0;
+        num += x[i] * wi;
+        den += wi;
+    }
+    num = blockSum(num, red);
+    den = blockSum(den, red);
+    if (threadIdx.x == 0)
+    {
+        out[2 * sgm] = num;
+        out[2 * sgm + 1] = den;
+    }
+}
+
 __global__ void k_zero_bytes(unsigned char* p, size_t n)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
@@ -316,6 +342,22 @@ int cfx_zero(cfx_ctx* c, void* p, size_t bytes)
         k_zero_bytes<<<std::max(1, std::min(4096, divUp((long long)bytes, 256))), 256, 0, c->ctx->stream()>>>(
             static_cast<unsigned char*>(p), bytes);
         CFX_CHECK_LAUNCH();
+    });
+}
+
+int cfx_segment_sums(cfx_ctx* c, int nSeg, const int64_t* d_segStart, const int64_t* d_idx, const double* d_x,
+                     const double* d_w, double* out)
+{
+    return guard(c, [&] {
+        need(nSeg >= 0 && (nSeg == 0 || (d_segStart && d_idx && d_x && out)), "bad segment arguments");
+        if (nSeg == 0) return;
+        DevBuf res(sizeof(double) * 2 * nSeg);
+        k_segment_sums<<<nSeg, 256, 0, c->ctx->stream()>>>(reinterpret_cast<const long long*>(d_segStart),
+                                                           reinterpret_cast<const long long*>(d_idx), d_x, d_w,
+                                                           res.as<double>());
+        CFX_CHECK_LAUNCH();
+        CFX_HIP(hipMemcpyAsync(out, res.p, sizeof(double) * 2 * nSeg, hipMemcpyDeviceToHost, c->ctx->stream()));
+        c->ctx->sync();
     });
 }
 

@@ -9,6 +9,10 @@
 #include <memory>
 
 #include <cstring>
+#include <vector>
+#include <thread>
+#include <set>
+#include <atomic>
 #include <string>
 
 struct cfx_hmesh
@@ -139,6 +143,51 @@ int cfx_detail_loss_at(cfx_detail* d, double q, const cfx_simple_controls* ctl, 
             *m = cfx_detail_measure{b.cells, b.sweeps, b.converged, b.imposedQ, b.inletArea, b.plugSpeed,
                                     b.dpPa, b.fluxIn, b.fluxOut};
         }
+    });
+}
+
+int cfx_detail_loss_at_many(int count, cfx_detail* const* blocks, const double* q, const cfx_simple_controls* ctl,
+                            int threads, cfx_detail_measure* m)
+{
+    return guardHost([&] {
+        if (count < 0 || (count > 0 && (!blocks || !q || !ctl))) throw std::invalid_argument("null argument");
+        if (threads < 1) throw std::invalid_argument("threads must be >= 1");
+        std::set<cfx_ctx*> ctxs;
+        for (int i = 0; i < count; ++i)
+        {
+            if (!blocks[i]) throw std::invalid_argument("null block");
+            ctxs.insert(blocks[i]->blk->context());
+        }
+        if (threads > 1 && static_cast<int>(ctxs.size()) != count)
+            throw std::invalid_argument("concurrent lossAt needs one context (stream) per block");
+        std::atomic<int> next{0};
+        std::vector<std::string> errs(static_cast<std::size_t>(count));
+        auto work = [&] {
+            for (int i; (i = next++) < count;)
+            {
+                try
+                {
+                    blocks[i]->blk->lossAt(q[i], *ctl);
+                    if (m)
+                    {
+                        const cf::BlockMeasure& b = blocks[i]->blk->lastMeasure();
+                        m[i] = cfx_detail_measure{b.cells, b.sweeps, b.converged, b.imposedQ, b.inletArea,
+                                                  b.plugSpeed, b.dpPa, b.fluxIn, b.fluxOut};
+                    }
+                }
+                catch (const std::exception& e)
+                {
+                    errs[static_cast<std::size_t>(i)] = e.what();
+                }
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < std::min(threads, count); ++t) pool.emplace_back(work);
+        work();
+        for (auto& t : pool) t.join();
+        for (int i = 0; i < count; ++i)
+            if (!errs[static_cast<std::size_t>(i)].empty())
+                throw cf::Error("block " + std::to_string(i) + ": " + errs[static_cast<std::size_t>(i)]);
     });
 }
 

@@ -32,23 +32,29 @@ def block_specs(count=64, cells=(20_000, 300_000), aspect=(4.0, 20.0), speed=(0.
     return out
 
 
+# The coupled cases' relaxation (relaxU 0.9, relaxP 0.03, pRelaxStart 0.02) diverges on these finer
+# blocks within 10-40 SIMPLE iterations, in the oracle as on the GPU (the pressure loop's gain grows
+# with resolution); relaxP 0.01 from a 0.005 floor converges on every block tried, the largest included.
 SIMPLE_CTL = dict(continuityTol=0, pcorrTol=0, momentumTol=0, linearTol=1e-6, linearMaxIter=3000,
-                  nonOrthCorrectors=1, relaxU=0.9, relaxP=0.03, pRelaxStart=0.02, pPrecision=N.FP32_IR)
+                  nonOrthCorrectors=1, relaxU=0.9, relaxP=0.01, pRelaxStart=0.005, pPrecision=N.FP32_IR)
 
 
-def build_blocks(ctx, specs, simple_iterations=3, jitter=0.1, nu=1e-3, rho=1000.0, log=None):
+def build_blocks(ctx, specs, simple_iterations=3, jitter=0.1, nu=1e-3, rho=1000.0, log=None, own_context=False):
     """Blocks with their last SIMPLE pressure system: list of dicts with the
-    Csr, device rhs, dp and sizes."""
+    Csr (on `ctx`), device rhs, dp and sizes.  With own_context each block's
+    DetailBlock runs on a context (stream) of its own and is kept ("detail",
+    "q"), for concurrent coupling passes."""
     blocks = []
     for k, s in enumerate(specs):
         t0 = time.perf_counter()
         hm = N.HostMesh.channel(s["nx"], s["ny"], s["nz"], (0, 0, 0), (s["length"], s["width"], s["width"]), jitter,
                                 s["seed"])
-        blk = N.Detail(ctx, hm, nu=nu, rho=rho, shape=s["shape"])
+        bctx = N.Context(0) if own_context else ctx
+        blk = N.Detail(bctx, hm, nu=nu, rho=rho, shape=s["shape"])
         dm = blk.mesh()
         nnz = dm.sizes()["nnz"]
         n = blk.nCells
-        dval, drhs = ctx.empty((simple_iterations, nnz)), ctx.empty((simple_iterations, n))
+        dval, drhs = bctx.empty((simple_iterations, nnz)), bctx.empty((simple_iterations, n))
         dm.set_pressure_dump(dval, drhs, 1, simple_iterations)
         q = s["speed"] * s["width"] * s["width"]
         dp, m = blk.loss_at(q, N.SimpleControls(maxIterations=simple_iterations, **SIMPLE_CTL))
@@ -60,7 +66,8 @@ def build_blocks(ctx, specs, simple_iterations=3, jitter=0.1, nu=1e-3, rho=1000.
         vals = dval.get()[last].copy()
         rhs = drhs.get()[last].copy()
         G = N.Csr(ctx, rp.astype(np.int64), cl.astype(np.int64), vals)
-        blocks.append(dict(csr=G, b=ctx.array(rhs), n=n, nnz=nnz, dp=dp, measure=m, spec=s))
+        blocks.append(dict(csr=G, b=ctx.array(rhs), n=n, nnz=nnz, dp=dp, measure=m, spec=s, q=q,
+                           detail=blk if own_context else None, ctx=bctx if own_context else None))
         del blk, dval, drhs
         if log:
             log(f"block {k}: {s['nx']}x{s['ny']}x{s['nz']} = {n} cells, {s['shape']}, dp {dp:.4g} Pa "
@@ -82,3 +89,12 @@ def solve_batched(ctx, blocks, tol=1e-8, maxIter=20000, schedule=None):
     reps, fin, st = N.batch_solve(ctx, [b["csr"] for b in blocks], [b["b"] for b in blocks], xs, tol, maxIter,
                                   schedule)
     return xs, reps, fin, st
+
+
+def coupling_pass(blocks, simple_iterations=3, threads=1):
+    """One pass of the coupling loop over the blocks: lossAt at each block's rate, warm-started
+    from its fields; returns (wall ms, dp list)."""
+    ctl = N.SimpleControls(maxIterations=simple_iterations, **SIMPLE_CTL)
+    t0 = time.perf_counter()
+    dps, _ = N.detail_loss_at_many([b["detail"] for b in blocks], [b["q"] for b in blocks], ctl, threads)
+    return (time.perf_counter() - t0) * 1e3, dps

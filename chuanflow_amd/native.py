@@ -210,6 +210,7 @@ _sig("cfx_hmesh_channel", C.c_int, _i64, _i64, _i64, _vp, _vp, C.c_double, C.c_u
 _sig("cfx_detail_create", C.c_int, _vp, _vp, C.c_double, C.c_double, C.c_char_p, C.POINTER(_vp))
 _sig("cfx_detail_destroy", C.c_int, _vp)
 _sig("cfx_detail_loss_at", C.c_int, _vp, C.c_double, _vp, _vp)
+_sig("cfx_detail_loss_at_many", C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, _vp)
 _sig("cfx_detail_info", C.c_int, _vp, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp, C.POINTER(_i64),
      C.POINTER(_i64))
 _sig("cfx_detail_inlet", C.c_int, _vp, _vp, _vp)
@@ -999,3 +1000,14 @@ class Detail:
 
     def mesh(self):
         return Mesh.borrowed(self.ctx, _L.cfx_detail_mesh(self.h), self.nCells, self.nFaces, self.nInternal)
+
+
+def detail_loss_at_many(blocks, qs, ctl: "SimpleControls", threads=1):
+    """lossAt of every block (Detail objects; with threads > 1 each needs its own Context):
+    returns (dp list, measure dicts)."""
+    n = len(blocks)
+    H = (_vp * n)(*[b.h for b in blocks])
+    q = np.ascontiguousarray(qs, np.float64)
+    m = (DetailMeasure * n)()
+    _host_check(_L.cfx_detail_loss_at_many(n, C.cast(H, _vp), _p(q), C.byref(ctl), int(threads), C.cast(m, _vp)))
+    return [x.dpPa for x in m], [x.as_dict() for x in m]

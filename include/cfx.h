@@ -73,6 +73,12 @@ int cfx_memcpy_dtoh(cfx_ctx* ctx, void* dst, const void* d_src, size_t bytes);
 int cfx_memcpy_dtod(cfx_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
 /* zero-fill by kernel (safe inside captured graphs) */
 int cfx_zero(cfx_ctx* ctx, void* d_ptr, size_t bytes);
+/* weighted sums over index segments, on the device: out[2 s] = sum x[i] w[i], out[2 s + 1] = sum w[i]
+ * for i = d_idx[d_segStart[s] .. d_segStart[s + 1]) (d_w null: weights 1); out is host [2 nSeg].
+ * The section means and boundary fluxes of a detail region (coupledNetwork.cpp:1154-1200) without
+ * copying its fields to the host; no counterpart in the reference. */
+int cfx_segment_sums(cfx_ctx* ctx, int nSeg, const int64_t* d_segStart, const int64_t* d_idx, const double* d_x,
+                     const double* d_w, double* out);
 
 /* ------------------------------------------------------------- mesh */
 /* Device mesh from a reference polyMesh's arrays

@@ -60,6 +60,12 @@ int cfx_detail_create(cfx_ctx* ctx, const cfx_hmesh* mesh, double nu, double rho
                       cfx_detail** out);
 int cfx_detail_destroy(cfx_detail* d);
 int cfx_detail_loss_at(cfx_detail* d, double q, const cfx_simple_controls* ctl, cfx_detail_measure* m);
+/* lossAt of `count` blocks (rates q[i], measures into m[i]; m may be null), the coupling loop's
+ * serial pass over its blocks (coupledNetwork.cpp:1461-1475) run by `threads` host threads; with
+ * threads > 1 every block needs its own context, so the blocks' kernels run on concurrent streams.
+ * An error in any block is reported after all have finished.  No counterpart in the reference. */
+int cfx_detail_loss_at_many(int count, cfx_detail* const* blocks, const double* q, const cfx_simple_controls* ctl,
+                            int threads, cfx_detail_measure* m);
 int cfx_detail_info(const cfx_detail* d, int* axis, int* flowSign, double* inletArea, int64_t* nSections,
                     int64_t* nInletFaces);
 /* inlet faces and their profile weights (flat: none) */

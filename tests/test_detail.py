@@ -75,3 +75,33 @@ def test_detail_block_matches_oracle(ctx, shape):
     assert abs(m["fluxIn"] - fino) <= 1e-9 * q and abs(m["fluxOut"] - fouto) <= 1e-6 * q
     Ug, pg, phig = blk.fields()
     assert np.abs(pg - p).max() <= 1e-6 * np.abs(p).max()
+
+
+@pytest.mark.gpu
+def test_detail_blocks_concurrent_loss_at_equals_serial():
+    """The coupling pass over several blocks run by host threads on one
+    stream per block (cfx_detail_loss_at_many) gives, block by block, the
+    pressure drop and fluxes of the blocks run one after another."""
+    from chuanflow_amd import native as N
+    print("\n[case] 6 detail blocks, lossAt serial vs 4 threads (one context per block)", flush=True)
+    ctl = N.SimpleControls(maxIterations=5, continuityTol=0, pcorrTol=0, momentumTol=0, linearTol=1e-8,
+                           linearMaxIter=3000, nonOrthCorrectors=1, relaxU=0.9, relaxP=0.03, pRelaxStart=0.02)
+    specs = [(12 + 2 * k, 5, 5, 0.001 * (k + 1), "parabolic" if k % 2 else "flat") for k in range(6)]
+
+    def make():
+        out = []
+        for k, (nx, ny, nz, q, shape) in enumerate(specs):
+            ctx = N.Context(0)
+            hm = N.HostMesh.channel(nx, ny, nz, (0, 0, 0), (1.0, 0.2, 0.2), 0.1, 40 + k)
+            out.append((ctx, N.Detail(ctx, hm, nu=1e-3, rho=1000.0, shape=shape), q))
+        return out
+
+    serial, conc = make(), make()
+    ref = [blk.loss_at(q, ctl)[1] for _, blk, q in serial]
+    dps, ms = N.detail_loss_at_many([b for _, b, _ in conc], [q for _, _, q in conc], ctl, threads=4)
+    for r, m, dp in zip(ref, ms, dps):
+        assert np.isfinite(dp) and dp == r["dpPa"] and m["fluxIn"] == r["fluxIn"] and m["fluxOut"] == r["fluxOut"]
+    with pytest.raises(RuntimeError, match="one context"):
+        shared = N.Context(0)
+        hm = N.HostMesh.channel(12, 5, 5, (0, 0, 0), (1.0, 0.2, 0.2), 0.1, 3)
+        N.detail_loss_at_many([N.Detail(shared, hm), N.Detail(shared, hm)], [0.001, 0.001], ctl, threads=2)
