@@ -213,10 +213,11 @@ def bench_c5(args):
     # the coupling loop's pass over its blocks (lossAt: 3 warm-started GPU SIMPLE iterations each), one
     # block after another and on concurrent streams (one context per block, host threads)
     passes = {}
-    for threads in (1, 4, 8):
-        ms, dps = DB.coupling_pass(blocks, 3, threads)
-        passes[threads] = ms
-        log(f"coupling pass, {threads} thread(s): {ms:.1f} ms, dp[0] {dps[0]:.6g} Pa")
+    for pname, prec in (("jacobi", N.FP32_IR), ("multigrid", N.FP32_MG)):
+        for threads in (1, 4, 8):
+            ms, dps = DB.coupling_pass(blocks, 3, threads, prec)
+            passes[f"{pname}_{threads}"] = ms
+            log(f"coupling pass, {pname} pressure, {threads} thread(s): {ms:.1f} ms, dp[0] {dps[0]:.6g} Pa")
     bitwise = all(np.array_equal(a.get(), b.get()) for a, b in zip(x1, xd))
     conv = all(r.converged for r in rb) and all(r.converged for r in r1)
     tb, to = float(np.mean(bat)), float(np.mean(one))
@@ -237,10 +238,10 @@ def bench_c5(args):
         "speedup_batched_vs_one_by_one": round(to / tb, 3), "autotune_s": round(tune_s, 3),
         "schedule": best.as_dict(), "candidates": [{"schedule": c, "ms": round(m, 4)} for c, m in table],
         "max_iterations": st.maxIterations, "all_converged": conv, "default_schedule_bitwise_equal": bitwise,
-        "coupling_pass_ms": {str(k): round(v, 1) for k, v in passes.items()},
+        "coupling_pass_ms": {k: round(v, 1) for k, v in passes.items()},
         "coupling_pass": "lossAt of every block (3 warm-started GPU SIMPLE iterations each, relaxU 0.9, relaxP 0.01; "
-                         "momentum BiCGStab, pressure Jacobi-PCG FP32+FP64 refinement to 1e-6), serial vs host "
-                         "threads on one stream per block",
+                         "momentum BiCGStab, pressure to 1e-6 by Jacobi-PCG FP32+FP64 refinement or multigrid FCG), "
+                         "serial vs host threads on one stream per block; passes continue one from another",
         "per_block": per_block,
     }
     print(json.dumps(result), flush=True)

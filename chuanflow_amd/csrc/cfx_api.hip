@@ -23,6 +23,7 @@ struct cfx_ctx
     std::string err;
     std::map<int, std::unique_ptr<Krylov>> krylov;  // workspace per system size
     bool graphs = true;
+    bool batchComponents = true;
     std::unique_ptr<BatchedPcg> batch;
 
     Krylov& solver(int n, int rowBlocks)
@@ -32,6 +33,7 @@ struct cfx_ctx
         // of the same size with more CSR-stream blocks
         if (!k || k->maxBlocks() < rowBlocks) k = std::make_unique<Krylov>(*ctx, n, std::max(rowBlocks, 1));
         k->useGraphs = graphs;
+        k->batchComponents = batchComponents;
         return *k;
     }
 };
@@ -846,6 +848,13 @@ int cfx_set_graphs(cfx_ctx* c, int enable)
 {
     if (!c) return CFX_E_INVALID;
     c->graphs = enable != 0;
+    return CFX_OK;
+}
+
+int cfx_set_component_batching(cfx_ctx* c, int enable)
+{
+    if (!c) return CFX_E_INVALID;
+    c->batchComponents = enable != 0;
     return CFX_OK;
 }
 

@@ -55,6 +55,16 @@ public:
     SolveReport bicgstab(const CsrView& A, const double* val, const double* b, double* x, double tol,
                          int maxIter);
 
+    // Jacobi-BiCGStab in FP64 of the ncomp (1..3) systems of one pattern: the
+    // components of a vector equation (fvMatrix::solve, fvMatrix.cpp:93-122),
+    // values at val + c*valStride, b and x as [c][n].  Each component runs
+    // exactly the arithmetic of bicgstab() on its own system (same kernels,
+    // same reductions), so x and reps[c] equal ncomp separate solves bit for
+    // bit; the components share each launch, a third of the launches.
+    void bicgstabComponents(const CsrView& A, const double* val, long long valStride, const double* b, double* x,
+                            int ncomp, double tol, int maxIter, SolveReport* reps);
+    static constexpr int kMaxComponents = 3;
+
     // Jacobi-BiCGStab in FP32 with FP64 iterative refinement: r = b - A x in
     // FP64, the FP32 solve of A d = r from d = 0 to `innerTol` relative (or to
     // the overall tolerance when that is closer), x += d; the reference's stop
@@ -94,6 +104,8 @@ public:
     int minBatch = 8, maxBatch = 256;
     // flexible CG: replay one captured iteration (true) or launch its kernels one by one
     bool useGraphs = true;
+    // the drivers' FP64 vector solves: all components in shared launches (true) or one by one
+    bool batchComponents = true;
 
 private:
     template <typename T>
@@ -113,6 +125,8 @@ private:
     DevBuf r32_, p32_, q32_, d32_, b32_, invD32_;
     DevBuf z32_, fcgPart_;  // allocated by the first fcgMixed
     DevBuf bf_[6], val32b_;  // FP32 BiCGStab vectors and matrix, allocated by the first bicgstabMixed
+    DevBuf wc_, stC_, partC_;  // component BiCGStab vectors, states, partials (first bicgstabComponents)
+    SolveState* hostC_ = nullptr;  // pinned
     // captured flexible-CG iteration, valid for the key (multigrid, buffers) and the stop parameters
     hipGraphExec_t fcgExec_ = nullptr;
     std::vector<const void*> fcgKey_;

@@ -335,13 +335,26 @@ __global__ __launch_bounds__(kVecBlock) void k_bicg_init_final(int G, const doub
     }
 }
 
+// The iteration kernels below solve blockIdx.y's system: component c uses the
+// vectors at offset c*vs, the matrix values at c*ms, the partials at c*ps
+// (doubles) and c*ips (ints) and state st[c]; a single solve is c = 0 with
+// zero strides.  Converged components return at once.
+
 // p = r + beta (p - omega v); z = p invD   (:267-272)
 template <typename T>
-__global__ __launch_bounds__(kVecBlock) void k_bicg_p(int n, const T* __restrict__ invD, const T* __restrict__ r,
-                                                      T* __restrict__ p, const T* __restrict__ v,
-                                                      T* __restrict__ z, const SolveState* st)
+__global__ __launch_bounds__(kVecBlock) void k_bicg_p(int n, long long vs, const T* __restrict__ invD,
+                                                      const T* __restrict__ r, T* __restrict__ p,
+                                                      const T* __restrict__ v, T* __restrict__ z,
+                                                      const SolveState* st)
 {
+    const long long o = blockIdx.y * vs;
+    st += blockIdx.y;
     if (st->done) return;
+    invD += o;
+    r += o;
+    p += o;
+    v += o;
+    z += o;
     const T beta = static_cast<T>(st->beta), omega = static_cast<T>(st->omega);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
@@ -356,14 +369,21 @@ __global__ __launch_bounds__(kVecBlock) void k_bicg_p(int n, const T* __restrict
 // which == 1: t = A sh, tt = t.t, ts = t.s, omega = ts/tt (:290-300)
 // (per-workgroup partials over rbPerWg row blocks; k_bicg_dot_final reduces them)
 template <typename T>
-__global__ __launch_bounds__(kSpmvBlock) void k_bicg_spmv(CsrView A, int rbPerWg, const T* __restrict__ val,
-                                                          const T* __restrict__ x, T* __restrict__ y,
-                                                          const T* __restrict__ w, const SolveState* st,
-                                                          double* part, int which)
+__global__ __launch_bounds__(kSpmvBlock) void k_bicg_spmv(CsrView A, int rbPerWg, long long ms, long long vs, int ps,
+                                                          const T* __restrict__ val, const T* __restrict__ x,
+                                                          T* __restrict__ y, const T* __restrict__ w,
+                                                          const SolveState* st, double* part, int which)
 {
     __shared__ T lds[kSpmvLds];
     __shared__ double red[kSpmvBlock / kWave];
+    st += blockIdx.y;
     if (st->done) return;
+    const long long o = blockIdx.y * vs;
+    val += blockIdx.y * ms;
+    x += o;
+    y += o;
+    w += o;
+    part += blockIdx.y * ps;
     const int G = gridDim.x;
     const int wg = xcdRemap(blockIdx.x, G);
     double d0 = 0, d1 = 0;
@@ -386,11 +406,13 @@ __global__ __launch_bounds__(kSpmvBlock) void k_bicg_spmv(CsrView A, int rbPerWg
     if (threadIdx.x == 0) { part[wg] = d0; part[G + wg] = d1; }
 }
 
-__global__ __launch_bounds__(kSpmvBlock) void k_bicg_dot_final(int G, const double* part, SolveState* st,
+__global__ __launch_bounds__(kSpmvBlock) void k_bicg_dot_final(int G, int ps, const double* part, SolveState* st,
                                                                int which)
 {
     __shared__ double red[kSpmvBlock / kWave];
+    st += blockIdx.y;
     if (st->done) return;
+    part += blockIdx.y * ps;
     const double D0 = sumPartials(part, G, red);
     const double D1 = sumPartials(part + G, G, red);
     if (threadIdx.x == 0)
@@ -413,12 +435,21 @@ __global__ __launch_bounds__(kSpmvBlock) void k_bicg_dot_final(int G, const doub
 
 // x += alpha z; s = r - alpha v; sh = s invD   (:278-289)
 template <typename T>
-__global__ __launch_bounds__(kVecBlock) void k_bicg_s(int n, const T* __restrict__ invD, T* __restrict__ x,
-                                                      const T* __restrict__ z, const T* __restrict__ r,
-                                                      const T* __restrict__ v, T* __restrict__ s,
-                                                      T* __restrict__ sh, const SolveState* st)
+__global__ __launch_bounds__(kVecBlock) void k_bicg_s(int n, long long vs, const T* __restrict__ invD,
+                                                      T* __restrict__ x, const T* __restrict__ z,
+                                                      const T* __restrict__ r, const T* __restrict__ v,
+                                                      T* __restrict__ s, T* __restrict__ sh, const SolveState* st)
 {
+    const long long o = blockIdx.y * vs;
+    st += blockIdx.y;
     if (st->done) return;
+    invD += o;
+    x += o;
+    z += o;
+    r += o;
+    v += o;
+    s += o;
+    sh += o;
     const T alpha = static_cast<T>(st->alpha);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
@@ -431,14 +462,25 @@ __global__ __launch_bounds__(kVecBlock) void k_bicg_s(int n, const T* __restrict
 
 // x += omega sh; r = s - omega t; ||r||inf; next rho = r0.r   (:301-314, :263-266)
 template <typename T>
-__global__ __launch_bounds__(kVecBlock) void k_bicg_r(int n, T* __restrict__ x, const T* __restrict__ sh,
-                                                      const T* __restrict__ s, const T* __restrict__ t,
-                                                      const T* __restrict__ r0, T* __restrict__ r,
-                                                      const SolveState* st, double* part, int* ipart)
+__global__ __launch_bounds__(kVecBlock) void k_bicg_r(int n, long long vs, int ps, int ips, T* __restrict__ x,
+                                                      const T* __restrict__ sh, const T* __restrict__ s,
+                                                      const T* __restrict__ t, const T* __restrict__ r0,
+                                                      T* __restrict__ r, const SolveState* st, double* part,
+                                                      int* ipart)
 {
     __shared__ double red[kVecBlock / kWave];
     __shared__ int ired[kVecBlock / kWave];
+    const long long o = blockIdx.y * vs;
+    st += blockIdx.y;
     if (st->done) return;
+    x += o;
+    sh += o;
+    s += o;
+    t += o;
+    r0 += o;
+    r += o;
+    part += blockIdx.y * ps;
+    ipart += blockIdx.y * ips;
     const bool tsmall = st->breakdown == 2;
     const T omega = static_cast<T>(st->omega);
     double rm = 0, rr0 = 0;
@@ -468,12 +510,15 @@ __global__ __launch_bounds__(kVecBlock) void k_bicg_r(int n, T* __restrict__ x, 
     if (threadIdx.x == 0) { part[blockIdx.x] = rm; part[G + blockIdx.x] = rr0; ipart[blockIdx.x] = nan; }
 }
 
-__global__ __launch_bounds__(kVecBlock) void k_bicg_r_final(int G, const double* part, const int* ipart,
-                                                            SolveState* st)
+__global__ __launch_bounds__(kVecBlock) void k_bicg_r_final(int G, int ps, int ips, const double* part,
+                                                            const int* ipart, SolveState* st)
 {
     __shared__ double red[kVecBlock / kWave];
     __shared__ int ired[kVecBlock / kWave];
+    st += blockIdx.y;
     if (st->done) return;
+    part += blockIdx.y * ps;
+    ipart += blockIdx.y * ips;
     const bool tsmall = st->breakdown == 2;
     const double RM = maxPartials(part, G, red);
     const double RR0 = sumPartials(part + G, G, red);
@@ -545,6 +590,27 @@ __global__ void k_add_float(int n, const float* __restrict__ d, double* __restri
 
 int vecGridFor(long long n) { return std::max(1, std::min(kVecGridMax, divUp(n, 4LL * kVecBlock))); }
 
+// One BiCGStab iteration (solveBiCGStab :267-314) of `nc` systems on the
+// pattern A: vectors w = {r, r0, p, v, z, s, sh, t} and x at stride vs,
+// values at stride ms, partials at strides ps / ips, states st[0..nc).
+template <typename T>
+void launchBicgIteration(hipStream_t s, const CsrView& A, int Gv, int Gs, int rbPerWg, int nc, long long vs,
+                         long long ms, int ps, int ips, const T* val, const T* invD, T* x, T* const* w,
+                         SolveState* st, double* part, int* ipart)
+{
+    T *r = w[0], *r0 = w[1], *p = w[2], *v = w[3], *z = w[4], *sv = w[5], *sh = w[6], *t = w[7];
+    const int n = A.n;
+    const dim3 gv(Gv, nc), gs(Gs, nc), g1(1, nc);
+    k_bicg_p<T><<<gv, kVecBlock, 0, s>>>(n, vs, invD, r, p, v, z, st);
+    k_bicg_spmv<T><<<gs, kSpmvBlock, 0, s>>>(A, rbPerWg, ms, vs, ps, val, z, v, r0, st, part, 0);
+    k_bicg_dot_final<<<g1, kSpmvBlock, 0, s>>>(Gs, ps, part, st, 0);
+    k_bicg_s<T><<<gv, kVecBlock, 0, s>>>(n, vs, invD, x, z, r, v, sv, sh, st);
+    k_bicg_spmv<T><<<gs, kSpmvBlock, 0, s>>>(A, rbPerWg, ms, vs, ps, val, sh, t, sv, st, part, 1);
+    k_bicg_dot_final<<<g1, kSpmvBlock, 0, s>>>(Gs, ps, part, st, 1);
+    k_bicg_r<T><<<gv, kVecBlock, 0, s>>>(n, vs, ps, ips, x, sh, sv, t, r0, r, st, part, ipart);
+    k_bicg_r_final<<<g1, kVecBlock, 0, s>>>(Gv, ps, ips, part, ipart, st);
+}
+
 } // namespace
 
 void computeInvD(Context& ctx, const CsrView& A, const double* val, double* invD, bool bicgGuard)
@@ -575,6 +641,7 @@ Krylov::Krylov(Context& ctx, int n, int maxRowBlocks) : ctx_(ctx), n_(n)
 
 Krylov::~Krylov()
 {
+    if (hostC_) (void)hipHostFree(hostC_);
     if (fcgExec_) (void)hipGraphExecDestroy(fcgExec_);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
@@ -664,14 +731,7 @@ SolveReport Krylov::bicgRun(const CsrView& A, const T* val, const T* invD, const
         const int batch = nextBatch(hs->iter, rInit, hs->rnorm, tol * bnorm, maxIter);
         for (int k = 0; k < batch; ++k)
         {
-            k_bicg_p<T><<<Gv, kVecBlock, 0, s>>>(n_, invD, r, p, v, z, st);
-            k_bicg_spmv<T><<<Gs, kSpmvBlock, 0, s>>>(A, rbPerWg, val, z, v, r0, st, part, 0);
-            k_bicg_dot_final<<<1, kSpmvBlock, 0, s>>>(Gs, part, st, 0);
-            k_bicg_s<T><<<Gv, kVecBlock, 0, s>>>(n_, invD, x, z, r, v, sv, sh, st);
-            k_bicg_spmv<T><<<Gs, kSpmvBlock, 0, s>>>(A, rbPerWg, val, sh, t, sv, st, part, 1);
-            k_bicg_dot_final<<<1, kSpmvBlock, 0, s>>>(Gs, part, st, 1);
-            k_bicg_r<T><<<Gv, kVecBlock, 0, s>>>(n_, x, sh, sv, t, r0, r, st, part, ipart);
-            k_bicg_r_final<<<1, kVecBlock, 0, s>>>(Gv, part, ipart, st);
+            launchBicgIteration<T>(s, A, Gv, Gs, rbPerWg, 1, 0, 0, 0, 0, val, invD, x, w, st, part, ipart);
         }
         CFX_CHECK_LAUNCH();
         launched += batch;
@@ -700,6 +760,99 @@ SolveReport Krylov::bicgstab(const CsrView& A, const double* val, const double* 
     CFX_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
     lastMs_ = rep.ms = ms;
     return rep;
+}
+
+void Krylov::bicgstabComponents(const CsrView& A, const double* val, long long valStride, const double* b, double* x,
+                                int ncomp, double tol, int maxIter, SolveReport* reps)
+{
+    if (ncomp < 1 || ncomp > kMaxComponents)
+        throw std::invalid_argument("bicgstabComponents: 1 to 3 components");
+    if (A.n != n_) throw std::invalid_argument("bicgstabComponents: pattern size differs from the solver's");
+    hipStream_t s = ctx_.stream();
+    CFX_HIP(hipEventRecord(ev0_, s));
+    const long long nn = n_;
+    const int ps = 4 * maxBlocks_, ips = maxBlocks_;
+    if (!wc_.p)
+    {
+        wc_.alloc(sizeof(double) * 9 * kMaxComponents * static_cast<std::size_t>(nn));
+        stC_.alloc(sizeof(SolveState) * kMaxComponents);
+        partC_.alloc((sizeof(double) * ps + sizeof(int) * ips) * kMaxComponents);
+        CFX_HIP(hipHostMalloc(reinterpret_cast<void**>(&hostC_), sizeof(SolveState) * kMaxComponents,
+                              hipHostMallocDefault));
+    }
+    // work vector j of component c at wc + (j * ncomp + c) n: each vector kind is [c][n] with stride n
+    double* wc = wc_.as<double>();
+    double* w[8];
+    for (int j = 0; j < 8; ++j) w[j] = wc + j * ncomp * nn;
+    double* invD = wc + 8 * ncomp * nn;
+    SolveState* st = stC_.as<SolveState>();
+    double* part = partC_.as<double>();
+    int* ipart = reinterpret_cast<int*>(part + static_cast<std::size_t>(ps) * kMaxComponents);
+    int Gs = std::min(A.nRowBlocks, kVecGridMax);
+    const int rbPerWg = divUp(A.nRowBlocks, Gs);
+    Gs = std::max(1, divUp(A.nRowBlocks, rbPerWg));
+    const int Gv = vecGridFor(n_);
+    SolveState h{};
+    h.tol = tol;
+    h.maxIter = maxIter;
+    for (int c = 0; c < ncomp; ++c)
+    {
+        // each component starts as bicgRun starts a single solve (:244-266)
+        const long long o = c * nn;
+        double* pc = part + static_cast<std::size_t>(c) * ps;
+        int* ic = ipart + static_cast<std::size_t>(c) * ips;
+        computeInvD(ctx_, A, val + c * valStride, invD + o, true);
+        CFX_HIP(hipMemcpyAsync(st + c, &h, offsetof(SolveState, ticket), hipMemcpyHostToDevice, s));
+        k_residual<double><<<Gs, kSpmvBlock, 0, s>>>(A, rbPerWg, val + c * valStride, x + o, b + o, w[0] + o, pc, ic);
+        k_residual_final<<<1, kSpmvBlock, 0, s>>>(Gs, pc, ic, st + c, 0);
+        k_bicg_init<double><<<Gv, kVecBlock, 0, s>>>(n_, w[0] + o, w[1] + o, w[2] + o, w[3] + o, st + c, pc);
+        k_bicg_init_final<<<1, kVecBlock, 0, s>>>(Gv, pc, st + c);
+    }
+    CFX_CHECK_LAUNCH();
+    auto pollC = [&] {
+        CFX_HIP(hipMemcpyAsync(hostC_, st, sizeof(SolveState) * ncomp, hipMemcpyDeviceToHost, s));
+        CFX_HIP(hipStreamSynchronize(s));
+    };
+    pollC();
+    double rInit[kMaxComponents], bnorm[kMaxComponents];
+    for (int c = 0; c < ncomp; ++c)
+    {
+        rInit[c] = hostC_[c].initialResidual;
+        bnorm[c] = hostC_[c].bnorm;
+    }
+    int launched = 0;
+    for (;;)
+    {
+        // launch enough iterations for the slowest component still running
+        int batch = 0;
+        for (int c = 0; c < ncomp; ++c)
+            if (!hostC_[c].done)
+                batch = std::max(batch, nextBatch(hostC_[c].iter, rInit[c], hostC_[c].rnorm, tol * bnorm[c], maxIter));
+        batch = std::min(batch, maxIter - launched);
+        if (batch <= 0) break;
+        for (int k = 0; k < batch; ++k)
+            launchBicgIteration<double>(s, A, Gv, Gs, rbPerWg, ncomp, nn, valStride, ps, ips, val, invD, x, w, st,
+                                        part, ipart);
+        CFX_CHECK_LAUNCH();
+        launched += batch;
+        pollC();
+    }
+    CFX_HIP(hipEventRecord(ev1_, s));
+    CFX_HIP(hipEventSynchronize(ev1_));
+    float ms = 0;
+    CFX_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+    lastMs_ = ms;
+    for (int c = 0; c < ncomp; ++c)
+    {
+        const SolveState& hs = hostC_[c];
+        SolveReport& rep = reps[c];
+        rep = SolveReport{};
+        rep.iterations = hs.iter;
+        rep.initialResidual = hs.initialResidual;
+        rep.finalResidual = hs.iter > 0 ? hs.rnorm : hs.initialResidual;
+        rep.converged = hs.converged;
+        rep.ms = ms;
+    }
 }
 
 SolveReport Krylov::bicgstabMixed(const CsrView& A, const double* val, const double* b, double* x, double tol,

@@ -440,6 +440,14 @@ SolveReport fvSolve(DevMesh& m, Krylov& K, const Ldu& A, double* val, double* rh
                                a.gradCorrected, a.sign, A.rhs + static_cast<long long>(k) * n, gradTmp, faceTmp);
         if (a.projectMean)
             for (int k = 0; k < A.ncomp; ++k) projectZeroMean(m, A.rhs + static_cast<long long>(k) * n);
+        if (a.solver == 1 && a.precision == kFP64 && A.ncomp > 1 && K.batchComponents)
+        {
+            // the components' BiCGStab solves in shared launches, equal to the loop below bit for bit
+            SolveReport reps[Krylov::kMaxComponents];
+            K.bicgstabComponents(view, val, nnz, A.rhs, x, A.ncomp, a.tol, a.maxIter, reps);
+            last = reps[A.ncomp - 1];
+            continue;
+        }
         for (int k = 0; k < A.ncomp; ++k)
         {
             const double* v = val + static_cast<long long>(k) * nnz;

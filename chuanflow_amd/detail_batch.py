@@ -91,10 +91,11 @@ def solve_batched(ctx, blocks, tol=1e-8, maxIter=20000, schedule=None):
     return xs, reps, fin, st
 
 
-def coupling_pass(blocks, simple_iterations=3, threads=1):
+def coupling_pass(blocks, simple_iterations=3, threads=1, pressure=N.FP32_IR):
     """One pass of the coupling loop over the blocks: lossAt at each block's rate, warm-started
-    from its fields; returns (wall ms, dp list)."""
-    ctl = N.SimpleControls(maxIterations=simple_iterations, **SIMPLE_CTL)
+    from its fields; pressure is the SIMPLE pressure solve (FP64 / FP32_IR Jacobi-PCG, FP32_MG
+    multigrid FCG); returns (wall ms, dp list)."""
+    ctl = N.SimpleControls(maxIterations=simple_iterations, **dict(SIMPLE_CTL, pPrecision=pressure))
     t0 = time.perf_counter()
     dps, _ = N.detail_loss_at_many([b["detail"] for b in blocks], [b["q"] for b in blocks], ctl, threads)
     return (time.perf_counter() - t0) * 1e3, dps

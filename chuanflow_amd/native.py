@@ -166,6 +166,7 @@ _sig("cfx_csr_residual_history", C.c_int, _vp, _vp, C.c_int)
 _sig("cfx_csr_set_inner_tol", C.c_int, _vp, C.c_double)
 _sig("cfx_mg_default_params", C.c_int, C.POINTER(MgParams))
 _sig("cfx_set_graphs", C.c_int, _vp, C.c_int)
+_sig("cfx_set_component_batching", C.c_int, _vp, C.c_int)
 _sig("cfx_batch_solve", C.c_int, _vp, C.c_int, _vp, _vp, _vp, C.c_double, C.c_int, _vp, _vp, _vp,
      C.POINTER(BatchStats))
 _sig("cfx_batch_autotune", C.c_int, _vp, C.c_int, _vp, _vp, _vp, C.c_int, C.POINTER(BatchSchedule), _vp, _vp,
@@ -297,6 +298,10 @@ class Context:
     def set_graphs(self, enable: bool):
         """Flexible CG: replay one captured iteration per step (True) or launch kernel by kernel."""
         self.check(_L.cfx_set_graphs(self.h, 1 if enable else 0))
+
+    def set_component_batching(self, enable: bool):
+        """FP64 BiCGStab of vector equations: all components in shared launches (True) or one by one."""
+        self.check(_L.cfx_set_component_batching(self.h, 1 if enable else 0))
 
     def device_info(self):
         buf = C.create_string_buffer(64)

@@ -224,6 +224,11 @@ int cfx_mg_default_params(cfx_mg_params* p);
 /* flexible CG replays one captured iteration per step (hipGraph, default 1)
  * or launches its kernels one by one (0); both give identical results */
 int cfx_set_graphs(cfx_ctx* ctx, int enable);
+/* the drivers' and fvMatrix solves of vector equations with FP64 BiCGStab
+ * (fvMatrix.cpp:93-122 solves component after component): all components in
+ * shared launches (default 1) or one after another (0); both give identical
+ * results.  No counterpart in the reference. */
+int cfx_set_component_batching(cfx_ctx* ctx, int enable);
 int cfx_csr_set_mg(cfx_csr* A, const cfx_mg_params* p);
 /* per-row finest-level a, s, c, t (device FP32 [n] each, or null for the
  * constant); the pointers are kept and read whenever the hierarchy's values

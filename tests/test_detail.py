@@ -78,14 +78,18 @@ def test_detail_block_matches_oracle(ctx, shape):
 
 
 @pytest.mark.gpu
-def test_detail_blocks_concurrent_loss_at_equals_serial():
+@pytest.mark.parametrize("pressure", ["jacobi_fp32ir", "multigrid"])
+def test_detail_blocks_concurrent_loss_at_equals_serial(pressure):
     """The coupling pass over several blocks run by host threads on one
     stream per block (cfx_detail_loss_at_many) gives, block by block, the
-    pressure drop and fluxes of the blocks run one after another."""
+    pressure drop and fluxes of the blocks run one after another, with the
+    reference's Jacobi-PCG pressure solve and with the multigrid FCG one."""
     from chuanflow_amd import native as N
-    print("\n[case] 6 detail blocks, lossAt serial vs 4 threads (one context per block)", flush=True)
+    print(f"\n[case] 6 detail blocks, lossAt serial vs 4 threads (one context per block), {pressure}", flush=True)
+    prec = N.FP32_IR if pressure == "jacobi_fp32ir" else N.FP32_MG
     ctl = N.SimpleControls(maxIterations=5, continuityTol=0, pcorrTol=0, momentumTol=0, linearTol=1e-8,
-                           linearMaxIter=3000, nonOrthCorrectors=1, relaxU=0.9, relaxP=0.03, pRelaxStart=0.02)
+                           linearMaxIter=3000, nonOrthCorrectors=1, relaxU=0.9, relaxP=0.03, pRelaxStart=0.02,
+                           pPrecision=prec)
     specs = [(12 + 2 * k, 5, 5, 0.001 * (k + 1), "parabolic" if k % 2 else "flat") for k in range(6)]
 
     def make():

@@ -152,3 +152,38 @@ def test_explicit_operator_matches_reference_definition(case):
         h = M.explicit_operator(ctx.array(x), comp).get()
         ref = A.rhs[comp] - (O.spmv(O.csr_from_ldu(om, A, comp), x) - A.diag[comp] * x)
         assert np.array_equal(h, ref)
+
+
+def test_vector_bicgstab_components_equal_separate_solves(case):
+    """The FP64 BiCGStab of a vector equation with its components in shared
+    launches gives the fields and report of the reference's component-after-
+    component loop (fvMatrix.cpp:93-122) bit for bit, with and without the
+    non-orthogonal corrector passes, and both track the oracle's solves."""
+    c, om, dm, ctx, N = case, case["om"], case["dm"], case["ctx"], case["N"]
+    phi = c["rng"].standard_normal(om.nFaces) * 1e-3
+    gf = O.harmonic_gamma_uniform(om, 1e-3)
+    old = c["rng"].standard_normal((om.nCells, 3))
+    A = _oracle_momentum(om, c, phi, gf, old, 2e-3, 0)
+    x0 = c["rng"].standard_normal((3, om.nCells)) * 1e-2
+    out = {}
+    for batched in (True, False):
+        ctx.set_component_batching(batched)
+        M = N.Matrix(dm, 3)
+        M.ddt(ctx.array(soa(old)), 2e-3)
+        M.div(ctx.array(phi), c["d"]["Ulc"], c["d"]["Ulk"], scheme=0, blend=0.7)
+        M.laplacian(ctx.array(gf), c["d"]["Ulc"], c["d"]["Ulk"], -1)
+        # None: no corrector (plain solve); 0 and 2: corrector applied before each of 1 and 3 passes
+        for nonOrth in (None, 0, 2):
+            x = ctx.array(x0)
+            corr = {} if nonOrth is None else dict(gammaFace=ctx.array(gf), linc=c["d"]["Ulc"], link=c["d"]["Ulk"])
+            rep = M.solve(x, solver=N.SOLVER_BICGSTAB, tol=1e-12, maxIter=400, nonOrth=nonOrth or 0, **corr)
+            out[batched, nonOrth] = (x.get(), rep.iterations, rep.finalResidual, rep.converged)
+    ctx.set_component_batching(True)
+    for nonOrth in (None, 0, 2):
+        a, b = out[True, nonOrth], out[False, nonOrth]
+        assert np.array_equal(a[0], b[0]) and a[1:] == b[1:], nonOrth
+        assert a[3] == 1
+    # the oracle's BiCGStab per component on the same system
+    for comp in range(3):
+        xo, _ = O.solve_bicgstab(O.csr_from_ldu(om, A, comp), A.rhs[comp].copy(), x0[comp].copy(), 1e-12, 400)
+        assert np.abs(out[True, None][0][comp] - xo).max() <= 1e-9 * max(np.abs(xo).max(), 1e-300), comp
