@@ -867,6 +867,7 @@ int cfx_mg_default_params(cfx_mg_params* p)
     return CFX_OK;
 }
 
+extern "C++" {  // helpers of the entry points: C++ linkage
 namespace
 {
 void setMgParams(MgCache& mg, const cfx_mg_params* p)
@@ -900,6 +901,7 @@ void setMgWeights(MgCache& mg, const float* pre, const float* scale, const float
     mg.valid = false;
 }
 } // namespace
+} // extern "C++"
 
 int cfx_csr_set_mg(cfx_csr* A, const cfx_mg_params* p)
 {
@@ -1012,6 +1014,7 @@ int cfx_csr_set_inner_tol(cfx_csr* A, double t)
     });
 }
 
+extern "C++" {  // helpers of the entry points: C++ linkage
 namespace
 {
 double timeSpmv(cfx_ctx* c, const CsrView& A, const double* val, DevBuf& val32, bool& v32ok, const void* x, void* y,
@@ -1046,6 +1049,7 @@ double timeSpmv(cfx_ctx* c, const CsrView& A, const double* val, DevBuf& val32, 
     return ms / reps;
 }
 } // namespace
+} // extern "C++"
 
 int cfx_csr_time_spmv(cfx_csr* A, const void* x, void* y, int reps, int precision, double* ms)
 {
@@ -1083,6 +1087,7 @@ int cfx_linearise(cfx_mesh* m, int ncomp, const int32_t* type, const double* val
     });
 }
 
+extern "C++" {  // helpers of the entry points: C++ linkage
 namespace
 {
 Schemes toSchemes(const cfx_schemes* s)
@@ -1098,6 +1103,7 @@ Schemes toSchemes(const cfx_schemes* s)
     return o;
 }
 } // namespace
+} // extern "C++"
 
 int cfx_simple_solve(cfx_mesh* m, double nu, const cfx_schemes* schemes, const cfx_simple_controls* ctl,
                      const int32_t* Utype, const double* Uvalue, const int32_t* ptype, const double* pvalue,
@@ -1268,6 +1274,7 @@ int cfx_mpnn_adam(cfx_mpnn* m, const float* grad, float lr)
     return guard(m->owner, [&] { m->net->adam(grad, lr); });
 }
 
+extern "C++" {  // helpers of the entry points: C++ linkage
 namespace
 {
 TrainOptions toTrain(const cfx_train_options* o, const MgParams& mg)
@@ -1286,6 +1293,7 @@ TrainOptions toTrain(const cfx_train_options* o, const MgParams& mg)
     return t;
 }
 } // namespace
+} // extern "C++"
 
 int cfx_mpnn_train(cfx_mpnn* m, cfx_csr* const* systems, const double* const* geo, const double* const* r0, int count,
                    const cfx_train_options* opt, double* lossHist)
@@ -1366,6 +1374,7 @@ int cfx_matrix_set_mpnn(cfx_matrix* A, cfx_mpnn* m)
 }
 
 // ------------------------------------------------------- batched solves
+extern "C++" {  // helpers of the entry points: C++ linkage
 namespace
 {
 std::vector<BatchSystem> batchSystems(cfx_ctx* c, int count, cfx_csr* const* systems, const double* const* b,
@@ -1384,6 +1393,7 @@ std::vector<BatchSystem> batchSystems(cfx_ctx* c, int count, cfx_csr* const* sys
     return v;
 }
 } // namespace
+} // extern "C++"
 
 int cfx_batch_solve(cfx_ctx* c, int count, cfx_csr* const* systems, const double* const* b, double* const* x,
                     double tol, int maxIter, const cfx_batch_schedule* sched, cfx_report* reps, double* finishMs,
