@@ -178,11 +178,8 @@ def parse_vtu(path):
     return arrays
 
 
-def main():
-    if not O.have_ref():
-        sys.exit("reference build missing: run `make -C oracle ref` first")
-    write_prism_41(os.path.join(HERE, "meshes", "prismTagged_41.msh"))
-    write_foam_box(os.path.join(HERE, "jitterBox_foam"))
+def ref_lib():
+    """The reference build with the mesh-reader entry points' signatures set."""
     lib = O.ref()
     lib.cfr_read_mesh.restype = C.c_void_p
     lib.cfr_read_mesh.argtypes = [C.c_int, C.c_char_p, C.c_char_p]
@@ -191,6 +188,15 @@ def main():
     lib.cfr_mesh_arrays.argtypes = [C.c_void_p] + [C.c_void_p] * 7 + [C.c_char_p, C.c_int64]
     lib.cfr_write_vtu.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_void_p, C.c_char_p, C.c_void_p]
     lib.cfr_mesh_free.argtypes = [C.c_void_p]
+    return lib
+
+
+def main():
+    if not O.have_ref():
+        sys.exit("reference build missing: run `make -C oracle ref` first")
+    write_prism_41(os.path.join(HERE, "meshes", "prismTagged_41.msh"))
+    write_foam_box(os.path.join(HERE, "jitterBox_foam"))
+    lib = ref_lib()
     golden = {}
     for name, (kind, rel, spec) in CASES.items():
         h, out = ref_read(lib, kind, os.path.join(HERE, rel), spec)

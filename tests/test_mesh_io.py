@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from helpers import oracle_mesh_from_arrays
+from helpers import oracle_mesh_from_arrays, write_gmsh_tet_channel
 
 N = pytest.importorskip("chuanflow_amd.native")
 
@@ -157,3 +157,25 @@ def test_gmsh_refusals(tmp_path):
            "$EndNodes\n$Elements\n2\n1 4 0 1 2 3 4\n2 4 0 2 3 4 5\n$EndElements\n")
     with pytest.raises(RuntimeError, match="no physical group"):
         N.HostMesh.read_gmsh(_write(tmp_path, "t.msh", tet))
+
+
+@pytest.mark.skipif(not O.have_ref(), reason="reference build (oracle/_ref) not built")
+def test_generated_tet_channel_reads_like_the_reference_reader(tmp_path):
+    """A Gmsh 2.2 tet channel written by the tests (6,144 Kuhn tets, jittered,
+    physical surfaces inlet/outlet/walls): the library's reader gives the
+    reference reader's arrays and patches bit for bit, and closed cells of
+    the box's volume."""
+    import make_mesh_golden as MG
+    path = str(tmp_path / "channel.msh")
+    write_gmsh_tet_channel(path, 16, 8, 8, jitter=0.1, seed=3)
+    hm = N.HostMesh.read_gmsh(path, {"inlet": "inlet", "outlet": "outlet"})
+    lib = MG.ref_lib()
+    h, ref = MG.ref_read(lib, 1, path, "inlet=inlet;outlet=outlet")
+    lib.cfr_mesh_free(h)
+    a = hm.arrays()
+    for k in ("points", "faceStart", "faceVerts", "owner", "neighbour", "patchStart", "patchSize"):
+        assert np.array_equal(np.asarray(a[k]).ravel(), ref[k]), k
+    assert a["nCells"] == int(ref["nCells"]) == 16 * 8 * 8 * 6
+    assert "".join(f"{n}={t};" for n, t in hm.patches()) == str(ref["patches"])
+    g = hm.geometry()
+    assert np.all(np.asarray(g["V"]) > 0) and abs(np.sum(g["V"]) - 0.0625) < 1e-14
