@@ -259,6 +259,127 @@ def cavity_bcs(patch_start, patch_size, n_internal, n_boundary, lid=1.0):
     return (Ut, Uv), (np.full(n_boundary, 2, np.int32), np.zeros((n_boundary, 1)))
 
 
+def channel_bcs(patch_start, patch_size, n_internal, n_boundary, u_in):
+    """Box channel along x: inlet (1st patch) fixed U = (u_in, 0, 0), outlet (2nd) zeroGradient U and
+    fixed p = 0, the other patches no-slip walls with zeroGradient p; per boundary face and per patch
+    (the reference driver's patch-level form)."""
+    pid = np.empty(n_boundary, np.int64)
+    for p, (s, n) in enumerate(zip(patch_start, patch_size)):
+        pid[s - n_internal:s - n_internal + n] = p
+    Ut = np.where(pid == 1, 2, 0).astype(np.int32)
+    Uv = np.zeros((n_boundary, 3))
+    Uv[pid == 0, 0] = u_in
+    pt = np.where(pid == 1, 0, 2).astype(np.int32)
+    pv = np.zeros((n_boundary, 1))
+    npatch = len(patch_start)
+    Upt = np.array([0, 2] + [0] * (npatch - 2), np.int32)
+    Upv = np.zeros((npatch, 3))
+    Upv[0, 0] = u_in
+    ppt = np.array([2, 0] + [2] * (npatch - 2), np.int32)
+    return (Ut, Uv), (pt, pv), (Upt, Upv, ppt, np.zeros(npatch))
+
+
+def bench_c1(args):
+    """Steady laminar channel, 40x20x20 hex (16,000 cells, 1 x 0.5 x 0.5 m, BlockMesh face order), nu 1e-3,
+    inlet 0.01 m/s: 200 SIMPLE iterations from rest per step.  Relaxation U 0.9 / p 0.03 from a 0.02
+    floor: the literal 0.7 / 0.3 diverges, in the reference build as on the GPU, at the same iteration.
+    The fields of a 20-iteration prefix are checked against the oracle; the reference build's SIMPLE on
+    the same case, timed over a 10-iteration sample, is the CPU baseline."""
+    from chuanflow_amd import native as N
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O  # noqa: E402  (checker and CPU baseline only)
+    ctx = N.Context(0)
+    hm = N.HostMesh.box(40, 20, 20, (0, 0, 0), (1.0, 0.5, 0.5), 0.0, 0, False)
+    arrays, geom = hm.arrays(), hm.geometry()
+    dm = N.Mesh(ctx, arrays, geom)
+    nI = len(arrays["neighbour"])
+    Ubc, pbc, patch_bcs = channel_bcs(arrays["patchStart"], arrays["patchSize"], nI, dm.nB, 0.01)
+    om = O.Mesh(arrays["points"], arrays["faceStart"], arrays["faceVerts"], arrays["owner"], arrays["neighbour"],
+                arrays["patchStart"], arrays["patchSize"], arrays["nCells"]).compute_geometry()
+
+    def rest():
+        U, p, phi = np.zeros((om.nCells, 3)), np.zeros(om.nCells), np.zeros(om.nFaces)
+        lc, lk = O.linearise(om, 3, *Ubc)
+        O.correct_boundary_flux(om, U, lc, lk, phi)
+        return U, p, phi
+
+    kw = dict(continuityTol=0, pcorrTol=0, momentumTol=0, linearTol=1e-8, linearMaxIter=2000,
+              nonOrthCorrectors=2, relaxU=0.9, relaxP=0.03, pRelaxStart=0.02)
+    sch = N.Schemes(0, 1, 1, 0.9)
+    runs = {}
+    variants = [v for v in (("fp64", N.FP64), ("fp32ir", N.FP32_IR), ("multigrid", N.FP32_MG))
+                if v[0] in args.c1_variants.split(",")]
+    if not variants or variants[0][0] != "fp64":
+        raise SystemExit("--c1-variants must start with fp64 (the reference's pressure solve, the metric)")
+    for name, pp in variants:
+        ctl = N.SimpleControls(maxIterations=args.c1_iterations, **kw, pPrecision=pp)
+        times, devs = [], []
+        for step in range(args.warmup + args.steps):
+            U, p, phi = rest()
+            dU, dp, dphi = ctx.array(np.ascontiguousarray(U.T)), ctx.array(p), ctx.array(phi)
+            ctx.sync()
+            t0 = time.perf_counter()
+            r = dm.simple(1e-3, sch, ctl, Ubc, pbc, dU, dp, dphi)
+            ctx.sync()
+            if step >= args.warmup:
+                times.append((time.perf_counter() - t0) * 1e3)
+                devs.append(r.ms)
+        runs[name] = dict(ms=float(np.mean(times)), device_ms=float(np.mean(devs)), iterations=r.iterations,
+                          final_continuity=r.finalContinuityError, diverged=bool(r.diverged))
+        log(f"C1 {args.c1_iterations} SIMPLE iterations, pressure {name}: {runs[name]['ms']:.1f} ms "
+            f"(device {runs[name]['device_ms']:.1f}), continuity {r.finalContinuityError:.3e}")
+    # parity: a 20-iteration prefix, GPU (reference pressure solve) against the oracle
+    nchk = 20
+    U, p, phi = rest()
+    dU, dp, dphi = ctx.array(np.ascontiguousarray(U.T)), ctx.array(p), ctx.array(phi)
+    dm.simple(1e-3, sch, N.SimpleControls(maxIterations=nchk, **kw), Ubc, pbc, dU, dp, dphi)
+    O.simple(om, U, p, phi, 1e-3, O.Schemes(0, 1, 1, 0.9), O.SimpleControls(maxIterations=nchk, **kw), Ubc, pbc)
+    rel = lambda a, b: float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+    parity = {"iterations": nchk, "U": rel(dU.get().T, U), "p": rel(dp.get(), p), "phi": rel(dphi.get(), phi)}
+    log(f"parity after {nchk} iterations: {parity}")
+    # CPU baseline: the reference build's SIMPLE (one core), a sample of iterations, scaled to the full run
+    sample = args.c1_cpu_sample
+    U, p, phi = rest()
+    ctl_vec = [sample, 0, 0, 0, 1e-8, 2000, 2, 0.9, 0.03, 0.02, 0, 0, 0]
+    if O.have_ref():
+        Upt, Upv, ppt, ppv = patch_bcs
+        rm = O.RefMesh(om)
+        t0 = time.perf_counter()
+        rm.simple(Upt, Upv, ppt, ppv, 1e-3, (0, 1, 1), ctl_vec, U, p, phi)
+        cpu_s = time.perf_counter() - t0
+        kind = "reference"
+    else:
+        t0 = time.perf_counter()
+        O.simple(om, U, p, phi, 1e-3, O.Schemes(0, 1, 1, 0.9), O.SimpleControls(maxIterations=sample, **kw),
+                 Ubc, pbc)
+        cpu_s = time.perf_counter() - t0
+        kind = "port"
+    cpu_ms = cpu_s * 1e3 * args.c1_iterations / sample
+    model, _ = cpu_info()
+    log(f"CPU ({kind}, 1 core, {model}): {sample} iterations in {cpu_s:.2f} s -> {cpu_ms / 1e3:.1f} s for "
+        f"{args.c1_iterations}")
+    g = runs["fp64"]
+    result = {
+        "metric": f"C1 steady channel: {args.c1_iterations} SIMPLE iterations, wall time",
+        "value": round(g["ms"], 2), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(g["ms"], 2), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "steady laminar channel 40x20x20 hex (16,000 cells, 1 x 0.5 x 0.5 m, BlockMesh "
+                               "face order), nu 1e-3, inlet 0.01 m/s, SIMPLE from rest, relaxation U 0.9 / p "
+                               "0.03 (floor 0.02), Jacobi-PCG pressure to 1e-8, 2 non-orthogonal correctors",
+                   "cells": dm.nCells, "iterations": args.c1_iterations, "parallelism": "single GPU"},
+        "device_ms": round(g["device_ms"], 2), "ms_per_iteration": round(g["ms"] / args.c1_iterations, 3),
+        "final_continuity": g["final_continuity"],
+        "pressure_fp32ir_ms": round(runs["fp32ir"]["ms"], 2) if "fp32ir" in runs else None,
+        "pressure_multigrid_ms": round(runs["multigrid"]["ms"], 2) if "multigrid" in runs else None,
+        "parity_vs_oracle": parity,
+        "cpu_baseline": {"value": round(cpu_ms, 1), "unit": "ms", "cores": 1, "kind": kind,
+                         "sample": f"{sample} SIMPLE iterations of the same case from rest, scaled to "
+                                   f"{args.c1_iterations}", "cpu": model},
+    }
+    print(json.dumps(result), flush=True)
+
+
 def kinetic_energy(V, U3):
     return 0.5 * float((V * (U3 ** 2).sum(axis=0)).sum())
 
@@ -385,9 +506,14 @@ def main():
     ap.add_argument("--mg-scale", type=float, default=1.8)
     ap.add_argument("--kcycle-depth", type=int, default=2)
     ap.add_argument("--mg-fusion", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c5", "spmv"],
-                    help="c2: the 128^3 pressure system (the headline metric); c3: PISO cavity on tets; "
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c5", "spmv"],
+                    help="c2: the 128^3 pressure system (the headline metric); c1: SIMPLE channel 40x20x20; "
+                         "c3: PISO cavity on tets; "
                          "c5: batched detail regions; spmv: SpMV and a copy of known size (traffic counters)")
+    ap.add_argument("--c1-iterations", type=int, default=200)
+    ap.add_argument("--c1-cpu-sample", type=int, default=10)
+    ap.add_argument("--c1-variants", default="fp64,fp32ir,multigrid",
+                    help="pressure solves timed: fp64 (reference Jacobi-PCG, the metric), fp32ir, multigrid")
     ap.add_argument("--c3-n", type=int, default=100)
     ap.add_argument("--c3-steps", type=int, default=500)
     ap.add_argument("--c3-pressure", default="mg", choices=["fp64", "fp32ir", "mg"])
@@ -404,6 +530,8 @@ def main():
               "not measured, in this build", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "c1":
+        return bench_c1(args)
     if args.workload == "c5":
         return bench_c5(args)
     if args.workload == "c3":
