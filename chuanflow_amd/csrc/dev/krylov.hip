@@ -367,7 +367,7 @@ __global__ __launch_bounds__(kVecBlock) void k_bicg_p(int n, long long vs, const
 // y = A x, then a dot with `w` (r0.v) or two dots (t.t, t.s) in the epilogue.
 // which == 0: v = A z, denom = r0.v, alpha = rho/denom   (:273-277)
 // which == 1: t = A sh, tt = t.t, ts = t.s, omega = ts/tt (:290-300)
-// (per-workgroup partials over rbPerWg row blocks; k_bicg_dot_final reduces them)
+// (per-workgroup partials over rbPerWg row blocks; the next vector kernel reduces them)
 template <typename T>
 __global__ __launch_bounds__(kSpmvBlock) void k_bicg_spmv(CsrView A, int rbPerWg, long long ms, long long vs, int ps,
                                                           const T* __restrict__ val, const T* __restrict__ x,
@@ -406,43 +406,43 @@ __global__ __launch_bounds__(kSpmvBlock) void k_bicg_spmv(CsrView A, int rbPerWg
     if (threadIdx.x == 0) { part[wg] = d0; part[G + wg] = d1; }
 }
 
-__global__ __launch_bounds__(kSpmvBlock) void k_bicg_dot_final(int G, int ps, const double* part, SolveState* st,
-                                                               int which)
+// the sum of partials[0..n) (sumPartials' fixed order) in every thread
+__device__ __forceinline__ double sumPartialsAll(const double* partials, int n, double* lds, double* slot)
 {
-    __shared__ double red[kSpmvBlock / kWave];
-    st += blockIdx.y;
-    if (st->done) return;
-    part += blockIdx.y * ps;
-    const double D0 = sumPartials(part, G, red);
-    const double D1 = sumPartials(part + G, G, red);
-    if (threadIdx.x == 0)
-    {
-        if (which == 0)
-        {
-            st->denom = D0;
-            if (fabs(D0) < 1e-300) st->done = 1;
-            else st->alpha = st->rho / D0;
-        }
-        else
-        {
-            st->tt = D0;
-            st->ts = D1;
-            if (D0 < 1e-300) st->breakdown = 2;  // r = s and stop (:292-299)
-            else st->omega = D1 / D0;
-        }
-    }
+    const double v = sumPartials(partials, n, lds);
+    if (threadIdx.x == 0) *slot = v;
+    __syncthreads();
+    return *slot;
 }
 
-// x += alpha z; s = r - alpha v; sh = s invD   (:278-289)
+// partials of k_bicg_r live past those of the SpMVs, which k_bicg_r reads
+constexpr int kRPartOff = 2 * kVecGridMax;
+
+// alpha = rho / r0.v from the first SpMV's Gs partials, reduced by every
+// workgroup in the same fixed order (workgroup 0 records it; r0.v ~ 0 ends
+// the solve, :273-277); x += alpha z; s = r - alpha v; sh = s invD (:278-289)
 template <typename T>
-__global__ __launch_bounds__(kVecBlock) void k_bicg_s(int n, long long vs, const T* __restrict__ invD,
-                                                      T* __restrict__ x, const T* __restrict__ z,
-                                                      const T* __restrict__ r, const T* __restrict__ v,
-                                                      T* __restrict__ s, T* __restrict__ sh, const SolveState* st)
+__global__ __launch_bounds__(kVecBlock) void k_bicg_s(int n, long long vs, int ps, int Gs,
+                                                      const T* __restrict__ invD, T* __restrict__ x,
+                                                      const T* __restrict__ z, const T* __restrict__ r,
+                                                      const T* __restrict__ v, T* __restrict__ s,
+                                                      T* __restrict__ sh, SolveState* st, const double* part)
 {
+    __shared__ double red[kVecBlock / kWave];
+    __shared__ double slot;
     const long long o = blockIdx.y * vs;
     st += blockIdx.y;
     if (st->done) return;
+    const double D0 = sumPartialsAll(part + blockIdx.y * ps, Gs, red, &slot);
+    const bool brk = fabs(D0) < 1e-300;
+    const double alphaD = brk ? 0.0 : st->rho / D0;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        st->denom = D0;
+        if (brk) st->done = 1;
+        else st->alpha = alphaD;
+    }
+    if (brk) return;
     invD += o;
     x += o;
     z += o;
@@ -450,7 +450,7 @@ __global__ __launch_bounds__(kVecBlock) void k_bicg_s(int n, long long vs, const
     v += o;
     s += o;
     sh += o;
-    const T alpha = static_cast<T>(st->alpha);
+    const T alpha = static_cast<T>(alphaD);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
         x[i] += alpha * z[i];
@@ -460,16 +460,20 @@ __global__ __launch_bounds__(kVecBlock) void k_bicg_s(int n, long long vs, const
     }
 }
 
-// x += omega sh; r = s - omega t; ||r||inf; next rho = r0.r   (:301-314, :263-266)
+// omega = t.s / t.t from the second SpMV's partials, reduced by every
+// workgroup (workgroup 0 records it; t.t ~ 0 is the breakdown that takes
+// r = s and stops, :290-300); x += omega sh; r = s - omega t; ||r||inf;
+// next rho = r0.r (:301-314, :263-266), partials at kRPartOff
 template <typename T>
-__global__ __launch_bounds__(kVecBlock) void k_bicg_r(int n, long long vs, int ps, int ips, T* __restrict__ x,
+__global__ __launch_bounds__(kVecBlock) void k_bicg_r(int n, long long vs, int ps, int ips, int Gs, T* __restrict__ x,
                                                       const T* __restrict__ sh, const T* __restrict__ s,
                                                       const T* __restrict__ t, const T* __restrict__ r0,
-                                                      T* __restrict__ r, const SolveState* st, double* part,
+                                                      T* __restrict__ r, SolveState* st, double* part,
                                                       int* ipart)
 {
     __shared__ double red[kVecBlock / kWave];
     __shared__ int ired[kVecBlock / kWave];
+    __shared__ double slot[2];
     const long long o = blockIdx.y * vs;
     st += blockIdx.y;
     if (st->done) return;
@@ -481,8 +485,19 @@ __global__ __launch_bounds__(kVecBlock) void k_bicg_r(int n, long long vs, int p
     r += o;
     part += blockIdx.y * ps;
     ipart += blockIdx.y * ips;
-    const bool tsmall = st->breakdown == 2;
-    const T omega = static_cast<T>(st->omega);
+    const double TT = sumPartialsAll(part, Gs, red, &slot[0]);
+    const double TS = sumPartialsAll(part + Gs, Gs, red, &slot[1]);
+    const bool tsmall = TT < 1e-300;
+    const double omegaD = tsmall ? 1.0 : TS / TT;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        st->tt = TT;
+        st->ts = TS;
+        if (tsmall) st->breakdown = 2;
+        else st->omega = omegaD;
+    }
+    part += kRPartOff;
+    const T omega = static_cast<T>(omegaD);
     double rm = 0, rr0 = 0;
     int nan = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -517,7 +532,7 @@ __global__ __launch_bounds__(kVecBlock) void k_bicg_r_final(int G, int ps, int i
     __shared__ int ired[kVecBlock / kWave];
     st += blockIdx.y;
     if (st->done) return;
-    part += blockIdx.y * ps;
+    part += blockIdx.y * ps + kRPartOff;
     ipart += blockIdx.y * ips;
     const bool tsmall = st->breakdown == 2;
     const double RM = maxPartials(part, G, red);
@@ -603,11 +618,9 @@ void launchBicgIteration(hipStream_t s, const CsrView& A, int Gv, int Gs, int rb
     const dim3 gv(Gv, nc), gs(Gs, nc), g1(1, nc);
     k_bicg_p<T><<<gv, kVecBlock, 0, s>>>(n, vs, invD, r, p, v, z, st);
     k_bicg_spmv<T><<<gs, kSpmvBlock, 0, s>>>(A, rbPerWg, ms, vs, ps, val, z, v, r0, st, part, 0);
-    k_bicg_dot_final<<<g1, kSpmvBlock, 0, s>>>(Gs, ps, part, st, 0);
-    k_bicg_s<T><<<gv, kVecBlock, 0, s>>>(n, vs, invD, x, z, r, v, sv, sh, st);
+    k_bicg_s<T><<<gv, kVecBlock, 0, s>>>(n, vs, ps, Gs, invD, x, z, r, v, sv, sh, st, part);
     k_bicg_spmv<T><<<gs, kSpmvBlock, 0, s>>>(A, rbPerWg, ms, vs, ps, val, sh, t, sv, st, part, 1);
-    k_bicg_dot_final<<<g1, kSpmvBlock, 0, s>>>(Gs, ps, part, st, 1);
-    k_bicg_r<T><<<gv, kVecBlock, 0, s>>>(n, vs, ps, ips, x, sh, sv, t, r0, r, st, part, ipart);
+    k_bicg_r<T><<<gv, kVecBlock, 0, s>>>(n, vs, ps, ips, Gs, x, sh, sv, t, r0, r, st, part, ipart);
     k_bicg_r_final<<<g1, kVecBlock, 0, s>>>(Gv, ps, ips, part, ipart, st);
 }
 
