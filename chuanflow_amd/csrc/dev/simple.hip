@@ -313,6 +313,23 @@ struct Work
     Ldu P() const { return {1, pdiag.as<double>(), prhs.as<double>(), pup.as<double>(), plo.as<double>()}; }
 };
 
+// The mesh's driver work buffers: allocated by the first driver call on the
+// mesh and kept (a time step of a 6M-cell mesh otherwise allocates and frees
+// ~3 GB, and every free waits for the device); the state slots start at zero
+// in every call, as a fresh allocation did.
+static Work& driverWork(DevMesh& m)
+{
+    if (!m.driverWork)
+    {
+        auto w = std::make_shared<Work>();
+        w->alloc(m);
+        m.driverWork = w;
+    }
+    Work& w = *static_cast<Work*>(m.driverWork.get());
+    CFX_HIP(hipMemsetAsync(w.sc.p, 0, sizeof(double) * kNumSlots, m.ctx->stream()));
+    return w;
+}
+
 // BCs: U, p from the caller; HbyA = fixedValue (U's face values) where U is
 // fixedValue, zeroGradient elsewhere (simpleSolver.cpp:175-187)
 void setupBcs(DevMesh& m, Work& w, const int* Utype, const double* Uvalue, const int* ptype, const double* pvalue,
@@ -493,8 +510,7 @@ SimpleResult simpleSolve(DevMesh& m, Krylov& K, double nu, const Schemes& sch, c
     Context& ctx = *m.ctx;
     hipStream_t s = ctx.stream();
     const int n = m.n;
-    Work w;
-    w.alloc(m);
+    Work& w = driverWork(m);
     bool needReference = true;
     setupBcs(m, w, Utype, Uvalue, ptype, pvalue, needReference);
     double* sc = w.sc.as<double>();
@@ -623,8 +639,7 @@ PisoResult pisoStep(DevMesh& m, Krylov& K, double nu, const Schemes& sch, const 
     Context& ctx = *m.ctx;
     hipStream_t s = ctx.stream();
     const int n = m.n;
-    Work w;
-    w.alloc(m);
+    Work& w = driverWork(m);
     bool needReference = true;
     setupBcs(m, w, Utype, Uvalue, ptype, pvalue, needReference);
     double* sc = w.sc.as<double>();

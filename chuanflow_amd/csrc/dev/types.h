@@ -131,6 +131,8 @@ struct DevMesh
     DevPattern pattern;
     DevBuf slotU, slotL;                   // int[nI]
     std::vector<double> hV;                // host copy of volumes
+    // the SIMPLE/PISO drivers' work buffers, kept between calls (allocated by the first)
+    std::shared_ptr<void> driverWork;
 };
 
 // Context: one HIP stream, one device, scratch for reductions.
