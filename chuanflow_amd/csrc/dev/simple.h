@@ -40,6 +40,10 @@ struct FvSolveArgs
     const double* geo = nullptr;
     float *wa = nullptr, *wt = nullptr;
     bool projectMean = false;  // subtract the mean of every pass' rhs (singular Neumann systems)
+    // the matrix values equal those of the previous solve on this mesh (a later PISO corrector): the
+    // multigrid hierarchy's values are kept instead of set up again (without a network only, whose
+    // weights depend on the right-hand side)
+    bool sameMatrix = false;
 };
 SolveReport fvSolve(DevMesh& m, Krylov& K, const Ldu& A, double* val, double* rhsBase, DevBuf& val32,
                     double* x, const FvSolveArgs& a, double* gradTmp, double* faceTmp);
@@ -123,8 +127,6 @@ struct PisoResult
     int converged = 0;
     double ms = 0;
 };
-
-class PisoWorkspace;  // buffers reused across time steps
 
 // solveTransientIncompressiblePiso (pisoSolver.cpp:13-235), laminar, one step.
 PisoResult pisoStep(DevMesh& m, Krylov& K, double nu, const Schemes& s, const PisoControls& c, const int* Utype,

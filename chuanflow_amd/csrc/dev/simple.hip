@@ -484,7 +484,9 @@ SolveReport fvSolve(DevMesh& m, Krylov& K, const Ldu& A, double* val, double* rh
                     a.mg->pre = a.mg->post = a.wa;
                     a.mg->scale = a.mg->restr = a.wt;
                 }
-                a.mg->valid = false;  // new values every call
+                // new values: set the hierarchy up again, except for the later passes of this solve and a
+                // repeated matrix when no network makes the weights depend on the right-hand side
+                if (a.net || (pass == 0 && !a.sameMatrix)) a.mg->valid = false;
                 Multigrid& M = a.mg->ready(*m.ctx, m.pattern, v);
                 last = K.fcgMixed(view, v, M, b, xk, a.tol, a.maxIter, a.delta, nullptr);
             }
@@ -714,6 +716,7 @@ PisoResult pisoStep(DevMesh& m, Krylov& K, double nu, const Schemes& sch, const 
         pa.link = w.plk.as<double>();
         pa.gradCorrected = sch.gradCorrected;
         pa.projectMean = c.projectNeumann && needReference;
+        pa.sameMatrix = corr > 1;  // rAUf, and so the pressure matrix, is the same for every corrector
         setPressureSolver(pa, c.pPrecision, c.pSolver);
         fvSolve(m, K, PE, w.pval.as<double>(), w.pbase.as<double>(), w.val32, p, pa, w.gradTmp.as<double>(),
                 w.faceTmp.as<double>());
