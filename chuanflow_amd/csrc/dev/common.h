@@ -158,4 +158,16 @@ __device__ __forceinline__ int orPartials(const int* partials, int n, int* lds)
 
 inline int divUp(long long a, long long b) { return static_cast<int>((a + b - 1) / b); }
 
+// Consecutive workgroup ids are dealt round-robin to the 8 XCDs of the
+// device; workgroup h of G works on chunk xcdRemap(h, G), so each XCD walks
+// one contiguous range of chunks and its own L2 holds the rows that range
+// gathers (neighbours of RCM-ordered rows are close in index).
+__device__ __forceinline__ int xcdRemap(int h, int G)
+{
+    constexpr int X = 8;
+    const int q = G / X, r = G % X;
+    const int x = h % X, idx = h / X;
+    return (x < r) ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
+}
+
 } // namespace cfx

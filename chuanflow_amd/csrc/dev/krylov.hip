@@ -68,16 +68,6 @@ namespace
 
 constexpr int kVecGridMax = 2048;
 
-// Consecutive workgroup ids are dealt round-robin to the 8 XCDs; remap so
-// each XCD walks one contiguous chunk of rows and its L2 holds that chunk's
-// gathered x entries.
-__device__ __forceinline__ int xcdRemap(int h, int G)
-{
-    constexpr int X = 8;
-    const int q = G / X, r = G % X;
-    const int x = h % X, idx = h / X;
-    return (x < r) ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
-}
 
 // Row sum of the CSR-stream block `blk` for the row this thread owns.
 // Products are formed once (coalesced val/col reads, gathered x) and staged

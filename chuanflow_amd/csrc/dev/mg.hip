@@ -19,14 +19,6 @@ constexpr int kCoarseMax = 2048;
 constexpr int kTailThreads = 1024;
 constexpr int kFuseRowsMax = 65536;  // largest level whose prolongation is fused into post-smoothing
 
-__device__ __forceinline__ int xcdRemap(int h, int G)
-{
-    constexpr int X = 8;
-    const int q = G / X, r = G % X;
-    const int x = h % X, idx = h / X;
-    return (x < r) ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
-}
-
 // Greedy aggregation of a symmetric pattern: (1) a row whose whole
 // neighbourhood is free becomes a root and takes it; (2) free rows join the
 // aggregate of their first aggregated neighbour (phase-1 aggregates only);

@@ -44,14 +44,6 @@ struct CgState
     int kcur, pad;       // flexible CG: iteration index of the running step (set by B, read by C)
 };
 
-__device__ __forceinline__ int xcdRemap(int h, int G)
-{
-    constexpr int X = 8;
-    const int q = G / X, r = G % X;
-    const int x = h % X, idx = h / X;
-    return (x < r) ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
-}
-
 // Deterministic all-workgroup reduction of G <= 2048 partials (blockDim 256):
 // 8 independent loads per thread, a fixed tree, broadcast through LDS.
 __device__ __forceinline__ double allSum(const double* __restrict__ p, int G, double* lds)
