@@ -65,6 +65,13 @@ public:
                             int ncomp, double tol, int maxIter, SolveReport* reps);
     static constexpr int kMaxComponents = 3;
 
+    // bicgstabMixed for the ncomp components in shared launches: per
+    // component the same true residuals, FP32 inner solves, corrections and
+    // FP64 fallback as bicgstabMixed on its system alone, bit for bit.
+    void bicgstabMixedComponents(const CsrView& A, const double* val, long long valStride, const double* b,
+                                 double* x, int ncomp, double tol, int maxIter, SolveReport* reps,
+                                 double innerTol = 1e-4);
+
     // Jacobi-BiCGStab in FP32 with FP64 iterative refinement: r = b - A x in
     // FP64, the FP32 solve of A d = r from d = 0 to `innerTol` relative (or to
     // the overall tolerance when that is closer), x += d; the reference's stop
@@ -112,6 +119,11 @@ private:
     SolveReport bicgRun(const CsrView& A, const T* val, const T* invD, const T* b, T* x, double tol, int maxIter,
                         T* const* work);
     template <typename T>
+    void bicgRunComponents(const CsrView& A, const T* val, long long valStride, const T* invD, const T* b, T* x,
+                           int ncomp, const double* tol, const int* maxIter, const bool* run, T* const* w,
+                           SolveReport* reps);
+    void reserveComponents();
+    template <typename T>
     SolveReport pcgRun(const CsrView& A, const double* val64, const T* valT, const T* invD,
                        const double* b, double* x64, T* xT, T* r, double tol, int maxIter,
                        double* resHist, bool mixed, double delta);
@@ -125,7 +137,8 @@ private:
     DevBuf r32_, p32_, q32_, d32_, b32_, invD32_;
     DevBuf z32_, fcgPart_;  // allocated by the first fcgMixed
     DevBuf bf_[6], val32b_;  // FP32 BiCGStab vectors and matrix, allocated by the first bicgstabMixed
-    DevBuf wc_, stC_, partC_;  // component BiCGStab vectors, states, partials (first bicgstabComponents)
+    DevBuf wc_, stC_, partC_;  // component BiCGStab vectors, states, partials (first component solve)
+    DevBuf wc32_, r64c_, val32c_;  // FP32 vectors, FP64 residuals and FP32 values of the mixed component solve
     SolveState* hostC_ = nullptr;  // pinned
     // captured flexible-CG iteration, valid for the key (multigrid, buffers) and the stop parameters
     hipGraphExec_t fcgExec_ = nullptr;

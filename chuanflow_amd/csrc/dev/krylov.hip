@@ -765,29 +765,31 @@ SolveReport Krylov::bicgstab(const CsrView& A, const double* val, const double* 
     return rep;
 }
 
-void Krylov::bicgstabComponents(const CsrView& A, const double* val, long long valStride, const double* b, double* x,
-                                int ncomp, double tol, int maxIter, SolveReport* reps)
+void Krylov::reserveComponents()
 {
-    if (ncomp < 1 || ncomp > kMaxComponents)
-        throw std::invalid_argument("bicgstabComponents: 1 to 3 components");
-    if (A.n != n_) throw std::invalid_argument("bicgstabComponents: pattern size differs from the solver's");
+    if (wc_.p) return;
+    const std::size_t nn = static_cast<std::size_t>(n_);
+    const int ps = 4 * maxBlocks_, ips = maxBlocks_;
+    wc_.alloc(sizeof(double) * 9 * kMaxComponents * nn);
+    // states: the component solves' [0, 3), the mixed solve's refinement states [3, 6)
+    stC_.alloc(sizeof(SolveState) * 2 * kMaxComponents);
+    partC_.alloc((sizeof(double) * ps + sizeof(int) * ips) * kMaxComponents);
+    CFX_HIP(hipHostMalloc(reinterpret_cast<void**>(&hostC_), sizeof(SolveState) * 2 * kMaxComponents,
+                          hipHostMallocDefault));
+}
+
+// BiCGStab (bicgRun's arithmetic) of the components c < ncomp with run[c]:
+// values at val + c valStride, invD, b, x and the work vectors w[j] as [c][n];
+// tolerance and iteration budget per component.  The others are left as they
+// are (their state starts done, so their workgroups return at once).
+template <typename T>
+void Krylov::bicgRunComponents(const CsrView& A, const T* val, long long valStride, const T* invD, const T* b, T* x,
+                               int ncomp, const double* tol, const int* maxIter, const bool* run, T* const* w,
+                               SolveReport* reps)
+{
     hipStream_t s = ctx_.stream();
-    CFX_HIP(hipEventRecord(ev0_, s));
     const long long nn = n_;
     const int ps = 4 * maxBlocks_, ips = maxBlocks_;
-    if (!wc_.p)
-    {
-        wc_.alloc(sizeof(double) * 9 * kMaxComponents * static_cast<std::size_t>(nn));
-        stC_.alloc(sizeof(SolveState) * kMaxComponents);
-        partC_.alloc((sizeof(double) * ps + sizeof(int) * ips) * kMaxComponents);
-        CFX_HIP(hipHostMalloc(reinterpret_cast<void**>(&hostC_), sizeof(SolveState) * kMaxComponents,
-                              hipHostMallocDefault));
-    }
-    // work vector j of component c at wc + (j * ncomp + c) n: each vector kind is [c][n] with stride n
-    double* wc = wc_.as<double>();
-    double* w[8];
-    for (int j = 0; j < 8; ++j) w[j] = wc + j * ncomp * nn;
-    double* invD = wc + 8 * ncomp * nn;
     SolveState* st = stC_.as<SolveState>();
     double* part = partC_.as<double>();
     int* ipart = reinterpret_cast<int*>(part + static_cast<std::size_t>(ps) * kMaxComponents);
@@ -795,20 +797,23 @@ void Krylov::bicgstabComponents(const CsrView& A, const double* val, long long v
     const int rbPerWg = divUp(A.nRowBlocks, Gs);
     Gs = std::max(1, divUp(A.nRowBlocks, rbPerWg));
     const int Gv = vecGridFor(n_);
-    SolveState h{};
-    h.tol = tol;
-    h.maxIter = maxIter;
+    int budget = 0;
     for (int c = 0; c < ncomp; ++c)
     {
+        SolveState h{};
+        h.tol = tol[c];
+        h.maxIter = maxIter[c];
+        h.done = run[c] ? 0 : 1;
+        CFX_HIP(hipMemcpyAsync(st + c, &h, offsetof(SolveState, ticket), hipMemcpyHostToDevice, s));
+        if (!run[c]) continue;
+        budget = std::max(budget, maxIter[c]);
         // each component starts as bicgRun starts a single solve (:244-266)
         const long long o = c * nn;
         double* pc = part + static_cast<std::size_t>(c) * ps;
         int* ic = ipart + static_cast<std::size_t>(c) * ips;
-        computeInvD(ctx_, A, val + c * valStride, invD + o, true);
-        CFX_HIP(hipMemcpyAsync(st + c, &h, offsetof(SolveState, ticket), hipMemcpyHostToDevice, s));
-        k_residual<double><<<Gs, kSpmvBlock, 0, s>>>(A, rbPerWg, val + c * valStride, x + o, b + o, w[0] + o, pc, ic);
+        k_residual<T><<<Gs, kSpmvBlock, 0, s>>>(A, rbPerWg, val + c * valStride, x + o, b + o, w[0] + o, pc, ic);
         k_residual_final<<<1, kSpmvBlock, 0, s>>>(Gs, pc, ic, st + c, 0);
-        k_bicg_init<double><<<Gv, kVecBlock, 0, s>>>(n_, w[0] + o, w[1] + o, w[2] + o, w[3] + o, st + c, pc);
+        k_bicg_init<T><<<Gv, kVecBlock, 0, s>>>(n_, w[0] + o, w[1] + o, w[2] + o, w[3] + o, st + c, pc);
         k_bicg_init_final<<<1, kVecBlock, 0, s>>>(Gv, pc, st + c);
     }
     CFX_CHECK_LAUNCH();
@@ -830,15 +835,207 @@ void Krylov::bicgstabComponents(const CsrView& A, const double* val, long long v
         int batch = 0;
         for (int c = 0; c < ncomp; ++c)
             if (!hostC_[c].done)
-                batch = std::max(batch, nextBatch(hostC_[c].iter, rInit[c], hostC_[c].rnorm, tol * bnorm[c], maxIter));
-        batch = std::min(batch, maxIter - launched);
+                batch = std::max(batch, nextBatch(hostC_[c].iter, rInit[c], hostC_[c].rnorm, tol[c] * bnorm[c],
+                                                  maxIter[c]));
+        batch = std::min(batch, budget - launched);
         if (batch <= 0) break;
         for (int k = 0; k < batch; ++k)
-            launchBicgIteration<double>(s, A, Gv, Gs, rbPerWg, ncomp, nn, valStride, ps, ips, val, invD, x, w, st,
-                                        part, ipart);
+            launchBicgIteration<T>(s, A, Gv, Gs, rbPerWg, ncomp, nn, valStride, ps, ips, val, invD, x, w, st, part,
+                                   ipart);
         CFX_CHECK_LAUNCH();
         launched += batch;
         pollC();
+    }
+    for (int c = 0; c < ncomp; ++c)
+    {
+        if (!run[c]) continue;
+        const SolveState& hs = hostC_[c];
+        SolveReport& rep = reps[c];
+        rep = SolveReport{};
+        rep.iterations = hs.iter;
+        rep.initialResidual = hs.initialResidual;
+        rep.finalResidual = hs.iter > 0 ? hs.rnorm : hs.initialResidual;
+        rep.converged = hs.converged;
+    }
+}
+
+void Krylov::bicgstabComponents(const CsrView& A, const double* val, long long valStride, const double* b, double* x,
+                                int ncomp, double tol, int maxIter, SolveReport* reps)
+{
+    if (ncomp < 1 || ncomp > kMaxComponents)
+        throw std::invalid_argument("bicgstabComponents: 1 to 3 components");
+    if (A.n != n_) throw std::invalid_argument("bicgstabComponents: pattern size differs from the solver's");
+    hipStream_t s = ctx_.stream();
+    CFX_HIP(hipEventRecord(ev0_, s));
+    reserveComponents();
+    const long long nn = n_;
+    // work vector j of component c at wc + (j * ncomp + c) n: each vector kind is [c][n] with stride n
+    double* wc = wc_.as<double>();
+    double* w[8];
+    for (int j = 0; j < 8; ++j) w[j] = wc + j * ncomp * nn;
+    double* invD = wc + 8 * ncomp * nn;
+    double tols[kMaxComponents];
+    int its[kMaxComponents];
+    bool run[kMaxComponents];
+    for (int c = 0; c < ncomp; ++c)
+    {
+        computeInvD(ctx_, A, val + c * valStride, invD + c * nn, true);
+        tols[c] = tol;
+        its[c] = maxIter;
+        run[c] = true;
+    }
+    bicgRunComponents<double>(A, val, valStride, invD, b, x, ncomp, tols, its, run, w, reps);
+    CFX_HIP(hipEventRecord(ev1_, s));
+    CFX_HIP(hipEventSynchronize(ev1_));
+    float ms = 0;
+    CFX_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+    lastMs_ = ms;
+    for (int c = 0; c < ncomp; ++c) reps[c].ms = ms;
+}
+
+void Krylov::bicgstabMixedComponents(const CsrView& A, const double* val, long long valStride, const double* b,
+                                     double* x, int ncomp, double tol, int maxIter, SolveReport* reps,
+                                     double innerTol)
+{
+    if (ncomp < 1 || ncomp > kMaxComponents)
+        throw std::invalid_argument("bicgstabMixedComponents: 1 to 3 components");
+    if (A.n != n_) throw std::invalid_argument("bicgstabMixedComponents: pattern size differs from the solver's");
+    hipStream_t s = ctx_.stream();
+    CFX_HIP(hipEventRecord(ev0_, s));
+    reserveComponents();
+    const long long nn = n_;
+    const std::size_t nnz = static_cast<std::size_t>(A.nnz);
+    if (!wc32_.p)
+    {
+        wc32_.alloc(sizeof(float) * 11 * kMaxComponents * static_cast<std::size_t>(nn));
+        r64c_.alloc(sizeof(double) * kMaxComponents * static_cast<std::size_t>(nn));
+    }
+    if (!val32c_.p || val32c_.bytes < sizeof(float) * kMaxComponents * nnz)
+        val32c_.alloc(sizeof(float) * kMaxComponents * nnz);
+    // FP32: 8 Krylov vectors, invD, the correction d and its right-hand side, each [c][n]
+    float* wf = wc32_.as<float>();
+    float* w32[8];
+    for (int j = 0; j < 8; ++j) w32[j] = wf + j * ncomp * nn;
+    float *invD32 = wf + 8 * ncomp * nn, *d = wf + 9 * ncomp * nn, *b32 = wf + 10 * ncomp * nn;
+    float* val32 = val32c_.as<float>();
+    double* wc = wc_.as<double>();
+    double* invD64 = wc + 8 * ncomp * nn;
+    double* r64 = r64c_.as<double>();
+    double* part = partC_.as<double>();
+    const int ps = 4 * maxBlocks_, ips = maxBlocks_;
+    int* ipart = reinterpret_cast<int*>(part + static_cast<std::size_t>(ps) * kMaxComponents);
+    SolveState* st = stC_.as<SolveState>() + kMaxComponents;  // refinement states
+    SolveState* hostR = hostC_ + kMaxComponents;
+    int Gs = std::min(A.nRowBlocks, kVecGridMax);
+    const int rbPerWg = divUp(A.nRowBlocks, Gs);
+    Gs = std::max(1, divUp(A.nRowBlocks, rbPerWg));
+    const int Gv = vecGridFor(n_);
+    for (int c = 0; c < ncomp; ++c)
+    {
+        // as bicgstabMixed: invD (BiCGStab guard), its FP32 copy and the FP32 values
+        computeInvD(ctx_, A, val + c * valStride, invD64 + c * nn, true);
+        castToFloat(ctx_, invD64 + c * nn, invD32 + c * nn, n_);
+        castToFloat(ctx_, val + c * valStride, val32 + c * nnz, A.nnz);
+    }
+    int its[kMaxComponents] = {0, 0, 0}, outer[kMaxComponents] = {0, 0, 0};
+    double last[kMaxComponents] = {INFINITY, INFINITY, INFINITY};
+    bool active[kMaxComponents], fallback[kMaxComponents] = {false, false, false};
+    for (int c = 0; c < ncomp; ++c)
+    {
+        reps[c] = SolveReport{};
+        active[c] = true;
+    }
+    for (;;)
+    {
+        // true FP64 residual r = b - A x of every running component and the reference's stop test on it
+        for (int c = 0; c < ncomp; ++c)
+        {
+            if (!active[c]) continue;
+            SolveState h{};
+            h.tol = tol;
+            if (outer[c] == 0)
+                CFX_HIP(hipMemcpyAsync(st + c, &h, offsetof(SolveState, ticket), hipMemcpyHostToDevice, s));
+            double* pc = part + static_cast<std::size_t>(c) * ps;
+            int* ic = ipart + static_cast<std::size_t>(c) * ips;
+            k_residual<double><<<Gs, kSpmvBlock, 0, s>>>(A, rbPerWg, val + c * valStride, x + c * nn, b + c * nn,
+                                                         r64 + c * nn, pc, ic);
+            k_residual_final<<<1, kSpmvBlock, 0, s>>>(Gs, pc, ic, st + c, outer[c] == 0 ? 0 : 1);
+        }
+        CFX_CHECK_LAUNCH();
+        CFX_HIP(hipMemcpyAsync(hostR, st, sizeof(SolveState) * ncomp, hipMemcpyDeviceToHost, s));
+        CFX_HIP(hipStreamSynchronize(s));
+        SolveState o[kMaxComponents];
+        for (int c = 0; c < ncomp; ++c) o[c] = hostR[c];
+        double inner[kMaxComponents];
+        int budget[kMaxComponents];
+        bool solve[kMaxComponents];
+        bool any = false;
+        for (int c = 0; c < ncomp; ++c)
+        {
+            solve[c] = false;
+            inner[c] = tol;
+            budget[c] = 0;
+            if (!active[c]) continue;
+            if (outer[c] == 0) reps[c].initialResidual = o[c].initialResidual;
+            reps[c].finalResidual = o[c].rnorm;
+            reps[c].converged = o[c].converged;
+            if (o[c].converged || !std::isfinite(o[c].rnorm) || its[c] >= maxIter)
+            {
+                active[c] = false;
+                continue;
+            }
+            if (outer[c] > 0 && !(o[c].rnorm < 0.5 * last[c]))
+            {
+                // the FP32 correction no longer reduces the FP64 residual: finish in FP64
+                fallback[c] = true;
+                active[c] = false;
+                continue;
+            }
+            last[c] = o[c].rnorm;
+            // FP32 BiCGStab on A d = r from d = 0, to innerTol relative (or to the overall tolerance)
+            castToFloat(ctx_, r64 + c * nn, b32 + c * nn, n_);
+            k_fill<float><<<Gv, kVecBlock, 0, s>>>(d + c * nn, 0.0f, n_);
+            inner[c] = std::max(innerTol, 0.5 * tol * o[c].bnorm / o[c].rnorm);
+            budget[c] = maxIter - its[c];
+            solve[c] = any = true;
+        }
+        if (!any) break;
+        SolveReport in[kMaxComponents];
+        bicgRunComponents<float>(A, val32, static_cast<long long>(nnz), invD32, b32, d, ncomp, inner, budget, solve,
+                                 w32, in);
+        for (int c = 0; c < ncomp; ++c)
+        {
+            if (!solve[c]) continue;
+            its[c] += in[c].iterations;
+            k_add_float<<<Gv, kVecBlock, 0, s>>>(n_, d + c * nn, x + c * nn);
+            ++outer[c];
+            if (in[c].iterations == 0) active[c] = false;  // nothing more the FP32 solve can do
+        }
+        CFX_CHECK_LAUNCH();
+    }
+    bool anyFallback = false;
+    int fbBudget[kMaxComponents];
+    double tols[kMaxComponents];
+    for (int c = 0; c < ncomp; ++c)
+    {
+        fbBudget[c] = maxIter - its[c];
+        tols[c] = tol;
+        anyFallback |= fallback[c];
+    }
+    if (anyFallback)
+    {
+        double* w64[8];
+        for (int j = 0; j < 8; ++j) w64[j] = wc + j * ncomp * nn;
+        SolveReport f64[kMaxComponents];
+        bicgRunComponents<double>(A, val, valStride, invD64, b, x, ncomp, tols, fbBudget, fallback, w64, f64);
+        for (int c = 0; c < ncomp; ++c)
+            if (fallback[c])
+            {
+                its[c] += f64[c].iterations;
+                reps[c].finalResidual = f64[c].finalResidual;
+                reps[c].converged = f64[c].converged;
+                reps[c].precondFallbacks = 1;
+            }
     }
     CFX_HIP(hipEventRecord(ev1_, s));
     CFX_HIP(hipEventSynchronize(ev1_));
@@ -847,14 +1044,9 @@ void Krylov::bicgstabComponents(const CsrView& A, const double* val, long long v
     lastMs_ = ms;
     for (int c = 0; c < ncomp; ++c)
     {
-        const SolveState& hs = hostC_[c];
-        SolveReport& rep = reps[c];
-        rep = SolveReport{};
-        rep.iterations = hs.iter;
-        rep.initialResidual = hs.initialResidual;
-        rep.finalResidual = hs.iter > 0 ? hs.rnorm : hs.initialResidual;
-        rep.converged = hs.converged;
-        rep.ms = ms;
+        reps[c].iterations = its[c];
+        reps[c].outerIterations = outer[c];
+        reps[c].ms = ms;
     }
 }
 

@@ -313,6 +313,37 @@ struct Work
     Ldu P() const { return {1, pdiag.as<double>(), prhs.as<double>(), pup.as<double>(), plo.as<double>()}; }
 };
 
+// device time of a driver call on its stream (events freed on every exit)
+class StepTimer
+{
+public:
+    explicit StepTimer(hipStream_t s) : s_(s)
+    {
+        CFX_HIP(hipEventCreate(&e0_));
+        CFX_HIP(hipEventCreate(&e1_));
+        CFX_HIP(hipEventRecord(e0_, s_));
+    }
+    ~StepTimer()
+    {
+        (void)hipEventDestroy(e0_);
+        (void)hipEventDestroy(e1_);
+    }
+    StepTimer(const StepTimer&) = delete;
+    StepTimer& operator=(const StepTimer&) = delete;
+    double stop()
+    {
+        CFX_HIP(hipEventRecord(e1_, s_));
+        CFX_HIP(hipEventSynchronize(e1_));
+        float ms = 0;
+        CFX_HIP(hipEventElapsedTime(&ms, e0_, e1_));
+        return ms;
+    }
+
+private:
+    hipStream_t s_;
+    hipEvent_t e0_ = nullptr, e1_ = nullptr;
+};
+
 // The mesh's driver work buffers: allocated by the first driver call on the
 // mesh and kept (a time step of a 6M-cell mesh otherwise allocates and frees
 // ~3 GB, and every free waits for the device); the state slots start at zero
@@ -457,11 +488,14 @@ SolveReport fvSolve(DevMesh& m, Krylov& K, const Ldu& A, double* val, double* rh
                                a.gradCorrected, a.sign, A.rhs + static_cast<long long>(k) * n, gradTmp, faceTmp);
         if (a.projectMean)
             for (int k = 0; k < A.ncomp; ++k) projectZeroMean(m, A.rhs + static_cast<long long>(k) * n);
-        if (a.solver == 1 && a.precision == kFP64 && A.ncomp > 1 && K.batchComponents)
+        if (a.solver == 1 && A.ncomp > 1 && K.batchComponents)
         {
             // the components' BiCGStab solves in shared launches, equal to the loop below bit for bit
             SolveReport reps[Krylov::kMaxComponents];
-            K.bicgstabComponents(view, val, nnz, A.rhs, x, A.ncomp, a.tol, a.maxIter, reps);
+            if (a.precision == kFP32IR)
+                K.bicgstabMixedComponents(view, val, nnz, A.rhs, x, A.ncomp, a.tol, a.maxIter, reps);
+            else
+                K.bicgstabComponents(view, val, nnz, A.rhs, x, A.ncomp, a.tol, a.maxIter, reps);
             last = reps[A.ncomp - 1];
             continue;
         }
@@ -518,10 +552,7 @@ SimpleResult simpleSolve(DevMesh& m, Krylov& K, double nu, const Schemes& sch, c
     double* sc = w.sc.as<double>();
     double* part = ctx.partials(0);
     harmonicGammaUniform(m, nu, w.gammaU.as<double>());
-    hipEvent_t e0, e1;
-    CFX_HIP(hipEventCreate(&e0));
-    CFX_HIP(hipEventCreate(&e1));
-    CFX_HIP(hipEventRecord(e0, s));
+    StepTimer timer(s);
 
     SimpleResult res;
     double contLag1 = INFINITY, contLag2 = INFINITY;
@@ -624,13 +655,7 @@ SimpleResult simpleSolve(DevMesh& m, Krylov& K, double nu, const Schemes& sch, c
         contLag2 = contLag1;
         contLag1 = contErr;
     }
-    CFX_HIP(hipEventRecord(e1, s));
-    CFX_HIP(hipEventSynchronize(e1));
-    float ms = 0;
-    CFX_HIP(hipEventElapsedTime(&ms, e0, e1));
-    res.ms = ms;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    res.ms = timer.stop();
     return res;
 }
 
@@ -647,10 +672,7 @@ PisoResult pisoStep(DevMesh& m, Krylov& K, double nu, const Schemes& sch, const 
     double* sc = w.sc.as<double>();
     double* part = ctx.partials(0);
     harmonicGammaUniform(m, nu, w.gammaU.as<double>());
-    hipEvent_t e0, e1;
-    CFX_HIP(hipEventCreate(&e0));
-    CFX_HIP(hipEventCreate(&e1));
-    CFX_HIP(hipEventRecord(e0, s));
+    StepTimer timer(s);
 
     PisoResult res;
     double* gradP = w.gradB.as<double>();
@@ -744,13 +766,7 @@ PisoResult pisoStep(DevMesh& m, Krylov& K, double nu, const Schemes& sch, const 
             break;
         }
     }
-    CFX_HIP(hipEventRecord(e1, s));
-    CFX_HIP(hipEventSynchronize(e1));
-    float ms = 0;
-    CFX_HIP(hipEventElapsedTime(&ms, e0, e1));
-    res.ms = ms;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    res.ms = timer.stop();
     return res;
 }
 

@@ -154,12 +154,16 @@ def test_explicit_operator_matches_reference_definition(case):
         assert np.array_equal(h, ref)
 
 
-def test_vector_bicgstab_components_equal_separate_solves(case):
-    """The FP64 BiCGStab of a vector equation with its components in shared
-    launches gives the fields and report of the reference's component-after-
-    component loop (fvMatrix.cpp:93-122) bit for bit, with and without the
-    non-orthogonal corrector passes, and both track the oracle's solves."""
+@pytest.mark.parametrize("precision", ["fp64", "fp32ir"])
+def test_vector_bicgstab_components_equal_separate_solves(case, precision):
+    """The BiCGStab of a vector equation (FP64, or FP32 with FP64 refinement)
+    with its components in shared launches gives the fields and report of the
+    reference's component-after-component loop (fvMatrix.cpp:93-122) bit for
+    bit, with and without the non-orthogonal corrector passes, and both track
+    the oracle's solves (the FP32 solve reaches tol 1e-12 in 2-3 refinement
+    sweeps here)."""
     c, om, dm, ctx, N = case, case["om"], case["dm"], case["ctx"], case["N"]
+    prec = N.FP64 if precision == "fp64" else N.FP32_IR
     phi = c["rng"].standard_normal(om.nFaces) * 1e-3
     gf = O.harmonic_gamma_uniform(om, 1e-3)
     old = c["rng"].standard_normal((om.nCells, 3))
@@ -176,8 +180,10 @@ def test_vector_bicgstab_components_equal_separate_solves(case):
         for nonOrth in (None, 0, 2):
             x = ctx.array(x0)
             corr = {} if nonOrth is None else dict(gammaFace=ctx.array(gf), linc=c["d"]["Ulc"], link=c["d"]["Ulk"])
-            rep = M.solve(x, solver=N.SOLVER_BICGSTAB, tol=1e-12, maxIter=400, nonOrth=nonOrth or 0, **corr)
-            out[batched, nonOrth] = (x.get(), rep.iterations, rep.finalResidual, rep.converged)
+            rep = M.solve(x, solver=N.SOLVER_BICGSTAB, tol=1e-12, maxIter=400, nonOrth=nonOrth or 0, precision=prec,
+                          **corr)
+            out[batched, nonOrth] = (x.get(), rep.iterations, rep.finalResidual, rep.converged,
+                                     rep.outerIterations, rep.precondFallbacks)
     ctx.set_component_batching(True)
     for nonOrth in (None, 0, 2):
         a, b = out[True, nonOrth], out[False, nonOrth]
@@ -187,3 +193,5 @@ def test_vector_bicgstab_components_equal_separate_solves(case):
     for comp in range(3):
         xo, _ = O.solve_bicgstab(O.csr_from_ldu(om, A, comp), A.rhs[comp].copy(), x0[comp].copy(), 1e-12, 400)
         assert np.abs(out[True, None][0][comp] - xo).max() <= 1e-9 * max(np.abs(xo).max(), 1e-300), comp
+    if precision == "fp32ir":
+        print(f"  FP32 refinement: outer {out[True, None][4]}, FP64 fallback {out[True, None][5]}", flush=True)
