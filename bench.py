@@ -218,6 +218,34 @@ def bench_c5(args):
             ms, dps = DB.coupling_pass(blocks, 3, threads, prec)
             passes[f"{pname}_{threads}"] = ms
             log(f"coupling pass, {pname} pressure, {threads} thread(s): {ms:.1f} ms, dp[0] {dps[0]:.6g} Pa")
+    # CPU baseline: the reference build's solveCG on every block's own system (one core), a bounded number of
+    # iterations per block (the CSR build, timed with 0 iterations, subtracted), extrapolated to the iterations
+    # that block takes
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O  # noqa: E402  (CPU baseline only)
+    cpu = None
+    if O.have_ref():
+        R = O.ref()
+        total_ms, sample = 0.0, 20
+        t_all = time.perf_counter()
+        for blk, rep in zip(blocks, rb):
+            rs, cl, vv, rhs = blk["host"]
+            x, res = np.zeros(blk["n"]), np.zeros(3)
+
+            def ref_run(k):
+                x[:] = 0
+                t = time.perf_counter()
+                R.cfr_solve_cg(O.C.c_int64(blk["n"]), O._ptr(rs, O.C.c_int64), O._ptr(cl, O.C.c_int64), O._ptr(vv),
+                               O._ptr(rhs), O._ptr(x), O.C.c_double(1e-8), O.C.c_int(k), O.C.c_int(0), O._ptr(res))
+                return time.perf_counter() - t
+            per_it = (ref_run(sample) - ref_run(0)) / sample
+            total_ms += per_it * rep.iterations * 1e3
+        model, _ = cpu_info()
+        log(f"cpu baseline (reference solveCG, one core): {total_ms / 1e3:.1f} s for the 64 blocks "
+            f"({time.perf_counter() - t_all:.1f} s of sampling)")
+        cpu = {"value": round(total_ms, 1), "unit": "ms", "cores": 1, "kind": "reference",
+               "sample": f"{sample} Jacobi-PCG iterations of every block's system with the reference build's "
+                         f"solveCG (FP64, 1 thread, {model}), each extrapolated to the iterations that block takes"}
     bitwise = all(np.array_equal(a.get(), b.get()) for a, b in zip(x1, xd))
     conv = all(r.converged for r in rb) and all(r.converged for r in r1)
     tb, to = float(np.mean(bat)), float(np.mean(one))
@@ -244,6 +272,8 @@ def bench_c5(args):
                          "serial vs host threads on one stream per block; passes continue one from another",
         "per_block": per_block,
     }
+    if cpu:
+        result["cpu_baseline"] = cpu
     print(json.dumps(result), flush=True)
 
 
@@ -433,7 +463,7 @@ def bench_c3(args):
             log(f"step {step + 1}: {times[-1]:.1f} ms, KE {ke[-1][1]:.6e}, continuity {r.continuityError:.2e}")
     # kinetic-energy drift against the FP64 CPU oracle: the same case on a small mesh, a step prefix
     sys.path.insert(0, os.path.join(REPO, "oracle"))
-    import oracle as O  # noqa: E402  (checker only)
+    import oracle as O  # noqa: E402  (checker and CPU baseline only)
     hs, a_s, g_s, ds, Ubs, pbs, sU, sp_, sphi = setup(args.c3_check_n, learned)
     om = O.Mesh(a_s["points"], a_s["faceStart"], a_s["faceVerts"], a_s["owner"], a_s["neighbour"], a_s["patchStart"],
                 a_s["patchSize"], a_s["nCells"]).compute_geometry()
@@ -449,6 +479,34 @@ def bench_c3(args):
         drift.append({"step": step + 1, "ke_gpu": kg, "ke_oracle": ko, "rel": abs(kg - ko) / ko})
     cpu_ms = (time.perf_counter() - tc) * 1e3 / max(1, args.c3_check_steps)
     mean_ms = float(np.mean(times[1:] if len(times) > 1 else times))
+    # CPU baseline: one step of the reference build's PISO (Jacobi-PCG pressure, FP64, one core) on a smaller
+    # instance of the same cavity, scaled by the cell count (a step of the 6M-tet case takes minutes on one core)
+    cpu = None
+    if O.have_ref() and args.c3_cpu_n > 0:
+        nc = args.c3_cpu_n
+        hc = N.HostMesh.box(nc, nc, nc, (0, 0, 0), (1, 1, 1), 0.10, SEED_MESH, True)
+        hc.renumber_rcm()
+        ac = hc.arrays()
+        mc = O.Mesh(ac["points"], ac["faceStart"], ac["faceVerts"], ac["owner"], ac["neighbour"], ac["patchStart"],
+                    ac["patchSize"], ac["nCells"]).compute_geometry()
+        rm = O.RefMesh(mc)
+        npatch = len(ac["patchStart"])
+        Upv = np.zeros((npatch, 3))
+        Upv[3, 0] = 1.0  # the lid, as cavity_bcs
+        Uc, pc, phic = np.zeros((mc.nCells, 3)), np.zeros(mc.nCells), np.zeros(mc.nFaces)
+        ctl_vec = [2e-3, 2, 1e-10, args.tol, 20000, 1, 0, 0, 0]
+        t0c = time.perf_counter()
+        rm.piso_step(np.zeros(npatch, np.int32), Upv, np.full(npatch, 2, np.int32), np.zeros(npatch), 1e-3,
+                     (0, corrected, 1), ctl_vec, Uc, pc, phic)
+        sec = time.perf_counter() - t0c
+        scaled = sec * 1e3 * dm.nCells / mc.nCells
+        model, _ = cpu_info()
+        log(f"cpu baseline (reference PISO, one core): {sec:.1f} s for one step on {mc.nCells} tets -> "
+            f"{scaled / 1e3:.1f} s per step scaled to {dm.nCells}")
+        cpu = {"value": round(scaled, 1), "unit": "ms", "cores": 1, "kind": "reference",
+               "sample": f"one PISO step (the first, from rest) of the same cavity on {nc}^3 hex -> {mc.nCells} "
+                         f"tets with the reference build (Jacobi-PCG pressure and BiCGStab momentum, FP64, 1 thread, "
+                         f"{model}), scaled by the cell count to {dm.nCells}"}
     result = {
         "metric": "PISO step time, lid-driven cavity Re 1000 on Kuhn tets; kinetic-energy drift vs FP64 CPU oracle",
         "value": round(mean_ms, 3), "unit": "ms", "n_gpus": 1, "steps": args.c3_steps, "warmup": 1,
@@ -463,6 +521,8 @@ def bench_c3(args):
         "ke_check": {"cells": ds.nCells, "steps": args.c3_check_steps, "max_rel_drift": max(d["rel"] for d in drift),
                      "per_step": drift, "oracle_step_ms": round(cpu_ms, 1)},
     }
+    if cpu:
+        result["cpu_baseline"] = cpu
     print(json.dumps(result), flush=True)
 
 
@@ -516,6 +576,8 @@ def main():
     ap.add_argument("--c1-variants", default="fp64,fp32ir,multigrid",
                     help="pressure solves timed: fp64 (reference Jacobi-PCG, the metric), fp32ir, multigrid")
     ap.add_argument("--c3-n", type=int, default=100)
+    ap.add_argument("--c3-cpu-n", type=int, default=40,
+                    help="hex cells per edge of the cavity the reference build's CPU step is timed on (0: skip)")
     ap.add_argument("--c3-network", action="store_true",
                     help="multigrid pressure solve with the network's finest-level weights (default: constant "
                          "weights, faster on these tets: 92.7 against 100.1 ms per step over 40 steps)")

@@ -67,7 +67,8 @@ def build_blocks(ctx, specs, simple_iterations=3, jitter=0.1, nu=1e-3, rho=1000.
         rhs = drhs.get()[last].copy()
         G = N.Csr(ctx, rp.astype(np.int64), cl.astype(np.int64), vals)
         blocks.append(dict(csr=G, b=ctx.array(rhs), n=n, nnz=nnz, dp=dp, measure=m, spec=s, q=q,
-                           detail=blk if own_context else None, ctx=bctx if own_context else None))
+                           detail=blk if own_context else None, ctx=bctx if own_context else None,
+                           host=(rp.astype(np.int64), cl.astype(np.int64), vals, rhs)))
         del blk, dval, drhs
         if log:
             log(f"block {k}: {s['nx']}x{s['ny']}x{s['nz']} = {n} cells, {s['shape']}, dp {dp:.4g} Pa "
