@@ -28,6 +28,16 @@ __global__ void k_absmax(int n, const double* __restrict__ v, double* __restrict
     if (threadIdx.x == 0) part[blockIdx.x] = m;
 }
 
+// max of the G partial maxima, by one workgroup (max is exact in any order)
+__global__ void k_absmax_final(int G, const double* __restrict__ part, double* __restrict__ out)
+{
+    __shared__ double red[B / kWave];
+    double m = 0.0;
+    for (int g = threadIdx.x; g < G; g += B) m = fmax(m, part[g]);
+    m = blockMax(m, red);
+    if (threadIdx.x == 0) *out = m;
+}
+
 // |a_ii| guarded away from zero (max(|a_ii|, 1e-300))
 __global__ void k_diag_abs(CsrView A, const double* __restrict__ val, double* __restrict__ d)
 {
@@ -45,19 +55,11 @@ constexpr int kFeatLanes = 8;
 static_assert(S::FIN == kFeatLanes, "one lane per node feature");
 __global__ __launch_bounds__(B) void k_features(CsrView A, const double* __restrict__ val,
                                                 const double* __restrict__ dabs, const double* __restrict__ geo,
-                                                const double* __restrict__ r0, const double* __restrict__ partMax,
-                                                int Gm, float* __restrict__ x0, float* __restrict__ ew,
+                                                const double* __restrict__ r0, const double* __restrict__ rmaxPtr,
+                                                float* __restrict__ x0, float* __restrict__ ew,
                                                 float* __restrict__ eg)
 {
-    __shared__ double red[B / kWave];
-    __shared__ double rmaxS;
-    double m = 0.0;
-    if (r0)
-        for (int g = threadIdx.x; g < Gm; g += B) m = fmax(m, partMax[g]);
-    m = blockMax(m, red);
-    if (threadIdx.x == 0) rmaxS = m;
-    __syncthreads();
-    const double rmax = rmaxS;
+    const double rmax = r0 ? *rmaxPtr : 0.0;
     const int i = blockIdx.x * (B / kFeatLanes) + static_cast<int>(threadIdx.x) / kFeatLanes;
     const int q = threadIdx.x % kFeatLanes;
     if (i >= A.n) return;  // all 8 lanes of a row leave together
@@ -167,35 +169,36 @@ __global__ __launch_bounds__(B) void k_layer_fwd(int n, CsrView A, const float* 
         if (i < n)
         {
             load16(hin + static_cast<long long>(i) * S::H + f0, self);
-            // two neighbours per step: both rows' loads are in flight before either is used; the
-            // sums still take the neighbours in slot order
+            // four neighbours per step: all four rows' loads are in flight before any is used; the sums
+            // still take the neighbours in slot order
             const int kEnd = A.rowPtr[i + 1];
-            for (int k = A.rowPtr[i]; k < kEnd; k += 2)
+            for (int k = A.rowPtr[i]; k < kEnd; k += 4)
             {
-                const bool two = k + 1 < kEnd;
-                const int j0 = A.col[k], j1 = two ? A.col[k + 1] : j0;
-                const float w0 = ew[k], g0 = eg[k];
-                const float w1 = two ? ew[k + 1] : 0.0f, g1 = two ? eg[k + 1] : 0.0f;
-                float v0[16], v1[16];
-                load16(hin + static_cast<long long>(j0) * S::H + f0, v0);
-                load16(hin + static_cast<long long>(j1) * S::H + f0, v1);
-                // fused multiply-adds (with FP16 rows they become v_fma_mix_f32, the conversion folded in)
-                if (j0 != i)
-                {
+                int j[4];
+                float w[4], gg[4];
 #pragma unroll
-                    for (int c = 0; c < 16; ++c)
-                    {
-                        m[c] = fmaf(w0, v0[c], m[c]);
-                        g[c] = fmaf(g0, v0[c], g[c]);
-                    }
+                for (int q = 0; q < 4; ++q)
+                {
+                    const bool ok = k + q < kEnd;
+                    j[q] = ok ? A.col[k + q] : i;
+                    w[q] = ok ? ew[k + q] : 0.0f;
+                    gg[q] = ok ? eg[k + q] : 0.0f;
                 }
-                if (two && j1 != i)
-                {
+                float v[4][16];
 #pragma unroll
-                    for (int c = 0; c < 16; ++c)
+                for (int q = 0; q < 4; ++q) load16(hin + static_cast<long long>(j[q]) * S::H + f0, v[q]);
+                // fused multiply-adds (with FP16 rows they become v_fma_mix_f32, the conversion folded in)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                {
+                    if (k + q < kEnd && j[q] != i)
                     {
-                        m[c] = fmaf(w1, v1[c], m[c]);
-                        g[c] = fmaf(g1, v1[c], g[c]);
+#pragma unroll
+                        for (int c = 0; c < 16; ++c)
+                        {
+                            m[c] = fmaf(w[q], v[q][c], m[c]);
+                            g[c] = fmaf(gg[q], v[q][c], g[c]);
+                        }
                     }
                 }
             }
@@ -549,10 +552,14 @@ void Mpnn::setGraph(const CsrView& A, const double* val, const double* geo, cons
     n_ = A.n;
     const int Gm = std::max(1, std::min(256, divUp(A.n, B)));
     double* pm = scal_.as<double>();
-    if (r0) k_absmax<<<Gm, B, 0, s>>>(A.n, r0, pm);
+    if (r0)
+    {
+        k_absmax<<<Gm, B, 0, s>>>(A.n, r0, pm);
+        k_absmax_final<<<1, B, 0, s>>>(Gm, pm, pm + 512);
+    }
     k_diag_abs<<<divUp(A.n, B), B, 0, s>>>(A, val, dabs_.as<double>());
-    k_features<<<divUp(A.n, B / kFeatLanes), B, 0, s>>>(A, val, dabs_.as<double>(), geo, r0, pm, Gm, x0_.as<float>(),
-                                           ew_.as<float>(), eg_.as<float>());
+    k_features<<<divUp(A.n, B / kFeatLanes), B, 0, s>>>(A, val, dabs_.as<double>(), geo, r0, pm + 512,
+                                                       x0_.as<float>(), ew_.as<float>(), eg_.as<float>());
     CFX_CHECK_LAUNCH();
 }
 
