@@ -199,6 +199,33 @@ __global__ void k_explicit_op(int n, const double* __restrict__ rhs, const doubl
     if (i < n) h[i] = rhs[i] - (ax[i] - diag[i] * x[i]);
 }
 
+// streaming yardsticks for the rooflines: a read-only pass (16-byte loads, grid-stride, one partial per
+// workgroup) and a copy, over n4 16-byte elements
+__global__ __launch_bounds__(256) void k_stream_read(const float4* __restrict__ a, long long n4,
+                                                     float* __restrict__ out)
+{
+    __shared__ float red[256];
+    float s = 0.f;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n4; i += gridDim.x * 256LL)
+    {
+        const float4 v = a[i];
+        s += (v.x + v.y) + (v.z + v.w);
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1)
+    {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_stream_copy(const float4* __restrict__ a, float4* __restrict__ b, long long n4)
+{
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n4; i += gridDim.x * 256LL) b[i] = a[i];
+}
+
 __global__ void k_add_vec(int n, const double* __restrict__ v, double* __restrict__ y)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -297,6 +324,47 @@ int cfx_device_info(cfx_ctx* c, char* name, size_t len, int* cus)
     return guard(c, [&] {
         if (name && len) { std::strncpy(name, c->ctx->deviceName.c_str(), len - 1); name[len - 1] = 0; }
         if (cus) *cus = c->ctx->numCUs();
+    });
+}
+
+int cfx_stream_bandwidth(cfx_ctx* c, size_t bytes, int reps, double* readGBps, double* copyGBps)
+{
+    return guard(c, [&] {
+        need(bytes >= (1u << 20) && reps > 0, "stream bandwidth: at least 1 MiB and one repetition");
+        hipStream_t s = c->ctx->stream();
+        const long long n4 = static_cast<long long>(bytes / 16);
+        DevBuf a(16 * static_cast<std::size_t>(n4)), b(16 * static_cast<std::size_t>(n4));
+        constexpr int kGrids[] = {1024, 2048, 4096};
+        DevBuf out(sizeof(float) * 4096);
+        CFX_HIP(hipMemsetAsync(a.p, 0, a.bytes, s));
+        hipEvent_t e0, e1;
+        CFX_HIP(hipEventCreate(&e0));
+        CFX_HIP(hipEventCreate(&e1));
+        double bestR = 0, bestC = 0;
+        // the best of a few grid sizes, each timed over reps launches after one warm-up
+        for (int g : kGrids)
+        {
+            float ms = 0;
+            k_stream_read<<<g, 256, 0, s>>>(a.as<float4>(), n4, out.as<float>());
+            CFX_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; ++r) k_stream_read<<<g, 256, 0, s>>>(a.as<float4>(), n4, out.as<float>());
+            CFX_HIP(hipEventRecord(e1, s));
+            CFX_HIP(hipEventSynchronize(e1));
+            CFX_HIP(hipEventElapsedTime(&ms, e0, e1));
+            bestR = std::max(bestR, 16.0 * n4 * reps / (ms * 1e-3) / 1e9);
+            k_stream_copy<<<g, 256, 0, s>>>(a.as<float4>(), b.as<float4>(), n4);
+            CFX_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; ++r) k_stream_copy<<<g, 256, 0, s>>>(a.as<float4>(), b.as<float4>(), n4);
+            CFX_HIP(hipEventRecord(e1, s));
+            CFX_HIP(hipEventSynchronize(e1));
+            CFX_HIP(hipEventElapsedTime(&ms, e0, e1));
+            bestC = std::max(bestC, 32.0 * n4 * reps / (ms * 1e-3) / 1e9);
+        }
+        CFX_CHECK_LAUNCH();
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (readGBps) *readGBps = bestR;
+        if (copyGBps) *copyGBps = bestC;
     });
 }
 

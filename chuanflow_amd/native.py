@@ -166,6 +166,7 @@ _sig("cfx_csr_residual_history", C.c_int, _vp, _vp, C.c_int)
 _sig("cfx_csr_set_inner_tol", C.c_int, _vp, C.c_double)
 _sig("cfx_mg_default_params", C.c_int, C.POINTER(MgParams))
 _sig("cfx_set_graphs", C.c_int, _vp, C.c_int)
+_sig("cfx_stream_bandwidth", C.c_int, _vp, C.c_size_t, C.c_int, _dp, _dp)
 _sig("cfx_set_component_batching", C.c_int, _vp, C.c_int)
 _sig("cfx_batch_solve", C.c_int, _vp, C.c_int, _vp, _vp, _vp, C.c_double, C.c_int, _vp, _vp, _vp,
      C.POINTER(BatchStats))
@@ -298,6 +299,12 @@ class Context:
     def set_graphs(self, enable: bool):
         """Flexible CG: replay one captured iteration per step (True) or launch kernel by kernel."""
         self.check(_L.cfx_set_graphs(self.h, 1 if enable else 0))
+
+    def stream_bandwidth(self, nbytes=1 << 30, reps=20):
+        """(read GB/s, copy GB/s read + written) of plain streams over nbytes: the roofline yardstick."""
+        rd, cp = C.c_double(0), C.c_double(0)
+        self.check(_L.cfx_stream_bandwidth(self.h, nbytes, reps, C.byref(rd), C.byref(cp)))
+        return rd.value, cp.value
 
     def set_component_batching(self, enable: bool):
         """FP64 BiCGStab of vector equations: all components in shared launches (True) or one by one."""

@@ -66,6 +66,12 @@ const char* cfx_last_error(const cfx_ctx* ctx);
 int cfx_synchronize(cfx_ctx* ctx);
 /* name: gcnArch string, cus: compute units */
 int cfx_device_info(cfx_ctx* ctx, char* name, size_t nameLen, int* cus);
+/* what a plain stream reaches on this device, the yardstick of the bandwidth
+ * rooflines: the best over a few grid sizes of a read-only pass and of a copy
+ * of `bytes` (>= 1 MiB), each timed over `reps` launches on the context's
+ * stream; GB/s of bytes read (copy: read + written).  No reference
+ * counterpart. */
+int cfx_stream_bandwidth(cfx_ctx* ctx, size_t bytes, int reps, double* readGBps, double* copyGBps);
 int cfx_malloc(cfx_ctx* ctx, size_t bytes, void** d_out);
 int cfx_free(cfx_ctx* ctx, void* d_ptr);
 int cfx_memcpy_htod(cfx_ctx* ctx, void* d_dst, const void* src, size_t bytes);

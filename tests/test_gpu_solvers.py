@@ -198,3 +198,15 @@ def test_failure_modes_report_not_throw(ctx):
     # tridiagonal Poisson: x = 1 (solverBackendApp.cpp:89-109)
     x, rep = G.solve(A @ np.ones(n), None, 1e-12, 500)
     assert rep.converged and np.abs(x - 1).max() < 1e-8
+
+
+def test_stream_bandwidth_yardstick(ctx):
+    """cfx_stream_bandwidth: plain read and copy streams land between a
+    fraction of the HBM peak and the peak itself (8 TB/s), and a read-only
+    stream is not slower than a copy's reads and writes together by more than
+    timing noise."""
+    print("\n[case] stream bandwidth, 256 MiB", flush=True)
+    rd, cp = ctx.stream_bandwidth(256 << 20, 5)
+    print(f"  read {rd:.0f} GB/s, copy {cp:.0f} GB/s", flush=True)
+    assert 1000 < rd < 8000 * 1.02 and 1000 < cp < 8000 * 1.02
+    assert rd > 0.8 * cp
