@@ -170,4 +170,33 @@ __device__ __forceinline__ int xcdRemap(int h, int G)
     return (x < r) ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
 }
 
+// CSR-stream staging of one row block's products: lds[k] = val[k0 + k] *
+// x[col[k0 + k]] for k < nnz (as TL), lane t taking k = t, t + B, ... Each lane
+// has four nonzeros' value, column and x loads in flight per step; every
+// product is the one of the plain loop.
+template <typename TL, typename TV, typename TX>
+__device__ __forceinline__ void stageProducts(TL* lds, const TV* __restrict__ val, const int* __restrict__ col,
+                                              const TX* __restrict__ x, int k0, int nnz)
+{
+    const int bd = static_cast<int>(blockDim.x);
+    int k = static_cast<int>(threadIdx.x);
+    for (; k + 3 * bd < nnz; k += 4 * bd)
+    {
+        TV v[4];
+        int c[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+        {
+            v[q] = val[k0 + k + q * bd];
+            c[q] = col[k0 + k + q * bd];
+        }
+        TX xv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) xv[q] = x[c[q]];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) lds[k + q * bd] = static_cast<TL>(v[q]) * static_cast<TL>(xv[q]);
+    }
+    for (; k < nnz; k += bd) lds[k] = static_cast<TL>(val[k0 + k]) * static_cast<TL>(x[col[k0 + k]]);
+}
+
 } // namespace cfx

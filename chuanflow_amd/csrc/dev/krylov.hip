@@ -84,7 +84,7 @@ __device__ __forceinline__ bool spmvRowSum(const CsrView& A, const T* __restrict
     row = r0 + static_cast<int>(threadIdx.x);
     if (nnz <= kSpmvLds)
     {
-        for (int k = threadIdx.x; k < nnz; k += blockDim.x) lds[k] = val[k0 + k] * x[A.col[k0 + k]];
+        stageProducts(lds, val, A.col, x, k0, nnz);
         __syncthreads();
         if (row < r1)
         {

@@ -118,7 +118,7 @@ __device__ __forceinline__ bool rowSum(const CsrView& A, const float* __restrict
     float s = 0.0f;
     if (nnz <= LDS)
     {
-        for (int k = threadIdx.x; k < nnz; k += B) lds[k] = valM[k0 + k] * v[A.col[k0 + k]];
+        stageProducts(lds, valM, A.col, v, k0, nnz);
         __syncthreads();
         if (row < r1)
             for (int k = A.rowPtr[row] - k0, e = A.rowPtr[row + 1] - k0; k < e; ++k) s += lds[k];

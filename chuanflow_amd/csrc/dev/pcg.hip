@@ -110,7 +110,7 @@ __device__ __forceinline__ void body_pcg_residual(CsrView A, int rbPerWg, const 
         double s = 0;
         if (nnz <= kSpmvLds)
         {
-            for (int k = threadIdx.x; k < nnz; k += kB) lds[k] = val[k0 + k] * x[A.col[k0 + k]];
+            stageProducts(lds, val, A.col, x, k0, nnz);
             __syncthreads();
             if (row < r1)
                 for (int k = A.rowPtr[row] - k0, e = A.rowPtr[row + 1] - k0; k < e; ++k) s += lds[k];
@@ -177,7 +177,7 @@ __device__ __forceinline__ void body_pcg_A(CsrView A, int rbPerWg, const T* __re
         T s = T(0);
         if (nnz <= kSpmvLds)
         {
-            for (int kk = threadIdx.x; kk < nnz; kk += kB) lds[kk] = val[k0 + kk] * p[A.col[k0 + kk]];
+            stageProducts(lds, val, A.col, p, k0, nnz);
             __syncthreads();
             if (row < r1)
                 for (int kk = A.rowPtr[row] - k0, e = A.rowPtr[row + 1] - k0; kk < e; ++kk) s += lds[kk];
@@ -401,8 +401,7 @@ __global__ __launch_bounds__(kB) void k_fcg_A(CsrView A, int rbPerWg, const doub
         double s = 0.0;
         if (nnz <= kSpmvLds)
         {
-            for (int kk = threadIdx.x; kk < nnz; kk += kB)
-                lds[kk] = val[k0 + kk] * static_cast<double>(p[A.col[k0 + kk]]);
+            stageProducts(lds, val, A.col, p, k0, nnz);
             __syncthreads();
             if (row < r1)
                 for (int kk = A.rowPtr[row] - k0, e = A.rowPtr[row + 1] - k0; kk < e; ++kk) s += lds[kk];
