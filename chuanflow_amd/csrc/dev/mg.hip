@@ -1029,7 +1029,9 @@ bool Multigrid::solveLevel(int l, const int* done, bool deferCombine)
         return false;
     }
     const CsrView A = L.pat.view();
-    const int Gv = std::max(1, std::min(64, divUp(L.n, 4 * B)));
+    // vector kernels of the K-cycle: about four elements per thread (each workgroup also reduces the
+    // level's partials, which costs little next to a thread walking a long stride of elements)
+    const int Gv = std::max(1, std::min(1024, divUp(L.n, 4 * B)));
     double* part = L.part.as<double>();
     visit(l, L.b.as<float>(), L.c1.as<float>(), nullptr, nullptr, L.G, L.rbPerWg, done);
     k_kc_spmv1<<<L.G, B, 0, s>>>(A, L.rbPerWg, L.val32.as<float>(), L.c1.as<float>(), L.b.as<float>(),
