@@ -28,6 +28,8 @@ Prints ONE JSON line on stdout; progress goes to stderr.
 from __future__ import annotations
 
 import argparse
+import contextlib
+import ctypes
 import json
 import os
 import platform
@@ -289,6 +291,23 @@ def cavity_bcs(patch_start, patch_size, n_internal, n_boundary, lid=1.0):
     return (Ut, Uv), (np.full(n_boundary, 2, np.int32), np.zeros((n_boundary, 1)))
 
 
+@contextlib.contextmanager
+def stdout_to_stderr():
+    """Send what native code writes to stdout (the reference build's progress lines) to stderr, so stdout
+    carries only the JSON line."""
+    sys.stdout.flush()
+    libc = ctypes.CDLL(None)
+    libc.fflush(None)
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        yield
+    finally:
+        libc.fflush(None)
+        os.dup2(saved, 1)
+        os.close(saved)
+
+
 def channel_bcs(patch_start, patch_size, n_internal, n_boundary, u_in):
     """Box channel along x: inlet (1st patch) fixed U = (u_in, 0, 0), outlet (2nd) zeroGradient U and
     fixed p = 0, the other patches no-slip walls with zeroGradient p; per boundary face and per patch
@@ -374,9 +393,10 @@ def bench_c1(args):
     if O.have_ref():
         Upt, Upv, ppt, ppv = patch_bcs
         rm = O.RefMesh(om)
-        t0 = time.perf_counter()
-        rm.simple(Upt, Upv, ppt, ppv, 1e-3, (0, 1, 1), ctl_vec, U, p, phi)
-        cpu_s = time.perf_counter() - t0
+        with stdout_to_stderr():
+            t0 = time.perf_counter()
+            rm.simple(Upt, Upv, ppt, ppv, 1e-3, (0, 1, 1), ctl_vec, U, p, phi)
+            cpu_s = time.perf_counter() - t0
         kind = "reference"
     else:
         t0 = time.perf_counter()
@@ -495,10 +515,11 @@ def bench_c3(args):
         Upv[3, 0] = 1.0  # the lid, as cavity_bcs
         Uc, pc, phic = np.zeros((mc.nCells, 3)), np.zeros(mc.nCells), np.zeros(mc.nFaces)
         ctl_vec = [2e-3, 2, 1e-10, args.tol, 20000, 1, 0, 0, 0]
-        t0c = time.perf_counter()
-        rm.piso_step(np.zeros(npatch, np.int32), Upv, np.full(npatch, 2, np.int32), np.zeros(npatch), 1e-3,
-                     (0, corrected, 1), ctl_vec, Uc, pc, phic)
-        sec = time.perf_counter() - t0c
+        with stdout_to_stderr():
+            t0c = time.perf_counter()
+            rm.piso_step(np.zeros(npatch, np.int32), Upv, np.full(npatch, 2, np.int32), np.zeros(npatch), 1e-3,
+                         (0, corrected, 1), ctl_vec, Uc, pc, phic)
+            sec = time.perf_counter() - t0c
         scaled = sec * 1e3 * dm.nCells / mc.nCells
         model, _ = cpu_info()
         log(f"cpu baseline (reference PISO, one core): {sec:.1f} s for one step on {mc.nCells} tets -> "
