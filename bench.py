@@ -46,7 +46,7 @@ METRIC = "p-solve time to 1e-8 FP64 residual: learned+mixed PCG vs plain PCG; Sp
 HBM_PEAK_GBPS = 8000.0
 # HBM bytes per FP64 SpMV launch on the 128^3 system from rocprofv3 PMC (profiles/spmv_128cube_hbm_traffic.md):
 # FETCH_SIZE 100,831.1 KiB / 0.50005 (the copy-kernel calibration) + WRITE_SIZE 16,384 KiB
-SPMV_FP64_TRAFFIC_128 = 223257689
+SPMV_FP64_TRAFFIC_128 = 220712939
 SEED_MESH, SEED_SHUFFLE, SEED_RHS = 128, 7, 2024
 
 
