@@ -53,7 +53,7 @@ struct MgParams
     float coarsestOmega = 2.0f / 3.0f;
     int maxLevels = 10, coarseMax = 128;
     int kcycleDepth = 2;              // coarse levels 1..kcycleDepth use the K-cycle (0: V-cycle)
-    int fusion = 3;                   // bit 0: the small coarse levels' cycle in one workgroup;
+    int fusion = 1;                   // bit 0: the small coarse levels' cycle in one workgroup;
                                       // bit 1: prolongation (and K-cycle combination) fused into the
                                       // post-smoothing of coarse levels of at most 65536 rows
 };
