@@ -77,7 +77,9 @@ public:
     // the overall tolerance when that is closer), x += d; the reference's stop
     // test is applied to the true FP64 residual only.  When a correction no
     // longer halves the FP64 residual the solve finishes in FP64
-    // (precondFallbacks = 1).  iterations counts the FP32 (and FP64) BiCGStab
+    // (precondFallbacks = 1), from the x before that correction if it made the
+    // residual worse or not finite; an FP32 solve that fails (non-finite) is
+    // not applied.  iterations counts the FP32 (and FP64) BiCGStab
     // iterations, outerIterations the corrections.
     SolveReport bicgstabMixed(const CsrView& A, const double* val, const double* b, double* x, double tol,
                               int maxIter, double innerTol = 1e-4);

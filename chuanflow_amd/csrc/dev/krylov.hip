@@ -908,7 +908,8 @@ void Krylov::bicgstabMixedComponents(const CsrView& A, const double* val, long l
     if (!wc32_.p)
     {
         wc32_.alloc(sizeof(float) * 11 * kMaxComponents * static_cast<std::size_t>(nn));
-        r64c_.alloc(sizeof(double) * kMaxComponents * static_cast<std::size_t>(nn));
+        // FP64 residuals [c][n], then each component's x before its last correction [c][n]
+        r64c_.alloc(sizeof(double) * 2 * kMaxComponents * static_cast<std::size_t>(nn));
     }
     if (!val32c_.p || val32c_.bytes < sizeof(float) * kMaxComponents * nnz)
         val32c_.alloc(sizeof(float) * kMaxComponents * nnz);
@@ -921,6 +922,7 @@ void Krylov::bicgstabMixedComponents(const CsrView& A, const double* val, long l
     double* wc = wc_.as<double>();
     double* invD64 = wc + 8 * ncomp * nn;
     double* r64 = r64c_.as<double>();
+    double* xPrev = r64 + kMaxComponents * nn;
     double* part = partC_.as<double>();
     const int ps = 4 * maxBlocks_, ips = maxBlocks_;
     int* ipart = reinterpret_cast<int*>(part + static_cast<std::size_t>(ps) * kMaxComponents);
@@ -940,6 +942,7 @@ void Krylov::bicgstabMixedComponents(const CsrView& A, const double* val, long l
     int its[kMaxComponents] = {0, 0, 0}, outer[kMaxComponents] = {0, 0, 0};
     double last[kMaxComponents] = {INFINITY, INFINITY, INFINITY};
     bool active[kMaxComponents], fallback[kMaxComponents] = {false, false, false};
+    bool restore[kMaxComponents] = {false, false, false};
     for (int c = 0; c < ncomp; ++c)
     {
         reps[c] = SolveReport{};
@@ -979,15 +982,17 @@ void Krylov::bicgstabMixedComponents(const CsrView& A, const double* val, long l
             if (outer[c] == 0) reps[c].initialResidual = o[c].initialResidual;
             reps[c].finalResidual = o[c].rnorm;
             reps[c].converged = o[c].converged;
-            if (o[c].converged || !std::isfinite(o[c].rnorm) || its[c] >= maxIter)
+            if (o[c].converged || its[c] >= maxIter || (outer[c] == 0 && !std::isfinite(o[c].rnorm)))
             {
                 active[c] = false;
                 continue;
             }
             if (outer[c] > 0 && !(o[c].rnorm < 0.5 * last[c]))
             {
-                // the FP32 correction no longer reduces the FP64 residual: finish in FP64
+                // the FP32 correction no longer halves the FP64 residual: finish in FP64, from the x before
+                // it when it made the residual worse or not finite
                 fallback[c] = true;
+                restore[c] = !(o[c].rnorm < last[c]);
                 active[c] = false;
                 continue;
             }
@@ -1007,6 +1012,13 @@ void Krylov::bicgstabMixedComponents(const CsrView& A, const double* val, long l
         {
             if (!solve[c]) continue;
             its[c] += in[c].iterations;
+            if (!std::isfinite(in[c].finalResidual))
+            {
+                fallback[c] = true;  // the FP32 solve itself failed: x is left as it is and finished in FP64
+                active[c] = false;
+                continue;
+            }
+            CFX_HIP(hipMemcpyAsync(xPrev + c * nn, x + c * nn, sizeof(double) * nn, hipMemcpyDeviceToDevice, s));
             k_add_float<<<Gv, kVecBlock, 0, s>>>(n_, d + c * nn, x + c * nn);
             ++outer[c];
             if (in[c].iterations == 0) active[c] = false;  // nothing more the FP32 solve can do
@@ -1018,9 +1030,12 @@ void Krylov::bicgstabMixedComponents(const CsrView& A, const double* val, long l
     double tols[kMaxComponents];
     for (int c = 0; c < ncomp; ++c)
     {
+        fallback[c] = fallback[c] && its[c] < maxIter;
         fbBudget[c] = maxIter - its[c];
         tols[c] = tol;
         anyFallback |= fallback[c];
+        if (fallback[c] && restore[c])
+            CFX_HIP(hipMemcpyAsync(x + c * nn, xPrev + c * nn, sizeof(double) * nn, hipMemcpyDeviceToDevice, s));
     }
     if (anyFallback)
     {
@@ -1080,6 +1095,8 @@ SolveReport Krylov::bicgstabMixed(const CsrView& A, const double* val, const dou
     SolveReport rep;
     int its = 0, outer = 0;
     double last = INFINITY;
+    double* xPrev = q_.as<double>();  // x before the last correction
+    bool fallback = false, restore = false;
     for (;;)
     {
         // true FP64 residual r = b - A x and the reference's stop test on it
@@ -1091,17 +1108,13 @@ SolveReport Krylov::bicgstabMixed(const CsrView& A, const double* val, const dou
         if (outer == 0) rep.initialResidual = o.initialResidual;
         rep.finalResidual = o.rnorm;
         rep.converged = o.converged;
-        if (o.converged || !std::isfinite(o.rnorm) || its >= maxIter) break;
+        if (o.converged || its >= maxIter || (outer == 0 && !std::isfinite(o.rnorm))) break;
         if (outer > 0 && !(o.rnorm < 0.5 * last))
         {
-            // the FP32 correction no longer reduces the FP64 residual: finish in FP64
-            double* w64[8] = {r_.as<double>(), r0_.as<double>(), p_.as<double>(), v_.as<double>(),
-                              z_.as<double>(), s_.as<double>(), sh_.as<double>(), t_.as<double>()};
-            const SolveReport f64 = bicgRun<double>(A, val, invDb_.as<double>(), b, x, tol, maxIter - its, w64);
-            its += f64.iterations;
-            rep.finalResidual = f64.finalResidual;
-            rep.converged = f64.converged;
-            rep.precondFallbacks = 1;
+            // the FP32 correction no longer halves the FP64 residual: finish in FP64, from the x before it
+            // when it made the residual worse or not finite
+            fallback = true;
+            restore = !(o.rnorm < last);
             break;
         }
         last = o.rnorm;
@@ -1112,9 +1125,26 @@ SolveReport Krylov::bicgstabMixed(const CsrView& A, const double* val, const dou
         const SolveReport in = bicgRun<float>(A, val32b_.as<float>(), invD32_.as<float>(), b32, d, inner,
                                               maxIter - its, w);
         its += in.iterations;
+        if (!std::isfinite(in.finalResidual))
+        {
+            fallback = true;  // the FP32 solve itself failed: x is left as it is and finished in FP64
+            break;
+        }
+        CFX_HIP(hipMemcpyAsync(xPrev, x, sizeof(double) * n_, hipMemcpyDeviceToDevice, s));
         k_add_float<<<Gv, kVecBlock, 0, s>>>(n_, d, x);
         ++outer;
         if (in.iterations == 0) break;  // nothing more the FP32 solve can do (breakdown at its start)
+    }
+    if (fallback && its < maxIter)
+    {
+        if (restore) CFX_HIP(hipMemcpyAsync(x, xPrev, sizeof(double) * n_, hipMemcpyDeviceToDevice, s));
+        double* w64[8] = {r_.as<double>(), r0_.as<double>(), p_.as<double>(), v_.as<double>(),
+                          z_.as<double>(), s_.as<double>(), sh_.as<double>(), t_.as<double>()};
+        const SolveReport f64 = bicgRun<double>(A, val, invDb_.as<double>(), b, x, tol, maxIter - its, w64);
+        its += f64.iterations;
+        rep.finalResidual = f64.finalResidual;
+        rep.converged = f64.converged;
+        rep.precondFallbacks = 1;
     }
     rep.iterations = its;
     rep.outerIterations = outer;

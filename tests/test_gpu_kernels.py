@@ -198,15 +198,15 @@ def test_vector_bicgstab_components_equal_separate_solves(case, precision):
 
 
 def test_mixed_vector_bicgstab_hard_systems_equal_separate_solves(case):
-    """Vector diffusion systems whose face coefficients span 1e-3 and 1e-6
-    (log-uniform, seeded), up to a few hundred BiCGStab iterations: the
-    FP32-refined component solve and the one-by-one solves agree bit for bit,
-    reports included, and converge (the FP64 fallback is not reached on these
-    systems; at a 1e-9 span on the tet mesh the FP32 solve diverges, in both
-    forms alike, and is reported unconverged)."""
+    """Vector diffusion systems whose face coefficients span 1e-3, 1e-6 and
+    1e-9 (log-uniform, seeded): the FP32-refined component solve and the
+    one-by-one solves agree bit for bit, reports included.  The first two
+    converge in FP32 refinement sweeps.  At 1e-9 the FP32 solve can fail (on
+    the tet mesh its iterates blow up): the failed correction is then not kept
+    and the solve finishes in FP64, so x stays finite."""
     c, om, dm, ctx, N = case, case["om"], case["dm"], case["ctx"], case["N"]
     rng = np.random.default_rng(23)
-    for span in (1e-3, 1e-6):
+    for span in (1e-3, 1e-6, 1e-9):
         gf = np.exp(rng.uniform(np.log(span), 0.0, om.nFaces)) * 1e-3
         x0 = rng.standard_normal((3, om.nCells)) * 1e-2
         src = rng.standard_normal((3, om.nCells))
@@ -225,4 +225,6 @@ def test_mixed_vector_bicgstab_hard_systems_equal_separate_solves(case):
         a, b = out[True], out[False]
         print(f"  span {span:g}: iterations {a[1]}, outer {a[4]}, fallback {a[5]}, converged {a[3]}", flush=True)
         assert np.array_equal(a[0], b[0]) and a[1:] == b[1:], span
-        assert a[3] == 1, span
+        assert np.all(np.isfinite(a[0])), span
+        if span > 1e-9:
+            assert a[3] == 1 and a[5] == 0, span
