@@ -423,7 +423,17 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const floa
         float *b = v + off[last], *e = b + C.n, *w = e + C.n;
         if (C.dense)
         {
-            for (int i = t; i < C.n; i += kTailThreads) e[i] = colDot8(C.dense, b, C.n, i);
+            // the inverse in LDS in one coalesced pass, then each row's column-order sum (colDot8's order)
+            __shared__ float dl[kDense * kDense];
+            const int nn = C.n * C.n;
+            for (int k = t; k < nn; k += kTailThreads) dl[k] = C.dense[k];
+            __syncthreads();
+            for (int i = t; i < C.n; i += kTailThreads)
+            {
+                float sd = 0.0f;
+                for (int j = 0; j < C.n; ++j) sd += dl[j * C.n + i] * b[j];
+                e[i] = sd;
+            }
             __syncthreads();
             ec = e;
         }
