@@ -40,6 +40,28 @@ def test_default_schedule_is_bitwise_the_single_solve(ctx, blocks):
         assert np.array_equal(a.get(), b.get())
 
 
+def test_batched_blocks_match_oracle_cg(ctx, blocks):
+    """Every block of the batched solve against the oracle's restatement of the
+    reference's solveCG (Matrix.cpp:153-222) on the same CSR system: at
+    tol 1e-13 the iteration counts agree within 2 and the solutions within
+    1e-10 relative (summation order is the only difference)."""
+    import oracle as O
+    from chuanflow_amd import detail_batch as DB
+    from chuanflow_amd import native as N
+    xb, rb, _, _ = DB.solve_batched(ctx, blocks, tol=1e-13, schedule=N.BatchSchedule(4, 2048, 1, 16))
+    for k, (b, x, r) in enumerate(zip(blocks, xb, rb)):
+        rp, cl, vals, rhs = b["host"]
+        A = O.Csr(b["n"], len(cl))
+        A.rowStart[:], A.cols[:], A.vals[:] = rp, cl, vals
+        xo, ro = O.solve_cg(A, rhs, None, 1e-13, 20000)
+        xg = x.get()
+        err = np.abs(xg - xo).max() / np.abs(xo).max()
+        print(f"block {k}: {b['n']} cells, its gpu {r.iterations} oracle {ro.iterations}, rel err {err:.2e}",
+              flush=True)
+        assert r.converged and ro.converged and abs(r.iterations - ro.iterations) <= 2
+        assert err <= 1e-10
+
+
 def test_autotuned_schedule_agrees(ctx, blocks):
     from chuanflow_amd import detail_batch as DB
     from chuanflow_amd import native as N
