@@ -587,7 +587,7 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--mg-scale", type=float, default=1.8)
     ap.add_argument("--kcycle-depth", type=int, default=2)
-    ap.add_argument("--mg-fusion", type=int, default=1)
+    ap.add_argument("--mg-fusion", type=int, default=29)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c5", "spmv"],
                     help="c2: the 128^3 pressure system (the headline metric); c1: SIMPLE channel 40x20x20; "
                          "c3: PISO cavity on tets; "

@@ -53,9 +53,15 @@ struct MgParams
     float coarsestOmega = 2.0f / 3.0f;
     int maxLevels = 10, coarseMax = 128;
     int kcycleDepth = 2;              // coarse levels 1..kcycleDepth use the K-cycle (0: V-cycle)
-    int fusion = 1;                   // bit 0: the small coarse levels' cycle in one workgroup;
+    int fusion = 29;                  // bit 0: the small coarse levels' cycle in one workgroup;
                                       // bit 1: prolongation (and K-cycle combination) fused into the
-                                      // post-smoothing of coarse levels of at most 65536 rows
+                                      // post-smoothing of coarse levels of at most 65536 rows;
+                                      // bit 2: that cycle (the tail) run by every workgroup of the
+                                      // post-smoothing of the level above it, from LDS;
+                                      // bit 3: the K-cycle's r2 = b - coef v1 formed in the second
+                                      // visit's pre-smoothing;
+                                      // bit 4: the K-cycle combination of a level of at most 256
+                                      // workgroups formed in the prolongation to the level above
 };
 
 // One level of the single-workgroup coarse tail (device pointers).
@@ -77,6 +83,26 @@ struct TailArgs
     int sweeps = 0;   // coarsest Jacobi sweeps and weight (no dense inverse)
     float omega = 0.0f;
     TailLevel L[kTailLevels];
+};
+// The arrays of a tail level, as indexed in the tail's pointer table.
+enum TailArray
+{
+    kTRowPtr, kTCol, kTValS, kTVal, kTWPre, kTWPost, kTSc, kTInvD, kTAgg, kTRRowPtr, kTRCol, kTRval, kTDense,
+    kTailArrays
+};
+// Which tail arrays a launch copies into LDS first (one coalesced pass over
+// all of them, many loads in flight), and where: word offset per
+// (level, array), -1 = read from global memory.
+constexpr int kTailUpArena = 36864;  // LDS words of the fused tail: its vectors and the staged arrays
+struct TailStage
+{
+    int chunks = 0;
+    int total4 = 0;                           // float4 slots over all chunks
+    int start4[kTailLevels * kTailArrays + 1];  // first float4 slot of each chunk
+    int words[kTailLevels * kTailArrays];
+    int dst[kTailLevels * kTailArrays];       // LDS word offset (a multiple of 4)
+    const void* src[kTailLevels * kTailArrays];
+    int slot[kTailLevels][kTailArrays];
 };
 
 struct MgLevel
@@ -157,15 +183,22 @@ private:
     int tailFrom_ = -1;
     std::vector<MgLevel> lv_;  // lv_[0] uses the fine pattern (pat unused)
     // out = cycle of level l applied to in (in, out distinct from the level's work vectors)
+    // kcB/kcV non-null: the K-cycle's second visit, whose right-hand side in = kcB - coef kcV is formed
+    // by the pre-smoothing kernel (fusion bit 3) and stored to `in`
     void visit(int l, const float* in, float* out, const float* q, double* partZQ, int Gs, int rbPerWg,
-               const int* done);
+               const int* done, const float* kcB = nullptr, const float* kcV = nullptr);
     // coarse correction of level l >= 1: L.e from L.b.  With deferCombine a
     // K-cycle level leaves e = k1 c1 + k2 c2 unformed (its partials hold the
     // coefficients) and returns true
     bool solveLevel(int l, const int* done, bool deferCombine = false);
     // the same for l >= tailLevel(), below the K-cycle levels: one single-workgroup launch
     bool useTail(int l) const;
+    bool tailAt(int l) const;  // levels l..last make a tail (whatever the fusion bits)
+    TailArgs tailArgs(int l) const;
     void launchTail(int l, const int* done);
+    // fusion bit 2: the tail below level l fused into l's post-smoothing; false where it does not apply
+    bool launchTailUp(int l, const float* in, float* out, const float* q, double* partZQ, int Gs, int rbPerWg,
+                      const int* done);
     DevBuf tv_[14];          // finest-level FP32 scratch of lossGrad / smoothSample
     DevBuf tpart_;
 };

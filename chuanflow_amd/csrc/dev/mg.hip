@@ -18,6 +18,28 @@ constexpr int kCoarseThreads = 1024;
 constexpr int kCoarseMax = 2048;
 constexpr int kTailThreads = 1024;
 constexpr int kFuseRowsMax = 65536;  // largest level whose prolongation is fused into post-smoothing
+constexpr int kDeferGroupsMax = 256; // largest K-cycle grid whose combination may move into the prolongation
+
+// the global address of array k of tail level T (null where the level has none)
+__host__ __device__ inline const void* tailGlobal(const TailLevel& T, int k)
+{
+    switch (k)
+    {
+    case kTRowPtr: return T.rowPtr;
+    case kTCol: return T.col;
+    case kTValS: return T.valS;
+    case kTVal: return T.val;
+    case kTWPre: return T.wPre;
+    case kTWPost: return T.wPost;
+    case kTSc: return T.sc;
+    case kTInvD: return T.invD;
+    case kTAgg: return T.agg;
+    case kTRRowPtr: return T.rRowPtr;
+    case kTRCol: return T.rCol;
+    case kTRval: return T.rval;
+    default: return T.dense;
+    }
+}
 
 // Greedy aggregation of a symmetric pattern: (1) a row whose whole
 // neighbourhood is free becomes a root and takes it; (2) free rows join the
@@ -210,16 +232,18 @@ __global__ __launch_bounds__(B) void k_mg_up(CsrView A, int rbPerWg, const float
     }
 }
 
-// sum over k in [k0, k1) of val[k] * x[col[k]], added in slot order; the
-// global loads go out 8 at a time (x is in LDS), so a row costs one or two
-// memory round trips instead of one per nonzero
+// sum over k in [k0, k1) of val[k] * x[col[k]], added in slot order.  The
+// loads go out 8 at a time: the x gathers are issued unconditionally
+// (padding lanes read x[0]) and only the accumulation is predicated, by a
+// select, so the compiler cannot sink each gather behind its own wait; the
+// sum is the plain loop's, bit for bit.
 __device__ __forceinline__ float rowDot8(const float* __restrict__ val, const int* __restrict__ col,
                                          const float* x, int k0, int k1)
 {
     float s = 0.0f;
     for (int k = k0; k < k1; k += 8)
     {
-        float v[8];
+        float v[8], xv[8];
         int c[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u)
@@ -229,10 +253,12 @@ __device__ __forceinline__ float rowDot8(const float* __restrict__ val, const in
             c[u] = ok ? col[k + u] : 0;
         }
 #pragma unroll
+        for (int u = 0; u < 8; ++u) xv[u] = x[c[u]];
+#pragma unroll
         for (int u = 0; u < 8; ++u)
         {
-            const float xv = x[c[u]];
-            if (k + u < k1) s += v[u] * xv;
+            const float t = s + v[u] * xv[u];
+            s = k + u < k1 ? t : s;
         }
     }
     return s;
@@ -247,11 +273,14 @@ __device__ __forceinline__ float colDot8(const float* __restrict__ m, const floa
         float v[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) v[u] = j + u < n ? m[(j + u) * n + i] : 0.0f;
+        float xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xv[u] = x[j + u < n ? j + u : 0];
 #pragma unroll
         for (int u = 0; u < 8; ++u)
         {
-            const float xv = x[j + u < n ? j + u : 0];
-            if (j + u < n) s += v[u] * xv;
+            const float t = s + v[u] * xv[u];
+            s = j + u < n ? t : s;
         }
     }
     return s;
@@ -376,17 +405,19 @@ __global__ __launch_bounds__(kDense) void k_mg_dense_apply(int n, const float* _
 }
 
 // ------------------------------------------------------------ coarse tail
-// The cycle of levels lt..last in ONE workgroup, vectors in LDS: the same
-// operations in the same order as k_mg_down, k_mg_restrict_csr, the coarsest
-// solve, k_mg_prolong and k_mg_up (row sums in slot order, member sums in
-// ascending member order, dense products in column order), so the result is
-// bit for bit that of the separate kernels; only the launches go.
-__global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const float* __restrict__ bIn,
-                                                          float* __restrict__ eOut, const int* done)
+// The cycle of the tail levels (0 = the tail's first level) in ONE workgroup,
+// vectors in LDS: the same operations in the same order as k_mg_down,
+// k_mg_restrict_csr, the coarsest solve, k_mg_prolong and k_mg_up (row sums
+// in slot order, member sums in ascending member order, dense products in
+// column order), so the result is bit for bit that of the separate kernels.
+// The level arrays come from the accessor (acc.get<T>(level, array): global
+// memory, or LDS at a known offset so the compiler issues LDS reads); v
+// holds b of tail level 0 at entry; the correction of level 0 is written to
+// eOut.
+template <typename Acc>
+__device__ __forceinline__ void tailCycle(const TailArgs& a, const Acc& acc, float* v, float* eOut)
 {
-    __shared__ float v[kTailLds];
-    if (done && *done) return;
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, nt = blockDim.x;
     const int last = a.count - 1;
     // level m: b at off[m], x at +n, r (later z) at +2n; coarsest: b, e (and the Jacobi ping-pong buffer)
     int off[kTailLevels];
@@ -398,40 +429,41 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const floa
             o += 3 * a.L[m].n;
         }
     }
-    for (int i = t; i < a.L[0].n; i += kTailThreads) v[off[0] + i] = bIn[i];
-    __syncthreads();
     for (int m = 0; m < last; ++m)
     {
-        const TailLevel& L = a.L[m];
-        float *b = v + off[m], *x = b + L.n, *r = x + L.n;
-        for (int i = t; i < L.n; i += kTailThreads)
+        const int n = a.L[m].n, nc = a.L[m + 1].n;
+        const int* rowPtr = acc.template get<int>(m, kTRowPtr);
+        const int* col = acc.template get<int>(m, kTCol);
+        const float* valS = acc.template get<float>(m, kTValS);
+        const float* wPre = acc.template get<float>(m, kTWPre);
+        float *b = v + off[m], *x = b + n, *r = x + n;
+        for (int i = t; i < n; i += nt)
         {
-            const float s = rowDot8(L.valS, L.col, b, L.rowPtr[i], L.rowPtr[i + 1]);
+            const float s = rowDot8(valS, col, b, rowPtr[i], rowPtr[i + 1]);
             const float bi = b[i];
-            x[i] = L.wPre[i] * bi;
+            x[i] = wPre[i] * bi;
             r[i] = bi - s;
         }
         __syncthreads();
+        const int* rRowPtr = acc.template get<int>(m, kTRRowPtr);
+        const int* rCol = acc.template get<int>(m, kTRCol);
+        const float* rval = acc.template get<float>(m, kTRval);
         float* bc = v + off[m + 1];
-        for (int c = t; c < a.L[m + 1].n; c += kTailThreads)
-            bc[c] = rowDot8(L.rval, L.rCol, r, L.rRowPtr[c], L.rRowPtr[c + 1]);
+        for (int c = t; c < nc; c += nt) bc[c] = rowDot8(rval, rCol, r, rRowPtr[c], rRowPtr[c + 1]);
         __syncthreads();
     }
     const float* ec;
     {
-        const TailLevel& C = a.L[last];
-        float *b = v + off[last], *e = b + C.n, *w = e + C.n;
-        if (C.dense)
+        const int n = a.L[last].n;
+        float *b = v + off[last], *e = b + n, *w = e + n;
+        const float* dense = acc.template get<float>(last, kTDense);
+        if (dense)
         {
-            // the inverse in LDS in one coalesced pass, then each row's column-order sum (colDot8's order)
-            __shared__ float dl[kDense * kDense];
-            const int nn = C.n * C.n;
-            for (int k = t; k < nn; k += kTailThreads) dl[k] = C.dense[k];
-            __syncthreads();
-            for (int i = t; i < C.n; i += kTailThreads)
+            // each row's column-order sum (colDot8's order)
+            for (int i = t; i < n; i += nt)
             {
                 float sd = 0.0f;
-                for (int j = 0; j < C.n; ++j) sd += dl[j * C.n + i] * b[j];
+                for (int j = 0; j < n; ++j) sd += dense[j * n + i] * b[j];
                 e[i] = sd;
             }
             __syncthreads();
@@ -440,15 +472,19 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const floa
         else
         {
             // damped Jacobi sweeps from zero (k_mg_coarse)
-            for (int i = t; i < C.n; i += kTailThreads) e[i] = 0.0f;
+            const int* rowPtr = acc.template get<int>(last, kTRowPtr);
+            const int* col = acc.template get<int>(last, kTCol);
+            const float* val = acc.template get<float>(last, kTVal);
+            const float* invD = acc.template get<float>(last, kTInvD);
+            for (int i = t; i < n; i += nt) e[i] = 0.0f;
             __syncthreads();
             float *cur = e, *nxt = w;
             for (int it = 0; it < a.sweeps; ++it)
             {
-                for (int i = t; i < C.n; i += kTailThreads)
+                for (int i = t; i < n; i += nt)
                 {
-                    const float s = rowDot8(C.val, C.col, cur, C.rowPtr[i], C.rowPtr[i + 1]);
-                    nxt[i] = cur[i] + a.omega * C.invD[i] * (b[i] - s);
+                    const float s = rowDot8(val, col, cur, rowPtr[i], rowPtr[i + 1]);
+                    nxt[i] = cur[i] + a.omega * invD[i] * (b[i] - s);
                 }
                 __syncthreads();
                 float* tmp = cur;
@@ -460,18 +496,318 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const floa
     }
     for (int m = last - 1; m >= 0; --m)
     {
-        const TailLevel& L = a.L[m];
-        float *b = v + off[m], *x = b + L.n;
-        float* z = m == 0 ? eOut : x + L.n;
-        for (int i = t; i < L.n; i += kTailThreads) x[i] += L.sc[i] * ec[L.agg[i]];
+        const int n = a.L[m].n;
+        const int* rowPtr = acc.template get<int>(m, kTRowPtr);
+        const int* col = acc.template get<int>(m, kTCol);
+        const float* val = acc.template get<float>(m, kTVal);
+        const float* wPost = acc.template get<float>(m, kTWPost);
+        const float* sc = acc.template get<float>(m, kTSc);
+        const int* agg = acc.template get<int>(m, kTAgg);
+        float *b = v + off[m], *x = b + n;
+        float* z = m == 0 ? eOut : x + n;
+        for (int i = t; i < n; i += nt) x[i] += sc[i] * ec[agg[i]];
         __syncthreads();
-        for (int i = t; i < L.n; i += kTailThreads)
+        for (int i = t; i < n; i += nt)
         {
-            const float s = rowDot8(L.val, L.col, x, L.rowPtr[i], L.rowPtr[i + 1]);
-            z[i] = x[i] + L.wPost[i] * (b[i] - s);
+            const float s = rowDot8(val, col, x, rowPtr[i], rowPtr[i + 1]);
+            z[i] = x[i] + wPost[i] * (b[i] - s);
         }
         __syncthreads();
         ec = z;
+    }
+}
+
+// tail arrays in global memory, the coarsest inverse in LDS
+struct TailGlobalAcc
+{
+    const TailArgs& a;
+    const float* dense;
+    template <typename T>
+    __device__ __forceinline__ const T* get(int m, int k) const
+    {
+        if (k == kTDense) return a.L[m].dense ? reinterpret_cast<const T*>(dense) : nullptr;
+        return static_cast<const T*>(tailGlobal(a.L[m], k));
+    }
+};
+
+// tail arrays staged in LDS at the word offsets slot[level][array]
+struct TailLdsAcc
+{
+    const float* arena;
+    const TailStage& st;
+    template <typename T>
+    __device__ __forceinline__ const T* get(int m, int k) const
+    {
+        // every array the cycle reads is staged; the coarsest inverse exists only where it was formed
+        return st.slot[m][k] >= 0 ? reinterpret_cast<const T*>(arena + st.slot[m][k]) : nullptr;
+    }
+};
+
+__global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const float* __restrict__ bIn,
+                                                          float* __restrict__ eOut, const int* done)
+{
+    __shared__ float v[kTailLds];
+    __shared__ float dl[kDense * kDense];
+    if (done && *done) return;
+    const int t = threadIdx.x;
+    const int last = a.count - 1;
+    for (int i = t; i < a.L[0].n; i += kTailThreads) v[i] = bIn[i];
+    // the coarsest inverse in LDS in one coalesced pass
+    if (a.L[last].dense)
+    {
+        const int nn = a.L[last].n * a.L[last].n;
+        for (int k = t; k < nn; k += kTailThreads) dl[k] = a.L[last].dense[k];
+    }
+    __syncthreads();
+    tailCycle(a, TailGlobalAcc{a, dl}, v, eOut);
+}
+
+// Copies the staged tail arrays into LDS.  The chunk table is read into LDS
+// first (one round trip); then the float4 slots of all chunks form one index
+// space, eight slots in flight per lane (a chunk's last slot may be partial
+// and is read word by word).
+__device__ __forceinline__ void stageTail(const TailStage& st, float* arena, int* cs4, int* cw, int* cd,
+                                          const float** csrc)
+{
+    const int t = threadIdx.x, nt = blockDim.x;
+    if (t <= st.chunks) cs4[t] = st.start4[t];
+    if (t < st.chunks)
+    {
+        cw[t] = st.words[t];
+        cd[t] = st.dst[t];
+        csrc[t] = static_cast<const float*>(st.src[t]);
+    }
+    __syncthreads();
+    int c = 0;
+    for (int base = t; base < st.total4; base += 8 * nt)
+    {
+        float4 val[8];
+        int dst[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+        {
+            const int s = base + u * nt;
+            dst[u] = -1;
+            if (s < st.total4)
+            {
+                while (s >= cs4[c + 1]) ++c;
+                const int j = s - cs4[c];
+                const int w = cw[c];
+                const float* src = csrc[c];
+                if (4 * j + 3 < w)
+                    val[u] = reinterpret_cast<const float4*>(src)[j];
+                else
+                {
+                    val[u].x = src[4 * j];
+                    val[u].y = 4 * j + 1 < w ? src[4 * j + 1] : 0.0f;
+                    val[u].z = 4 * j + 2 < w ? src[4 * j + 2] : 0.0f;
+                    val[u].w = 0.0f;
+                }
+                dst[u] = cd[c] + 4 * j;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (dst[u] >= 0) *reinterpret_cast<float4*>(arena + dst[u]) = val[u];
+    }
+}
+
+// sum over k in [k0, k1) of val[k] * x[col[k]] in slot order (rowDot8's
+// sums), the loads of 32 nonzeros in flight together; unit: val is all ones
+// (the product is then x itself, exactly)
+template <bool UNIT>
+__device__ __forceinline__ float rowDot32(const float* __restrict__ val, const int* __restrict__ col,
+                                          const float* __restrict__ x, int k0, int k1)
+{
+    float s = 0.0f;
+    for (int k = k0; k < k1; k += 32)
+    {
+        float v[32], xv[32];
+        int c[32];
+#pragma unroll
+        for (int u = 0; u < 32; ++u)
+        {
+            const bool ok = k + u < k1;
+            c[u] = ok ? col[k + u] : 0;
+            v[u] = UNIT ? 1.0f : (ok ? val[k + u] : 0.0f);
+        }
+#pragma unroll
+        for (int u = 0; u < 32; ++u) xv[u] = x[c[u]];
+#pragma unroll
+        for (int u = 0; u < 32; ++u)
+        {
+            const float t = s + (UNIT ? xv[u] : v[u] * xv[u]);
+            s = k + u < k1 ? t : s;
+        }
+    }
+    return s;
+}
+
+// Copies two arrays (wa and wb words) into LDS as one index space of float4
+// slots, eight in flight per lane; a last partial slot is read word by word.
+__device__ __forceinline__ void copyTwo(float* da, const float* __restrict__ sa, int wa, float* db,
+                                        const float* __restrict__ sb, int wb)
+{
+    const int nt = blockDim.x;
+    const int a4 = (wa + 3) / 4, total = a4 + (wb + 3) / 4;
+    for (int base = threadIdx.x; base < total; base += 8 * nt)
+    {
+        float4 v[8];
+        float* d[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+        {
+            const int s = base + u * nt;
+            d[u] = nullptr;
+            if (s < total)
+            {
+                const bool first = s < a4;
+                const int j = first ? s : s - a4;
+                const int w = first ? wa : wb;
+                const float* src = first ? sa : sb;
+                if (4 * j + 3 < w)
+                    v[u] = reinterpret_cast<const float4*>(src)[j];
+                else
+                {
+                    v[u].x = src[4 * j];
+                    v[u].y = 4 * j + 1 < w ? src[4 * j + 1] : 0.0f;
+                    v[u].z = 4 * j + 2 < w ? src[4 * j + 2] : 0.0f;
+                    v[u].w = 0.0f;
+                }
+                d[u] = (first ? da : db) + 4 * j;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (d[u]) *reinterpret_cast<float4*>(d[u]) = v[u];
+    }
+}
+
+// The tail fused into the post-smoothing of the level above it (level l):
+// every workgroup copies l's residual and the transfer's member list into
+// LDS, restricts into the tail, stages the tail's arrays over them, runs the
+// tail cycle on its own from LDS, forms the prolongated iterate
+// xp = x + S P e of all of l's rows in LDS, then post-smooths four row blocks
+// (one per 256 threads) the CSR-stream way: products staged in LDS from
+// coalesced value and column loads, each row summed in slot order.  Every
+// value is the float expression of k_mg_restrict_csr, the tail,
+// k_mg_prolong and k_mg_up in their summation orders, so z is bit for bit
+// theirs; the launches and the tail's chains of dependent global loads go.
+constexpr int kTailUpThreads = 1024;
+constexpr int kTailUpBlocks = kTailUpThreads / B;  // row blocks per workgroup
+__global__ __launch_bounds__(kTailUpThreads) void k_mg_tail_up(
+    CsrView A, const float* __restrict__ val32, const float* __restrict__ x, const float* __restrict__ b,
+    const float* __restrict__ wD, const int* __restrict__ agg, const float* __restrict__ sc,
+    const float* __restrict__ r, const int* __restrict__ rRowPtr, const int* __restrict__ rCol,
+    const float* __restrict__ rval, int unitRestrict, TailArgs ta, TailStage st, int rOff, int cOff, int xpOff,
+    int prodOff, float* __restrict__ z, const int* done)
+{
+    __shared__ float arena[kTailUpArena];
+    __shared__ int cs4[kTailLevels * kTailArrays + 1], cw[kTailLevels * kTailArrays], cd[kTailLevels * kTailArrays];
+    __shared__ const float* csrc[kTailLevels * kTailArrays];
+    if (done && *done) return;
+    const int t = threadIdx.x;
+    const int n = A.n, n0 = ta.L[0].n;
+    // l's residual and the members of every coarse row
+    float* rl = arena + rOff;
+    int* cl = reinterpret_cast<int*>(arena + cOff);
+    copyTwo(rl, r, n, reinterpret_cast<float*>(cl), reinterpret_cast<const float*>(rCol), n);
+    __syncthreads();
+    // b of the tail's first level = P^T (T r), every row, members in ascending order (k_mg_restrict_csr's sums)
+    for (int c = t; c < n0; c += kTailUpThreads)
+    {
+        const int k0 = rRowPtr[c], k1 = rRowPtr[c + 1];
+        float sum = 0.0f;
+        for (int k = k0; k < k1; k += 8)
+        {
+            int m[8];
+            float v[8], w[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+            {
+                m[u] = k + u < k1 ? cl[k + u] : 0;
+                w[u] = unitRestrict || k + u >= k1 ? 1.0f : rval[k + u];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = rl[m[u]];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+            {
+                const float t2 = sum + (unitRestrict ? v[u] : w[u] * v[u]);
+                sum = k + u < k1 ? t2 : sum;
+            }
+        }
+        arena[c] = sum;
+    }
+    __syncthreads();
+    stageTail(st, arena, cs4, cw, cd, csrc);
+    __syncthreads();
+    float* e = arena + 2 * n0;  // the tail's level-0 residual slot, free once restricted
+    tailCycle(ta, TailLdsAcc{arena, st}, arena, e);
+    // xp of every row of level l, over the staged arrays (no longer read)
+    float* xp = arena + xpOff;
+    for (int j0 = t; j0 < n; j0 += 16 * kTailUpThreads)
+    {
+        float xv[16], sv[16];
+        int av[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+        {
+            const int j = j0 + u * kTailUpThreads;
+            const bool ok = j < n;
+            xv[u] = ok ? x[j] : 0.0f;
+            sv[u] = ok ? sc[j] : 0.0f;
+            av[u] = ok ? agg[j] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+        {
+            const int j = j0 + u * kTailUpThreads;
+            if (j < n) xp[j] = xv[u] + sv[u] * e[av[u]];
+        }
+    }
+    __syncthreads();
+    // post-smoothing of this workgroup's row blocks, one per 256 threads
+    const int q = t / B, lt = t % B;
+    const int blk = blockIdx.x * kTailUpBlocks + q;
+    float* lq = arena + prodOff + q * kSpmvLds;
+    int r0 = 0, r1 = 0, k0 = 0, nnz = 0;
+    if (blk < A.nRowBlocks)
+    {
+        r0 = A.rowBlocks[blk];
+        r1 = A.rowBlocks[blk + 1];
+        k0 = A.rowPtr[r0];
+        nnz = A.rowPtr[r1] - k0;
+    }
+    const bool staged = nnz <= kSpmvLds;
+    if (staged)
+    {
+        int k = lt;
+        for (; k + 3 * B < nnz; k += 4 * B)
+        {
+            float v[4];
+            int c[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+            {
+                v[u] = val32[k0 + k + u * B];
+                c[u] = A.col[k0 + k + u * B];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) lq[k + u * B] = v[u] * xp[c[u]];
+        }
+        for (; k < nnz; k += B) lq[k] = val32[k0 + k] * xp[A.col[k0 + k]];
+    }
+    __syncthreads();
+    const int row = r0 + lt;
+    if (row < r1)
+    {
+        float s = 0.0f;
+        if (staged)
+            for (int k = A.rowPtr[row] - k0, ke = A.rowPtr[row + 1] - k0; k < ke; ++k) s += lq[k];
+        else
+            for (int k = A.rowPtr[row]; k < A.rowPtr[row + 1]; ++k) s += val32[k] * xp[A.col[k]];
+        z[row] = xp[row] + wD[row] * (b[row] - s);
     }
 }
 
@@ -613,6 +949,96 @@ __global__ __launch_bounds__(B) void k_kc_combine(int n, const float* __restrict
     float k1, k2;
     kcCoefficients(part, G, red, k1, k2);
     for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) e[i] = k1 * c1[i] + k2 * c2[i];
+}
+
+// Pre-smoothing of a K-cycle's second visit with its right-hand side formed
+// where it is read: r2 = b - coef v1, coef = c1.b / c1.v1 from the partials
+// (every workgroup reduces them in k_kc_axpy's order), so x = W r2 and
+// r = r2 - (A W) r2 are k_kc_axpy + k_mg_down bit for bit; r2 of the
+// workgroup's rows is stored for the rest of the visit.
+__global__ __launch_bounds__(B) void k_mg_down_kc(CsrView A, const float* __restrict__ valS,
+                                                  const float* __restrict__ b, const float* __restrict__ v1,
+                                                  const double* __restrict__ part, int G,
+                                                  const float* __restrict__ wD, float* __restrict__ r2,
+                                                  float* __restrict__ x, float* __restrict__ r, const int* done)
+{
+    __shared__ float lds[kSpmvLds];
+    __shared__ double red[B / kWave + 1];
+    if (done && *done) return;
+    const double rho = redAll(part, G, red);
+    const double a1 = redAll(part + Multigrid::kPartMax, G, red);
+    const float coef = static_cast<float>(rho > 0.0 ? a1 / rho : 0.0);
+    const int blk = xcdRemap(blockIdx.x, gridDim.x);
+    const int r0 = A.rowBlocks[blk], r1 = A.rowBlocks[blk + 1];
+    const int k0 = A.rowPtr[r0], nnz = A.rowPtr[r1] - k0;
+    const int row = r0 + static_cast<int>(threadIdx.x);
+    float s = 0.0f;
+    if (nnz <= kSpmvLds)
+    {
+        int k = threadIdx.x;
+        for (; k + 3 * B < nnz; k += 4 * B)
+        {
+            float v[4], bb[4], vv[4];
+            int c[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+            {
+                v[u] = valS[k0 + k + u * B];
+                c[u] = A.col[k0 + k + u * B];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+            {
+                bb[u] = b[c[u]];
+                vv[u] = v1[c[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) lds[k + u * B] = v[u] * (bb[u] - coef * vv[u]);
+        }
+        for (; k < nnz; k += B)
+        {
+            const int c = A.col[k0 + k];
+            lds[k] = valS[k0 + k] * (b[c] - coef * v1[c]);
+        }
+        __syncthreads();
+        if (row < r1)
+            for (int k = A.rowPtr[row] - k0, e = A.rowPtr[row + 1] - k0; k < e; ++k) s += lds[k];
+    }
+    else if (row < r1)
+    {
+        for (int k = A.rowPtr[row]; k < A.rowPtr[row + 1]; ++k)
+        {
+            const int c = A.col[k];
+            s += valS[k] * (b[c] - coef * v1[c]);
+        }
+    }
+    if (row < r1)
+    {
+        const float bi = b[row] - coef * v1[row];
+        r2[row] = bi;
+        x[row] = wD[row] * bi;
+        r[row] = bi - s;
+    }
+}
+
+// Prolongation of a deferred K-cycle correction: x += S P (k1 c1 + k2 c2),
+// the coefficients reduced by every workgroup (k_kc_combine + k_mg_prolong
+// bit for bit).
+__global__ __launch_bounds__(B) void k_mg_prolong_kc(int n, const int* __restrict__ agg,
+                                                     const float* __restrict__ sc, const float* __restrict__ c1,
+                                                     const float* __restrict__ c2, const double* __restrict__ kcPart,
+                                                     int kcG, float* __restrict__ x, const int* done)
+{
+    __shared__ double red[B / kWave + 1];
+    if (done && *done) return;
+    float k1, k2;
+    kcCoefficients(kcPart, kcG, red, k1, k2);
+    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B)
+    {
+        const int a = agg[i];
+        const float e = k1 * c1[a] + k2 * c2[a];
+        x[i] += sc[i] * e;
+    }
 }
 
 // Prolongation fused into the post-smoothing of a coarse level: with
@@ -970,12 +1396,14 @@ void Multigrid::setMatrix(const double* fineVal64, const float* pre, const float
     CFX_CHECK_LAUNCH();
 }
 
-bool Multigrid::useTail(int l) const
+bool Multigrid::tailAt(int l) const
 {
-    return (p_.fusion & 1) && tailFrom_ > 0 && l >= tailFrom_ && l > p_.kcycleDepth && l < levels() - 1;
+    return tailFrom_ > 0 && l >= tailFrom_ && l > p_.kcycleDepth && l < levels() - 1;
 }
 
-void Multigrid::launchTail(int l, const int* done)
+bool Multigrid::useTail(int l) const { return (p_.fusion & 1) && tailAt(l); }
+
+TailArgs Multigrid::tailArgs(int l) const
 {
     const int last = levels() - 1;
     TailArgs a;
@@ -1007,7 +1435,79 @@ void Multigrid::launchTail(int l, const int* done)
         else
             T.dense = L.dense.p ? L.dense.as<float>() : nullptr;
     }
-    k_mg_tail<<<1, kTailThreads, 0, ctx_.stream()>>>(a, lv_[l].b.as<float>(), lv_[l].e.as<float>(), done);
+    return a;
+}
+
+void Multigrid::launchTail(int l, const int* done)
+{
+    k_mg_tail<<<1, kTailThreads, 0, ctx_.stream()>>>(tailArgs(l), lv_[l].b.as<float>(), lv_[l].e.as<float>(), done);
+}
+
+bool Multigrid::launchTailUp(int l, const float* in, float* out, const float* q, double* partZQ, int Gs,
+                             int rbPerWg, const int* done)
+{
+    (void)q;
+    (void)Gs;
+    (void)rbPerWg;
+    // z.q partials follow the grid of the separate post-smoothing: with them the separate kernels run
+    if (!(p_.fusion & 4) || partZQ || !tailAt(l + 1)) return false;
+    const MgLevel& L = lv_[l];
+    const CsrView A = l == 0 ? fine_.view() : L.pat.view();
+    // every workgroup repeats the tail: only where the level takes a single wave of workgroups
+    const int G = divUp(A.nRowBlocks, kTailUpBlocks);
+    if (G > 256) return false;
+    const TailArgs ta = tailArgs(l + 1);
+    const int last = ta.count - 1;
+    int vec = 0;
+    for (int m = 0; m < ta.count; ++m) vec += 3 * ta.L[m].n;
+    auto up4 = [](int w) { return (w + 3) / 4 * 4; };
+    // LDS: the tail's vectors, then l's residual and member list (restriction), later the tail's staged
+    // arrays, later l's prolongated iterate and the products of four row blocks
+    const int rOff = up4(vec), cOff = rOff + up4(L.n);
+    const int xpOff = up4(vec), prodOff = xpOff + up4(L.n);
+    if (cOff + up4(L.n) > kTailUpArena || prodOff + kTailUpBlocks * kSpmvLds > kTailUpArena) return false;
+    TailStage st;
+    for (auto& row : st.slot)
+        for (int& v : row) v = -1;
+    int off = up4(vec);
+    st.start4[0] = 0;
+    for (int m = 0; m < ta.count; ++m)
+    {
+        const TailLevel& T = ta.L[m];
+        const int n = T.n, nnz = lv_[l + 1 + m].nnz, nc = m < last ? ta.L[m + 1].n : 0;
+        struct Want
+        {
+            int k, words;
+        };
+        std::vector<Want> want;
+        if (m < last)
+            want = {{kTRowPtr, n + 1}, {kTCol, nnz}, {kTValS, nnz}, {kTWPre, n}, {kTRRowPtr, nc + 1}, {kTRCol, n},
+                    {kTRval, n}, {kTSc, n}, {kTAgg, n}, {kTVal, nnz}, {kTWPost, n}};
+        else if (T.dense)
+            want = {{kTDense, n * n}};
+        else
+            want = {{kTRowPtr, n + 1}, {kTCol, nnz}, {kTVal, nnz}, {kTInvD, n}};
+        for (const Want& w : want)
+        {
+            const int words4 = (w.words + 3) / 4;
+            if (off + 4 * words4 > kTailUpArena) return false;  // the tail does not fit in LDS
+            const int c = st.chunks++;
+            st.src[c] = tailGlobal(T, w.k);
+            st.words[c] = w.words;
+            st.dst[c] = off;
+            st.start4[c + 1] = st.start4[c] + words4;
+            st.slot[m][w.k] = off;
+            off += 4 * words4;
+        }
+    }
+    st.total4 = st.start4[st.chunks];
+    const CsrView R = L.rpat.view();
+    // off the finest level the restriction scale is 1 (k_restrict_vals): its products are exact
+    k_mg_tail_up<<<G, kTailUpThreads, 0, ctx_.stream()>>>(
+        A, L.val32.as<float>(), L.x.as<float>(), in, L.wPost32.as<float>(), L.agg.as<int>(), L.s32.as<float>(),
+        L.r.as<float>(), R.rowPtr, R.col, L.rval.as<float>(), l > 0 ? 1 : 0, ta, st, rOff, cOff, xpOff, prodOff, out,
+        done);
+    return true;
 }
 
 bool Multigrid::solveLevel(int l, const int* done, bool deferCombine)
@@ -1046,8 +1546,17 @@ bool Multigrid::solveLevel(int l, const int* done, bool deferCombine)
     visit(l, L.b.as<float>(), L.c1.as<float>(), nullptr, nullptr, L.G, L.rbPerWg, done);
     k_kc_spmv1<<<L.G, B, 0, s>>>(A, L.rbPerWg, L.val32.as<float>(), L.c1.as<float>(), L.b.as<float>(),
                                  L.v1.as<float>(), part, done);
-    k_kc_axpy<<<Gv, B, 0, s>>>(L.n, L.b.as<float>(), L.v1.as<float>(), L.r2.as<float>(), part, L.G, done);
-    visit(l, L.r2.as<float>(), L.c2.as<float>(), nullptr, nullptr, L.G, L.rbPerWg, done);
+    if (p_.fusion & 8)
+    {
+        // the second visit's pre-smoothing forms r2 = b - coef v1 itself
+        visit(l, L.r2.as<float>(), L.c2.as<float>(), nullptr, nullptr, L.G, L.rbPerWg, done, L.b.as<float>(),
+              L.v1.as<float>());
+    }
+    else
+    {
+        k_kc_axpy<<<Gv, B, 0, s>>>(L.n, L.b.as<float>(), L.v1.as<float>(), L.r2.as<float>(), part, L.G, done);
+        visit(l, L.r2.as<float>(), L.c2.as<float>(), nullptr, nullptr, L.G, L.rbPerWg, done);
+    }
     k_kc_spmv2<<<L.G, B, 0, s>>>(A, L.rbPerWg, L.val32.as<float>(), L.c2.as<float>(), L.v1.as<float>(),
                                  L.r2.as<float>(), part, done);
     if (deferCombine) return true;  // the caller forms e = k1 c1 + k2 c2 where it reads it
@@ -1056,22 +1565,31 @@ bool Multigrid::solveLevel(int l, const int* done, bool deferCombine)
 }
 
 void Multigrid::visit(int l, const float* in, float* out, const float* q, double* partZQ, int Gs, int rbPerWg,
-                      const int* done)
+                      const int* done, const float* kcB, const float* kcV)
 {
     hipStream_t s = ctx_.stream();
     MgLevel& L = lv_[l];
     const CsrView A = l == 0 ? fine_.view() : L.pat.view();
-    k_mg_down<<<A.nRowBlocks, B, 0, s>>>(A, L.valS32.as<float>(), in, L.wPre32.as<float>(), L.x.as<float>(),
-                                         L.r.as<float>(), done);
+    if (kcB)
+        k_mg_down_kc<<<A.nRowBlocks, B, 0, s>>>(A, L.valS32.as<float>(), kcB, kcV, L.part.as<double>(), L.G,
+                                                L.wPre32.as<float>(), const_cast<float*>(in), L.x.as<float>(),
+                                                L.r.as<float>(), done);
+    else
+        k_mg_down<<<A.nRowBlocks, B, 0, s>>>(A, L.valS32.as<float>(), in, L.wPre32.as<float>(), L.x.as<float>(),
+                                             L.r.as<float>(), done);
     if (l + 1 < levels())
     {
+        if (launchTailUp(l, in, out, q, partZQ, Gs, rbPerWg, done)) return;
         MgLevel& C = lv_[l + 1];
         k_mg_restrict_csr<<<L.rpat.nRowBlocks, B, 0, s>>>(L.rpat.view(), L.rval.as<float>(), L.r.as<float>(),
                                                           C.b.as<float>(), done);
         // fused only where the level is small: on a large level forming S P e per nonzero costs more
         // gathers than the separate prolongation saves
         const bool fuse = (p_.fusion & 2) && l >= 1 && L.n <= kFuseRowsMax;
-        const bool deferred = solveLevel(l + 1, done, fuse);
+        // a K-cycle below may leave its combination to the prolongation (its coefficients are reduced by
+        // every workgroup there, so only for a level of few workgroups)
+        const bool deferKc = fuse || ((p_.fusion & 16) && C.G <= kDeferGroupsMax);
+        const bool deferred = solveLevel(l + 1, done, deferKc);
         if (fuse)
         {
             k_mg_prolong_up<<<Gs, B, 0, s>>>(A, rbPerWg, L.val32.as<float>(), L.x.as<float>(), in,
@@ -1080,8 +1598,13 @@ void Multigrid::visit(int l, const float* in, float* out, const float* q, double
                                              deferred ? C.part.as<double>() : nullptr, C.G, out, done);
             return;
         }
-        k_mg_prolong<<<divUp(L.n, B), B, 0, s>>>(L.n, L.agg.as<int>(), L.s32.as<float>(), C.e.as<float>(),
-                                                  L.x.as<float>(), done);
+        if (deferred)
+            k_mg_prolong_kc<<<std::max(1, std::min(1024, divUp(L.n, B))), B, 0, s>>>(
+                L.n, L.agg.as<int>(), L.s32.as<float>(), C.c1.as<float>(), C.c2.as<float>(), C.part.as<double>(), C.G,
+                L.x.as<float>(), done);
+        else
+            k_mg_prolong<<<divUp(L.n, B), B, 0, s>>>(L.n, L.agg.as<int>(), L.s32.as<float>(), C.e.as<float>(),
+                                                      L.x.as<float>(), done);
     }
     k_mg_up<<<Gs, B, 0, s>>>(A, rbPerWg, L.val32.as<float>(), L.x.as<float>(), in, L.wPost32.as<float>(), out, q,
                              partZQ, done);

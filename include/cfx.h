@@ -223,8 +223,14 @@ typedef struct cfx_mg_params {
     int fusion;                         /* bit 0: the cycle of the small coarsest levels (below the K-cycle
                                            levels) in one workgroup launch; bit 1: prolongation and K-cycle
                                            combination fused into the post-smoothing of coarse levels up to
-                                           65536 rows (default 1: bit 1 measured 1 % slower at 128^3); results
-                                           are bit for bit those of fusion 0 */
+                                           65536 rows; bit 2: that small-level cycle run from LDS by every
+                                           workgroup of the post-smoothing of the level above it (no z.q
+                                           partials there); bit 3: the K-cycle's second right-hand side
+                                           formed inside the pre-smoothing; bit 4: the K-cycle combination of
+                                           a level of at most 256 workgroups formed inside the prolongation
+                                           (default 29 = bits 0, 2, 3, 4: 13.5 against 14.5 ms for bit 0
+                                           alone at 128^3; bit 1 measured slower); results are bit for bit
+                                           those of fusion 0 */
 } cfx_mg_params;
 int cfx_mg_default_params(cfx_mg_params* p);
 /* flexible CG replays one captured iteration per step (hipGraph, default 1)

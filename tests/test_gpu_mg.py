@@ -176,10 +176,13 @@ def test_graph_replay_equals_eager_launches(ctx, systems):
 @pytest.mark.parametrize("case", ["lap40-k1", "fv24n-k0", "lap30-jacobi", "fv32-k2"])
 def test_fused_kernels_equal_separate_launches(ctx, systems, case):
     """Launch fusion changes no bit: the small coarsest levels' cycle in one
-    workgroup (fusion bit 0; dense and Jacobi-swept coarsest level) and the
+    workgroup (fusion bit 0; dense and Jacobi-swept coarsest level), the
     prolongation with the K-cycle combination folded into coarse
-    post-smoothing (bit 1) give the cycle of the separate kernels exactly,
-    and the same flexible-CG iterates."""
+    post-smoothing (bit 1), that cycle repeated from LDS by every workgroup
+    of the post-smoothing above it (bit 2), the K-cycle's r2 formed in the
+    second visit's pre-smoothing (bit 3) and its combination formed in the
+    prolongation (bit 4), alone and together, give the cycle of the separate
+    kernels exactly, and the same flexible-CG iterates."""
     if case == "lap40-k1":
         A, b = systems["lap40"][0], None
         kw = dict(kcycleDepth=1)
@@ -208,7 +211,8 @@ def test_fused_kernels_equal_separate_launches(ctx, systems, case):
         assert rows[-1] > 128  # coarsest level swept, not inverted
     r = ctx.array(np.random.default_rng(3).standard_normal(n).astype(np.float32))
     zs, xs, its = [], [], []
-    for fusion in (0, 1, 2, 3):
+    fusions = (0, 1, 2, 3, 4, 8, 16, 29, 31)
+    for fusion in fusions:
         G.set_mg(scale=1.8, coarseScale=1.8, fusion=fusion, **kw)
         z = ctx.empty(n, np.float32)
         G.mg_apply(r, z)
@@ -217,6 +221,6 @@ def test_fused_kernels_equal_separate_launches(ctx, systems, case):
         assert rep.converged
         xs.append(x)
         its.append(rep.iterations)
-    for k in (1, 2, 3):
-        assert np.array_equal(zs[0], zs[k]), k
-        assert its[0] == its[k] and np.array_equal(xs[0], xs[k]), k
+    for k in range(1, len(fusions)):
+        assert np.array_equal(zs[0], zs[k]), fusions[k]
+        assert its[0] == its[k] and np.array_equal(xs[0], xs[k]), fusions[k]
