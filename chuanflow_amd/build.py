@@ -24,7 +24,7 @@ OBJ_DIR = os.path.join(HERE, "_build")
 LIB_NAME = "libchuanflow_amd.so"
 ARCH = os.environ.get("CFX_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["dev/krylov.hip", "dev/pcg.hip", "dev/mg.hip", "dev/dic.hip", "dev/mpnn.hip", "dev/train.hip", "dev/fvops.hip", "dev/mesh.hip", "dev/simple.hip", "cfx_api.hip"]
+HIP_SOURCES = ["dev/krylov.hip", "dev/pcg.hip", "dev/mg.hip", "dev/dic.hip", "dev/mpnn.hip", "dev/train.hip", "dev/fvops.hip", "dev/mesh.hip", "dev/simple.hip", "dev/noalloc.hip", "cfx_api.hip"]
 CXX_SOURCES = ["cf/PolyMesh.cpp", "cf/DetailBlock.cpp", "cf/MeshIO.cpp", "cfx_host_api.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function",

@@ -7,6 +7,7 @@
 #include "dev/krylov.h"
 #include "dev/mesh.h"
 #include "dev/mg.h"
+#include "dev/noalloc.h"
 #include "dev/simple.h"
 #include "dev/train.h"
 
@@ -878,6 +879,16 @@ int cfx_csr_solve_device(cfx_csr* A, const double* b, double* x, double tol, int
             s = solveOn(A->owner, A->pattern.view(), A->val.as<double>(), A->val32, A->val32Valid, b, x, tol, maxIter,
                         solver, precond, precision, A->innerTol, hist);
         A->lastIters = hist ? s.iterations : -1;
+        toReport(s, rep);
+    });
+}
+
+int cfx_pcg_solve_noalloc(cfx_ctx* c, int n, int nnz, const int* d_rowPtr, const int* d_col, const double* d_val,
+                          const double* d_b, double* d_x, double tol, int maxIter, cfx_report* rep)
+{
+    static_assert(CFX_NOALLOC_MAX_ROWS == kNoAllocMaxRows, "the header's row limit differs from the library's");
+    return guard(c, [&] {
+        const SolveReport s = pcgNoAlloc(*c->ctx, n, nnz, d_rowPtr, d_col, d_val, d_b, d_x, tol, maxIter);
         toReport(s, rep);
     });
 }

@@ -151,6 +151,26 @@ private:
     double lastMs_ = 0;
 };
 
+// Workspace of one Jacobi-PCG run: 4 x 2048 partials, the device state
+// (kPcgStateBytes), the direction and product vectors (n entries of the
+// Krylov type each) and, for FP32 runs with FP64 refinement, the FP64
+// residual.
+constexpr std::size_t kPcgStateBytes = 256;
+constexpr int kPcgPartials = 4 * 2048;
+struct PcgWork
+{
+    double* part;
+    void* state;
+    void* p;
+    void* q;
+    double* r64;
+};
+// FP64 Jacobi-PCG (solveCG, Matrix.cpp:153-222) on a caller's workspace: the
+// kernels and the host loop of Krylov::cg, nothing allocated.  invD holds the
+// CG-guarded inverse diagonal, r is the residual vector (n doubles).
+SolveReport pcgSolveWith(Context& ctx, const CsrView& A, const double* val, const double* invD, const double* b,
+                         double* x, double tol, int maxIter, const PcgWork& w, double* r);
+
 // invD from a CSR diagonal: CG guard |d| > 1e-300 (Matrix.cpp:160-165),
 // BiCGStab guard d > 1e-300 (Matrix.cpp:231-236).
 void computeInvD(Context& ctx, const CsrView& A, const double* val, double* invD, bool bicgGuard);

@@ -1,5 +1,6 @@
 // mesh.hip -- device mesh construction and the execution context.
 #include "mesh.h"
+#include "noalloc.h"
 
 #include <algorithm>
 #include <cmath>
@@ -26,6 +27,7 @@ Context::Context(int device) : device_(device)
     CFX_HIP(hipMemsetAsync(tickets_.p, 0, sizeof(double) * 64, stream_));
     CFX_HIP(hipHostMalloc(reinterpret_cast<void**>(&hostState_), 4 * sizeof(SolveState), hipHostMallocDefault));
     CFX_HIP(hipStreamSynchronize(stream_));
+    noallocSetup(device);  // the no-allocation solve's scratch address on this device (device current)
 }
 
 Context::~Context()

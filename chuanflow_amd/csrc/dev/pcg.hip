@@ -624,6 +624,16 @@ __global__ __launch_bounds__(kB) void k_bpcg_C(const BatchSysDev* __restrict__ S
 
 } // namespace
 
+namespace
+{
+__global__ void k_clear_words(unsigned int* __restrict__ p, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+} // namespace
+
+static_assert(sizeof(CgState) <= kPcgStateBytes, "the PCG state outgrows its workspace slot");
+
 // ------------------------------------------------------------------- host
 struct PcgLaunch
 {
@@ -640,26 +650,28 @@ static PcgLaunch pcgGrid(const CsrView& A, int n)
     return g;
 }
 
+// The Jacobi-PCG driver on an explicit workspace (no allocation here): the
+// state is cleared by a kernel, the host polls pinned state between batches.
 template <typename T>
-SolveReport Krylov::pcgRun(const CsrView& A, const double* val64, const T* valT, const T* invD, const double* b,
-                           double* x64, T* xT, T* r, double tol, int maxIter, double* resHist, bool mixed,
-                           double delta)
+static SolveReport pcgCore(Context& ctx, int n_, const PcgWork& w, int minBatch, int maxBatch, const CsrView& A,
+                           const double* val64, const T* valT, const T* invD, const double* b, double* x64, T* xT,
+                           T* r, double tol, int maxIter, double* resHist, bool mixed, double delta)
 {
-    hipStream_t s = ctx_.stream();
+    hipStream_t s = ctx.stream();
     const PcgLaunch g = pcgGrid(A, n_);
     if (g.Gs > kRedMax || g.Gv > kRedMax) throw std::invalid_argument("pcg: grid exceeds the partial buffer");
-    double* part = partials_.as<double>();
+    double* part = w.part;
     double *partRZ = part, *partRM = part + kRedMax, *partBM = part + 2 * kRedMax, *partPQ = part + 3 * kRedMax;
-    CgState* st = reinterpret_cast<CgState*>(state_.as<char>() + 2 * sizeof(SolveState));
-    CFX_HIP(hipMemsetAsync(st, 0, sizeof(CgState), s));
-    T* p = reinterpret_cast<T*>(mixed ? p32_.p : p_.p);
-    T* q = reinterpret_cast<T*>(mixed ? q32_.p : q_.p);
-    double* r64 = mixed ? r0_.as<double>() : nullptr;
+    CgState* st = static_cast<CgState*>(w.state);
+    k_clear_words<<<1, kB, 0, s>>>(reinterpret_cast<unsigned int*>(st), sizeof(CgState) / 4);
+    T* p = static_cast<T*>(w.p);
+    T* q = static_cast<T*>(w.q);
+    double* r64 = w.r64;
 
     // initial residual (FP64); for FP64 runs r is the residual itself
     k_pcg_residual<T><<<g.Gs, kB, 0, s>>>(A, g.rbPerWg, val64, x64, b, r64, r, invD, partRZ, partRM, partBM);
     CFX_CHECK_LAUNCH();
-    CgState* hs = reinterpret_cast<CgState*>(ctx_.hostState() + 2);
+    CgState* hs = reinterpret_cast<CgState*>(ctx.hostState() + 2);
     auto pollState = [&] {
         CFX_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
         CFX_HIP(hipStreamSynchronize(s));
@@ -725,6 +737,23 @@ SolveReport Krylov::pcgRun(const CsrView& A, const double* val64, const T* valT,
     rep.outerIterations = mixed ? outer : 0;
     if (mixed) rep.trueResidual = hs->converged ? hs->rnorm / hs->bnorm : -1;
     return rep;
+}
+
+template <typename T>
+SolveReport Krylov::pcgRun(const CsrView& A, const double* val64, const T* valT, const T* invD, const double* b,
+                           double* x64, T* xT, T* r, double tol, int maxIter, double* resHist, bool mixed,
+                           double delta)
+{
+    const PcgWork w{partials_.as<double>(), state_.as<char>() + 2 * sizeof(SolveState), mixed ? p32_.p : p_.p,
+                    mixed ? q32_.p : q_.p, mixed ? r0_.as<double>() : nullptr};
+    return pcgCore<T>(ctx_, n_, w, minBatch, maxBatch, A, val64, valT, invD, b, x64, xT, r, tol, maxIter, resHist,
+                      mixed, delta);
+}
+
+SolveReport pcgSolveWith(Context& ctx, const CsrView& A, const double* val, const double* invD, const double* b,
+                         double* x, double tol, int maxIter, const PcgWork& w, double* r)
+{
+    return pcgCore<double>(ctx, A.n, w, 8, 256, A, val, val, invD, b, x, x, r, tol, maxIter, nullptr, false, 0.0);
 }
 
 SolveReport Krylov::cg(const CsrView& A, const double* val, const double* b, double* x, double tol, int maxIter,

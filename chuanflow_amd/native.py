@@ -163,6 +163,8 @@ _sig("cfx_csr_solve", C.c_int, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, C.c_
 _sig("cfx_csr_solve_device", C.c_int, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
      C.POINTER(Report))
 _sig("cfx_csr_residual_history", C.c_int, _vp, _vp, C.c_int)
+_sig("cfx_pcg_solve_noalloc", C.c_int, _vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int,
+     C.POINTER(Report))
 _sig("cfx_csr_set_inner_tol", C.c_int, _vp, C.c_double)
 _sig("cfx_mg_default_params", C.c_int, C.POINTER(MgParams))
 _sig("cfx_set_graphs", C.c_int, _vp, C.c_int)
@@ -924,6 +926,20 @@ class Mpnn:
                                             _p(r0) if r0 is not None else None, C.byref(o), samples,
                                             1 if with_network else 0, C.byref(loss)))
         return loss.value
+
+
+NOALLOC_MAX_ROWS = 524288
+
+
+def pcg_solve_noalloc(ctx: Context, d_rowPtr, d_col, d_val, d_b, d_x, tol=1e-8, maxIter=2000):
+    """cfx_pcg_solve_noalloc: FP64 Jacobi-PCG on a device CSR system (int32 d_rowPtr[n+1], d_col[nnz],
+    FP64 d_val[nnz], d_b[n], d_x[n] = initial guess and result) with the library's static scratch."""
+    n = int(np.prod(d_b.shape))
+    nnz = int(np.prod(d_col.shape))
+    rep = Report()
+    ctx.check(_L.cfx_pcg_solve_noalloc(ctx.h, n, nnz, _p(d_rowPtr), _p(d_col), _p(d_val), _p(d_b), _p(d_x), tol,
+                                       maxIter, C.byref(rep)))
+    return rep
 
 
 def _ptr_array(items):

@@ -194,6 +194,22 @@ int cfx_csr_solve_device(cfx_csr* A, const double* d_b, double* d_x, double tol,
                          int solver, int precond, int precision, cfx_report* rep);
 /* per-iteration ||r||inf of the last FP64 CG solve: hist[0..iterations] */
 int cfx_csr_residual_history(cfx_csr* A, double* hist, int maxLen);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving loops, code run under stream capture, harnesses that
+ * forbid hipMalloc): the FP64 Jacobi-PCG of cfx_csr_solve_device (CG,
+ * CFX_PRECOND_JACOBI, CFX_FP64; solveCG's stop test and guards) on a device
+ * CSR system the caller owns -- int32 d_rowPtr[n+1], d_col[nnz], FP64
+ * d_val[nnz], d_b[n], d_x[n] (initial guess and result) -- with every
+ * temporary in a library-owned static device array of fixed size: nothing is
+ * allocated.  At most CFX_NOALLOC_MAX_ROWS rows (larger: CFX_E_INVALID).
+ * Each GPU holds one such array (about 21 MB) until the process ends; all
+ * calls on a GPU share it, so calls that overlap on one GPU race.  The
+ * result equals cfx_csr_solve_device's to rounding (the row blocks differ,
+ * so dot products are grouped differently); rep->ms is host wall time. */
+#define CFX_NOALLOC_MAX_ROWS 524288
+int cfx_pcg_solve_noalloc(cfx_ctx* ctx, int n, int nnz, const int* d_rowPtr, const int* d_col,
+                          const double* d_val, const double* d_b, double* d_x, double tol, int maxIter,
+                          cfx_report* rep);
 /* CFX_FP32_IR: the factor by which the FP32 recurrence residual must fall
  * below the last true FP64 residual before the next reliable update
  * (default 1e-2) */
