@@ -47,11 +47,29 @@ HBM_PEAK_GBPS = 8000.0
 # HBM bytes per FP64 SpMV launch on the 128^3 system from rocprofv3 PMC (profiles/spmv_128cube_hbm_traffic.md):
 # FETCH_SIZE 100,831.1 KiB / 0.50005 (the copy-kernel calibration) + WRITE_SIZE 16,384 KiB
 SPMV_FP64_TRAFFIC_128 = 220712939
+# HBM bytes per k_fcg_A launch on the 128^3 system, the same way: FETCH_SIZE 105,493.0 KiB / 0.50005 + WRITE_SIZE
+# 8,320.1 KiB (profiles/spmv_128cube_hbm_traffic.md)
+FCG_A_TRAFFIC_128 = 224547574
 SEED_MESH, SEED_SHUFFLE, SEED_RHS = 128, 7, 2024
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def canonical_fcg_bytes(n, nnz):
+    # flexible CG's q = A p: FP64 value + int32 column per nonzero, int32 row pointer, FP32 p read once,
+    # FP32 r read (p.r partial), FP32 q written
+    return 12 * nnz + 4 * (n + 1) + 12 * n
+
+
+def fine_level_bytes(n, nnz):
+    # per FCG iteration on the finest level: k_fcg_A, then pre-smoothing (FP32 column-scaled values + columns,
+    # row pointer, b, w read; x, r written) and post-smoothing (FP32 values + columns, row pointer, x, b, w
+    # read; z written)
+    down = 8 * nnz + 4 * (n + 1) + 16 * n
+    up = 8 * nnz + 4 * (n + 1) + 16 * n
+    return canonical_fcg_bytes(n, nnz) + down + up
 
 
 def canonical_spmv_bytes(n, nnz, s):
@@ -86,7 +104,9 @@ def build_system(N, ctx, n):
     return hm, arrays, geom, dm, A, b, bw_shuffled
 
 
-def time_solve(N, ctx, A, d_x, precision, steps, warmup, tol, maxIter, precond=None):
+def time_solve(N, ctx, A, d_x, precision, steps, warmup, tol, maxIter, precond=None, warm=None):
+    """Solves from x = 0, `warmup` untimed then `steps` timed: [(wall ms, report)]; the warm-up walls are
+    appended to `warm` when given."""
     reps = []
     precond = N.PRECOND_JACOBI if precond is None else precond
     for i in range(warmup + steps):
@@ -98,6 +118,8 @@ def time_solve(N, ctx, A, d_x, precision, steps, warmup, tol, maxIter, precond=N
         wall = (time.perf_counter() - t0) * 1e3
         if i >= warmup:
             reps.append((wall, rep.as_dict()))
+        elif warm is not None:
+            warm.append(wall)
         log(f"  {'warmup' if i < warmup else 'step'} {i}: {wall:.2f} ms wall, {rep.ms:.2f} ms device, "
             f"{rep.iterations} it, outer {rep.outerIterations}, conv {rep.converged}")
     return reps
@@ -548,7 +570,8 @@ def bench_c3(args):
 
 
 def bench_spmv(args):
-    """SpMV alone on the 128^3 pressure system (FP64 and FP32 values, args.spmv_reps launches each) and a
+    """SpMV alone on the 128^3 pressure system (FP64 and FP32 values, and the flexible CG's fine-level
+    k_fcg_A; args.spmv_reps launches each) and a
     device-to-device copy of a known byte count: the workload the HBM traffic counters are read on
     (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE; the copy's counters give the scale to bytes)."""
     from chuanflow_amd import native as N
@@ -561,6 +584,7 @@ def bench_spmv(args):
     xs32 = ctx.array(np.random.default_rng(1).standard_normal(n).astype(np.float32))
     ys32 = ctx.empty(n, np.float32)
     ms32 = A.time_spmv(0, xs32, ys32, args.spmv_reps, N.FP32_IR)
+    msf = A.time_fcg_spmv(args.spmv_reps)
     copy_bytes = 256 << 20
     src, dst = ctx.zeros(copy_bytes // 8), ctx.empty(copy_bytes // 8)
     for _ in range(args.spmv_reps):
@@ -572,6 +596,8 @@ def bench_spmv(args):
                       "spmv_fp64_GBps": round(b64 / (ms64 * 1e-3) / 1e9, 1),
                       "spmv_fp32_us": round(ms32 * 1e3, 2), "spmv_fp32_bytes": b32,
                       "spmv_fp32_GBps": round(b32 / (ms32 * 1e-3) / 1e9, 1),
+                      "fcg_a_us": round(msf * 1e3, 2), "fcg_a_bytes": canonical_fcg_bytes(n, nnz),
+                      "fcg_a_GBps": round(canonical_fcg_bytes(n, nnz) / (msf * 1e-3) / 1e9, 1),
                       "copy_bytes_read": copy_bytes, "copy_bytes_written": copy_bytes}), flush=True)
 
 
@@ -642,7 +668,8 @@ def main():
     xmix = d_x.get()
     log("multigrid-preconditioned flexible CG (FP32 + FP64 refinement), tuned constants")
     A.set_mg(scale=args.mg_scale, coarseScale=args.mg_scale, kcycleDepth=args.kcycle_depth, fusion=args.mg_fusion)
-    rmg = time_solve(N, ctx, A, d_x, N.FP32_IR, args.steps, args.warmup, args.tol, maxIter, N.PRECOND_LEARNED)
+    warm_mg = []
+    rmg = time_solve(N, ctx, A, d_x, N.FP32_IR, args.steps, args.warmup, args.tol, maxIter, N.PRECOND_MG, warm_mg)
     xmg = d_x.get()
     log("learned: network weights on the finest level, flexible CG (FP32 + FP64 refinement)")
     from chuanflow_amd import learned as LN
@@ -651,6 +678,20 @@ def main():
     rln = time_solve(N, ctx, A, d_x, N.FP32_IR, args.steps, args.warmup, args.tol, maxIter, N.PRECOND_LEARNED)
     xln = d_x.get()
 
+    # the learned solve's fine-level kernel (k_fcg_A: FP64 values on FP32 vectors, once per FCG iteration),
+    # timed inside one more learned solve (not in `value`): that solve launches its kernels one by one with
+    # HIP events around every k_fcg_A on the library stream, so each launch sees the caches the solve leaves
+    ctx.set_fcg_probe(True)
+    ctx.check(N.lib().cfx_zero(ctx.h, d_x.ptr, d_x.nbytes))
+    A.solve(d_x, N.SOLVER_CG, args.tol, maxIter, 0, None, None, None, -1, True, N.PRECOND_LEARNED, N.FP32_IR)
+    ctx.sync()
+    ctx.set_fcg_probe(False)
+    ms_fcg, n_fcg = A.fcg_probe()
+    ms_fcg_loop = A.time_fcg_spmv(args.spmv_reps)
+    b_fcg = canonical_fcg_bytes(n, nnz)
+    gbs_fcg = b_fcg / (ms_fcg * 1e-3) / 1e9
+    log(f"k_fcg_A in the solve {ms_fcg*1e3:.1f} us ({n_fcg} launches) = {gbs_fcg:.0f} GB/s (canonical bytes); "
+        f"back to back {ms_fcg_loop*1e3:.1f} us")
     # SpMV roofline: average launch time over reps on the library stream (HIP events)
     xs = ctx.array(np.random.default_rng(1).standard_normal(n))
     ys = ctx.empty(n)
@@ -728,10 +769,26 @@ def main():
         "spmv_fp64_vs_stream_read": round(gbs64 / read_gbs, 3),
         "device_stream_read_spmv_bytes_GBps": round(read_ws_gbs, 1),
         "spmv_fp64_vs_stream_read_same_bytes": round(gbs64 / read_ws_gbs, 3),
-        "roofline": {"bound": "hbm", "achieved": round(gbs64, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(gbs64 / HBM_PEAK_GBPS, 4),
-                     "traffic": SPMV_FP64_TRAFFIC_128 if args.n == 128 else None,
-                     "kernel": "k_spmv<double> (CSR-stream SpMV, FP64)", "bytes_per_launch": b64},
+        "roofline": {"bound": "hbm", "achieved": round(gbs_fcg, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(gbs_fcg / HBM_PEAK_GBPS, 4),
+                     "traffic": FCG_A_TRAFFIC_128 if args.n == 128 else None,
+                     "kernel": "k_fcg_A (the learned solve's fine-level q = A p: FP64 values, FP32 vectors, "
+                               "p.q and p.r partials; once per FCG iteration)",
+                     "bytes_per_launch": b_fcg, "us_per_launch": round(ms_fcg * 1e3, 2),
+                     "launches_timed": n_fcg, "us_back_to_back": round(ms_fcg_loop * 1e3, 2)},
+        "spmv_fp64_roofline": {"achieved": round(gbs64, 1), "frac": round(gbs64 / HBM_PEAK_GBPS, 4),
+                               "traffic": SPMV_FP64_TRAFFIC_128 if args.n == 128 else None,
+                               "kernel": "k_spmv<double> (CSR-stream SpMV, FP64; the plain PCG's product)",
+                               "bytes_per_launch": b64},
+        # one-time cost of a new pattern, not in `value`: the first multigrid solve builds the aggregation
+        # hierarchy on the host and captures the FCG iteration as a graph (later solves on the pattern reuse both)
+        "setup_ms": round(warm_mg[0] - tmg, 1) if warm_mg else None,
+        "first_solve_ms": round(warm_mg[0], 1) if warm_mg else None,
+        # the three fine-level matrix passes of every learned FCG iteration (k_fcg_A, pre- and post-smoothing)
+        # at their canonical bytes, over the whole learned solve's wall time: a floor on what the solve streams
+        "learned_fine_level_GBps": round(
+            (rln[-1][1]["iterations"] + rln[-1][1]["outerIterations"]) * fine_level_bytes(n, nnz) / (tln * 1e-3) / 1e9,
+            1),
     }
     if not args.no_cpu:
         chk = cpu_baseline_and_check(arrays, b, x64, xmix, it64, args.cpu_sample_iters, A, xmg, xln)

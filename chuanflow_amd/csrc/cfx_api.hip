@@ -11,6 +11,7 @@
 #include "dev/simple.h"
 #include "dev/train.h"
 
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -25,6 +26,7 @@ struct cfx_ctx
     std::map<int, std::unique_ptr<Krylov>> krylov;  // workspace per system size
     bool graphs = true;
     bool batchComponents = true;
+    bool probeFcgA = false;
     std::unique_ptr<BatchedPcg> batch;
 
     Krylov& solver(int n, int rowBlocks)
@@ -35,6 +37,7 @@ struct cfx_ctx
         if (!k || k->maxBlocks() < rowBlocks) k = std::make_unique<Krylov>(*ctx, n, std::max(rowBlocks, 1));
         k->useGraphs = graphs;
         k->batchComponents = batchComponents;
+        k->probeFcgA = probeFcgA;
         return *k;
     }
 };
@@ -67,6 +70,8 @@ struct cfx_matrix
     cfx_mesh* mesh = nullptr;
     int ncomp = 1;
     DevBuf diag, rhs, rhsBase, upper, lower, val, val32, tmp;
+    double relaxation = 1.0;     // Schemes::relaxation of this matrix
+    DevBuf relaxOld;             // the solution before the final pass ([ncomp][n])
     MgCache mg;
     cfx_mpnn* net = nullptr;     // learned finest-level weights (precond LEARNED)
     DevBuf geo, wa, wt;          // per-slot face geometry, network outputs
@@ -117,6 +122,12 @@ int fail(cfx_ctx* c, int code, const std::string& msg)
     return code;
 }
 
+// an object in the wrong state for the call (CFX_E_STATE)
+struct StateError : std::logic_error
+{
+    using std::logic_error::logic_error;
+};
+
 template <typename F>
 int guard(cfx_ctx* c, F&& f)
 {
@@ -134,6 +145,10 @@ int guard(cfx_ctx* c, F&& f)
     catch (const std::invalid_argument& e)
     {
         return fail(c, CFX_E_INVALID, e.what());
+    }
+    catch (const StateError& e)
+    {
+        return fail(c, CFX_E_STATE, e.what());
     }
     catch (const std::exception& e)
     {
@@ -682,15 +697,21 @@ int cfx_matrix_solve(cfx_matrix* A, double* x, int solver, double tol, int maxIt
         DevMesh& m = *A->mesh->m;
         need(solver == CFX_SOLVER_CG || solver == CFX_SOLVER_BICGSTAB, "unknown solver");
         need(precision == CFX_FP64 || precision == CFX_FP32_IR, "unknown precision");
-        need(precond == CFX_PRECOND_JACOBI || precond == CFX_PRECOND_LEARNED,
+        need(precond == CFX_PRECOND_JACOBI || precond == CFX_PRECOND_LEARNED || precond == CFX_PRECOND_MG,
              "preconditioner not available in this build");
-        need(precond != CFX_PRECOND_LEARNED || (solver == CFX_SOLVER_CG && precision == CFX_FP32_IR),
+        const bool multigrid = precond == CFX_PRECOND_LEARNED || precond == CFX_PRECOND_MG;
+        need(!multigrid || (solver == CFX_SOLVER_CG && precision == CFX_FP32_IR),
              "the multigrid preconditioner runs inside flexible CG in FP32 with FP64 refinement");
+        if (precond == CFX_PRECOND_LEARNED && !A->net)
+            throw StateError("CFX_PRECOND_LEARNED needs a network attached (cfx_matrix_set_mpnn); "
+                             "CFX_PRECOND_MG solves with the multigrid weights alone");
         FvSolveArgs a;
-        if (precond == CFX_PRECOND_LEARNED)
+        if (multigrid)
         {
             a.mg = &A->mg;
-            if (A->net)
+            if (precond == CFX_PRECOND_MG)
+                A->mg.useWeights(A->mg.userPre, A->mg.userScale, A->mg.userPost, A->mg.userRestr);
+            else
             {
                 if (!A->geo.p)
                 {
@@ -715,10 +736,26 @@ int cfx_matrix_solve(cfx_matrix* A, double* x, int solver, double tol, int maxIt
         a.sign = sign;
         a.gradCorrected = gradCorrected != 0;
         a.precision = precision;
+        a.relaxation = A->relaxation;
+        if (A->relaxation < 1.0)
+        {
+            const std::size_t need = sizeof(double) * A->ncomp * m.n;
+            if (!A->relaxOld.p || A->relaxOld.bytes < need) A->relaxOld.alloc(need);
+            a.relaxOld = A->relaxOld.as<double>();
+        }
         const SolveReport last = fvSolve(m, c->solver(m.n, m.pattern.nRowBlocks), A->ldu(), A->val.as<double>(),
                                          A->rhsBase.as<double>(), A->val32, x, a, A->mesh->gradTmp.as<double>(),
                                          A->mesh->faceTmp.as<double>());
         toReport(last, rep);
+    });
+}
+
+int cfx_matrix_set_relaxation(cfx_matrix* A, double relaxation)
+{
+    if (!A) return CFX_E_INVALID;
+    return guard(A->mesh->owner, [&] {
+        need(relaxation > 0.0 && relaxation <= 1.0 && std::isfinite(relaxation), "relaxation must be in (0, 1]");
+        A->relaxation = relaxation;
     });
 }
 
@@ -853,12 +890,17 @@ int cfx_csr_solve_device(cfx_csr* A, const double* b, double* x, double tol, int
                     .fcgMixed(A->pattern.view(), A->val.as<double>(), A->dicReady(), b, x, tol, maxIter, A->innerTol,
                               nullptr);
         }
-        else if (precond == CFX_PRECOND_LEARNED)
+        else if (precond == CFX_PRECOND_LEARNED || precond == CFX_PRECOND_MG)
         {
             need(solver == CFX_SOLVER_CG, "the multigrid preconditioner runs inside flexible CG");
             need(precision == CFX_FP32_IR, "the multigrid preconditioner runs in FP32 with FP64 refinement");
+            if (precond == CFX_PRECOND_LEARNED && !A->net)
+                throw StateError("CFX_PRECOND_LEARNED needs a network attached (cfx_csr_set_mpnn); "
+                                 "CFX_PRECOND_MG solves with the multigrid weights alone");
             Context& ctx = *A->owner->ctx;
-            if (A->net)
+            if (precond == CFX_PRECOND_MG)
+                A->mg.useWeights(A->mg.userPre, A->mg.userScale, A->mg.userPost, A->mg.userRestr);
+            else
             {
                 if (!A->wa.p)
                 {
@@ -973,6 +1015,10 @@ void setMgParams(MgCache& mg, const cfx_mg_params* p)
 
 void setMgWeights(MgCache& mg, const float* pre, const float* scale, const float* post, const float* restr)
 {
+    mg.userPre = pre;
+    mg.userScale = scale;
+    mg.userPost = post;
+    mg.userRestr = restr;
     mg.pre = pre;
     mg.scale = scale;
     mg.post = post;
@@ -1148,6 +1194,37 @@ int cfx_matrix_time_spmv(cfx_matrix* A, int comp, const void* x, void* y, int re
         bool ok = false;
         *ms = timeSpmv(A->mesh->owner, m.pattern.view(), A->val.as<double>() + (long long)comp * m.pattern.nnz,
                        A->val32, ok, x, y, reps, precision);
+    });
+}
+
+int cfx_matrix_time_fcg_spmv(cfx_matrix* A, int reps, double* ms)
+{
+    if (!A || !ms) return CFX_E_INVALID;
+    return guard(A->mesh->owner, [&] {
+        need(A->ncomp == 1, "the flexible CG solves scalar matrices");
+        DevMesh& m = *A->mesh->m;
+        *ms = A->mesh->owner->solver(m.pattern.n, m.pattern.nRowBlocks)
+                  .timeFcgSpmv(m.pattern.view(), A->val.as<double>(), reps);
+    });
+}
+
+int cfx_set_fcg_probe(cfx_ctx* c, int enable)
+{
+    if (!c) return CFX_E_INVALID;
+    c->probeFcgA = enable != 0;
+    for (auto& k : c->krylov)
+        if (k.second) k.second->probeFcgA = c->probeFcgA;
+    return CFX_OK;
+}
+
+int cfx_matrix_fcg_probe(cfx_matrix* A, double* avgMs, int* launches)
+{
+    if (!A || !avgMs || !launches) return CFX_E_INVALID;
+    return guard(A->mesh->owner, [&] {
+        DevMesh& m = *A->mesh->m;
+        Krylov& k = A->mesh->owner->solver(m.pattern.n, m.pattern.nRowBlocks);
+        *avgMs = k.probeMs;
+        *launches = k.probeCount;
     });
 }
 

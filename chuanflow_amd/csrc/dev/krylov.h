@@ -102,6 +102,11 @@ public:
     SolveReport fcgMixed(const CsrView& A, const double* val, Preconditioner& mg, const double* b, double* x,
                          double tol, int maxIter, double delta, double* resHist = nullptr);
 
+    // Average device time (ms) of the flexible CG's fine-level kernel (q = A p with FP64 values on FP32
+    // vectors, partials p.q and p.r) over `reps` launches on this solver's vectors, HIP events on the
+    // context's stream: the roofline yardstick of the multigrid solves.
+    double timeFcgSpmv(const CsrView& A, const double* val, int reps);
+
     // True residual ||b - A x|| in FP64: returns {inf-norm rel, 2-norm rel}.
     void trueResidual(const CsrView& A, const double* val, const double* b, const double* x,
                       double& infRel, double& l2Rel);
@@ -113,6 +118,11 @@ public:
     int minBatch = 8, maxBatch = 256;
     // flexible CG: replay one captured iteration (true) or launch its kernels one by one
     bool useGraphs = true;
+    // measurement: flexible CG launches eagerly and times every fine-level k_fcg_A launch with HIP events;
+    // probeMs / probeCount: average device time and count of those that ran (not the ones after the stop)
+    bool probeFcgA = false;
+    double probeMs = 0;
+    int probeCount = 0;
     // the drivers' FP64 vector solves: all components in shared launches (true) or one by one
     bool batchComponents = true;
 
@@ -144,6 +154,8 @@ private:
     SolveState* hostC_ = nullptr;  // pinned
     // captured flexible-CG iteration, valid for the key (multigrid, buffers) and the stop parameters
     hipGraphExec_t fcgExec_ = nullptr;
+    std::vector<hipEvent_t> probeEv_;  // event pairs of the k_fcg_A probe
+    int probeUsed_ = 0;
     std::vector<const void*> fcgKey_;
     double fcgTol_ = -1, fcgDelta_ = -1;
     int fcgMaxIter_ = -1;

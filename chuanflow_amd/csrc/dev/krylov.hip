@@ -644,6 +644,7 @@ Krylov::Krylov(Context& ctx, int n, int maxRowBlocks) : ctx_(ctx), n_(n)
 
 Krylov::~Krylov()
 {
+    for (hipEvent_t e : probeEv_) (void)hipEventDestroy(e);
     if (hostC_) (void)hipHostFree(hostC_);
     if (fcgExec_) (void)hipGraphExecDestroy(fcgExec_);
     if (ev0_) (void)hipEventDestroy(ev0_);

@@ -212,7 +212,21 @@ struct MgCache
     std::unique_ptr<Multigrid> mg;
     MgParams p;
     const float *pre = nullptr, *scale = nullptr, *post = nullptr, *restr = nullptr;
+    // the caller's own weights (null: the constants); a learned solve points pre..restr at the network's
+    // outputs, a plain multigrid solve back at these
+    const float *userPre = nullptr, *userScale = nullptr, *userPost = nullptr, *userRestr = nullptr;
     bool valid = false;
+
+    // point the finest-level weights at w (pre = w0, scale = w1, post = w2, restrict = w3); a change
+    // invalidates the hierarchy's values
+    void useWeights(const float* a, const float* s, const float* c, const float* t)
+    {
+        if (a != pre || s != scale || c != post || t != restr) valid = false;
+        pre = a;
+        scale = s;
+        post = c;
+        restr = t;
+    }
     std::pair<int, int> built{-1, -1};
 
     Multigrid& ready(Context& ctx, const DevPattern& pat, const double* val)

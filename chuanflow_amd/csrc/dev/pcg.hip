@@ -1,14 +1,13 @@
-// pcg.hip -- Jacobi-PCG with two kernels per iteration and no atomics.
+// pcg.hip -- Jacobi-PCG with three kernels per iteration and no atomics.
 //
 // Iteration k runs
-//   A(k): reduce the previous update's partials (r.z, max|r|) -- every
-//         workgroup redundantly, in one fixed order, so all agree bit for
-//         bit -- take the stop decision, beta = rz_k / rz_{k-1}, and compute
-//         q = A p with p = invD r + beta p_old formed on the fly while the
-//         products are staged in LDS (CSR-stream), writing p for its own
-//         rows; partial p.q
-//   B(k): reduce p.q, alpha = rz / p.q, x += alpha p, r -= alpha q,
-//         partials r.(invD r) and max|r|
+//   A(k): q = A p (CSR-stream: each row block's products staged in LDS, each
+//         row summed in column order), partial p.q per workgroup
+//   B(k): reduce the p.q partials -- every workgroup redundantly, in one
+//         fixed order, so all agree bit for bit -- alpha = rz / p.q,
+//         x += alpha p, r -= alpha q, partials r.(invD r) and max|r|
+//   C(k): reduce those partials the same way, take the stop decision,
+//         beta = rz_k / rz_{k-1}, p = invD r + beta p
 // Workgroup 0 records the scalars for the host, which only polls `done`
 // between batches of iterations.  Scalars that a kernel reads while its own
 // workgroup 0 writes the next value are double-buffered by iteration parity.
@@ -24,6 +23,8 @@
 #include "batch.h"
 #include "krylov.h"
 #include "mg.h"
+
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -756,6 +757,33 @@ SolveReport pcgSolveWith(Context& ctx, const CsrView& A, const double* val, cons
     return pcgCore<double>(ctx, A.n, w, 8, 256, A, val, val, invD, b, x, x, r, tol, maxIter, nullptr, false, 0.0);
 }
 
+double Krylov::timeFcgSpmv(const CsrView& A, const double* val, int reps)
+{
+    hipStream_t s = ctx_.stream();
+    const PcgLaunch g = pcgGrid(A, n_);
+    if (A.n != n_ || reps < 1) throw std::invalid_argument("timeFcgSpmv: bad matrix or repetition count");
+    if (!fcgPart_.p)
+    {
+        fcgPart_.alloc(sizeof(double) * 5 * kRedMax);
+        z32_.alloc(sizeof(float) * static_cast<std::size_t>(n_));
+    }
+    double* part = fcgPart_.as<double>();
+    CgState* st = reinterpret_cast<CgState*>(state_.as<char>() + 2 * sizeof(SolveState));
+    k_clear_words<<<1, kB, 0, s>>>(reinterpret_cast<unsigned int*>(st), sizeof(CgState) / 4);  // not done
+    k_fcg_A<<<g.Gs, kB, 0, s>>>(A, g.rbPerWg, val, p32_.as<float>(), r32_.as<float>(), q32_.as<float>(), st, part,
+                                part + kRedMax);  // warm
+    CFX_HIP(hipEventRecord(ev0_, s));
+    for (int i = 0; i < reps; ++i)
+        k_fcg_A<<<g.Gs, kB, 0, s>>>(A, g.rbPerWg, val, p32_.as<float>(), r32_.as<float>(), q32_.as<float>(), st,
+                                    part, part + kRedMax);
+    CFX_HIP(hipEventRecord(ev1_, s));
+    CFX_HIP(hipEventSynchronize(ev1_));
+    CFX_CHECK_LAUNCH();
+    float ms = 0;
+    CFX_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+    return ms / reps;
+}
+
 SolveReport Krylov::cg(const CsrView& A, const double* val, const double* b, double* x, double tol, int maxIter,
                        double* resHist)
 {
@@ -816,6 +844,7 @@ SolveReport Krylov::fcgMixed(const CsrView& A, const double* val, Preconditioner
            *partZQ = part + 4 * kRedMax;
     double* partRZ = partials_.as<double>();  // written by the residual kernel, unused here
     CgState* st = reinterpret_cast<CgState*>(state_.as<char>() + 2 * sizeof(SolveState));
+    probeUsed_ = 0;
     CFX_HIP(hipMemsetAsync(st, 0, sizeof(CgState), s));
     CFX_HIP(hipMemsetAsync(d32_.p, 0, sizeof(float) * n_, s));
     float *p = p32_.as<float>(), *q = q32_.as<float>(), *r = r32_.as<float>(), *z = z32_.as<float>(),
@@ -848,11 +877,27 @@ SolveReport Krylov::fcgMixed(const CsrView& A, const double* val, Preconditioner
         batch = std::max(2, std::min(64, batch));
         batch = std::min(batch, maxIter - k);
         if (batch <= 0) break;
-        if (!useGraphs)
+        if (!useGraphs || probeFcgA)
         {
             for (int j = 0; j < batch; ++j)
             {
-                k_fcg_A<<<g.Gs, kB, 0, s>>>(A, g.rbPerWg, val, p, r, q, st, partPQ, partPR);
+                if (probeFcgA)
+                {
+                    // measurement mode: the launch carries HIP start and stop events, which the command
+                    // processor records at the kernel's own start and end on the solver's stream
+                    while (static_cast<int>(probeEv_.size()) < 2 * (probeUsed_ + 1))
+                    {
+                        hipEvent_t e;
+                        CFX_HIP(hipEventCreate(&e));
+                        probeEv_.push_back(e);
+                    }
+                    hipExtLaunchKernelGGL(k_fcg_A, dim3(g.Gs), dim3(kB), 0, s, probeEv_[2 * probeUsed_],
+                                          probeEv_[2 * probeUsed_ + 1], 0, A, g.rbPerWg, val, p, r, q,
+                                          static_cast<const CgState*>(st), partPQ, partPR);
+                    ++probeUsed_;
+                }
+                else
+                    k_fcg_A<<<g.Gs, kB, 0, s>>>(A, g.rbPerWg, val, p, r, q, st, partPQ, partPR);
                 k_fcg_B<<<g.Gv, kB, 0, s>>>(n_, x32, r, p, q, st, partPQ, partPR, g.Gs, partRM);
                 mg.apply(r, z, q, partZQ, g.Gs, g.rbPerWg, done);
                 k_fcg_C<<<g.Gv, kB, 0, s>>>(n_, z, p, st, partZQ, g.Gs, partRM, partBM, g.Gv, -1, tol, maxIter,
@@ -862,7 +907,27 @@ SolveReport Krylov::fcgMixed(const CsrView& A, const double* val, Preconditioner
         else
         {
             // one iteration (A, B, the cycle, C) as a captured graph, replayed per step
-            const std::vector<const void*> key{&mg, val, resHist, reinterpret_cast<const void*>(mg.id()), A.rowPtr};
+            // every address and size the captured kernels take: the matrix pattern and values, the
+            // preconditioner (its id changes with its buffers and launch structure), this solver's vectors,
+            // partials and state, the history buffer and the grids
+            const std::vector<const void*> key{&mg,
+                                               reinterpret_cast<const void*>(mg.id()),
+                                               val,
+                                               resHist,
+                                               A.rowPtr,
+                                               A.col,
+                                               A.rowBlocks,
+                                               reinterpret_cast<const void*>(static_cast<std::uintptr_t>(A.n)),
+                                               reinterpret_cast<const void*>(static_cast<std::uintptr_t>(A.nRowBlocks)),
+                                               p,
+                                               q,
+                                               r,
+                                               z,
+                                               x32,
+                                               part,
+                                               st,
+                                               reinterpret_cast<const void*>(static_cast<std::uintptr_t>(g.Gs)),
+                                               reinterpret_cast<const void*>(static_cast<std::uintptr_t>(g.Gv))};
             if (!fcgExec_ || key != fcgKey_ || tol != fcgTol_ || maxIter != fcgMaxIter_ || delta != fcgDelta_)
             {
                 if (fcgExec_) CFX_HIP(hipGraphExecDestroy(fcgExec_));
@@ -905,6 +970,28 @@ SolveReport Krylov::fcgMixed(const CsrView& A, const double* val, Preconditioner
             continue;
         }
         break;
+    }
+    if (probeFcgA && probeUsed_ > 0)
+    {
+        // launches after the stop decision return at once: average over those that ran (> 20 % of the longest)
+        CFX_HIP(hipStreamSynchronize(s));
+        std::vector<float> t(probeUsed_);
+        float mx = 0;
+        for (int i = 0; i < probeUsed_; ++i)
+        {
+            CFX_HIP(hipEventElapsedTime(&t[i], probeEv_[2 * i], probeEv_[2 * i + 1]));
+            mx = std::max(mx, t[i]);
+        }
+        double sum = 0;
+        int cnt = 0;
+        for (float v : t)
+            if (v > 0.2f * mx)
+            {
+                sum += v;
+                ++cnt;
+            }
+        probeMs = cnt ? sum / cnt : 0.0;
+        probeCount = cnt;
     }
     const bool failed = hs->nan || hs->breakdown;
     if (!hs->converged && !failed)

@@ -44,6 +44,10 @@ struct FvSolveArgs
     // multigrid hierarchy's values are kept instead of set up again (without a network only, whose
     // weights depend on the right-hand side)
     bool sameMatrix = false;
+    // Schemes::relaxation (FvMatrix.h:30-38): after the final pass x = xOld + relaxation (x - xOld) with
+    // xOld the solution before that pass (fvMatrix.cpp:110-118); relaxOld holds ncomp * n doubles
+    double relaxation = 1.0;
+    double* relaxOld = nullptr;
 };
 SolveReport fvSolve(DevMesh& m, Krylov& K, const Ldu& A, double* val, double* rhsBase, DevBuf& val32,
                     double* x, const FvSolveArgs& a, double* gradTmp, double* faceTmp);

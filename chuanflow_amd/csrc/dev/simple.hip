@@ -468,6 +468,16 @@ static void dumpPressure(DevMesh& m, const Ldu& PE, const double* pval, Pressure
     ++d->count;
 }
 
+namespace
+{
+// x = xOld + rl (x - xOld), the reference's expression (fvMatrix.cpp:116)
+__global__ void k_relax(long long n, const double* __restrict__ xOld, double* __restrict__ x, double rl)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        x[i] = xOld[i] + rl * (x[i] - xOld[i]);
+}
+} // namespace
+
 SolveReport fvSolve(DevMesh& m, Krylov& K, const Ldu& A, double* val, double* rhsBase, DevBuf& val32, double* x,
                     const FvSolveArgs& a, double* gradTmp, double* faceTmp)
 {
@@ -479,8 +489,13 @@ SolveReport fvSolve(DevMesh& m, Krylov& K, const Ldu& A, double* val, double* rh
     const int passes = a.gammaFace ? std::max(1, a.nonOrth + 1) : 1;
     const CsrView view = m.pattern.view();
     SolveReport last;
+    const bool relax = a.relaxation < 1.0;
+    if (relax && !a.relaxOld) throw std::invalid_argument("fvSolve: relaxation needs a buffer for the old solution");
     for (int pass = 0; pass < passes; ++pass)
     {
+        const bool finalPass = pass + 1 == passes;
+        // the solution before the final pass's solves (each component's solve leaves the others untouched)
+        if (finalPass && relax) CFX_HIP(hipMemcpyAsync(a.relaxOld, x, bytes, hipMemcpyDeviceToDevice, s));
         CFX_HIP(hipMemcpyAsync(A.rhs, rhsBase, bytes, hipMemcpyDeviceToDevice, s));
         if (a.gammaFace)
             for (int k = 0; k < A.ncomp; ++k)
@@ -535,6 +550,12 @@ SolveReport fvSolve(DevMesh& m, Krylov& K, const Ldu& A, double* val, double* rh
                 last = K.cgMixed(view, v, val32.as<float>(), b, xk, a.tol, a.maxIter, a.delta, false);
             }
         }
+    }
+    if (relax)
+    {
+        const long long total = static_cast<long long>(A.ncomp) * n;
+        k_relax<<<std::max(1, std::min(4096, divUp(total, 256))), 256, 0, s>>>(total, a.relaxOld, x, a.relaxation);
+        CFX_CHECK_LAUNCH();
     }
     return last;
 }

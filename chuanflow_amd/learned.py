@@ -156,7 +156,8 @@ def channel_run_device(ctx, hm, Uin, iterations, every=1, max_dumps=None, nu=1e-
 
 def _fcg_iterations(G, b, net, dgeo, maxIter):
     G.set_mpnn(net, dgeo if net is not None else None)
-    _, rep = G.solve(b, None, 1e-8, maxIter, N.SOLVER_CG, N.PRECOND_LEARNED, N.FP32_IR)
+    _, rep = G.solve(b, None, 1e-8, maxIter, N.SOLVER_CG, N.PRECOND_LEARNED if net is not None else N.PRECOND_MG,
+                     N.FP32_IR)
     G.set_mpnn(None)
     return rep.iterations if rep.converged and rep.precondFallbacks == 0 else None
 
@@ -249,7 +250,8 @@ def heldout_iterations(ctx, nets, channels=2, size=(128, 128, 128), length=(1.0,
             for name, net in [("tuned", None)] + list(nets.items()):
                 G.set_mg(**TUNED_MG)
                 G.set_mpnn(net, dgeo if net is not None else None)
-                _, rep = G.solve(b, None, tol, 2000, N.SOLVER_CG, N.PRECOND_LEARNED, N.FP32_IR)
+                _, rep = G.solve(b, None, tol, 2000, N.SOLVER_CG,
+                                 N.PRECOND_LEARNED if net is not None else N.PRECOND_MG, N.FP32_IR)
                 row[name] = rep.iterations
                 row[name + "_converged"] = bool(rep.converged)
             G.set_mpnn(None)

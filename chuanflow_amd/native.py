@@ -22,7 +22,7 @@ if not os.path.exists(LIB_PATH):
 _L = C.CDLL(LIB_PATH)
 
 CFX_OK, CFX_E_INVALID, CFX_E_HIP, CFX_E_STATE, CFX_E_NUMERIC, CFX_E_INTERNAL = range(6)
-PRECOND_JACOBI, PRECOND_DIC, PRECOND_LEARNED = 0, 1, 2
+PRECOND_JACOBI, PRECOND_DIC, PRECOND_LEARNED, PRECOND_MG = 0, 1, 2, 3
 FP64, FP32_IR, FP32_MG = 0, 1, 2
 SOLVER_CG, SOLVER_BICGSTAB = 0, 1
 
@@ -151,6 +151,7 @@ _sig("cfx_matrix_device_ptrs", C.c_int, _vp, C.POINTER(_vp), C.POINTER(_vp))
 _sig("cfx_nonorth_correct", C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp)
 _sig("cfx_matrix_solve", C.c_int, _vp, _vp, C.c_int, C.c_double, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int,
      C.c_int, C.c_int, C.c_int, C.POINTER(Report))
+_sig("cfx_matrix_set_relaxation", C.c_int, _vp, C.c_double)
 _sig("cfx_matrix_explicit_operator", C.c_int, _vp, _vp, C.c_int, _vp)
 _sig("cfx_matrix_spmv", C.c_int, _vp, C.c_int, _vp, _vp)
 _sig("cfx_csr_create", C.c_int, _vp, _i64, _vp, _vp, _vp, C.POINTER(_vp))
@@ -208,6 +209,9 @@ _sig("cfx_piso_step", C.c_int, _vp, C.c_double, C.POINTER(Schemes), C.POINTER(Pi
      _vp, _vp, _vp, C.POINTER(PisoResult))
 _sig("cfx_csr_time_spmv", C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, _dp)
 _sig("cfx_matrix_time_spmv", C.c_int, _vp, C.c_int, _vp, _vp, C.c_int, C.c_int, _dp)
+_sig("cfx_matrix_time_fcg_spmv", C.c_int, _vp, C.c_int, C.POINTER(C.c_double))
+_sig("cfx_set_fcg_probe", C.c_int, _vp, C.c_int)
+_sig("cfx_matrix_fcg_probe", C.c_int, _vp, C.POINTER(C.c_double), C.POINTER(C.c_int))
 # host mesh
 _sig("cfx_hmesh_box", C.c_int, _i64, _i64, _i64, _vp, _vp, C.c_double, C.c_uint64, C.c_int, C.POINTER(_vp))
 _sig("cfx_hmesh_channel", C.c_int, _i64, _i64, _i64, _vp, _vp, C.c_double, C.c_uint64, C.POINTER(_vp))
@@ -301,6 +305,10 @@ class Context:
     def set_graphs(self, enable: bool):
         """Flexible CG: replay one captured iteration per step (True) or launch kernel by kernel."""
         self.check(_L.cfx_set_graphs(self.h, 1 if enable else 0))
+
+    def set_fcg_probe(self, enable: bool):
+        """Measurement mode: multigrid solves time each fine-level k_fcg_A launch with HIP events."""
+        self.check(_L.cfx_set_fcg_probe(self.h, 1 if enable else 0))
 
     def stream_bandwidth(self, nbytes=1 << 30, reps=20):
         """(read GB/s, copy GB/s read + written) of plain streams over nbytes: the roofline yardstick."""
@@ -667,7 +675,7 @@ class Matrix:
             pass
 
     def set_mg(self, **kw):
-        """Multigrid constants for precond=PRECOND_LEARNED (cfx_mg_params fields)."""
+        """Multigrid constants for precond=PRECOND_MG / PRECOND_LEARNED (cfx_mg_params fields)."""
         p = mg_default_params()
         for k, v in kw.items():
             if not hasattr(p, k):
@@ -719,6 +727,21 @@ class Matrix:
     def time_spmv(self, comp, x, y, reps=20, precision=FP64):
         ms = C.c_double(0)
         self.ctx.check(_L.cfx_matrix_time_spmv(self.h, comp, _p(x), _p(y), reps, precision, C.byref(ms)))
+        return ms.value
+
+    def fcg_probe(self):
+        """(average ms, launches) of the fine-level k_fcg_A in the last probed multigrid solve."""
+        ms, cnt = C.c_double(0), C.c_int(0)
+        self.ctx.check(_L.cfx_matrix_fcg_probe(self.h, C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value
+
+    def set_relaxation(self, relaxation):
+        """Schemes::relaxation of this fvMatrix: applied after the final pass of solve()."""
+        self.ctx.check(_L.cfx_matrix_set_relaxation(self.h, relaxation))
+
+    def time_fcg_spmv(self, reps=20):
+        ms = C.c_double(0)
+        self.ctx.check(_L.cfx_matrix_time_fcg_spmv(self.h, reps, C.byref(ms)))
         return ms.value
 
     def explicit_operator(self, x, comp, out=None):

@@ -54,7 +54,11 @@ typedef struct cfx_report {
     double ms;              /* device time of the solve */
 } cfx_report;
 
-enum { CFX_PRECOND_JACOBI = 0, CFX_PRECOND_DIC = 1, CFX_PRECOND_LEARNED = 2 };
+/* CFX_PRECOND_LEARNED: multigrid flexible CG with the attached network's
+ * finest-level weights (CFX_E_STATE without one); CFX_PRECOND_MG: the same
+ * solver with the multigrid's own weights (cfx_mg_params constants or the
+ * per-row arrays of *_set_mg_weights), any network ignored */
+enum { CFX_PRECOND_JACOBI = 0, CFX_PRECOND_DIC = 1, CFX_PRECOND_LEARNED = 2, CFX_PRECOND_MG = 3 };
 enum { CFX_FP64 = 0, CFX_FP32_IR = 1, CFX_FP32_MG = 2 /* drivers' pressure solve: multigrid FCG */ };
 enum { CFX_SOLVER_CG = 0, CFX_SOLVER_BICGSTAB = 1 };
 
@@ -164,6 +168,11 @@ int cfx_nonorth_correct(cfx_mesh* mesh, const double* d_gammaFace, const double*
 int cfx_matrix_solve(cfx_matrix* A, double* d_x, int solver, double tol, int maxIter, int nonOrth,
                      const double* d_gammaFace, const double* d_linc, const double* d_link,
                      int sign, int gradCorrected, int precond, int precision, cfx_report* rep);
+/* Schemes::relaxation of this matrix (FvMatrix.h:30-38, default 1): after the
+ * final pass of cfx_matrix_solve each component becomes
+ * xOld + relaxation (x - xOld), xOld its value before that pass
+ * (fvMatrix.cpp:110-118).  relaxation in (0, 1]. */
+int cfx_matrix_set_relaxation(cfx_matrix* A, double relaxation);
 /* explicitOperator (FvMatrix.h:86, fvMatrix.cpp:131-146): h = rhs - (A x - D x) */
 int cfx_matrix_explicit_operator(cfx_matrix* A, const double* d_x, int comp, double* d_h);
 /* y = A x with the matrix's current CSR values of one component */
@@ -465,6 +474,21 @@ int cfx_piso_step(cfx_mesh* mesh, double nu, const cfx_schemes* schemes, const c
 int cfx_csr_time_spmv(cfx_csr* A, const void* d_x, void* d_y, int reps, int precision, double* msPerLaunch);
 int cfx_matrix_time_spmv(cfx_matrix* A, int comp, const void* d_x, void* d_y, int reps, int precision,
                          double* msPerLaunch);
+/* average device time (ms, HIP events on the context's stream) of the
+ * flexible CG's fine-level kernel on this matrix -- q = A p with the FP64
+ * values on FP32 vectors plus the p.q and p.r partials, the kernel the
+ * multigrid-preconditioned solves (CFX_PRECOND_LEARNED) run once per
+ * iteration -- over reps launches on the solver's own vectors.  A
+ * measurement helper with no reference counterpart. */
+int cfx_matrix_time_fcg_spmv(cfx_matrix* A, int reps, double* ms);
+/* measurement mode (no reference counterpart): with enable != 0 the
+ * multigrid-preconditioned solves launch their kernels one by one and time
+ * every fine-level k_fcg_A launch with HIP events on the context's stream;
+ * cfx_matrix_fcg_probe returns the average device time (ms) and the count of
+ * the launches that ran in the last such solve on this matrix's size (those
+ * queued after the stop decision return at once and are not counted). */
+int cfx_set_fcg_probe(cfx_ctx* ctx, int enable);
+int cfx_matrix_fcg_probe(cfx_matrix* A, double* avgMs, int* launches);
 
 #ifdef __cplusplus
 }

@@ -139,7 +139,7 @@ def ref():
         R.cfr_mesh_free.argtypes = [C.c_void_p]
         R.cfr_mesh_ncells.restype = C.c_int64
         R.cfr_mesh_ncells.argtypes = [C.c_void_p]
-        for name in ("cfr_geometry", "cfr_owner_weights", "cfr_grad", "cfr_laplacian_face",
+        for name in ("cfr_geometry", "cfr_owner_weights", "cfr_grad", "cfr_laplacian_face", "cfr_laplacian_face_relaxed",
                      "cfr_flux_ops", "cfr_simple", "cfr_piso_step"):
             getattr(R, name).argtypes = None
         _ref = R
@@ -466,19 +466,19 @@ class RefMesh:
         return out
 
     def laplacian_face(self, gammaFace, ptype, pval, lapCorrected, gradCorrected, sign, extraRhs,
-                       tol, maxIter, nonOrth, x0, u):
+                       tol, maxIter, nonOrth, x0, u, relaxation=1.0):
         n = self.m.nCells
         x = np.array(x0, np.float64)
         diag, rhs, offU = np.zeros(n), np.zeros(n), np.zeros(n)
         it = C.c_int(0)
-        conv = ref().cfr_laplacian_face(
+        conv = ref().cfr_laplacian_face_relaxed(
             C.c_void_p(self.h), _ptr(np.ascontiguousarray(gammaFace, np.float64)),
             _ptr(np.ascontiguousarray(ptype, np.int32), C.c_int32),
             _ptr(np.ascontiguousarray(pval, np.float64)), C.c_int(int(lapCorrected)),
             C.c_int(int(gradCorrected)), C.c_int(sign),
             _ptr(None if extraRhs is None else np.ascontiguousarray(extraRhs, np.float64)),
             C.c_double(tol), C.c_int(maxIter), C.c_int(nonOrth), _ptr(x), _ptr(diag), _ptr(rhs),
-            _ptr(np.ascontiguousarray(u, np.float64)), _ptr(offU), C.byref(it))
+            _ptr(np.ascontiguousarray(u, np.float64)), _ptr(offU), C.byref(it), C.c_double(relaxation))
         return dict(x=x, diag=diag, rhs=rhs, offU=offU, iterations=it.value, converged=bool(conv))
 
     def flux_ops(self, U, p, gradP, rAU, Utype, Uval, ptype, pval, lapCorrected):

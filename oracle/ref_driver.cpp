@@ -285,17 +285,18 @@ void cfr_grad(void* mh, const double* data, const int32_t* bcType, const double*
 // solve(Controls{tol, maxIter, nonOrth}) from the initial guess x (in/out).
 // Outputs aDiag, rhs_ after solve (= h(0)), and offU = -(Off * u) for a
 // probe vector u (= h(u) - h(0)).
-int cfr_laplacian_face(void* mh, const double* gammaFace, const int32_t* bcType,
-                       const double* bcValue, int lapCorrected, int gradScheme, int sign,
-                       const double* extraRhs, double tol, int maxIter, int nonOrth, double* x,
-                       double* diagOut, double* rhsOut, const double* u, double* offU,
-                       int* iters)
+int cfr_laplacian_face_relaxed(void* mh, const double* gammaFace, const int32_t* bcType,
+                               const double* bcValue, int lapCorrected, int gradScheme, int sign,
+                               const double* extraRhs, double tol, int maxIter, int nonOrth, double* x,
+                               double* diagOut, double* rhsOut, const double* u, double* offU,
+                               int* iters, double relaxation)
 {
     const polyMesh& m = *static_cast<polyMesh*>(mh);
     volScalarField T(m, "T");
     setScalarBC(T, bcType, bcValue);
     fvMatrix eq(m, viewOf(T), "TEqn");
     eq.schemes() = schemesOf(0, lapCorrected, gradScheme);
+    eq.schemes().relaxation = relaxation;
     std::memcpy(eq.solution()[0].data(), x, sizeof(double) * m.nCells());
     Field<scalar> gf(gammaFace, gammaFace + m.nFaces());
     fvm::laplacianFace(eq, gf, lapCorrected ? "corrected" : "none", sign);
@@ -314,6 +315,16 @@ int cfr_laplacian_face(void* mh, const double* gammaFace, const int32_t* bcType,
         offU[c] = hu[c] - h0[c];
     }
     return rep.converged ? 1 : 0;
+}
+
+int cfr_laplacian_face(void* mh, const double* gammaFace, const int32_t* bcType,
+                       const double* bcValue, int lapCorrected, int gradScheme, int sign,
+                       const double* extraRhs, double tol, int maxIter, int nonOrth, double* x,
+                       double* diagOut, double* rhsOut, const double* u, double* offU,
+                       int* iters)
+{
+    return cfr_laplacian_face_relaxed(mh, gammaFace, bcType, bcValue, lapCorrected, gradScheme, sign, extraRhs, tol,
+                                      maxIter, nonOrth, x, diagOut, rhsOut, u, offU, iters, 1.0);
 }
 
 int cfr_solve_cg(int64_t n, const int64_t* rowStart, const int64_t* cols, const double* vals,

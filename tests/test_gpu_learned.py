@@ -58,7 +58,7 @@ def test_ablation_trained_network_vs_constants(ctx, heldout):
                               ("learned", LN.TUNED_MG, True)):
             G.set_mg(**mg)
             G.set_mpnn(net if use else None, g if use else None)
-            x, rep = G.solve(b, None, 1e-8, 500, N.SOLVER_CG, N.PRECOND_LEARNED, N.FP32_IR)
+            x, rep = G.solve(b, None, 1e-8, 500, N.SOLVER_CG, N.PRECOND_LEARNED if use else N.PRECOND_MG, N.FP32_IR)
             Ax = np.bincount(A_rows, weights=v * x[cl], minlength=len(b))
             assert rep.converged and rep.precondFallbacks == 0
             assert np.abs(b - Ax).max() / np.abs(b).max() < 1e-8

@@ -97,13 +97,17 @@ def test_fcg_converges_to_true_fp64_residual(ctx, systems, name):
         b = np.random.default_rng(2).standard_normal(A.shape[0])
     N, G = make(ctx, A)
     G.set_mg(scale=1.8, coarseScale=1.8)
-    x, rep = G.solve(b, None, 1e-8, 500, N.SOLVER_CG, N.PRECOND_LEARNED, N.FP32_IR)
+    x, rep = G.solve(b, None, 1e-8, 500, N.SOLVER_CG, N.PRECOND_MG, N.FP32_IR)
     true = np.abs(b - A @ x).max() / np.abs(b).max()
     print(f"[{name}] fcg its {rep.iterations} outer {rep.outerIterations} true {true:.2e}", flush=True)
     assert rep.converged and rep.precondFallbacks == 0
     assert true < 1e-8 and abs(rep.trueResidual - true) <= 1e-3 * true + 1e-15
     _, rj = G.solve(b, None, 1e-8, 5000, N.SOLVER_CG, N.PRECOND_JACOBI, N.FP32_IR)
     assert rj.converged and rep.iterations * 5 < rj.iterations
+    # the learned preconditioner without a network is refused (CFX_E_STATE), not silently replaced
+    with pytest.raises(N.CfxError, match="network") as e:
+        G.solve(b, None, 1e-8, 500, N.SOLVER_CG, N.PRECOND_LEARNED, N.FP32_IR)
+    assert e.value.code == 3
     # same iterates as the FP64 oracle's FCG up to FP32 rounding: iteration counts agree closely
     H = MG.hierarchy(A)
     p = MG.Params(scale=1.8, coarseScale=1.8)
@@ -117,10 +121,10 @@ def test_identity_network_limit_equals_jacobi_fcg(ctx, systems):
     n = A.shape[0]
     one, zero = ctx.array(np.ones(n, np.float32)), ctx.array(np.zeros(n, np.float32))
     G.set_mg_weights(one, zero, zero)
-    _, rl = G.solve(b, None, 1e-8, 5000, N.SOLVER_CG, N.PRECOND_LEARNED, N.FP32_IR)
+    _, rl = G.solve(b, None, 1e-8, 5000, N.SOLVER_CG, N.PRECOND_MG, N.FP32_IR)
     N2, J = make(ctx, A)
     J.set_mg(maxLevels=1, omegaPre=1.0, omegaPost=0.0)
-    _, rj = J.solve(b, None, 1e-8, 5000, N.SOLVER_CG, N.PRECOND_LEARNED, N.FP32_IR)
+    _, rj = J.solve(b, None, 1e-8, 5000, N.SOLVER_CG, N.PRECOND_MG, N.FP32_IR)
     # FP64 Jacobi-PCG: FCG with a fixed SPD preconditioner has PCG's iterates in exact arithmetic
     _, rp = J.solve(b, None, 1e-8, 5000, N.SOLVER_CG, N.PRECOND_JACOBI, N.FP64)
     print(f"identity-limit FCG {rl.iterations}, Jacobi-FCG {rj.iterations}, FP64 Jacobi-PCG {rp.iterations}",
@@ -138,7 +142,7 @@ def test_nonfinite_weights_fall_back_to_jacobi_pcg(ctx, systems):
     w[n // 3] = np.nan
     dw = ctx.array(w)
     G.set_mg_weights(dw, None, None)
-    x, rep = G.solve(b, None, 1e-8, 5000, N.SOLVER_CG, N.PRECOND_LEARNED, N.FP32_IR)
+    x, rep = G.solve(b, None, 1e-8, 5000, N.SOLVER_CG, N.PRECOND_MG, N.FP32_IR)
     true = np.abs(b - A @ x).max() / np.abs(b).max()
     assert rep.precondFallbacks == 1 and rep.converged and true < 1e-8
 
@@ -150,9 +154,9 @@ def test_new_values_on_same_pattern_rebuild_galerkin_values(ctx, systems):
     G.set_mg(scale=1.8, coarseScale=1.8)
     A2 = A.copy()
     A2.data = A2.data * np.where(A2.indices == np.repeat(np.arange(A.shape[0]), np.diff(A.indptr)), 1.5, 1.0)
-    G.solve(b, None, 1e-8, 500, N.SOLVER_CG, N.PRECOND_LEARNED, N.FP32_IR)
+    G.solve(b, None, 1e-8, 500, N.SOLVER_CG, N.PRECOND_MG, N.FP32_IR)
     G.set_values(A2.data)
-    x, rep = G.solve(b, None, 1e-8, 500, N.SOLVER_CG, N.PRECOND_LEARNED, N.FP32_IR)
+    x, rep = G.solve(b, None, 1e-8, 500, N.SOLVER_CG, N.PRECOND_MG, N.FP32_IR)
     assert rep.converged and np.abs(b - A2 @ x).max() / np.abs(b).max() < 1e-8
     _, _, vv, _ = G.mg_level(1)
     np.testing.assert_allclose(vv, MG.hierarchy(A2).A[1].data, rtol=1e-13, atol=1e-12)
@@ -165,7 +169,7 @@ def test_graph_replay_equals_eager_launches(ctx, systems):
     xs, reps = [], []
     for graphs in (True, False, True):
         ctx.set_graphs(graphs)
-        x, rep = G.solve(b, None, 1e-8, 500, N.SOLVER_CG, N.PRECOND_LEARNED, N.FP32_IR)
+        x, rep = G.solve(b, None, 1e-8, 500, N.SOLVER_CG, N.PRECOND_MG, N.FP32_IR)
         xs.append(x)
         reps.append(rep.iterations)
     ctx.set_graphs(True)
@@ -217,7 +221,7 @@ def test_fused_kernels_equal_separate_launches(ctx, systems, case):
         z = ctx.empty(n, np.float32)
         G.mg_apply(r, z)
         zs.append(z.get())
-        x, rep = G.solve(b, None, 1e-8, 500, N.SOLVER_CG, N.PRECOND_LEARNED, N.FP32_IR)
+        x, rep = G.solve(b, None, 1e-8, 500, N.SOLVER_CG, N.PRECOND_MG, N.FP32_IR)
         assert rep.converged
         xs.append(x)
         its.append(rep.iterations)

@@ -60,7 +60,9 @@ def test_simple_tracks_oracle_tight(ctx, relax):
     r, hg = dm.simple(1e-3, N.Schemes(0, 1, 1, 0.9), N.SimpleControls(**kw), Ubc, pbc, dU, dp, dphi, hist=True)
     ro, ho = O.simple(om, U, p, phi, 1e-3, O.Schemes(0, 1, 1, 0.9), O.SimpleControls(**kw), Ubc, pbc, hist=True)
     assert r.iterations == ro.iterations == iters and not r.diverged
-    assert rel(aos(dU.get()), U) < 1e-9 and rel(dp.get(), p) < 1e-9 and rel(dphi.get(), phi) < 1e-9
+    eU, ep, ephi = rel(aos(dU.get()), U), rel(dp.get(), p), rel(dphi.get(), phi)
+    print(f"SIMPLE {relax}: rel diff U {eU:.2e} p {ep:.2e} phi {ephi:.2e}", flush=True)
+    assert eU < 1e-10 and ep < 1e-10 and ephi < 1e-10
     np.testing.assert_allclose(hg, ho, rtol=1e-6, atol=1e-14)
 
 
@@ -138,7 +140,9 @@ def test_piso_tracks_oracle(ctx):
         r = dm.piso_step(1e-3, N.Schemes(0, 1, 1, 0.9), N.PisoControls(**kw), Ubc, pbc, dU, dp, dphi)
         ro = O.piso_step(om, U, p, phi, 1e-3, O.Schemes(0, 1, 1, 0.9), O.PisoControls(**kw), Ubc, pbc)
         assert r.correctorsUsed == ro.correctorsUsed
-        assert rel(aos(dU.get()), U) < 1e-9 and rel(dp.get(), p) < 1e-9 and rel(dphi.get(), phi) < 1e-9, step
+        eU, ep, ephi = rel(aos(dU.get()), U), rel(dp.get(), p), rel(dphi.get(), phi)
+        print(f"PISO step {step}: rel diff U {eU:.2e} p {ep:.2e} phi {ephi:.2e}", flush=True)
+        assert eU < 1e-10 and ep < 1e-10 and ephi < 1e-10, step
 
 
 def test_piso_inviscid_body_force_closed_form(ctx):
@@ -262,7 +266,7 @@ def test_simple_on_gmsh_tet_channel_matches_oracle(ctx, tmp_path):
     for x, y, name in ((aos(dU.get()), U, "U"), (dp.get(), p, "p"), (dphi.get(), phi, "phi")):
         e = rel(x, y)
         print(f"  {name}: max rel diff {e:.2e}", flush=True)
-        assert e < 1e-9, name
+        assert e < 1e-10, name
     assert np.abs(U).max() > 0.01  # the flow is driven through the imported patches
 
 

@@ -166,6 +166,37 @@ def test_oracle_matches_reference_build_on_fresh_case():
     assert np.array_equal(U, Ur) and np.array_equal(p, pr) and np.array_equal(phi, phir)
 
 
+@pytest.mark.skipif(not O.have_ref(), reason="reference build not present")
+def test_reference_relaxation_is_applied_after_the_final_pass_only():
+    """Schemes::relaxation (fvMatrix.cpp:110-118) in the reference build: three
+    passes of the oracle's corrected laplacian solve, then
+    x = xOld + 0.7 (x - xOld) with xOld the solution before the final pass,
+    equal bit for bit to the reference's relaxed fvMatrix::solve (the
+    unrelaxed one equals the oracle's pass loop)."""
+    m = O.blockmesh(7, 5, 4, (0, 0, 0), (1, 0.7, 0.6), jitter=0.15, seed=5).compute_geometry()
+    rng = np.random.default_rng(3)
+    _, _, pt, pv, _, _, ppt, ppv = channel_bcs(m)
+    lc, lk = O.linearise(m, 1, pt, pv)
+    gf = rng.uniform(0.5, 2.0, m.nFaces)
+    src = rng.standard_normal(m.nCells)
+    A = O.Ldu(m)
+    O.assemble_laplacian(m, A, gf, lc, lk, -1)
+    A.rhs[0] += src
+    csr = O.csr_from_ldu(m, A)
+    x = np.zeros(m.nCells)
+    for _ in range(3):
+        xOld = x.copy()
+        rhs = A.rhs[0].copy()
+        O.nonorth_correct(m, gf, x, lc, lk, rhs=rhs)
+        x, _ = O.solve_cg(csr, rhs, x, 1e-12, 2000)
+    rm = O.RefMesh(m)
+    z = np.zeros(m.nCells)
+    r1 = rm.laplacian_face(gf, ppt, ppv, 1, 1, -1, src, 1e-12, 2000, 2, z, z)
+    r7 = rm.laplacian_face(gf, ppt, ppv, 1, 1, -1, src, 1e-12, 2000, 2, z, z, relaxation=0.7)
+    assert np.array_equal(r1["x"], x)
+    assert np.array_equal(r7["x"], xOld + 0.7 * (x - xOld)) and not np.array_equal(r7["x"], x)
+
+
 # ----------------------------------------------------------- closed forms
 def test_unit_cube_geometry_closed_forms():
     # kernelTests.cpp:58-83
