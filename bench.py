@@ -755,6 +755,15 @@ def main():
         "learned_true_residual": rln[-1][1]["trueResidual"],
         "learned_precond_fallbacks": rln[-1][1]["precondFallbacks"],
         "speedup_learned_vs_plain": round(t64 / tln, 3),
+        # what the network buys and costs, measured: FCG device time (the solver's own events, network not
+        # included) saved against the multigrid with constants, and the time outside the solver (network
+        # inference and the finest-level weight setup) over the constant-weight solve's
+        "learned_fcg_device_ms": round(float(np.mean([r["ms"] for _, r in rln])), 3),
+        "mg_fcg_device_ms": round(float(np.mean([r["ms"] for _, r in rmg])), 3),
+        "learned_saved_fcg_ms": round(float(np.mean([r["ms"] for _, r in rmg]) - np.mean([r["ms"] for _, r in rln])),
+                                      3),
+        "learned_inference_ms": round(float(np.mean([w - r["ms"] for w, r in rln]) -
+                                            np.mean([w - r["ms"] for w, r in rmg])), 3),
         "learned_preconditioner": "3-layer MPNN (width 32, FP16 MFMA) giving the finest-level smoother weight and "
                                   "transfer scale of an aggregation K-cycle inside flexible CG; weights "
                                   "chuanflow_amd/weights/mpnn_pressure.npy trained on the GPU on pressure "

@@ -70,6 +70,11 @@ struct cfx_matrix
     cfx_mesh* mesh = nullptr;
     int ncomp = 1;
     DevBuf diag, rhs, rhsBase, upper, lower, val, val32, tmp;
+    // values version: bumped by every call that changes the matrix values; a solve whose version equals
+    // the last solve's reuses what depends on the values alone (multigrid hierarchy, network features).
+    // Raw device access to the values (cfx_matrix_device_ptrs) turns the reuse off for good.
+    std::uint64_t version = 1, solvedVersion = 0;
+    bool rawAccess = false;
     double relaxation = 1.0;     // Schemes::relaxation of this matrix
     DevBuf relaxOld;             // the solution before the final pass ([ncomp][n])
     MgCache mg;
@@ -98,6 +103,8 @@ struct cfx_csr
     MgCache mg;
     cfx_mpnn* net = nullptr;     // learned finest-level weights (precond LEARNED)
     const double* geo = nullptr; // caller's per-slot geometry (optional)
+    // values version (bumped by set_values) and the version the network's features were formed for
+    std::uint64_t version = 1, netVersion = 0;
     DevBuf wa, wt;
     std::unique_ptr<Dic> dic;    // precond DIC
     bool dicValid = false;
@@ -615,13 +622,19 @@ int cfx_matrix_destroy(cfx_matrix* A)
 int cfx_matrix_reset(cfx_matrix* A)
 {
     if (!A) return CFX_E_INVALID;
-    return guard(A->mesh->owner, [&] { lduReset(*A->mesh->m, A->ldu()); });
+    return guard(A->mesh->owner, [&] {
+        ++A->version;
+        lduReset(*A->mesh->m, A->ldu());
+    });
 }
 
 int cfx_assemble_laplacian(cfx_matrix* A, const double* gf, const double* linc, const double* link, int sign)
 {
     if (!A) return CFX_E_INVALID;
-    return guard(A->mesh->owner, [&] { assembleLaplacian(*A->mesh->m, A->ldu(), gf, linc, link, sign); });
+    return guard(A->mesh->owner, [&] {
+        ++A->version;
+        assembleLaplacian(*A->mesh->m, A->ldu(), gf, linc, link, sign);
+    });
 }
 
 int cfx_assemble_div(cfx_matrix* A, const double* phi, int scheme, double blend, const double* linc,
@@ -630,6 +643,7 @@ int cfx_assemble_div(cfx_matrix* A, const double* phi, int scheme, double blend,
     if (!A) return CFX_E_INVALID;
     return guard(A->mesh->owner, [&] {
         need(scheme >= 0 && scheme <= 2, "div scheme must be 0 upwind, 1 linear, 2 blend");
+        ++A->version;
         assembleDiv(*A->mesh->m, A->ldu(), phi, scheme, blend, linc, link, sign);
     });
 }
@@ -639,6 +653,7 @@ int cfx_assemble_ddt(cfx_matrix* A, const double* old, double dt, int sign)
     if (!A) return CFX_E_INVALID;
     return guard(A->mesh->owner, [&] {
         need(dt > 0, "fvm::ddt: non-positive time step");
+        ++A->version;
         assembleDdt(*A->mesh->m, A->ldu(), old, dt, sign);
     });
 }
@@ -672,6 +687,7 @@ int cfx_matrix_device_ptrs(cfx_matrix* A, double** d_diag, double** d_rhs)
 {
     if (!A) return CFX_E_INVALID;
     return guard(A->mesh->owner, [&] {
+        A->rawAccess = true;  // the caller may change the diagonal at any time
         if (d_diag) *d_diag = A->diag.as<double>();
         if (d_rhs) *d_rhs = A->rhs.as<double>();
     });
@@ -737,6 +753,9 @@ int cfx_matrix_solve(cfx_matrix* A, double* x, int solver, double tol, int maxIt
         a.gradCorrected = gradCorrected != 0;
         a.precision = precision;
         a.relaxation = A->relaxation;
+        need(!multigrid || A->ncomp == 1, "the multigrid preconditioner solves scalar matrices");
+        a.sameMatrix = !A->rawAccess && A->version == A->solvedVersion;
+        A->solvedVersion = A->version;
         if (A->relaxation < 1.0)
         {
             const std::size_t need = sizeof(double) * A->ncomp * m.n;
@@ -841,6 +860,7 @@ int cfx_csr_set_values(cfx_csr* A, const double* vals)
     return guard(A->owner, [&] {
         CFX_HIP(hipMemcpyAsync(A->val.p, vals, sizeof(double) * A->pattern.nnz, hipMemcpyHostToDevice,
                                A->owner->ctx->stream()));
+        ++A->version;
         A->val32Valid = false;
         A->mg.valid = false;
         A->dicValid = false;
@@ -854,6 +874,7 @@ int cfx_csr_set_values_device(cfx_csr* A, const double* d_vals)
     return guard(A->owner, [&] {
         CFX_HIP(hipMemcpyAsync(A->val.p, d_vals, sizeof(double) * A->pattern.nnz, hipMemcpyDeviceToDevice,
                                A->owner->ctx->stream()));
+        ++A->version;
         A->val32Valid = false;
         A->mg.valid = false;
         A->dicValid = false;
@@ -907,11 +928,12 @@ int cfx_csr_solve_device(cfx_csr* A, const double* b, double* x, double tol, int
                     A->wa.alloc(sizeof(float) * A->pattern.n);
                     A->wt.alloc(sizeof(float) * A->pattern.n);
                 }
+                // features from the matrix alone are reused while its values are those of the last learned solve
                 learnedWeights(*A->net->net, A->pattern, A->val.as<double>(), A->geo, b, A->wa.as<float>(),
-                               A->wt.as<float>());
-                A->mg.pre = A->mg.post = A->wa.as<float>();
-                A->mg.scale = A->mg.restr = A->wt.as<float>();
-                A->mg.valid = false;
+                               A->wt.as<float>(), A->netVersion == A->version);
+                A->netVersion = A->version;
+                A->mg.useWeights(A->wa.as<float>(), A->wt.as<float>(), A->wa.as<float>(), A->wt.as<float>());
+                A->mg.weightsValid = false;  // new contents
             }
             Multigrid& mg = A->mg.ready(ctx, A->pattern, A->val.as<double>());
             s = A->owner->solver(A->pattern.n, A->pattern.nRowBlocks)
@@ -1023,7 +1045,7 @@ void setMgWeights(MgCache& mg, const float* pre, const float* scale, const float
     mg.scale = scale;
     mg.post = post;
     mg.restr = restr;
-    mg.valid = false;
+    mg.weightsValid = false;
 }
 } // namespace
 } // extern "C++"
@@ -1504,8 +1526,8 @@ int cfx_csr_set_mpnn(cfx_csr* A, cfx_mpnn* m, const double* geo)
         need(!m || m->owner == A->owner, "network of another context");
         A->net = m;
         A->geo = geo;
-        if (!m) A->mg.pre = A->mg.post = A->mg.scale = A->mg.restr = nullptr;
-        A->mg.valid = false;
+        A->netVersion = 0;  // new network or geometry: its features are formed again
+        if (!m) A->mg.useWeights(A->mg.userPre, A->mg.userScale, A->mg.userPost, A->mg.userRestr);
     });
 }
 
@@ -1524,8 +1546,8 @@ int cfx_matrix_set_mpnn(cfx_matrix* A, cfx_mpnn* m)
     return guard(A->mesh->owner, [&] {
         need(!m || m->owner == A->mesh->owner, "network of another context");
         A->net = m;
-        if (!m) A->mg.pre = A->mg.post = A->mg.scale = A->mg.restr = nullptr;
-        A->mg.valid = false;
+        A->solvedVersion = 0;  // the next solve forms the network's features again
+        if (!m) A->mg.useWeights(A->mg.userPre, A->mg.userScale, A->mg.userPost, A->mg.userRestr);
     });
 }
 

@@ -155,6 +155,12 @@ public:
     // a, s, c, t; null takes the constant from `p`.
     void setMatrix(const double* fineVal64, const float* pre, const float* scale, const float* post,
                    const float* restr, const MgParams& p);
+    // The two halves of setMatrix: what depends on the matrix values (Galerkin values, coarse-level setup
+    // with the constants, the coarsest inverse, the finest FP32 copy), and what depends on the finest-level
+    // weights (its inverse diagonal and weights, column-scaled values and restriction values), which reads
+    // the finest values given to the last setValues.  New weights on unchanged values need only the second.
+    void setValues(const double* fineVal64, const MgParams& p);
+    void setWeights(const float* pre, const float* scale, const float* post, const float* restr, const MgParams& p);
     // z = V-cycle(r) on the finest level.  When partZQ is non-null also writes
     // per-workgroup partials of z.q (grid Gs with rbPerWg row blocks each).
     void apply(const float* r, float* z, const float* q, double* partZQ, int Gs, int rbPerWg,
@@ -180,6 +186,7 @@ private:
     MgParams p_;
     std::uintptr_t id_ = 0;
     std::uintptr_t structVersion_ = 0;
+    const double* fineVal64_ = nullptr;  // the finest values of the last setValues
     int tailFrom_ = -1;
     std::vector<MgLevel> lv_;  // lv_[0] uses the fine pattern (pat unused)
     // out = cycle of level l applied to in (in, out distinct from the level's work vectors)
@@ -215,13 +222,14 @@ struct MgCache
     // the caller's own weights (null: the constants); a learned solve points pre..restr at the network's
     // outputs, a plain multigrid solve back at these
     const float *userPre = nullptr, *userScale = nullptr, *userPost = nullptr, *userRestr = nullptr;
-    bool valid = false;
+    bool valid = false;         // the hierarchy holds the current matrix values (and weights)
+    bool weightsValid = false;  // it holds the current finest-level weights
 
-    // point the finest-level weights at w (pre = w0, scale = w1, post = w2, restrict = w3); a change
-    // invalidates the hierarchy's values
+    // point the finest-level weights at the arrays (pre a, scale s, post c, restrict t); a change of
+    // arrays invalidates the weights (new contents of the same arrays: the caller clears weightsValid)
     void useWeights(const float* a, const float* s, const float* c, const float* t)
     {
-        if (a != pre || s != scale || c != post || t != restr) valid = false;
+        if (a != pre || s != scale || c != post || t != restr) weightsValid = false;
         pre = a;
         scale = s;
         post = c;
@@ -241,7 +249,12 @@ struct MgCache
         if (!valid)
         {
             mg->setMatrix(val, pre, scale, post, restr, p);
-            valid = true;
+            valid = weightsValid = true;
+        }
+        else if (!weightsValid)
+        {
+            mg->setWeights(pre, scale, post, restr, p);
+            weightsValid = true;
         }
         return *mg;
     }

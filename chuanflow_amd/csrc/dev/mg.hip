@@ -124,8 +124,8 @@ __global__ void k_scale_vals(CsrView A, const double* __restrict__ val64, float*
          k += static_cast<long long>(gridDim.x) * blockDim.x)
     {
         const float v = static_cast<float>(val64[k]);
-        val32[k] = v;
-        valS[k] = v * wD32[A.col[k]];
+        if (val32) val32[k] = v;
+        if (valS) valS[k] = v * wD32[A.col[k]];
     }
 }
 
@@ -1350,49 +1350,72 @@ Multigrid::Multigrid(Context& ctx, const DevPattern& fine, int maxLevels, int co
 void Multigrid::setMatrix(const double* fineVal64, const float* pre, const float* scale, const float* post,
                           const float* restr, const MgParams& p)
 {
+    setValues(fineVal64, p);
+    setWeights(pre, scale, post, restr, p);
+}
+
+void Multigrid::setValues(const double* fineVal64, const MgParams& p)
+{
     hipStream_t s = ctx_.stream();
     if (p.kcycleDepth != p_.kcycleDepth || p.fusion != p_.fusion || p.coarseSweeps != p_.coarseSweeps ||
         p.coarsestOmega != p_.coarsestOmega)
         ++structVersion_;
     p_ = p;
+    fineVal64_ = fineVal64;
     for (std::size_t l = 0; l < lv_.size(); ++l)
     {
         MgLevel& L = lv_[l];
         const CsrView A = l == 0 ? fine_.view() : L.pat.view();
         const double* v64 = l == 0 ? fineVal64 : L.val64.as<double>();
-        if (l > 0)
+        if (l == 0)
         {
-            const MgLevel& F = lv_[l - 1];
-            const double* fv = l == 1 ? fineVal64 : F.val64.as<double>();
-            // lanes per coarse nonzero from the mean number of fine nonzeros it sums
-            const double per = static_cast<double>(F.nnz) / std::max(1, L.nnz);
-            if (per >= 48.0)
-                k_galerkin<64><<<divUp(64LL * L.nnz, B), B, 0, s>>>(L.nnz, F.gStart.as<int>(), F.gPerm.as<int>(),
-                                                                   fv, L.val64.as<double>());
-            else if (per >= 3.0)
-                k_galerkin<8><<<divUp(8LL * L.nnz, B), B, 0, s>>>(L.nnz, F.gStart.as<int>(), F.gPerm.as<int>(),
-                                                                 fv, L.val64.as<double>());
-            else
-                k_galerkin<1><<<divUp(L.nnz, B), B, 0, s>>>(L.nnz, F.gStart.as<int>(), F.gPerm.as<int>(), fv,
-                                                           L.val64.as<double>());
+            // the FP32 copy of the finest values (its weight-dependent arrays are setWeights')
+            k_scale_vals<<<std::max(1, std::min(8192, divUp(L.nnz, B))), B, 0, s>>>(A, v64, L.val32.as<float>(),
+                                                                                   nullptr, nullptr);
+            continue;
         }
-        const bool fineLevel = l == 0;
-        const float aC = fineLevel ? p.omegaPre : p.coarseOmega;
-        const float cC = fineLevel ? p.omegaPost : p.coarseOmega;
-        const float sC = fineLevel ? p.scale : p.coarseScale;
-        k_level_setup<<<divUp(L.n, B), B, 0, s>>>(A, v64, L.invD32.as<float>(),
-                                                  L.wPre32.as<float>(), L.wPost32.as<float>(), L.s32.as<float>(),
-                                                  L.t32.as<float>(), fineLevel ? pre : nullptr, aC,
-                                                  fineLevel ? scale : nullptr, sC, fineLevel ? post : nullptr, cC,
-                                                  fineLevel ? restr : nullptr, fineLevel ? p.restrictScale : 1.0f);
+        const MgLevel& F = lv_[l - 1];
+        const double* fv = l == 1 ? fineVal64 : F.val64.as<double>();
+        // lanes per coarse nonzero from the mean number of fine nonzeros it sums
+        const double per = static_cast<double>(F.nnz) / std::max(1, L.nnz);
+        if (per >= 48.0)
+            k_galerkin<64><<<divUp(64LL * L.nnz, B), B, 0, s>>>(L.nnz, F.gStart.as<int>(), F.gPerm.as<int>(), fv,
+                                                               L.val64.as<double>());
+        else if (per >= 3.0)
+            k_galerkin<8><<<divUp(8LL * L.nnz, B), B, 0, s>>>(L.nnz, F.gStart.as<int>(), F.gPerm.as<int>(), fv,
+                                                             L.val64.as<double>());
+        else
+            k_galerkin<1><<<divUp(L.nnz, B), B, 0, s>>>(L.nnz, F.gStart.as<int>(), F.gPerm.as<int>(), fv,
+                                                       L.val64.as<double>());
+        // coarse levels take the constants
+        k_level_setup<<<divUp(L.n, B), B, 0, s>>>(A, v64, L.invD32.as<float>(), L.wPre32.as<float>(),
+                                                  L.wPost32.as<float>(), L.s32.as<float>(), L.t32.as<float>(), nullptr,
+                                                  p.coarseOmega, nullptr, p.coarseScale, nullptr, p.coarseOmega,
+                                                  nullptr, 1.0f);
         k_scale_vals<<<std::max(1, std::min(8192, divUp(L.nnz, B))), B, 0, s>>>(A, v64, L.val32.as<float>(),
                                                                                L.wPre32.as<float>(),
                                                                                L.valS32.as<float>());
         if (L.dense.p) k_mg_dense_inverse<<<1, 1024, 0, s>>>(A, L.val64.as<double>(), L.dense.as<float>());
         if (L.rval.p)
-            k_restrict_vals<<<divUp(L.n, B), B, 0, s>>>(L.n, L.aggMem.as<int>(), l == 0 ? L.t32.as<float>() : nullptr,
-                                                        L.rval.as<float>());
+            k_restrict_vals<<<divUp(L.n, B), B, 0, s>>>(L.n, L.aggMem.as<int>(), nullptr, L.rval.as<float>());
     }
+    CFX_CHECK_LAUNCH();
+}
+
+void Multigrid::setWeights(const float* pre, const float* scale, const float* post, const float* restr,
+                           const MgParams& p)
+{
+    if (!fineVal64_) throw std::logic_error("multigrid: weights set before the matrix values");
+    hipStream_t s = ctx_.stream();
+    MgLevel& L = lv_[0];
+    const CsrView A = fine_.view();
+    k_level_setup<<<divUp(L.n, B), B, 0, s>>>(A, fineVal64_, L.invD32.as<float>(), L.wPre32.as<float>(),
+                                              L.wPost32.as<float>(), L.s32.as<float>(), L.t32.as<float>(), pre,
+                                              p.omegaPre, scale, p.scale, post, p.omegaPost, restr, p.restrictScale);
+    k_scale_vals<<<std::max(1, std::min(8192, divUp(L.nnz, B))), B, 0, s>>>(A, fineVal64_, nullptr,
+                                                                           L.wPre32.as<float>(), L.valS32.as<float>());
+    if (L.rval.p)
+        k_restrict_vals<<<divUp(L.n, B), B, 0, s>>>(L.n, L.aggMem.as<int>(), L.t32.as<float>(), L.rval.as<float>());
     CFX_CHECK_LAUNCH();
 }
 

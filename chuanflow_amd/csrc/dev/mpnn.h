@@ -53,7 +53,9 @@ public:
 
     // Inputs of a graph: CSR values (FP64), optional per-slot geometry g and
     // optional residual r0 (FP64, device).  Computes node/edge features.
-    void setGraph(const CsrView& A, const double* val, const double* geo, const double* r0);
+    // sameGraph: the values are those of the previous call (same arrays, same
+    // contents), so only the residual feature is computed again.
+    void setGraph(const CsrView& A, const double* val, const double* geo, const double* r0, bool sameGraph = false);
     // a, t per row (device FP32).  `train` keeps the activations for backward.
     void forward(float* a, float* t, bool train);
     // dL/dparams (device FP32 [nParams]) from dL/da, dL/dt (device FP32 [n])
@@ -73,6 +75,8 @@ private:
     DevBuf h_[2], xs_[MpnnShape::L + 1], mask_[MpnnShape::L + 1], o_;
     DevBuf dh_[2], dmg_, part_;
     int adamStep_ = 0;
+    bool featuresValid_ = false;  // x0_, ew_, eg_ hold the features of (A_, lastVal_, lastGeo_)
+    const double *lastVal_ = nullptr, *lastGeo_ = nullptr;
     void reserve(int n, int nnz);
     void refreshT();
 };

@@ -526,16 +526,19 @@ SolveReport fvSolve(DevMesh& m, Krylov& K, const Ldu& A, double* val, double* rh
             }
             else if (a.mg)
             {
+                // the values are those of the hierarchy's last setup in the later passes of this solve and for
+                // a repeated matrix (one component: a multigrid solve is scalar)
+                const bool sameValues = (pass > 0 || a.sameMatrix) && A.ncomp == 1;
                 if (a.net)
                 {
-                    a.net->setGraph(view, v, a.geo, b);
+                    // the network's weights follow the right-hand side: new weights on every pass; its
+                    // matrix-only features and the hierarchy's values stay while the values do
+                    a.net->setGraph(view, v, a.geo, b, sameValues);
                     a.net->forward(a.wa, a.wt, false);
-                    a.mg->pre = a.mg->post = a.wa;
-                    a.mg->scale = a.mg->restr = a.wt;
+                    a.mg->useWeights(a.wa, a.wt, a.wa, a.wt);
+                    a.mg->weightsValid = false;
                 }
-                // new values: set the hierarchy up again, except for the later passes of this solve and a
-                // repeated matrix when no network makes the weights depend on the right-hand side
-                if (a.net || (pass == 0 && !a.sameMatrix)) a.mg->valid = false;
+                if (!sameValues) a.mg->valid = false;
                 Multigrid& M = a.mg->ready(*m.ctx, m.pattern, v);
                 last = K.fcgMixed(view, v, M, b, xk, a.tol, a.maxIter, a.delta, nullptr);
             }

@@ -61,8 +61,10 @@ private:
 void meshSlotGeometry(const DevMesh& m, double* geo);
 
 // weights of a learned solve: features of (pattern, values, geometry, rhs),
-// network forward, tied weights a (pre = post) and t (restrict = prolong)
+// network forward, tied weights a (pre = post) and t (restrict = prolong);
+// sameGraph: the values are those of the network's previous graph (only the
+// residual feature is formed again)
 void learnedWeights(Mpnn& net, const DevPattern& pat, const double* val, const double* geo, const double* rhs,
-                    float* a, float* t);
+                    float* a, float* t, bool sameGraph = false);
 
 } // namespace cfx

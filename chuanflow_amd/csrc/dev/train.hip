@@ -66,9 +66,9 @@ void meshSlotGeometry(const DevMesh& m, double* geo)
 }
 
 void learnedWeights(Mpnn& net, const DevPattern& pat, const double* val, const double* geo, const double* rhs,
-                    float* a, float* t)
+                    float* a, float* t, bool sameGraph)
 {
-    net.setGraph(pat.view(), val, geo, rhs);
+    net.setGraph(pat.view(), val, geo, rhs, sameGraph);
     net.forward(a, t, false);
 }
 
