@@ -215,7 +215,8 @@ def bench_c5(args):
     best, table = N.batch_autotune(ctx, [b["csr"] for b in blocks], [b["b"] for b in blocks], xs, iters=20)
     tune_s = time.perf_counter() - tt
     log(f"autotuned {best.as_dict()} in {tune_s:.2f}s")
-    one, bat, bdef = [], [], []
+    one, bat, bdef, b3 = [], [], [], []
+    persist = N.BatchSchedule(4, 2048, 1, 16, persistent=1)  # one launch for the whole batch
     for i in range(args.warmup + args.steps):
         ctx.sync()
         t1 = time.perf_counter()
@@ -225,19 +226,24 @@ def bench_c5(args):
         xb, rb, fin, st = DB.solve_batched(ctx, blocks, args.tol, schedule=best)
         ctx.sync()
         t3 = time.perf_counter()
-        xd, rd, find, std = DB.solve_batched(ctx, blocks, args.tol, schedule=N.BatchSchedule(4, 2048, 1, 16))
+        xd, rd, find, std = DB.solve_batched(ctx, blocks, args.tol, schedule=N.BatchSchedule())
         ctx.sync()
         t4 = time.perf_counter()
+        x3, r3, fin3, st3 = DB.solve_batched(ctx, blocks, args.tol, schedule=persist)
+        ctx.sync()
+        t5 = time.perf_counter()
         if i >= args.warmup:
             one.append((t2 - t1) * 1e3)
             bat.append((t3 - t2) * 1e3)
             bdef.append((t4 - t3) * 1e3)
+            b3.append((t5 - t4) * 1e3)
         log(f"step {i}: one by one {(t2 - t1) * 1e3:.1f} ms (device {sum(r.ms for r in r1):.1f}), batched tuned "
-            f"{(t3 - t2) * 1e3:.1f} ms (device {st.ms:.1f}), batched default {(t4 - t3) * 1e3:.1f} ms")
+            f"{(t3 - t2) * 1e3:.1f} ms (device {st.ms:.1f}), batched default {(t4 - t3) * 1e3:.1f} ms "
+            f"(device {std.ms:.1f}), persistent {(t5 - t4) * 1e3:.1f} ms (device {st3.ms:.1f})")
     # the coupling loop's pass over its blocks (lossAt: 3 warm-started GPU SIMPLE iterations each), one
     # block after another and on concurrent streams (one context per block, host threads)
     passes = {}
-    for pname, prec in (("jacobi", N.FP32_IR), ("multigrid", N.FP32_MG)):
+    for pname, prec in (("jacobi", N.FP32_IR), ("multigrid", N.FP32_MG)) if args.c5_passes else ():
         for threads in (1, 4, 8):
             ms, dps = DB.coupling_pass(blocks, 3, threads, prec)
             passes[f"{pname}_{threads}"] = ms
@@ -248,7 +254,7 @@ def bench_c5(args):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O  # noqa: E402  (CPU baseline only)
     cpu = None
-    if O.have_ref():
+    if O.have_ref() and not args.no_cpu:
         R = O.ref()
         total_ms, sample = 0.0, 20
         t_all = time.perf_counter()
@@ -270,13 +276,13 @@ def bench_c5(args):
         cpu = {"value": round(total_ms, 1), "unit": "ms", "cores": 1, "kind": "reference",
                "sample": f"{sample} Jacobi-PCG iterations of every block's system with the reference build's "
                          f"solveCG (FP64, 1 thread, {model}), each extrapolated to the iterations that block takes"}
-    bitwise = all(np.array_equal(a.get(), b.get()) for a, b in zip(x1, xd))
+    bitwise = all(np.array_equal(a.get(), b.get()) and np.array_equal(a.get(), c.get()) for a, b, c in zip(x1, xd, x3))
     conv = all(r.converged for r in rb) and all(r.converged for r in r1)
     tb, to = float(np.mean(bat)), float(np.mean(one))
     per_block = [{"cells": b["n"], "shape": b["spec"]["shape"], "aspect": round(b["spec"]["length"] / b["spec"]["width"], 2),
                   "iterations": r.iterations, "alone_ms": round(ra.ms, 3), "batched_finish_ms": round(f, 3),
-                  "dp_Pa": round(b["dp"], 6)}
-                 for b, r, ra, f in zip(blocks, rb, r1, fin)]
+                  "persistent_finish_ms": round(fp, 3), "dp_Pa": round(b["dp"], 6)}
+                 for b, r, ra, f, fp in zip(blocks, rb, r1, fin, fin3)]
     result = {
         "metric": "detail-region pressure systems solved to 1e-8: batched vs one by one",
         "value": round(tb, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -287,6 +293,11 @@ def bench_c5(args):
                                f"systems each; FP64 Jacobi-PCG to 1e-8", "cells": cells, "blocks": len(blocks),
                    "parallelism": "single GPU"},
         "one_by_one_ms": round(to, 3), "batched_default_ms": round(float(np.mean(bdef)), 3),
+        "batched_persistent_ms": round(float(np.mean(b3)), 3),
+        "batched_forms": "default: A, B, C launches per iteration over all systems (captured hipGraph), host "
+                         "polling every 16 iterations; persistent: one launch (captured hipGraph) for the whole "
+                         "batch, workgroups claiming chunks of any unfinished system; both bitwise equal to the "
+                         "solves alone",
         "speedup_batched_vs_one_by_one": round(to / tb, 3), "autotune_s": round(tune_s, 3),
         "schedule": best.as_dict(), "candidates": [{"schedule": c, "ms": round(m, 4)} for c, m in table],
         "max_iterations": st.maxIterations, "all_converged": conv, "default_schedule_bitwise_equal": bitwise,
@@ -637,6 +648,8 @@ def main():
     ap.add_argument("--c3-momentum", default="fp32ir", choices=["fp32ir", "fp64"],
                     help="momentum BiCGStab: FP32 with FP64 refinement, or FP64")
     ap.add_argument("--blocks", type=int, default=64)
+    ap.add_argument("--c5-passes", type=int, default=1, choices=[0, 1],
+                    help="c5: also time the coupling pass over the blocks")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is described, "

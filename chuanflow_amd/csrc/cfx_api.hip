@@ -1573,6 +1573,11 @@ std::vector<BatchSystem> batchSystems(cfx_ctx* c, int count, cfx_csr* const* sys
 } // namespace
 } // extern "C++"
 
+static cfx_batch_schedule toSchedule(const BatchSchedule& b)
+{
+    return cfx_batch_schedule{b.vecElems, b.wgCap, b.graph, b.pollEvery, b.persistent, b.claim, b.wgs};
+}
+
 int cfx_batch_solve(cfx_ctx* c, int count, cfx_csr* const* systems, const double* const* b, double* const* x,
                     double tol, int maxIter, const cfx_batch_schedule* sched, cfx_report* reps, double* finishMs,
                     cfx_batch_stats* stats)
@@ -1590,6 +1595,9 @@ int cfx_batch_solve(cfx_ctx* c, int count, cfx_csr* const* systems, const double
             sc.wgCap = sched->wgCap;
             sc.graph = sched->graph;
             sc.pollEvery = sched->pollEvery;
+            sc.persistent = sched->persistent;
+            sc.claim = sched->claim;
+            sc.wgs = sched->wgs;
         }
         else
             sc = c->batch->autotune(sys, 20, nullptr);
@@ -1603,7 +1611,7 @@ int cfx_batch_solve(cfx_ctx* c, int count, cfx_csr* const* systems, const double
         {
             stats->ms = r.ms;
             stats->maxIterations = r.maxIterations;
-            stats->used = cfx_batch_schedule{sc.vecElems, sc.wgCap, sc.graph, sc.pollEvery};
+            stats->used = toSchedule(sc);
         }
     });
 }
@@ -1618,11 +1626,10 @@ int cfx_batch_autotune(cfx_ctx* c, int count, cfx_csr* const* systems, const dou
         std::vector<BatchSystem> sys = batchSystems(c, count, systems, b, x);
         std::vector<std::pair<BatchSchedule, double>> log;
         const BatchSchedule b = c->batch->autotune(sys, iters, &log);
-        *best = cfx_batch_schedule{b.vecElems, b.wgCap, b.graph, b.pollEvery};
+        *best = toSchedule(b);
         for (int k = 0; k < std::min<int>(maxCand, static_cast<int>(log.size())); ++k)
         {
-            if (cand) cand[k] = cfx_batch_schedule{log[k].first.vecElems, log[k].first.wgCap, log[k].first.graph,
-                                                   log[k].first.pollEvery};
+            if (cand) cand[k] = toSchedule(log[k].first);
             if (candMs) candMs[k] = log[k].second;
         }
     });

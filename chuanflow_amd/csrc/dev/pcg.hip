@@ -623,6 +623,240 @@ __global__ __launch_bounds__(kB) void k_bpcg_C(const BatchSysDev* __restrict__ S
                        s.st->kcur, tol, maxIter, 0.0, 0, nullptr, t.y, s.Gv);
 }
 
+// ------------------------------------------- persistent batched Jacobi-PCG
+// The whole batch in one launch.  Each system walks the steps residual, C(0),
+// then A, B, C per iteration; step `st` of system s consists of the chunks
+// that the launches above would give their workgroups (Gs for residual and
+// A, Gv for C(0), B and C), and chunk c runs exactly the body workgroup c
+// runs there, so every partial and every iterate is the same.  Workgroups
+// are not tied to systems: wave 0 of each workgroup looks at every system's
+// control word (step << 32 | chunks claimed), claims a run of chunks (the
+// system's claim size) of a step that has some left with one atomic add,
+// runs them, and counts them finished; the workgroup that finishes a step's last chunk moves the system
+// to its next step (or retires it once C has set `done`).  Release/acquire
+// fences at agent scope order each step's results before the next step's
+// claims across the XCDs' L2s.  No workgroup ever waits for another one, so
+// the launch needs no co-residency; one that finds nothing to claim sleeps
+// briefly and looks again.  It exits once no system is active, or -- the
+// bound on every wait -- once no step anywhere has finished for `stallTicks`
+// of the wall clock, which sets the abort word for the host to report.
+constexpr unsigned int kStepDone = 0xffffffffu;
+// each system's claim word and finish counter sit on cache lines of their own
+constexpr int kCtlStride = 256;
+
+struct BatchCtl
+{
+    char* word;                  // [K], stride kCtlStride: u64 step << 32 | chunks claimed
+    char* fin;                   // [K], stride kCtlStride: u32 chunks finished of the current step
+    long long* finish;           // [K] wall-clock tick at which the system stopped
+    unsigned int* glob;          // active systems, abort, steps finished
+    long long* t0;               // start tick
+    const int4* chunks;          // [K] (Gs, Gv, chunks per claim of residual/A steps, of C/B steps)
+};
+__device__ __forceinline__ unsigned long long* ctlWord(const BatchCtl& c, int s)
+{
+    return reinterpret_cast<unsigned long long*>(c.word + static_cast<std::size_t>(s) * kCtlStride);
+}
+__device__ __forceinline__ unsigned int* ctlFin(const BatchCtl& c, int s)
+{
+    return reinterpret_cast<unsigned int*>(c.fin + static_cast<std::size_t>(s) * kCtlStride);
+}
+
+// step kinds: 0 residual, 1 C(0), then A, B, C; residual and A have Gs chunks, the others Gv
+__device__ __forceinline__ int stepKind(unsigned int step) { return step < 2 ? static_cast<int>(step) : 2 + static_cast<int>((step - 2) % 3); }
+__device__ __forceinline__ bool spmvStep(unsigned int step)
+{
+    const int kind = stepKind(step);
+    return kind == 0 || kind == 2;
+}
+__device__ __forceinline__ unsigned int stepChunks(int4 g, unsigned int step)
+{
+    return static_cast<unsigned int>(spmvStep(step) ? g.x : g.y);
+}
+__device__ __forceinline__ unsigned int stepClaim(int4 g, unsigned int step)
+{
+    return static_cast<unsigned int>(spmvStep(step) ? g.z : g.w);
+}
+
+__global__ void k_bpcg_ctl_init(BatchCtl c, CgState* __restrict__ st, int K)
+{
+    for (int s = threadIdx.x; s < K; s += blockDim.x)
+    {
+        *ctlWord(c, s) = 0ull;
+        *ctlFin(c, s) = 0u;
+        c.finish[s] = -1;
+        unsigned int* w = reinterpret_cast<unsigned int*>(st + s);
+        for (int i = 0; i < static_cast<int>(sizeof(CgState) / 4); ++i) w[i] = 0u;
+    }
+    if (threadIdx.x == 0)
+    {
+        c.glob[0] = static_cast<unsigned int>(K);
+        c.glob[1] = 0u;
+        c.glob[2] = 0u;
+        *c.t0 = wall_clock64();
+    }
+}
+
+__global__ __launch_bounds__(kB) void k_bpcg_persistent(const BatchSysDev* __restrict__ S, int K, BatchCtl c,
+                                                        const int* __restrict__ home, double tol, int maxIter,
+                                                        long long stallTicks)
+{
+    __shared__ int sh[4];
+    int cursor = home[blockIdx.x];
+    bool idle = false;
+    unsigned int idleProgress = 0;
+    long long idleSince = 0;
+    int backoff = 1;  // sleeps of about 0.2 us between looks while nothing can be claimed (doubles to 32)
+    for (;;)
+    {
+        // ---- wave 0 looks for chunks to claim
+        if (threadIdx.x < kWave)
+        {
+            const int lane = static_cast<int>(threadIdx.x);
+            int got = -2;  // >= 0: system, -1: leave, -2: nothing to claim now
+            unsigned int gStep = 0, gChunk = 0, gCount = 0;
+            const bool over = __hip_atomic_load(c.glob, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u ||
+                              __hip_atomic_load(c.glob + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+            if (over) got = -1;
+            for (int base = 0; got == -2 && base < K; base += kWave)
+            {
+                const int s = (cursor + base + lane) % K;
+                bool avail = false;
+                unsigned int seen = 0;  // the step this lane saw
+                if (base + lane < K)
+                {
+                    const unsigned long long w = __hip_atomic_load(ctlWord(c, s), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    seen = static_cast<unsigned int>(w >> 32);
+                    avail = seen != kStepDone && static_cast<unsigned int>(w) < stepChunks(c.chunks[s], seen);
+                }
+                unsigned long long mask = __ballot(avail);
+                while (mask && got == -2)
+                {
+                    const int first = __ffsll(static_cast<long long>(mask)) - 1;
+                    mask &= mask - 1;
+                    int res = -2;
+                    unsigned int rs = 0, rc = 0, rn = 0;
+                    if (lane == first)
+                    {
+                        // the claim size is that of the step this lane saw; the add returns the step it
+                        // claims in, whose chunk count bounds what is taken
+                        const unsigned int take = stepClaim(c.chunks[s], seen);
+                        const unsigned long long w2 = __hip_atomic_fetch_add(
+                            ctlWord(c, s), static_cast<unsigned long long>(take), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        rs = static_cast<unsigned int>(w2 >> 32);
+                        rc = static_cast<unsigned int>(w2);
+                        if (rs != kStepDone)
+                        {
+                            const unsigned int n = stepChunks(c.chunks[s], rs);
+                            if (rc < n)
+                            {
+                                res = s;
+                                rn = min(take, n - rc);
+                            }
+                        }
+                    }
+                    got = __shfl(res, first);
+                    gStep = __shfl(rs, first);
+                    gChunk = __shfl(rc, first);
+                    gCount = __shfl(rn, first);
+                }
+            }
+            if (lane == 0)
+            {
+                if (got >= 0)
+                {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    idle = false;
+                    backoff = 1;
+                    cursor = got;
+                }
+                else if (got == -2)
+                {
+                    // bounded wait: give up only when no step anywhere has finished for stallTicks
+                    const unsigned int prog = __hip_atomic_load(c.glob + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const long long now = wall_clock64();
+                    if (!idle || prog != idleProgress)
+                    {
+                        idle = true;
+                        idleProgress = prog;
+                        idleSince = now;
+                    }
+                    else if (now - idleSince > stallTicks)
+                    {
+                        __hip_atomic_store(c.glob + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        got = -1;
+                    }
+                    for (int z = 0; z < backoff; ++z) __builtin_amdgcn_s_sleep(8);
+                    backoff = min(2 * backoff, 32);
+                }
+                sh[0] = got;
+                sh[1] = static_cast<int>(gStep);
+                sh[2] = static_cast<int>(gChunk);
+                sh[3] = static_cast<int>(gCount);
+            }
+        }
+        __syncthreads();
+        const int s = sh[0];
+        const unsigned int step = static_cast<unsigned int>(sh[1]);
+        const int c0 = sh[2], cn = sh[3];
+        __syncthreads();
+        if (s == -1) return;
+        if (s == -2) continue;
+        // ---- run chunks c0 .. c0 + cn - 1 of the step, as workgroups c of the launch
+        const BatchSysDev d = S[s];
+        const int kind = stepKind(step);
+        for (int ch = c0; ch < c0 + cn; ++ch)
+        {
+            if (kind == 0)
+                body_pcg_residual<double>(d.A, d.rbPerWg, d.val, d.x, d.b, nullptr, d.r, d.invD, d.part,
+                                          d.part + kRedMax, d.part + 2 * kRedMax, ch, d.Gs);
+            else if (kind == 1)
+                body_pcg_C<double>(d.n, d.invD, d.r, d.p, d.st, d.part, d.part + kRedMax, d.part + 2 * kRedMax, d.Gs, 0,
+                                   tol, maxIter, 0.0, 0, nullptr, ch, d.Gv);
+            else if (kind == 2)
+                body_pcg_A<double>(d.A, d.rbPerWg, d.val, d.p, d.q, d.st, d.part + 3 * kRedMax, ch, d.Gs);
+            else if (kind == 3)
+            {
+                if (!d.st->done)
+                {
+                    const int k = d.st->iter + 1;
+                    if (ch == 0 && threadIdx.x == 0) d.st->kcur = k;
+                    body_pcg_B<double>(d.n, d.invD, d.x, d.r, d.p, d.q, d.st, d.part + 3 * kRedMax, d.Gs, d.part,
+                                       d.part + kRedMax, k, ch, d.Gv);
+                }
+            }
+            else if (!d.st->done)
+                body_pcg_C<double>(d.n, d.invD, d.r, d.p, d.st, d.part, d.part + kRedMax, d.part + 2 * kRedMax, d.Gv,
+                                   d.st->kcur, tol, maxIter, 0.0, 0, nullptr, ch, d.Gv);
+            __syncthreads();
+        }
+        // ---- count them finished; the last one moves the system on.  Every wave
+        // releases its own stores (a workgroup barrier does not wait for them).
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            const unsigned int n = stepChunks(c.chunks[s], step);
+            const unsigned int f = __hip_atomic_fetch_add(ctlFin(c, s), static_cast<unsigned int>(cn), __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT);
+            if (f + static_cast<unsigned int>(cn) == n)
+            {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                __hip_atomic_store(ctlFin(c, s), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const bool stop = (kind == 1 || kind == 4) &&
+                                  __hip_atomic_load(&d.st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+                if (stop) c.finish[s] = wall_clock64();
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                __hip_atomic_store(ctlWord(c, s),
+                                   static_cast<unsigned long long>(stop ? kStepDone : step + 1) << 32,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (stop) __hip_atomic_fetch_sub(c.glob, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(c.glob + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+}
+
 } // namespace
 
 namespace
@@ -1071,7 +1305,7 @@ BatchResult BatchedPcg::solve(const std::vector<BatchSystem>& sys, double tol, i
     hipStream_t s = ctx_.stream();
     const std::size_t K = sys.size();
     if (K == 0) throw std::invalid_argument("batch: no systems");
-    if (sc.vecElems < 1 || sc.wgCap < 1 || sc.wgCap > kRedMax || sc.pollEvery < 1)
+    if (sc.vecElems < 1 || sc.wgCap < 1 || sc.wgCap > kRedMax || sc.pollEvery < 1 || sc.claim < 1 || sc.wgs < 0)
         throw std::invalid_argument("batch: bad schedule");
     std::size_t N = 0;
     for (const BatchSystem& b : sys) N += static_cast<std::size_t>(b.A.n);
@@ -1105,6 +1339,14 @@ BatchResult BatchedPcg::solve(const std::vector<BatchSystem>& sys, double tol, i
         off += static_cast<std::size_t>(b.A.n);
     }
     CFX_HIP(hipMemcpyAsync(sys_.p, hs.data(), sizeof(BatchSysDev) * K, hipMemcpyHostToDevice, s));
+    BatchResult res;
+    if (sc.persistent)
+    {
+        std::vector<int2> ch(K);
+        for (std::size_t q = 0; q < K; ++q) ch[q] = make_int2(hs[q].Gs, hs[q].Gv);
+        runPersistent(sys, ch, sc, tol, maxIter, N, res, res.finishMs);
+        return res;
+    }
     tabS_ = upload(tabS, s);
     tabV_ = upload(tabV, s);
     CFX_HIP(hipMemsetAsync(st, 0, sizeof(CgState) * K, s));
@@ -1113,7 +1355,6 @@ BatchResult BatchedPcg::solve(const std::vector<BatchSystem>& sys, double tol, i
     const int2* TS = tabS_.as<int2>();
     const int2* TV = tabV_.as<int2>();
     std::vector<CgState> h(K);
-    BatchResult res;
     res.finishMs.assign(K, -1.0);
     // capture (and instantiate) the iteration before the timed region
     hipGraphExec_t exec = nullptr;
@@ -1185,6 +1426,111 @@ BatchResult BatchedPcg::solve(const std::vector<BatchSystem>& sys, double tol, i
     return res;
 }
 
+void BatchedPcg::runPersistent(const std::vector<BatchSystem>& sys, const std::vector<int2>& grids,
+                               const BatchSchedule& sc, double tol, int maxIter, std::size_t N, BatchResult& res,
+                               std::vector<double>& finishMs)
+{
+    hipStream_t s = ctx_.stream();
+    const int K = static_cast<int>(sys.size());
+    int dev = 0, cus = 0, perCu = 0, rateKHz = 0;
+    CFX_HIP(hipGetDevice(&dev));
+    CFX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    CFX_HIP(hipDeviceGetAttribute(&rateKHz, hipDeviceAttributeWallClockRate, dev));
+    CFX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, reinterpret_cast<const void*>(k_bpcg_persistent), kB, 0));
+    const int G = sc.wgs > 0 ? sc.wgs : std::max(1, perCu) * std::max(1, cus);
+    if (rateKHz <= 0) throw std::runtime_error("batch: no device wall clock rate");
+    // first system each workgroup looks at: the one holding its share of the rows
+    std::vector<int> home(G);
+    {
+        std::size_t q = 0, end = static_cast<std::size_t>(sys[0].A.n);
+        for (int g = 0; g < G; ++g)
+        {
+            const std::size_t row = N * static_cast<std::size_t>(g) / static_cast<std::size_t>(G);
+            while (row >= end && q + 1 < sys.size()) end += static_cast<std::size_t>(sys[++q].A.n);
+            home[g] = static_cast<int>(q);
+        }
+    }
+    home_ = upload(home, s);
+    // chunks per claim: about `claim` claims per workgroup and step, the workgroups dealt to systems by rows
+    std::vector<int4> ch(sys.size());
+    for (std::size_t q = 0; q < sys.size(); ++q)
+    {
+        const double share = static_cast<double>(G) * sys[q].A.n / static_cast<double>(N);
+        const int wgs = std::max(1, static_cast<int>(share));
+        ch[q] = make_int4(grids[q].x, grids[q].y, std::max(1, divUp(grids[q].x, static_cast<long long>(sc.claim) * wgs)),
+                          std::max(1, divUp(grids[q].y, static_cast<long long>(sc.claim) * wgs)));
+    }
+    chunks_ = upload(ch, s);
+    // control block: claim words and finish counters (a cache line each), finish ticks (8 B) per system,
+    // then the start tick and 4 words
+    const std::size_t Kz = static_cast<std::size_t>(K);
+    const std::size_t bytes = 2 * kCtlStride * Kz + 8 * Kz + 16 + 16;
+    if (!ctl_.p || ctl_.bytes < bytes) ctl_.alloc(bytes);
+    char* base = ctl_.as<char>();
+    BatchCtl c;
+    c.word = base;
+    c.fin = base + kCtlStride * Kz;
+    c.finish = reinterpret_cast<long long*>(base + 2 * kCtlStride * Kz);
+    c.t0 = reinterpret_cast<long long*>(base + 2 * kCtlStride * Kz + 8 * Kz);
+    c.glob = reinterpret_cast<unsigned int*>(base + 2 * kCtlStride * Kz + 8 * Kz + 16);
+    c.chunks = chunks_.as<int4>();
+    CgState* st = state_.as<CgState>();
+    const BatchSysDev* S = sys_.as<BatchSysDev>();
+    // a wait ends after one second in which no step anywhere finished
+    const long long stallTicks = static_cast<long long>(rateKHz) * 1000;
+    auto launch = [&] {
+        k_bpcg_ctl_init<<<1, kB, 0, s>>>(c, st, K);
+        k_bpcg_persistent<<<G, kB, 0, s>>>(S, K, c, home_.as<int>(), tol, maxIter, stallTicks);
+    };
+    hipGraphExec_t exec = nullptr;
+    if (sc.graph)
+    {
+        hipGraph_t g;
+        CFX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        launch();
+        CFX_HIP(hipStreamEndCapture(s, &g));
+        CFX_HIP(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+        CFX_HIP(hipGraphDestroy(g));
+    }
+    CFX_HIP(hipEventRecord(e0_, s));
+    if (exec)
+        CFX_HIP(hipGraphLaunch(exec, s));
+    else
+        launch();
+    CFX_CHECK_LAUNCH();
+    CFX_HIP(hipEventRecord(e1_, s));
+    CFX_HIP(hipEventSynchronize(e1_));
+    if (exec) CFX_HIP(hipGraphExecDestroy(exec));
+    float ms = 0;
+    CFX_HIP(hipEventElapsedTime(&ms, e0_, e1_));
+    std::vector<CgState> h(Kz);
+    std::vector<long long> fin(Kz + 2);
+    unsigned int glob[4];
+    CFX_HIP(hipMemcpyAsync(h.data(), st, sizeof(CgState) * Kz, hipMemcpyDeviceToHost, s));
+    CFX_HIP(hipMemcpyAsync(fin.data(), c.finish, sizeof(long long) * (Kz + 1), hipMemcpyDeviceToHost, s));
+    CFX_HIP(hipMemcpyAsync(glob, c.glob, sizeof(glob), hipMemcpyDeviceToHost, s));
+    CFX_HIP(hipStreamSynchronize(s));
+    if (glob[1])
+        throw std::runtime_error("batch: the persistent solve stopped making progress (" + std::to_string(glob[0]) +
+                                 " systems unfinished)");
+    const long long t0 = fin[Kz];
+    res.ms = ms;
+    res.reports.resize(Kz);
+    finishMs.assign(Kz, ms);
+    for (std::size_t q = 0; q < Kz; ++q)
+    {
+        SolveReport& r = res.reports[q];
+        r.iterations = h[q].iter;
+        r.initialResidual = h[q].initialResidual;
+        r.finalResidual = h[q].iter > 0 ? h[q].rnorm : h[q].initialResidual;
+        r.converged = h[q].converged;
+        r.breakdown = h[q].breakdown;
+        if (fin[q] >= 0) finishMs[q] = static_cast<double>(fin[q] - t0) / rateKHz;
+        r.ms = finishMs[q];
+        res.maxIterations = std::max(res.maxIterations, r.iterations);
+    }
+}
+
 BatchSchedule BatchedPcg::autotune(const std::vector<BatchSystem>& sys, int iters,
                                    std::vector<std::pair<BatchSchedule, double>>* log)
 {
@@ -1205,28 +1551,45 @@ BatchSchedule BatchedPcg::autotune(const std::vector<BatchSystem>& sys, int iter
         b.x = xs_.as<double>() + off;
         off += static_cast<std::size_t>(b.A.n);
     }
-    BatchSchedule best;
-    double bestMs = 1e300;
+    std::vector<BatchSchedule> cands;
+    for (int ve : {2, 4, 8})
+        for (int cap : {512, 1024, 2048})
+            for (int claim : {1, 2, 4})
+            {
+                BatchSchedule c;
+                c.persistent = 1;
+                c.vecElems = ve;
+                c.wgCap = cap;
+                c.claim = claim;
+                cands.push_back(c);
+            }
     for (int graph : {1, 0})
         for (int ve : {2, 4, 8})
             for (int cap : {512, 1024, 2048})
             {
                 BatchSchedule c;
+                c.persistent = 0;
                 c.vecElems = ve;
                 c.wgCap = cap;
                 c.graph = graph;
                 c.pollEvery = iters;
-                for (std::size_t q = 0; q < sys.size(); ++q)
-                    CFX_HIP(hipMemcpyAsync(tmp[q].x, sys[q].x, sizeof(double) * sys[q].A.n, hipMemcpyDeviceToDevice, s));
-                // tol 0: no system stops early, every candidate runs `iters` iterations
-                const BatchResult r = solve(tmp, 0.0, iters, c);
-                if (log) log->push_back({c, r.ms});
-                if (r.ms < bestMs)
-                {
-                    bestMs = r.ms;
-                    best = c;
-                }
+                cands.push_back(c);
             }
+    BatchSchedule best;
+    double bestMs = 1e300;
+    for (const BatchSchedule& c : cands)
+    {
+        for (std::size_t q = 0; q < sys.size(); ++q)
+            CFX_HIP(hipMemcpyAsync(tmp[q].x, sys[q].x, sizeof(double) * sys[q].A.n, hipMemcpyDeviceToDevice, s));
+        // tol 0: no system stops early, every candidate runs `iters` iterations
+        const BatchResult r = solve(tmp, 0.0, iters, c);
+        if (log) log->push_back({c, r.ms});
+        if (r.ms < bestMs)
+        {
+            bestMs = r.ms;
+            best = c;
+        }
+    }
     best.pollEvery = 16;
     tuned_[key] = best;
     return best;

@@ -85,8 +85,9 @@ def solve_one_by_one(ctx, blocks, tol=1e-8, maxIter=20000):
     return xs, reps
 
 
-def solve_batched(ctx, blocks, tol=1e-8, maxIter=20000, schedule=None):
-    xs = [ctx.zeros(b["n"]) for b in blocks]
+def solve_batched(ctx, blocks, tol=1e-8, maxIter=20000, schedule=None, x0=None):
+    """x0: host initial guesses (default zero)."""
+    xs = [ctx.zeros(b["n"]) if x0 is None else ctx.array(x0[k]) for k, b in enumerate(blocks)]
     reps, fin, st = N.batch_solve(ctx, [b["csr"] for b in blocks], [b["b"] for b in blocks], xs, tol, maxIter,
                                   schedule)
     return xs, reps, fin, st

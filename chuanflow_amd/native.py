@@ -90,7 +90,16 @@ class TrainOptions(C.Structure):
 
 
 class BatchSchedule(C.Structure):
-    _fields_ = [("vecElems", C.c_int), ("wgCap", C.c_int), ("graph", C.c_int), ("pollEvery", C.c_int)]
+    """cfx_batch_schedule; unnamed fields take the library's defaults (three-launch form)."""
+    _fields_ = [("vecElems", C.c_int), ("wgCap", C.c_int), ("graph", C.c_int), ("pollEvery", C.c_int),
+                ("persistent", C.c_int), ("claim", C.c_int), ("wgs", C.c_int)]
+    _defaults = (4, 2048, 1, 16, 0, 2, 0)
+
+    def __init__(self, *args, **kw):
+        vals = dict(zip([f for f, _ in self._fields_], self._defaults))
+        vals.update(zip([f for f, _ in self._fields_], args))
+        vals.update(kw)
+        super().__init__(**vals)
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -992,10 +1001,10 @@ def batch_autotune(ctx, systems, bs, xs, iters=20):
     B = _ptr_array(bs)
     X = _ptr_array(xs)
     best = BatchSchedule()
-    cand = (BatchSchedule * 32)()
-    ms = np.zeros(32)
+    cand = (BatchSchedule * 64)()
+    ms = np.zeros(64)
     ctx.check(_L.cfx_batch_autotune(ctx.h, K, C.cast(S, _vp), C.cast(B, _vp), C.cast(X, _vp), iters, C.byref(best),
-                                    C.cast(cand, _vp), _p(ms), 32))
+                                    C.cast(cand, _vp), _p(ms), 64))
     n = int((ms > 0).sum())
     return best, [(cand[i].as_dict(), ms[i]) for i in range(n)]
 

@@ -358,19 +358,28 @@ int cfx_mesh_set_mpnn(cfx_mesh* m, cfx_mpnn* net);
 
 /* ----------------------------------------------------- batched solves */
 /* K independent systems (the pressure systems of many detail regions)
- * solved by Jacobi-PCG together: each of the three PCG kernels is one
- * launch over all systems, every system keeps its own scalars and stop
- * test (solveCG's, Matrix.cpp:153-222), and the iteration is replayed as a
- * captured hipGraph.  With the default schedule {4, 2048, 1, 16} each
- * system's result is bit for bit that of cfx_csr_solve_device (FP64
- * Jacobi-PCG); sched = null picks the schedule with the autotuner.  The
- * reference solves its detail blocks one after another
- * (coupledNetwork.cpp:1461-1475); it has no batched solver. */
+ * solved by Jacobi-PCG together, every system with its own scalars and stop
+ * test (solveCG's, Matrix.cpp:153-222).  Three-launch form (default): each
+ * of the three PCG kernels is one launch over all systems per iteration,
+ * replayed as a captured hipGraph, the host polling between batches of
+ * iterations.  Persistent form (persistent = 1): the whole batch is one
+ * launch, captured as a hipGraph; its workgroups claim chunks of whichever
+ * systems have work, and each system retires at its own stop test.  Both
+ * forms, with the default schedule {4, 2048, 1, 16, 0, 2, 0} or any other
+ * pollEvery, graph, persistent, claim and wgs, give each system the result
+ * of cfx_csr_solve_device (FP64 Jacobi-PCG) bit for bit; vecElems and wgCap
+ * change the summation grouping.  sched = null picks the schedule with the
+ * autotuner, which times both forms.  The reference solves its detail blocks one after
+ * another (coupledNetwork.cpp:1461-1475); it has no batched solver. */
 typedef struct cfx_batch_schedule {
     int vecElems;   /* elements per thread of the vector kernels */
     int wgCap;      /* SpMV workgroups per system at most (<= 2048) */
-    int graph;      /* 1: replay the captured iteration */
-    int pollEvery;  /* iterations between host polls */
+    int graph;      /* 1: replay the captured iteration (persistent: the captured solve) */
+    int pollEvery;  /* three-launch form: iterations between host polls */
+    int persistent; /* 1: one launch for the whole batch; 0: three launches per iteration */
+    int claim;      /* persistent: about this many claims per workgroup and step (>= 1); sets the chunks
+                       each claim takes */
+    int wgs;        /* persistent: workgroups of the launch (0: as many as fit the device at once) */
 } cfx_batch_schedule;
 typedef struct cfx_batch_stats {
     double ms;              /* device time of the whole batch */

@@ -1,7 +1,9 @@
 """Batched solves of many detail-region pressure systems: with the default
 schedule each system's iterates are bit for bit those of its solve alone
-(same kernels, same per-system grid); other schedules (autotuned) agree to
-1e-9 relative at tol 1e-12; graph replay equals kernel-by-kernel launches."""
+(same kernel bodies, same per-system chunks), in the three-launch form and
+in the persistent form (one launch for the whole batch, any number of
+workgroups and claims); other schedules (autotuned) agree to 1e-9 relative
+at tol 1e-12; graph replay equals kernel-by-kernel launches."""
 import numpy as np
 import pytest
 
@@ -34,10 +36,39 @@ def test_default_schedule_is_bitwise_the_single_solve(ctx, blocks):
         assert np.array_equal(a.get(), b.get())
     assert st.maxIterations == max(r.iterations for r in r1)
     assert np.all(fin > 0) and np.all(fin <= st.ms + 1e-9)
-    # graph replay and kernel-by-kernel launches give the same bits
-    xn, rn, _, _ = DB.solve_batched(ctx, blocks, schedule=N.BatchSchedule(4, 2048, 0, 7))
-    for a, b in zip(xb, xn):
+    # the three-launch form (graph replay and kernel-by-kernel launches) and the persistent
+    # form (few workgroups and one claim each per step, no graph; many workgroups, four claims): the same bits
+    for sched in (N.BatchSchedule(4, 2048, 1, 16, persistent=0), N.BatchSchedule(4, 2048, 0, 7, persistent=0),
+                  N.BatchSchedule(4, 2048, 0, persistent=1, claim=1, wgs=7),
+                  N.BatchSchedule(4, 2048, 1, persistent=1, claim=4, wgs=300)):
+        print(f"schedule {sched.as_dict()}", flush=True)
+        xn, rn, fn, sn = DB.solve_batched(ctx, blocks, schedule=sched)
+        for a, b, ra, rbb in zip(xb, xn, rb, rn):
+            assert ra.iterations == rbb.iterations and rbb.converged
+            assert np.array_equal(a.get(), b.get())
+        assert np.all(fn > 0) and np.all(fn <= sn.ms + 1e-9)
+
+
+@pytest.mark.parametrize("persistent", [0, 1])
+def test_batch_stops_at_the_cap_and_at_once(ctx, blocks, persistent):
+    """The iteration cap (C(k) stops at k = maxIter) and an initial guess that
+    already meets the tolerance (C(0) stops) retire systems -- inside the one
+    launch in the persistent form -- exactly as the single solves do."""
+    from chuanflow_amd import detail_batch as DB
+    from chuanflow_amd import native as N
+    sched = N.BatchSchedule(persistent=persistent)
+    x1, r1 = DB.solve_one_by_one(ctx, blocks, tol=1e-8, maxIter=5)
+    xb, rb, _, st = DB.solve_batched(ctx, blocks, tol=1e-8, maxIter=5, schedule=sched)
+    for a, b, ra, rbb in zip(x1, xb, r1, rb):
+        assert ra.iterations == rbb.iterations == 5 and not rbb.converged
         assert np.array_equal(a.get(), b.get())
+    assert st.maxIterations == 5
+    # start from converged solutions: no iteration, x untouched
+    xs, _ = DB.solve_one_by_one(ctx, blocks, tol=1e-12)
+    starts = [x.get() for x in xs]
+    xc, rc, _, _ = DB.solve_batched(ctx, blocks, tol=1e-8, x0=starts, schedule=sched)
+    for s0, x, r in zip(starts, xc, rc):
+        assert r.converged and r.iterations == 0 and np.array_equal(s0, x.get())
 
 
 def test_batched_blocks_match_oracle_cg(ctx, blocks):
@@ -68,7 +99,8 @@ def test_autotuned_schedule_agrees(ctx, blocks):
     xs = [ctx.zeros(b["n"]) for b in blocks]
     best, table = N.batch_autotune(ctx, [b["csr"] for b in blocks], [b["b"] for b in blocks], xs, iters=10)
     print(f"autotuned {best.as_dict()} over {len(table)} candidates", flush=True)
-    assert len(table) == 18 and min(ms for _, ms in table) > 0
+    assert len(table) == 45 and min(ms for _, ms in table) > 0
+    assert {c["persistent"] for c, _ in table} == {0, 1}
     x1, r1 = DB.solve_one_by_one(ctx, blocks, tol=1e-12)
     xb, rb, _, st = DB.solve_batched(ctx, blocks, tol=1e-12, schedule=best)
     for a, b, ra, rbb in zip(x1, xb, r1, rb):
