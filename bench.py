@@ -256,7 +256,7 @@ def bench_c5(args):
     cpu = None
     if O.have_ref() and not args.no_cpu:
         R = O.ref()
-        total_ms, sample = 0.0, 20
+        total_ms, sample = 0.0, args.c5_cpu_sample
         t_all = time.perf_counter()
         for blk, rep in zip(blocks, rb):
             rs, cl, vv, rhs = blk["host"]
@@ -309,7 +309,7 @@ def bench_c5(args):
     }
     if cpu:
         result["cpu_baseline"] = cpu
-    print(json.dumps(result), flush=True)
+    return result
 
 
 def cavity_bcs(patch_start, patch_size, n_internal, n_boundary, lid=1.0):
@@ -460,7 +460,7 @@ def bench_c1(args):
                          "sample": f"{sample} SIMPLE iterations of the same case from rest, scaled to "
                                    f"{args.c1_iterations}", "cpu": model},
     }
-    print(json.dumps(result), flush=True)
+    return result
 
 
 def kinetic_energy(V, U3):
@@ -650,6 +650,11 @@ def main():
     ap.add_argument("--blocks", type=int, default=64)
     ap.add_argument("--c5-passes", type=int, default=1, choices=[0, 1],
                     help="c5: also time the coupling pass over the blocks")
+    ap.add_argument("--c5-cpu-sample", type=int, default=20,
+                    help="c5: reference solveCG iterations sampled per block for the CPU baseline")
+    ap.add_argument("--legs", default="c1,c5",
+                    help="c2: also run short C1 (fp64 pressure, one timed run) and C5 (no coupling pass) legs and "
+                         "report them as sub-objects of the line (empty: none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is described, "
@@ -657,9 +662,11 @@ def main():
         sys.exit(2)
 
     if args.workload == "c1":
-        return bench_c1(args)
+        print(json.dumps(bench_c1(args)), flush=True)
+        return
     if args.workload == "c5":
-        return bench_c5(args)
+        print(json.dumps(bench_c5(args)), flush=True)
+        return
     if args.workload == "c3":
         return bench_c3(args)
     if args.workload == "spmv":
@@ -816,7 +823,36 @@ def main():
         chk = cpu_baseline_and_check(arrays, b, x64, xmix, it64, args.cpu_sample_iters, A, xmg, xln)
         result["cpu_baseline"] = chk.pop("cpu_baseline")
         result["checks"] = chk
+    result["legs"] = run_legs(args)
     print(json.dumps(result), flush=True)
+
+
+def run_legs(args):
+    """Short runs of the other configurations, reported beside the headline: C1 (200 SIMPLE iterations, the
+    reference's Jacobi-PCG pressure solve, one warm-up and one timed run) and C5 (64 batched detail regions,
+    without the coupling-pass timing), each with its oracle check and its own CPU baseline sample."""
+    import copy
+    legs = {}
+    names = [x for x in args.legs.split(",") if x]
+    if "c1" in names:
+        a = copy.copy(args)
+        a.c1_variants, a.steps, a.warmup, a.c1_cpu_sample = "fp64", 1, 1, 3
+        log("leg C1")
+        r = bench_c1(a)
+        legs["c1"] = {k: r.get(k) for k in ("metric", "value", "unit", "ms_per_iteration", "device_ms",
+                                            "final_continuity", "parity_vs_oracle", "cpu_baseline", "config")}
+    if "c5" in names:
+        a = copy.copy(args)
+        a.c5_passes, a.steps, a.warmup, a.c5_cpu_sample = 0, 1, 1, 5
+        log("leg C5")
+        r = bench_c5(a)
+        legs["c5"] = {k: r.get(k) for k in ("metric", "value", "unit", "one_by_one_ms", "batched_default_ms",
+                                            "batched_persistent_ms", "speedup_batched_vs_one_by_one", "schedule",
+                                            "max_iterations", "all_converged", "default_schedule_bitwise_equal",
+                                            "cpu_baseline", "config")}
+        fin = [b["batched_finish_ms"] for b in r["per_block"]]
+        legs["c5"]["block_finish_ms"] = {"first": min(fin), "last": max(fin)}
+    return legs
 
 
 if __name__ == "__main__":
