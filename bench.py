@@ -625,6 +625,9 @@ def main():
     ap.add_argument("--mg-scale", type=float, default=1.8)
     ap.add_argument("--kcycle-depth", type=int, default=2)
     ap.add_argument("--mg-fusion", type=int, default=29)
+    ap.add_argument("--inner-tol", type=float, default=0.0,
+                    help="reliable-update factor of the FP32-refined solves (0: each solver's default, 1e-2 for "
+                         "Jacobi-PCG and 1e-4 for flexible CG)")
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c5", "spmv"],
                     help="c2: the 128^3 pressure system (the headline metric); c1: SIMPLE channel 40x20x20; "
                          "c3: PISO cavity on tets; "
@@ -676,6 +679,7 @@ def main():
     arch, cus = ctx.device_info()
     log(f"device {arch}, {cus} CUs")
     hm, arrays, geom, dm, A, b, bw_shuffled = build_system(N, ctx, args.n)
+    A.set_inner_tol(args.inner_tol)
     n, nnz = dm.nCells, dm.sizes()["nnz"]
     maxIter = 20000
 

@@ -76,6 +76,7 @@ struct cfx_matrix
     std::uint64_t version = 1, solvedVersion = 0;
     bool rawAccess = false;
     double relaxation = 1.0;     // Schemes::relaxation of this matrix
+    double innerTol = 0.0;       // reliable-update factor of its FP32-refined solves (0: each solver's default)
     DevBuf relaxOld;             // the solution before the final pass ([ncomp][n])
     MgCache mg;
     cfx_mpnn* net = nullptr;     // learned finest-level weights (precond LEARNED)
@@ -97,7 +98,7 @@ struct cfx_csr
     cfx_ctx* owner = nullptr;
     DevPattern pattern;
     DevBuf val, val32, b, x, resHist;
-    double innerTol = 1e-2;  // reliable-update factor of the mixed-precision PCG
+    double innerTol = 0.0;  // reliable-update factor of the mixed-precision solves (0: each solver's default)
     int lastIters = -1;
     bool val32Valid = false;
     MgCache mg;
@@ -753,6 +754,7 @@ int cfx_matrix_solve(cfx_matrix* A, double* x, int solver, double tol, int maxIt
         a.gradCorrected = gradCorrected != 0;
         a.precision = precision;
         a.relaxation = A->relaxation;
+        a.delta = A->innerTol;
         need(!multigrid || A->ncomp == 1, "the multigrid preconditioner solves scalar matrices");
         a.sameMatrix = !A->rawAccess && A->version == A->solvedVersion;
         A->solvedVersion = A->version;
@@ -775,6 +777,15 @@ int cfx_matrix_set_relaxation(cfx_matrix* A, double relaxation)
     return guard(A->mesh->owner, [&] {
         need(relaxation > 0.0 && relaxation <= 1.0 && std::isfinite(relaxation), "relaxation must be in (0, 1]");
         A->relaxation = relaxation;
+    });
+}
+
+int cfx_matrix_set_inner_tol(cfx_matrix* A, double t)
+{
+    if (!A) return CFX_E_INVALID;
+    return guard(A->mesh->owner, [&] {
+        need(t == 0 || (t > 0 && t < 1), "inner tolerance must be 0 (the solver's default) or in (0,1)");
+        A->innerTol = t;
     });
 }
 
@@ -1156,7 +1167,7 @@ int cfx_csr_set_inner_tol(cfx_csr* A, double t)
 {
     if (!A) return CFX_E_INVALID;
     return guard(A->owner, [&] {
-        need(t > 0 && t < 1, "inner tolerance must be in (0,1)");
+        need(t == 0 || (t > 0 && t < 1), "inner tolerance must be 0 (the solver's default) or in (0,1)");
         A->innerTol = t;
     });
 }

@@ -89,7 +89,8 @@ public:
     // accumulates a correction; each time its residual has fallen by `delta`
     // below the last true residual, the correction is folded into the FP64 x
     // and r = b - A x is recomputed in FP64 and replaces the FP32 residual.
-    // Convergence is declared on true FP64 residuals only.
+    // Convergence is declared on true FP64 residuals only.  delta <= 0
+    // takes kPcgDelta.
     SolveReport cgMixed(const CsrView& A, const double* val, const float* val32, const double* b,
                         double* x, double tol, int maxIter, double delta, bool project);
 
@@ -98,7 +99,9 @@ public:
     // same reliable FP64 updates as cgMixed.  `mg` must hold the hierarchy
     // of `val` (Multigrid::setMatrix).  On a breakdown or a non-finite value
     // the solve continues from the last FP64 iterate with cgMixed (Jacobi)
-    // and reports precondFallbacks = 1.
+    // and reports precondFallbacks = 1.  delta <= 0: the update comes after
+    // a drop of kFcgDelta, or after one of kPcgDelta once kFcgSegment
+    // iterations have passed (the Jacobi fallback takes kPcgDelta).
     SolveReport fcgMixed(const CsrView& A, const double* val, Preconditioner& mg, const double* b, double* x,
                          double tol, int maxIter, double delta, double* resHist = nullptr);
 
@@ -162,6 +165,19 @@ private:
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     double lastMs_ = 0;
 };
+
+// Default reliable-update factors.  Each update of the flexible CG
+// re-applies the preconditioner to the new true residual (about one
+// iteration's cost).  Over a few iterations its FP32 recurrence stays
+// accurate through a drop of 1e-5 (multigrid on the 128^3 pressure system:
+// 21 iterations at 1e-2 .. 1e-5, 22 at 1e-6), but over long runs it drifts
+// as Jacobi-PCG's does (711 iterations at 1e-2, 795 at 1e-4, 2,615 at 1e-6;
+// Jacobi-preconditioned flexible CG 295 at 1e-2, 309 at 1e-4).  So the
+// automatic rule updates after a drop of kFcgDelta, or of kPcgDelta once
+// kFcgSegment iterations have passed since the last true residual.
+constexpr double kFcgDelta = 1e-4;
+constexpr double kPcgDelta = 1e-2;
+constexpr int kFcgSegment = 16;
 
 // Workspace of one Jacobi-PCG run: 4 x 2048 partials, the device state
 // (kPcgStateBytes), the direction and product vectors (n entries of the

@@ -41,8 +41,10 @@ struct CgState
 {
     double rz[2];        // r.z after iteration k (slot k & 1)
     double bnorm, initialResidual, rnorm, alpha, pAp, updAbs, beta;
+    double updFast;      // flexible CG, automatic factor: the early reliable-update threshold
     int iter, done, converged, breakdown, paused, nan;
-    int kcur, pad;       // flexible CG: iteration index of the running step (set by B, read by C)
+    int kcur;            // flexible CG: iteration index of the running step (set by B, read by C)
+    int kRef;            // flexible CG: iteration of the last true residual
 };
 
 // Deterministic all-workgroup reduction of G <= 2048 partials (blockDim 256):
@@ -546,13 +548,22 @@ __global__ __launch_bounds__(kB) void k_fcg_C(int n, const float* __restrict__ z
             if (lead) st->done = 1;
             return;
         }
-        if (!trueRes && (conv || RM < st->updAbs))
+        // automatic factor (delta < 0): the update comes after a drop of kFcgDelta, or after one of
+        // kPcgDelta once kFcgSegment iterations have passed, whichever is first
+        const bool due = delta > 0 ? RM < st->updAbs
+                                   : (RM < st->updFast || (RM < st->updAbs && k - st->kRef >= kFcgSegment));
+        if (!trueRes && (conv || due))
         {
             if (lead) { st->paused = 1; st->done = 1; }
             return;
         }
     }
-    if (trueRes && lead) st->updAbs = delta * RM;
+    if (trueRes && lead)
+    {
+        st->updAbs = (delta > 0 ? delta : kPcgDelta) * RM;
+        st->updFast = kFcgDelta * RM;
+        st->kRef = k;
+    }
     const double beta = k == 0 ? 0.0 : -ZQ / st->pAp;
     if (lead) st->beta = beta;
     const float tb = static_cast<float>(beta);
@@ -1037,6 +1048,7 @@ SolveReport Krylov::cg(const CsrView& A, const double* val, const double* b, dou
 SolveReport Krylov::cgMixed(const CsrView& A, const double* val, const float* val32, const double* b, double* x,
                             double tol, int maxIter, double delta, bool /*project*/)
 {
+    if (delta <= 0) delta = kPcgDelta;
     hipStream_t s = ctx_.stream();
     CFX_HIP(hipEventRecord(ev0_, s));
     k_inv_diag32<<<divUp(A.n, 256), 256, 0, s>>>(A, val, invD32_.as<float>(), invD_.as<double>());
@@ -1061,8 +1073,9 @@ SolveReport Krylov::cgMixed(const CsrView& A, const double* val, const float* va
 
 
 SolveReport Krylov::fcgMixed(const CsrView& A, const double* val, Preconditioner& mg, const double* b, double* x,
-                             double tol, int maxIter, double delta, double* resHist)
+                             double tol, int maxIter, double deltaArg, double* resHist)
 {
+    const double delta = deltaArg > 0 ? deltaArg : -1.0;  // negative: the automatic rule of k_fcg_C
     hipStream_t s = ctx_.stream();
     CFX_HIP(hipEventRecord(ev0_, s));
     const PcgLaunch g = pcgGrid(A, n_);
@@ -1251,7 +1264,7 @@ SolveReport Krylov::fcgMixed(const CsrView& A, const double* val, Preconditioner
         CFX_HIP(hipMemsetAsync(d32_.p, 0, sizeof(float) * n_, s));
         CFX_CHECK_LAUNCH();
         rep = pcgRun<float>(A, val, val32, invD32_.as<float>(), b, x, d32_.as<float>(), r32_.as<float>(), tol,
-                            std::max(0, maxIter - used), nullptr, true, delta);
+                            std::max(0, maxIter - used), nullptr, true, deltaArg > 0 ? deltaArg : kPcgDelta);
         rep.iterations += used;
         rep.initialResidual = init;
         rep.precondFallbacks = 1;

@@ -34,7 +34,7 @@ struct FvSolveArgs
     int sign = -1;
     bool gradCorrected = true;
     int precision = kFP64;
-    double delta = 1e-2;
+    double delta = 0.0;  // reliable-update factor of FP32-refined solves (0: the solver's default)
     MgCache* mg = nullptr;  // non-null: flexible CG with the multigrid preconditioner (FP32 + FP64 refinement)
     Mpnn* net = nullptr;    // non-null: finest-level weights from the network (per-slot geometry geo)
     const double* geo = nullptr;

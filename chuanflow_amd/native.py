@@ -161,6 +161,7 @@ _sig("cfx_nonorth_correct", C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int, 
 _sig("cfx_matrix_solve", C.c_int, _vp, _vp, C.c_int, C.c_double, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int,
      C.c_int, C.c_int, C.c_int, C.POINTER(Report))
 _sig("cfx_matrix_set_relaxation", C.c_int, _vp, C.c_double)
+_sig("cfx_matrix_set_inner_tol", C.c_int, _vp, C.c_double)
 _sig("cfx_matrix_explicit_operator", C.c_int, _vp, _vp, C.c_int, _vp)
 _sig("cfx_matrix_spmv", C.c_int, _vp, C.c_int, _vp, _vp)
 _sig("cfx_csr_create", C.c_int, _vp, _i64, _vp, _vp, _vp, C.POINTER(_vp))
@@ -747,6 +748,10 @@ class Matrix:
     def set_relaxation(self, relaxation):
         """Schemes::relaxation of this fvMatrix: applied after the final pass of solve()."""
         self.ctx.check(_L.cfx_matrix_set_relaxation(self.h, relaxation))
+
+    def set_inner_tol(self, t):
+        """Reliable-update factor of this matrix's FP32-refined solves (0: each solver's default)."""
+        self.ctx.check(_L.cfx_matrix_set_inner_tol(self.h, t))
 
     def time_fcg_spmv(self, reps=20):
         ms = C.c_double(0)

@@ -173,6 +173,11 @@ int cfx_matrix_solve(cfx_matrix* A, double* d_x, int solver, double tol, int max
  * xOld + relaxation (x - xOld), xOld its value before that pass
  * (fvMatrix.cpp:110-118).  relaxation in (0, 1]. */
 int cfx_matrix_set_relaxation(cfx_matrix* A, double relaxation);
+/* CFX_FP32_IR solves of this matrix: the factor by which the FP32 recurrence
+ * residual must fall below the last true FP64 residual before the next
+ * reliable update (0, the default: each solver's own, as
+ * cfx_csr_set_inner_tol).  No counterpart in the reference. */
+int cfx_matrix_set_inner_tol(cfx_matrix* A, double innerTol);
 /* explicitOperator (FvMatrix.h:86, fvMatrix.cpp:131-146): h = rhs - (A x - D x) */
 int cfx_matrix_explicit_operator(cfx_matrix* A, const double* d_x, int comp, double* d_h);
 /* y = A x with the matrix's current CSR values of one component */
@@ -220,8 +225,10 @@ int cfx_pcg_solve_noalloc(cfx_ctx* ctx, int n, int nnz, const int* d_rowPtr, con
                           const double* d_val, const double* d_b, double* d_x, double tol, int maxIter,
                           cfx_report* rep);
 /* CFX_FP32_IR: the factor by which the FP32 recurrence residual must fall
- * below the last true FP64 residual before the next reliable update
- * (default 1e-2) */
+ * below the last true FP64 residual before the next reliable update; 0 (the
+ * default) takes each solver's own: 1e-2 for Jacobi-PCG, 1e-4 for the
+ * flexible CG of the multigrid, learned and DIC preconditioners (whose every
+ * update costs one more preconditioner application) */
 int cfx_csr_set_inner_tol(cfx_csr* A, double innerTol);
 
 /* ------------------------------- multigrid / learned preconditioner */
