@@ -166,9 +166,9 @@ private:
     double lastMs_ = 0;
 };
 
-// Default reliable-update factors.  Each update of the flexible CG
-// re-applies the preconditioner to the new true residual (about one
-// iteration's cost).  Over a few iterations its FP32 recurrence stays
+// Default reliable-update factors.  An update of the flexible CG after a
+// long run re-applies the preconditioner to the new true residual (about
+// one iteration's cost).  Over a few iterations its FP32 recurrence stays
 // accurate through a drop of 1e-5 (multigrid on the 128^3 pressure system:
 // 21 iterations at 1e-2 .. 1e-5, 22 at 1e-6), but over long runs it drifts
 // as Jacobi-PCG's does (711 iterations at 1e-2, 795 at 1e-4, 2,615 at 1e-6;

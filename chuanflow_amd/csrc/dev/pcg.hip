@@ -1110,6 +1110,7 @@ SolveReport Krylov::fcgMixed(const CsrView& A, const double* val, Preconditioner
         CFX_HIP(hipStreamSynchronize(s));
     };
     int k = 0, outer = 1;
+    int k0Ref = 0;  // iteration of the last true residual
     double r0 = -1;
     for (;;)
     {
@@ -1203,13 +1204,20 @@ SolveReport Krylov::fcgMixed(const CsrView& A, const double* val, Preconditioner
         if (!hs->done) continue;
         if (hs->paused && !hs->converged)
         {
-            // reliable update at iteration hs->iter; the direction p_{k-1} and q = A p_{k-1} are kept
+            // reliable update at iteration hs->iter: the direction p_{k-1} and q = A p_{k-1} are kept; r
+            // becomes the true residual, from which C(k) decides and B takes the next step length.  After a
+            // short run of FP32 iterations (at most kFcgSegment) the recurrence residual is still close to
+            // the true one, so z = M r of the paused iteration and its z.q partials are kept too (flexible
+            // CG takes any preconditioned vector for z; applying M again would cost one more cycle); after
+            // a longer run z is recomputed from the true residual
+            const int kPrev = k0Ref;
             k = hs->iter;
+            k0Ref = k;
             k_fold<<<g.Gv, kB, 0, s>>>(n_, x, x32);
             k_pcg_residual<float><<<g.Gs, kB, 0, s>>>(A, g.rbPerWg, val, x, b, nullptr, r, invD32, partRZ, partRM,
                                                       partBM);
             k_resume<<<1, 1, 0, s>>>(st);
-            mg.apply(r, z, q, partZQ, g.Gs, g.rbPerWg, done);
+            if (k - kPrev > kFcgSegment) mg.apply(r, z, q, partZQ, g.Gs, g.rbPerWg, done);
             k_fcg_C<<<g.Gv, kB, 0, s>>>(n_, z, p, st, partZQ, g.Gs, partRM, partBM, g.Gs, k, tol, maxIter, delta, 3,
                                         resHist);
             CFX_CHECK_LAUNCH();
