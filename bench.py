@@ -624,7 +624,7 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--mg-scale", type=float, default=1.8)
     ap.add_argument("--kcycle-depth", type=int, default=2)
-    ap.add_argument("--mg-fusion", type=int, default=29)
+    ap.add_argument("--mg-fusion", type=int, default=61)
     ap.add_argument("--inner-tol", type=float, default=0.0,
                     help="reliable-update factor of the FP32-refined solves (0: each solver's default, 1e-2 for "
                          "Jacobi-PCG and 1e-4 for flexible CG)")

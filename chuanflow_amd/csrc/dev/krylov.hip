@@ -42,7 +42,34 @@ std::vector<int> buildRowBlocks(const std::vector<int>& rowPtr)
     return b;
 }
 
-void DevPattern::build(std::vector<int> rowPtrH, std::vector<int> colH, hipStream_t s)
+std::vector<int> buildRowBlocksGrouped(const std::vector<int>& rowPtr, const std::vector<int>& group)
+{
+    const int n = static_cast<int>(rowPtr.size()) - 1;
+    std::vector<int> b{0};
+    int rows = 0;
+    long long nnz = 0;
+    for (int g0 = 0; g0 < n;)
+    {
+        int g1 = g0 + 1;
+        while (g1 < n && group[g1] == group[g0]) ++g1;
+        const int len = g1 - g0;
+        const long long gnz = rowPtr[g1] - rowPtr[g0];
+        if (len > kSpmvBlock) return {};
+        if (rows > 0 && (rows + len > kSpmvBlock || nnz + gnz > kSpmvLds))
+        {
+            b.push_back(g0);
+            rows = 0;
+            nnz = 0;
+        }
+        rows += len;
+        nnz += gnz;
+        g0 = g1;
+    }
+    if (n > 0) b.push_back(n);
+    return b;
+}
+
+void DevPattern::build(std::vector<int> rowPtrH, std::vector<int> colH, hipStream_t s, std::vector<int> blocks)
 {
     static std::atomic<std::uint64_t> counter{0};
     uid = ++counter;
@@ -52,7 +79,7 @@ void DevPattern::build(std::vector<int> rowPtrH, std::vector<int> colH, hipStrea
     for (int r = 0; r < n; ++r)
         for (int k = rowPtrH[r]; k < rowPtrH[r + 1]; ++k)
             if (colH[k] == r) { diag[r] = k; break; }
-    const std::vector<int> blocks = buildRowBlocks(rowPtrH);
+    if (blocks.empty()) blocks = buildRowBlocks(rowPtrH);
     nRowBlocks = static_cast<int>(blocks.size()) - 1;
     rowPtr = upload(rowPtrH, s);
     col = upload(colH, s);

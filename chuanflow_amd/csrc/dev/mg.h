@@ -53,7 +53,7 @@ struct MgParams
     float coarsestOmega = 2.0f / 3.0f;
     int maxLevels = 10, coarseMax = 128;
     int kcycleDepth = 2;              // coarse levels 1..kcycleDepth use the K-cycle (0: V-cycle)
-    int fusion = 29;                  // bit 0: the small coarse levels' cycle in one workgroup;
+    int fusion = 61;                  // bit 0: the small coarse levels' cycle in one workgroup;
                                       // bit 1: prolongation (and K-cycle combination) fused into the
                                       // post-smoothing of coarse levels of at most 65536 rows;
                                       // bit 2: that cycle (the tail) run by every workgroup of the
@@ -61,7 +61,9 @@ struct MgParams
                                       // bit 3: the K-cycle's r2 = b - coef v1 formed in the second
                                       // visit's pre-smoothing;
                                       // bit 4: the K-cycle combination of a level of at most 256
-                                      // workgroups formed in the prolongation to the level above
+                                      // workgroups formed in the prolongation to the level above;
+                                      // bit 5: the restriction from a coarse level formed by its
+                                      // pre-smoothing (rows numbered aggregate by aggregate)
 };
 
 // One level of the single-workgroup coarse tail (device pointers).
@@ -94,14 +96,15 @@ enum TailArray
 // all of them, many loads in flight), and where: word offset per
 // (level, array), -1 = read from global memory.
 constexpr int kTailUpArena = 36864;  // LDS words of the fused tail: its vectors and the staged arrays
+constexpr int kTailChunks = kTailLevels * kTailArrays + 1;  // the arrays and the tail's right-hand side
 struct TailStage
 {
     int chunks = 0;
-    int total4 = 0;                           // float4 slots over all chunks
-    int start4[kTailLevels * kTailArrays + 1];  // first float4 slot of each chunk
-    int words[kTailLevels * kTailArrays];
-    int dst[kTailLevels * kTailArrays];       // LDS word offset (a multiple of 4)
-    const void* src[kTailLevels * kTailArrays];
+    int total4 = 0;                    // float4 slots over all chunks
+    int start4[kTailChunks + 1];       // first float4 slot of each chunk
+    int words[kTailChunks];
+    int dst[kTailChunks];              // LDS word offset (a multiple of 4)
+    const void* src[kTailChunks];
     int slot[kTailLevels][kTailArrays];
 };
 
@@ -122,6 +125,9 @@ struct MgLevel
     DevBuf gStart, gPerm;          // coarse nonzero c = sum of fine nonzeros gPerm[gStart[c]..gStart[c+1])
     DevBuf dense;                  // coarsest level: FP32 inverse [n][n] (symmetric)
     std::vector<int> hAgg;
+    // a coarse level above the coarsest: rows numbered aggregate by aggregate and row blocks of whole
+    // aggregates, so its pre-smoothing can form the coarse right-hand side itself
+    bool groupedRows = false;
 };
 
 class Multigrid : public Preconditioner
@@ -204,8 +210,9 @@ private:
     TailArgs tailArgs(int l) const;
     void launchTail(int l, const int* done);
     // fusion bit 2: the tail below level l fused into l's post-smoothing; false where it does not apply
+    // restricted: l's pre-smoothing formed the tail's right-hand side (fusion bit 5)
     bool launchTailUp(int l, const float* in, float* out, const float* q, double* partZQ, int Gs, int rbPerWg,
-                      const int* done);
+                      const int* done, bool restricted);
     DevBuf tv_[14];          // finest-level FP32 scratch of lossGrad / smoothSample
     DevBuf tpart_;
 };

@@ -79,6 +79,75 @@ int aggregate(const std::vector<int>& rp, const std::vector<int>& col, std::vect
     return na;
 }
 
+// The coarse pattern of aggregation `agg` (na aggregates) of the pattern (rp, cl): coarse rows in
+// aggregate order, each row's columns ascending.  With gStart/gPerm also the Galerkin map: coarse
+// nonzero c sums the fine nonzeros gPerm[gStart[c] .. gStart[c+1]), in ascending slot order.
+void coarsePattern(const std::vector<int>& rp, const std::vector<int>& cl, const std::vector<int>& agg, int na,
+                   std::vector<int>& crp, std::vector<int>& ccol, std::vector<int>* gStart, std::vector<int>* gPerm)
+{
+    const int n = static_cast<int>(rp.size()) - 1;
+    std::vector<int> aStart(na + 1, 0), aMem(n);
+    for (int i = 0; i < n; ++i) ++aStart[agg[i] + 1];
+    for (int c = 0; c < na; ++c) aStart[c + 1] += aStart[c];
+    {
+        std::vector<int> fill(aStart.begin(), aStart.end() - 1);
+        for (int i = 0; i < n; ++i) aMem[fill[agg[i]]++] = i;
+    }
+    crp.assign(na + 1, 0);
+    ccol.clear();
+    if (gPerm)
+    {
+        gStart->clear();
+        gPerm->clear();
+        gPerm->reserve(rp[n]);
+        std::vector<std::pair<int, int>> pairs;
+        for (int c = 0; c < na; ++c)
+        {
+            pairs.clear();
+            for (int t = aStart[c]; t < aStart[c + 1]; ++t)
+            {
+                const int i = aMem[t];
+                for (int k = rp[i]; k < rp[i + 1]; ++k) pairs.emplace_back(agg[cl[k]], k);
+            }
+            std::sort(pairs.begin(), pairs.end());
+            for (std::size_t t = 0; t < pairs.size(); ++t)
+            {
+                if (t == 0 || pairs[t].first != pairs[t - 1].first)
+                {
+                    gStart->push_back(static_cast<int>(gPerm->size()));
+                    ccol.push_back(pairs[t].first);
+                }
+                gPerm->push_back(pairs[t].second);
+            }
+            crp[c + 1] = static_cast<int>(ccol.size());
+        }
+        gStart->push_back(static_cast<int>(gPerm->size()));
+        return;
+    }
+    // the pattern alone: distinct coarse columns per coarse row (a marker per column), then sorted
+    std::vector<int> mark(na, -1), row;
+    for (int c = 0; c < na; ++c)
+    {
+        row.clear();
+        for (int t = aStart[c]; t < aStart[c + 1]; ++t)
+        {
+            const int i = aMem[t];
+            for (int k = rp[i]; k < rp[i + 1]; ++k)
+            {
+                const int cc = agg[cl[k]];
+                if (mark[cc] != c)
+                {
+                    mark[cc] = c;
+                    row.push_back(cc);
+                }
+            }
+        }
+        std::sort(row.begin(), row.end());
+        ccol.insert(ccol.end(), row.begin(), row.end());
+        crp[c + 1] = static_cast<int>(ccol.size());
+    }
+}
+
 // ------------------------------------------------------------------ kernels
 // coarse nonzero c = sum of its fine nonzeros gPerm[gStart[c] .. gStart[c+1]): L lanes per coarse
 // nonzero (lane q takes every L-th fine slot), then a fixed butterfly over the L lanes, so the result
@@ -154,21 +223,74 @@ __device__ __forceinline__ bool rowSum(const CsrView& A, const float* __restrict
     return row < r1;
 }
 
-// pre-smoothing from x = 0 and residual: x = W b, r = b - (A W) b
+// The restriction of one row block of a level numbered aggregate by
+// aggregate (the block holds whole aggregates): bc[c] = sum of r over the
+// rows of aggregate c, in row order from zero -- k_mg_restrict_csr's sum with
+// unit weights, bit for bit.  The aggregate range and member starts are read
+// at kernel entry (blockAggregates), before the block's products.
+struct BlockAggregates
+{
+    int r0 = 0, c0 = 0, cEnd = -1, m0 = 0, m1 = 0;  // first row, aggregates [c0, cEnd], this thread's members
+};
+__device__ __forceinline__ BlockAggregates blockAggregates(const CsrView& A, int blk, const int* __restrict__ agg,
+                                                           const int* __restrict__ aStart)
+{
+    BlockAggregates g;
+    const int r0 = A.rowBlocks[blk], r1 = A.rowBlocks[blk + 1];
+    g.r0 = r0;
+    if (r1 > r0)
+    {
+        g.c0 = agg[r0];
+        g.cEnd = agg[r1 - 1];
+        const int c = g.c0 + static_cast<int>(threadIdx.x);
+        if (c <= g.cEnd)
+        {
+            g.m0 = aStart[c];
+            g.m1 = aStart[c + 1];
+        }
+    }
+    return g;
+}
+__device__ __forceinline__ void restrictBlock(const BlockAggregates& g, float ri, bool mine, float* rl,
+                                              float* __restrict__ bc)
+{
+    if (mine) rl[threadIdx.x] = ri;
+    __syncthreads();
+    const int c = g.c0 + static_cast<int>(threadIdx.x);
+    if (c <= g.cEnd)
+    {
+        float sum = 0.0f;
+        for (int i = g.m0; i < g.m1; ++i) sum += rl[i - g.r0];
+        bc[c] = sum;
+    }
+}
+
+// pre-smoothing from x = 0 and residual: x = W b, r = b - (A W) b; with bc
+// (a level numbered aggregate by aggregate, unit restriction weights) also
+// the coarse right-hand side bc = P^T r of the block's aggregates
 __global__ __launch_bounds__(B) void k_mg_down(CsrView A, const float* __restrict__ valS,
                                                const float* __restrict__ b, const float* __restrict__ wD,
-                                               float* __restrict__ x, float* __restrict__ r, const int* done)
+                                               float* __restrict__ x, float* __restrict__ r,
+                                               const int* __restrict__ agg, const int* __restrict__ aStart,
+                                               float* __restrict__ bc, const int* done)
 {
     __shared__ float lds[kSpmvLds];
+    __shared__ float rl[B];
     if (done && *done) return;
+    const int blk = xcdRemap(blockIdx.x, gridDim.x);
+    BlockAggregates g;
+    if (bc) g = blockAggregates(A, blk, agg, aStart);
     int row;
-    float s;
-    if (rowSum<kSpmvLds>(A, valS, b, xcdRemap(blockIdx.x, gridDim.x), lds, row, s))
+    float s, ri = 0.0f;
+    const bool mine = rowSum<kSpmvLds>(A, valS, b, blk, lds, row, s);
+    if (mine)
     {
         const float bi = b[row];
         x[row] = wD[row] * bi;
-        r[row] = bi - s;
+        ri = bi - s;
+        r[row] = ri;
     }
+    if (bc) restrictBlock(g, ri, mine, rl, bc);
 }
 
 // bc = P^T (T r) as a CSR-stream product over the transfer pattern
@@ -699,47 +821,51 @@ __global__ __launch_bounds__(kTailUpThreads) void k_mg_tail_up(
     CsrView A, const float* __restrict__ val32, const float* __restrict__ x, const float* __restrict__ b,
     const float* __restrict__ wD, const int* __restrict__ agg, const float* __restrict__ sc,
     const float* __restrict__ r, const int* __restrict__ rRowPtr, const int* __restrict__ rCol,
-    const float* __restrict__ rval, int unitRestrict, TailArgs ta, TailStage st, int rOff, int cOff, int xpOff,
-    int prodOff, float* __restrict__ z, const int* done)
+    const float* __restrict__ rval, int unitRestrict, int restricted, TailArgs ta, TailStage st, int rOff, int cOff,
+    int xpOff, int prodOff, float* __restrict__ z, const int* done)
 {
     __shared__ float arena[kTailUpArena];
-    __shared__ int cs4[kTailLevels * kTailArrays + 1], cw[kTailLevels * kTailArrays], cd[kTailLevels * kTailArrays];
-    __shared__ const float* csrc[kTailLevels * kTailArrays];
+    __shared__ int cs4[kTailChunks + 1], cw[kTailChunks], cd[kTailChunks];
+    __shared__ const float* csrc[kTailChunks];
     if (done && *done) return;
     const int t = threadIdx.x;
     const int n = A.n, n0 = ta.L[0].n;
-    // l's residual and the members of every coarse row
-    float* rl = arena + rOff;
-    int* cl = reinterpret_cast<int*>(arena + cOff);
-    copyTwo(rl, r, n, reinterpret_cast<float*>(cl), reinterpret_cast<const float*>(rCol), n);
-    __syncthreads();
-    // b of the tail's first level = P^T (T r), every row, members in ascending order (k_mg_restrict_csr's sums)
-    for (int c = t; c < n0; c += kTailUpThreads)
+    // restricted: the tail's right-hand side was formed by l's pre-smoothing and is a staged chunk
+    if (!restricted)
     {
-        const int k0 = rRowPtr[c], k1 = rRowPtr[c + 1];
-        float sum = 0.0f;
-        for (int k = k0; k < k1; k += 8)
+        // l's residual and the members of every coarse row
+        float* rl = arena + rOff;
+        int* cl = reinterpret_cast<int*>(arena + cOff);
+        copyTwo(rl, r, n, reinterpret_cast<float*>(cl), reinterpret_cast<const float*>(rCol), n);
+        __syncthreads();
+        // b of the tail's first level = P^T (T r), every row, members in ascending order (k_mg_restrict_csr's sums)
+        for (int c = t; c < n0; c += kTailUpThreads)
         {
-            int m[8];
-            float v[8], w[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u)
+            const int k0 = rRowPtr[c], k1 = rRowPtr[c + 1];
+            float sum = 0.0f;
+            for (int k = k0; k < k1; k += 8)
             {
-                m[u] = k + u < k1 ? cl[k + u] : 0;
-                w[u] = unitRestrict || k + u >= k1 ? 1.0f : rval[k + u];
-            }
+                int m[8];
+                float v[8], w[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) v[u] = rl[m[u]];
+                for (int u = 0; u < 8; ++u)
+                {
+                    m[u] = k + u < k1 ? cl[k + u] : 0;
+                    w[u] = unitRestrict || k + u >= k1 ? 1.0f : rval[k + u];
+                }
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
-            {
-                const float t2 = sum + (unitRestrict ? v[u] : w[u] * v[u]);
-                sum = k + u < k1 ? t2 : sum;
+                for (int u = 0; u < 8; ++u) v[u] = rl[m[u]];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                {
+                    const float t2 = sum + (unitRestrict ? v[u] : w[u] * v[u]);
+                    sum = k + u < k1 ? t2 : sum;
+                }
             }
+            arena[c] = sum;
         }
-        arena[c] = sum;
+        __syncthreads();
     }
-    __syncthreads();
     stageTail(st, arena, cs4, cw, cd, csrc);
     __syncthreads();
     float* e = arena + 2 * n0;  // the tail's level-0 residual slot, free once restricted
@@ -960,11 +1086,16 @@ __global__ __launch_bounds__(B) void k_mg_down_kc(CsrView A, const float* __rest
                                                   const float* __restrict__ b, const float* __restrict__ v1,
                                                   const double* __restrict__ part, int G,
                                                   const float* __restrict__ wD, float* __restrict__ r2,
-                                                  float* __restrict__ x, float* __restrict__ r, const int* done)
+                                                  float* __restrict__ x, float* __restrict__ r,
+                                                  const int* __restrict__ agg, const int* __restrict__ aStart,
+                                                  float* __restrict__ bc, const int* done)
 {
     __shared__ float lds[kSpmvLds];
     __shared__ double red[B / kWave + 1];
+    __shared__ float rl[B];
     if (done && *done) return;
+    BlockAggregates g;
+    if (bc) g = blockAggregates(A, xcdRemap(blockIdx.x, gridDim.x), agg, aStart);
     const double rho = redAll(part, G, red);
     const double a1 = redAll(part + Multigrid::kPartMax, G, red);
     const float coef = static_cast<float>(rho > 0.0 ? a1 / rho : 0.0);
@@ -1012,13 +1143,16 @@ __global__ __launch_bounds__(B) void k_mg_down_kc(CsrView A, const float* __rest
             s += valS[k] * (b[c] - coef * v1[c]);
         }
     }
+    float ri = 0.0f;
     if (row < r1)
     {
         const float bi = b[row] - coef * v1[row];
         r2[row] = bi;
         x[row] = wD[row] * bi;
-        r[row] = bi - s;
+        ri = bi - s;
+        r[row] = ri;
     }
+    if (bc) restrictBlock(g, ri, row < r1, rl, bc);
 }
 
 // Prolongation of a deferred K-cycle correction: x += S P (k1 c1 + k2 c2),
@@ -1250,18 +1384,49 @@ Multigrid::Multigrid(Context& ctx, const DevPattern& fine, int maxLevels, int co
     id_ = ++counter;
     hipStream_t s = ctx.stream();
     coarseMax = std::min(coarseMax, kCoarseMax);
+    // 1. the aggregation of every level, each coarse level numbered in the order its aggregates were made
+    std::vector<std::vector<int>> RP{fine.hRowPtr}, CL{fine.hCol}, AG;
+    while (true)
+    {
+        const int n = static_cast<int>(RP.back().size()) - 1;
+        if (n <= coarseMax || static_cast<int>(RP.size()) >= maxLevels) break;
+        std::vector<int> agg;
+        const int na = aggregate(RP.back(), CL.back(), agg);
+        if (na >= n) break;  // no coarsening possible
+        std::vector<int> crp, ccol;
+        coarsePattern(RP.back(), CL.back(), agg, na, crp, ccol, nullptr, nullptr);
+        AG.push_back(std::move(agg));
+        RP.push_back(std::move(crp));
+        CL.push_back(std::move(ccol));
+    }
+    const int last = static_cast<int>(RP.size()) - 1;
+    // 2. every coarse level above the coarsest renumbered aggregate by aggregate (rows sorted by their
+    //    aggregate, then by their first number), from the top down so that each level is sorted by the
+    //    final numbers of the level above it: the same hierarchy, whose restriction from such a level sums
+    //    contiguous rows and whose aggregates follow its rows in order
+    for (int l = last - 1; l >= 1; --l)
+    {
+        const int n = static_cast<int>(AG[l].size()), na = static_cast<int>(RP[l + 1].size()) - 1;
+        std::vector<int> start(na + 1, 0), pi(n);
+        for (int i = 0; i < n; ++i) ++start[AG[l][i] + 1];
+        for (int c = 0; c < na; ++c) start[c + 1] += start[c];
+        for (int i = 0; i < n; ++i) pi[i] = start[AG[l][i]]++;
+        for (int& a : AG[l - 1]) a = pi[a];
+        std::vector<int> moved(n);
+        for (int i = 0; i < n; ++i) moved[pi[i]] = AG[l][i];
+        AG[l].swap(moved);
+    }
+    // 3. patterns, transfers and Galerkin maps in the final numbering
     std::vector<int> rp = fine.hRowPtr, cl = fine.hCol;
     lv_.emplace_back();
     lv_[0].n = fine.n;
     lv_[0].nnz = fine.nnz;
-    while (true)
+    for (int l = 0; l < last; ++l)
     {
         MgLevel& L = lv_.back();
         const int n = L.n;
-        if (n <= coarseMax || static_cast<int>(lv_.size()) >= maxLevels) break;
-        std::vector<int> agg;
-        const int na = aggregate(rp, cl, agg);
-        if (na >= n) break;  // no coarsening possible
+        const std::vector<int>& agg = AG[l];
+        const int na = static_cast<int>(RP[l + 1].size()) - 1;
         // members of each aggregate
         std::vector<int> aStart(na + 1, 0), aMem(n);
         for (int i = 0; i < n; ++i) ++aStart[agg[i] + 1];
@@ -1270,33 +1435,8 @@ Multigrid::Multigrid(Context& ctx, const DevPattern& fine, int maxLevels, int co
             std::vector<int> fill(aStart.begin(), aStart.end() - 1);
             for (int i = 0; i < n; ++i) aMem[fill[agg[i]]++] = i;
         }
-        // coarse pattern and the Galerkin map, row by coarse row
-        std::vector<int> crp(na + 1, 0), ccol, gStart, gPerm;
-        gPerm.reserve(rp[n]);
-        std::vector<std::pair<int, int>> pairs;
-        for (int c = 0; c < na; ++c)
-        {
-            pairs.clear();
-            for (int t = aStart[c]; t < aStart[c + 1]; ++t)
-            {
-                const int i = aMem[t];
-                for (int k = rp[i]; k < rp[i + 1]; ++k) pairs.emplace_back(agg[cl[k]], k);
-            }
-            std::sort(pairs.begin(), pairs.end());
-            // one coarse nonzero per distinct coarse column, summing its fine
-            // nonzeros in ascending slot order
-            for (std::size_t t = 0; t < pairs.size(); ++t)
-            {
-                if (t == 0 || pairs[t].first != pairs[t - 1].first)
-                {
-                    gStart.push_back(static_cast<int>(gPerm.size()));
-                    ccol.push_back(pairs[t].first);
-                }
-                gPerm.push_back(pairs[t].second);
-            }
-            crp[c + 1] = static_cast<int>(ccol.size());
-        }
-        gStart.push_back(static_cast<int>(gPerm.size()));
+        std::vector<int> crp, ccol, gStart, gPerm;
+        coarsePattern(rp, cl, agg, na, crp, ccol, &gStart, &gPerm);
         L.agg = upload(agg, s);
         L.aggStart = upload(aStart, s);
         L.aggMem = upload(aMem, s);
@@ -1305,18 +1445,25 @@ Multigrid::Multigrid(Context& ctx, const DevPattern& fine, int maxLevels, int co
         L.gStart = upload(gStart, s);
         L.gPerm = upload(gPerm, s);
         L.hAgg = agg;
+        if (l > 0)
+        {
+            // this coarse level's rows and row blocks: numbered aggregate by aggregate, blocks of whole
+            // aggregates (its restriction then runs inside its pre-smoothing)
+            std::vector<int> blocks = buildRowBlocksGrouped(rp, agg);
+            L.groupedRows = !blocks.empty();
+            L.pat.build(rp, cl, s, std::move(blocks));
+        }
         MgLevel C;
         C.n = na;
         C.nnz = crp[na];
         lv_.push_back(std::move(C));
-        lv_.back().pat.build(crp, ccol, s);
         rp.swap(crp);
         cl.swap(ccol);
     }
+    if (last > 0) lv_.back().pat.build(rp, cl, s);
     if (lv_.size() > 1 && lv_.back().n > kCoarseMax)
         throw std::invalid_argument("multigrid: coarsest level exceeds 2048 rows (raise maxLevels)");
     // the coarse tail: the first level from which every level fits the tail's LDS
-    const int last = static_cast<int>(lv_.size()) - 1;
     for (int l = 1; l < last && tailFrom_ < 0; ++l)
     {
         long long rows = 0;
@@ -1467,7 +1614,7 @@ void Multigrid::launchTail(int l, const int* done)
 }
 
 bool Multigrid::launchTailUp(int l, const float* in, float* out, const float* q, double* partZQ, int Gs,
-                             int rbPerWg, const int* done)
+                             int rbPerWg, const int* done, bool restricted)
 {
     (void)q;
     (void)Gs;
@@ -1488,12 +1635,24 @@ bool Multigrid::launchTailUp(int l, const float* in, float* out, const float* q,
     // arrays, later l's prolongated iterate and the products of four row blocks
     const int rOff = up4(vec), cOff = rOff + up4(L.n);
     const int xpOff = up4(vec), prodOff = xpOff + up4(L.n);
-    if (cOff + up4(L.n) > kTailUpArena || prodOff + kTailUpBlocks * kSpmvLds > kTailUpArena) return false;
+    if ((!restricted && cOff + up4(L.n) > kTailUpArena) || prodOff + kTailUpBlocks * kSpmvLds > kTailUpArena)
+        return false;
     TailStage st;
     for (auto& row : st.slot)
         for (int& v : row) v = -1;
     int off = up4(vec);
     st.start4[0] = 0;
+    if (restricted)
+    {
+        // the tail's right-hand side, formed by l's pre-smoothing, staged with its arrays: the tail's
+        // level-0 b at word 0 (the rest of the vector slots is written by the cycle before it is read)
+        const int words4 = (ta.L[0].n + 3) / 4;
+        st.src[0] = lv_[l + 1].b.as<float>();
+        st.words[0] = ta.L[0].n;
+        st.dst[0] = 0;
+        st.start4[1] = words4;
+        st.chunks = 1;
+    }
     for (int m = 0; m < ta.count; ++m)
     {
         const TailLevel& T = ta.L[m];
@@ -1528,8 +1687,8 @@ bool Multigrid::launchTailUp(int l, const float* in, float* out, const float* q,
     // off the finest level the restriction scale is 1 (k_restrict_vals): its products are exact
     k_mg_tail_up<<<G, kTailUpThreads, 0, ctx_.stream()>>>(
         A, L.val32.as<float>(), L.x.as<float>(), in, L.wPost32.as<float>(), L.agg.as<int>(), L.s32.as<float>(),
-        L.r.as<float>(), R.rowPtr, R.col, L.rval.as<float>(), l > 0 ? 1 : 0, ta, st, rOff, cOff, xpOff, prodOff, out,
-        done);
+        L.r.as<float>(), R.rowPtr, R.col, L.rval.as<float>(), l > 0 ? 1 : 0, restricted ? 1 : 0, ta, st, rOff, cOff,
+        xpOff, prodOff, out, done);
     return true;
 }
 
@@ -1593,19 +1752,24 @@ void Multigrid::visit(int l, const float* in, float* out, const float* q, double
     hipStream_t s = ctx_.stream();
     MgLevel& L = lv_[l];
     const CsrView A = l == 0 ? fine_.view() : L.pat.view();
+    // fusion bit 5: a coarse level numbered aggregate by aggregate restricts inside its pre-smoothing (its
+    // restriction weights are 1)
+    const bool fuseR = (p_.fusion & 32) && l >= 1 && L.groupedRows && l + 1 < levels();
+    float* bc = fuseR ? lv_[l + 1].b.as<float>() : nullptr;
     if (kcB)
         k_mg_down_kc<<<A.nRowBlocks, B, 0, s>>>(A, L.valS32.as<float>(), kcB, kcV, L.part.as<double>(), L.G,
                                                 L.wPre32.as<float>(), const_cast<float*>(in), L.x.as<float>(),
-                                                L.r.as<float>(), done);
+                                                L.r.as<float>(), L.agg.as<int>(), L.aggStart.as<int>(), bc, done);
     else
         k_mg_down<<<A.nRowBlocks, B, 0, s>>>(A, L.valS32.as<float>(), in, L.wPre32.as<float>(), L.x.as<float>(),
-                                             L.r.as<float>(), done);
+                                             L.r.as<float>(), L.agg.as<int>(), L.aggStart.as<int>(), bc, done);
     if (l + 1 < levels())
     {
-        if (launchTailUp(l, in, out, q, partZQ, Gs, rbPerWg, done)) return;
+        if (launchTailUp(l, in, out, q, partZQ, Gs, rbPerWg, done, fuseR)) return;
         MgLevel& C = lv_[l + 1];
-        k_mg_restrict_csr<<<L.rpat.nRowBlocks, B, 0, s>>>(L.rpat.view(), L.rval.as<float>(), L.r.as<float>(),
-                                                          C.b.as<float>(), done);
+        if (!fuseR)
+            k_mg_restrict_csr<<<L.rpat.nRowBlocks, B, 0, s>>>(L.rpat.view(), L.rval.as<float>(), L.r.as<float>(),
+                                                              C.b.as<float>(), done);
         // fused only where the level is small: on a large level forming S P e per nonzero costs more
         // gathers than the separate prolongation saves
         const bool fuse = (p_.fusion & 2) && l >= 1 && L.n <= kFuseRowsMax;

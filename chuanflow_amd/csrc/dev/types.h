@@ -100,13 +100,19 @@ struct DevPattern
     {
         return {n, nnz, nRowBlocks, rowPtr.as<int>(), col.as<int>(), diagSlot.as<int>(), rowBlocks.as<int>()};
     }
-    void build(std::vector<int> rowPtrH, std::vector<int> colH, hipStream_t s);
+    // blocks: CSR-stream row blocks to use (empty: buildRowBlocks)
+    void build(std::vector<int> rowPtrH, std::vector<int> colH, hipStream_t s, std::vector<int> blocks = {});
 };
 
 // Builds rowBlocks for CSR-stream from a host row pointer: each block holds
 // at most kSpmvBlock rows and kSpmvLds nonzeros (a longer row gets a block
 // of its own and is summed directly from global memory).
 std::vector<int> buildRowBlocks(const std::vector<int>& rowPtr);
+// The same for rows numbered group by group (group[i] non-decreasing): a
+// block ends only where the group changes, so each block holds whole groups
+// (at most kSpmvBlock rows; a group whose rows exceed kSpmvLds nonzeros gets
+// a block of its own).  Empty when a group has more than kSpmvBlock rows.
+std::vector<int> buildRowBlocksGrouped(const std::vector<int>& rowPtr, const std::vector<int>& group);
 
 class Context;
 

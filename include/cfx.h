@@ -259,10 +259,11 @@ typedef struct cfx_mg_params {
                                            workgroup of the post-smoothing of the level above it (no z.q
                                            partials there); bit 3: the K-cycle's second right-hand side
                                            formed inside the pre-smoothing; bit 4: the K-cycle combination of
-                                           a level of at most 256 workgroups formed inside the prolongation
-                                           (default 29 = bits 0, 2, 3, 4: 13.5 against 14.5 ms for bit 0
-                                           alone at 128^3; bit 1 measured slower); results are bit for bit
-                                           those of fusion 0 */
+                                           a level of at most 256 workgroups formed inside the prolongation;
+                                           bit 5: the restriction from a coarse level (whose rows are
+                                           numbered aggregate by aggregate) formed by its pre-smoothing
+                                           (default 61 = bits 0, 2, 3, 4, 5; bit 1 measured slower);
+                                           results are bit for bit those of fusion 0 */
 } cfx_mg_params;
 int cfx_mg_default_params(cfx_mg_params* p);
 /* flexible CG replays one captured iteration per step (hipGraph, default 1)

@@ -101,20 +101,40 @@ class Hierarchy:
 
 
 def hierarchy(A, maxLevels=10, coarseMax=128):
+    """Aggregation hierarchy.  Each level is aggregated with its rows numbered
+    in the order the aggregates of the level below were made; then every coarse
+    level above the coarsest is renumbered aggregate by aggregate (rows sorted
+    by their aggregate, ties by their number), from the top down, so that each
+    level's rows follow the final numbers of the level above.  The renumbering
+    permutes coarse rows only: the operators are the same up to that
+    permutation."""
     A = sp.csr_matrix(A)
     A.sort_indices()
-    H = Hierarchy(A=[A])
-    while A.shape[0] > coarseMax and len(H.A) < maxLevels:
+    levels, aggs = [A], []
+    while A.shape[0] > coarseMax and len(levels) < maxLevels:
         agg, na = aggregate(A.indptr, A.indices)
         if na >= A.shape[0]:
             break
-        P = prolongation(agg, na)
-        Ac = sp.csr_matrix(P.T @ A @ P)
+        Ac = sp.csr_matrix(prolongation(agg, na).T @ A @ prolongation(agg, na))
         Ac.sort_indices()
-        H.agg.append(agg)
+        aggs.append(agg)
+        levels.append(Ac)
+        A = Ac
+    last = len(levels) - 1
+    for l in range(last - 1, 0, -1):
+        order = np.argsort(aggs[l], kind="stable")
+        pi = np.empty_like(order)
+        pi[order] = np.arange(len(order))
+        aggs[l - 1] = pi[aggs[l - 1]]
+        aggs[l] = aggs[l][order]
+    H = Hierarchy(A=[levels[0]])
+    for l in range(last):
+        P = prolongation(aggs[l], levels[l + 1].shape[0])
+        Ac = sp.csr_matrix(P.T @ H.A[l] @ P)
+        Ac.sort_indices()
+        H.agg.append(aggs[l])
         H.P.append(P)
         H.A.append(Ac)
-        A = Ac
     return H
 
 

@@ -1,6 +1,6 @@
 """The multigrid preconditioner and flexible CG on the GPU against the
 multigrid oracle: hierarchy (aggregation bit-exact, Galerkin values to
-1e-13), one V-cycle in FP32 against FP64 (rel 1e-4), FCG convergence to a
+1e-13), one V-cycle in FP32 against FP64 (rel 2e-4), FCG convergence to a
 true FP64 residual of 1e-8, the Jacobi limit, and the Jacobi-PCG fallback
 on a non-finite preconditioner."""
 import numpy as np
@@ -87,7 +87,9 @@ def test_vcycle_fp32_matches_oracle(ctx, systems, name, maxLevels):
     ref = MG.vcycle(H, r.astype(np.float64), p, pre.astype(np.float64), scale.astype(np.float64),
                     post.astype(np.float64), restrict=restrict.astype(np.float64))
     err = np.abs(z.get() - ref).max() / np.abs(ref).max()
-    assert err < 1e-4, err
+    # an FP32 cycle against FP64: on the pinned (nearly singular) fv32 system its rounding, amplified by the
+    # exact coarsest solve, spans 4e-5 to 1.2e-4 of max|z| over weight seeds, for every launch fusion
+    assert err < 2e-4, err
 
 
 @pytest.mark.parametrize("name", ["lap40", "fv32", "fv24n"])
@@ -185,8 +187,10 @@ def test_fused_kernels_equal_separate_launches(ctx, systems, case):
     post-smoothing (bit 1), that cycle repeated from LDS by every workgroup
     of the post-smoothing above it (bit 2), the K-cycle's r2 formed in the
     second visit's pre-smoothing (bit 3) and its combination formed in the
-    prolongation (bit 4), alone and together, give the cycle of the separate
-    kernels exactly, and the same flexible-CG iterates."""
+    prolongation (bit 4), the restriction from a coarse level formed by its
+    pre-smoothing (bit 5, alone and with the tail's LDS cycle, which then
+    stages that right-hand side), alone and together, give the cycle of the
+    separate kernels exactly, and the same flexible-CG iterates."""
     if case == "lap40-k1":
         A, b = systems["lap40"][0], None
         kw = dict(kcycleDepth=1)
@@ -215,7 +219,7 @@ def test_fused_kernels_equal_separate_launches(ctx, systems, case):
         assert rows[-1] > 128  # coarsest level swept, not inverted
     r = ctx.array(np.random.default_rng(3).standard_normal(n).astype(np.float32))
     zs, xs, its = [], [], []
-    fusions = (0, 1, 2, 3, 4, 8, 16, 29, 31)
+    fusions = (0, 1, 2, 3, 4, 8, 16, 29, 31, 32, 36, 61, 63)
     for fusion in fusions:
         G.set_mg(scale=1.8, coarseScale=1.8, fusion=fusion, **kw)
         z = ctx.empty(n, np.float32)

@@ -56,6 +56,42 @@ def test_phase_one_roots_own_their_whole_neighbourhood():
     assert np.all(agg[nb] == 0)
 
 
+def test_coarse_levels_are_numbered_aggregate_by_aggregate_and_equal_the_plain_hierarchy():
+    """Every coarse level above the coarsest lists its rows aggregate by
+    aggregate, in the order of the level above (so a block of consecutive rows
+    restricts to a consecutive range of coarse rows); the renumbering is a
+    permutation of each coarse level: with Q_l the permutation matrix, the
+    operators are Q_l A_l Q_l^T of the hierarchy numbered as its aggregates
+    were made (aggregation and Galerkin products repeated here without it)."""
+    A = laplacian3d(24, 22, 20, seed=5)
+    H = MG.hierarchy(A, coarseMax=8)
+    assert len(H.A) >= 4
+    for l in range(1, len(H.A) - 1):
+        assert np.all(np.diff(H.agg[l]) >= 0), l  # aggregate ids non-decreasing along the rows
+        assert np.array_equal(np.unique(H.agg[l]), np.arange(H.A[l + 1].shape[0]))
+    # the plain hierarchy
+    plain, aggs = [sp.csr_matrix(A)], []
+    while plain[-1].shape[0] > 8:
+        agg, na = MG.aggregate(plain[-1].indptr, plain[-1].indices)
+        P = MG.prolongation(agg, na)
+        plain.append(sp.csr_matrix(P.T @ plain[-1] @ P))
+        aggs.append(agg)
+    assert [m.shape[0] for m in plain] == [m.shape[0] for m in H.A]
+    # recover each permutation from the aggregate maps: old row of every new row
+    old_of_new = [np.arange(A.shape[0])]
+    for l in range(1, len(H.A)):
+        new_agg = H.agg[l - 1]  # new level-l row of each (final-numbered) level-(l-1) row
+        old_agg = aggs[l - 1][old_of_new[l - 1]]  # old level-l row of the same rows
+        m = np.full(H.A[l].shape[0], -1)
+        m[new_agg] = old_agg
+        assert np.all(m >= 0)
+        old_of_new.append(m)
+        Pm = H.A[l].toarray()
+        Po = plain[l].toarray()[np.ix_(m, m)]
+        np.testing.assert_allclose(Pm, Po, rtol=1e-13, atol=1e-13 * np.abs(Po).max())
+    assert np.array_equal(old_of_new[-1], np.arange(H.A[-1].shape[0]))  # the coarsest keeps its numbering
+
+
 def test_hierarchy_galerkin_is_ptap_and_preserves_row_sums():
     A = laplacian3d(14, 13, 12, seed=3)
     H = MG.hierarchy(A, coarseMax=50)
