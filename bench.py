@@ -483,7 +483,7 @@ def bench_c3(args):
         Ubc, pbc = cavity_bcs(arrays["patchStart"], arrays["patchSize"], len(arrays["neighbour"]), dm.nB)
         dU, dp, dphi = ctx.zeros((3, dm.nCells)), ctx.zeros(dm.nCells), ctx.zeros(dm.nFaces)
         dm.set_mg(scale=args.mg_scale, coarseScale=args.mg_scale, kcycleDepth=args.kcycle_depth,
-                  fusion=args.mg_fusion)
+                  fusion=args.mg_fusion, smoothFine=args.smooth_fine, smoothCoarse=args.smooth_coarse)
         if learned:
             dm.set_mpnn(LN.load(ctx))
         return hm, arrays, geom, dm, Ubc, pbc, dU, dp, dphi
@@ -625,6 +625,8 @@ def main():
     ap.add_argument("--mg-scale", type=float, default=1.8)
     ap.add_argument("--kcycle-depth", type=int, default=2)
     ap.add_argument("--mg-fusion", type=int, default=61)
+    ap.add_argument("--smooth-fine", type=int, default=3, help="multigrid smoothing sweeps on the finest level")
+    ap.add_argument("--smooth-coarse", type=int, default=2, help="multigrid smoothing sweeps on the K-cycle levels")
     ap.add_argument("--inner-tol", type=float, default=0.0,
                     help="reliable-update factor of the FP32-refined solves (0: each solver's default, 1e-2 for "
                          "Jacobi-PCG and 1e-4 for flexible CG)")
@@ -691,7 +693,8 @@ def main():
     rmix = time_solve(N, ctx, A, d_x, N.FP32_IR, args.steps, args.warmup, args.tol, maxIter)
     xmix = d_x.get()
     log("multigrid-preconditioned flexible CG (FP32 + FP64 refinement), tuned constants")
-    A.set_mg(scale=args.mg_scale, coarseScale=args.mg_scale, kcycleDepth=args.kcycle_depth, fusion=args.mg_fusion)
+    A.set_mg(scale=args.mg_scale, coarseScale=args.mg_scale, kcycleDepth=args.kcycle_depth, fusion=args.mg_fusion,
+             smoothFine=args.smooth_fine, smoothCoarse=args.smooth_coarse)
     warm_mg = []
     rmg = time_solve(N, ctx, A, d_x, N.FP32_IR, args.steps, args.warmup, args.tol, maxIter, N.PRECOND_MG, warm_mg)
     xmg = d_x.get()

@@ -1017,7 +1017,8 @@ int cfx_mg_default_params(cfx_mg_params* p)
     if (!p) return CFX_E_INVALID;
     const MgParams d;
     *p = cfx_mg_params{d.omegaPre, d.scale, d.omegaPost, d.restrictScale, d.coarseOmega, d.coarseScale,
-                       d.coarseSweeps, d.coarsestOmega, d.maxLevels, d.coarseMax, d.kcycleDepth, d.fusion};
+                       d.coarseSweeps, d.coarsestOmega, d.maxLevels, d.coarseMax, d.kcycleDepth, d.fusion,
+                       d.smoothFine, d.smoothCoarse};
     return CFX_OK;
 }
 
@@ -1043,6 +1044,10 @@ void setMgParams(MgCache& mg, const cfx_mg_params* p)
     need(p->kcycleDepth >= 0, "kcycleDepth < 0");
     m.kcycleDepth = p->kcycleDepth;
     m.fusion = p->fusion;
+    need(p->smoothFine >= 1 && p->smoothFine <= 8 && p->smoothCoarse >= 1 && p->smoothCoarse <= 8,
+         "smoothFine and smoothCoarse must be in [1,8]");
+    m.smoothFine = p->smoothFine;
+    m.smoothCoarse = p->smoothCoarse;
     mg.valid = false;
 }
 
@@ -1524,6 +1529,7 @@ int cfx_csr_mg_loss_grad(cfx_csr* A, const float* e, const float* a, const float
     return guard(A->owner, [&] {
         MgParams p = A->mg.p;
         p.kcycleDepth = 0;
+        p.smoothFine = p.smoothCoarse = 1;  // the objective's cycle: one sweep each side
         Multigrid mg(*A->owner->ctx, A->pattern, p.maxLevels, p.coarseMax);
         mg.setMatrix(A->val.as<double>(), a, t, a, t, p);
         *loss = mg.lossGrad(e, a, t, ga, gt);

@@ -53,6 +53,8 @@ struct MgParams
     float coarsestOmega = 2.0f / 3.0f;
     int maxLevels = 10, coarseMax = 128;
     int kcycleDepth = 2;              // coarse levels 1..kcycleDepth use the K-cycle (0: V-cycle)
+    int smoothFine = 3;               // smoothing sweeps before and after the coarse correction: finest level,
+    int smoothCoarse = 2;             // K-cycle levels (the levels below them take one)
     int fusion = 61;                  // bit 0: the small coarse levels' cycle in one workgroup;
                                       // bit 1: prolongation (and K-cycle combination) fused into the
                                       // post-smoothing of coarse levels of at most 65536 rows;
@@ -115,6 +117,7 @@ struct MgLevel
     DevBuf val64, val32, valS32;   // FP64 Galerkin values, FP32 copy, A Wpre (FP32)
     DevBuf invD32, wPre32, wPost32, s32, t32;  // D^-1, a D^-1, c D^-1, prolongation / restriction scale
     DevBuf b, x, r, e;             // FP32: rhs, iterate, residual, coarse-correction result
+    DevBuf t2;                     // FP32: second residual / smoothing buffer (sweeps > 1)
     DevBuf c1, c2, v1, r2, part;   // K-cycle vectors and 5 x kPartMax dot partials
     int G = 1, rbPerWg = 1;        // SpMV grid of this level (G <= kPartMax)
     // transfer to level l+1
@@ -210,9 +213,10 @@ private:
     TailArgs tailArgs(int l) const;
     void launchTail(int l, const int* done);
     // fusion bit 2: the tail below level l fused into l's post-smoothing; false where it does not apply
-    // restricted: l's pre-smoothing formed the tail's right-hand side (fusion bit 5)
-    bool launchTailUp(int l, const float* in, float* out, const float* q, double* partZQ, int Gs, int rbPerWg,
-                      const int* done, bool restricted);
+    // res: l's residual after its pre-smoothing; restricted: l's pre-smoothing formed the tail's right-hand
+    // side (fusion bit 5); out: l's first post-smoothing sweep
+    bool launchTailUp(int l, const float* in, float* out, const float* res, const double* partZQ, const int* done,
+                      bool restricted);
     DevBuf tv_[14];          // finest-level FP32 scratch of lossGrad / smoothSample
     DevBuf tpart_;
 };

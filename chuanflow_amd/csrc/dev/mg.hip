@@ -293,6 +293,33 @@ __global__ __launch_bounds__(B) void k_mg_down(CsrView A, const float* __restric
     if (bc) restrictBlock(g, ri, mine, rl, bc);
 }
 
+// a further pre-smoothing sweep: x += W r, r' = r - (A W) r (= b - A x of the
+// new x); with bc as in k_mg_down, the coarse right-hand side from r'
+__global__ __launch_bounds__(B) void k_mg_down_more(CsrView A, const float* __restrict__ valS,
+                                                    const float* __restrict__ rIn, const float* __restrict__ wD,
+                                                    float* __restrict__ x, float* __restrict__ rOut,
+                                                    const int* __restrict__ agg, const int* __restrict__ aStart,
+                                                    float* __restrict__ bc, const int* done)
+{
+    __shared__ float lds[kSpmvLds];
+    __shared__ float rl[B];
+    if (done && *done) return;
+    const int blk = xcdRemap(blockIdx.x, gridDim.x);
+    BlockAggregates g;
+    if (bc) g = blockAggregates(A, blk, agg, aStart);
+    int row;
+    float s, ri = 0.0f;
+    const bool mine = rowSum<kSpmvLds>(A, valS, rIn, blk, lds, row, s);
+    if (mine)
+    {
+        const float r0 = rIn[row];
+        x[row] += wD[row] * r0;
+        ri = r0 - s;
+        rOut[row] = ri;
+    }
+    if (bc) restrictBlock(g, ri, mine, rl, bc);
+}
+
 // bc = P^T (T r) as a CSR-stream product over the transfer pattern
 __global__ __launch_bounds__(B) void k_mg_restrict_csr(CsrView R, const float* __restrict__ rval,
                                                        const float* __restrict__ r, float* __restrict__ bc,
@@ -403,6 +430,39 @@ __device__ __forceinline__ float colDot8(const float* __restrict__ m, const floa
         {
             const float t = s + v[u] * xv[u];
             s = j + u < n ? t : s;
+        }
+    }
+    return s;
+}
+
+// rowDot8 over arrays in LDS: the value and column loads are unconditional
+// (each staged array is followed by kLdsPad readable words) and only their
+// use is masked, so the loads of a row go out together without a branch per
+// load; the sum is rowDot8's, bit for bit.
+constexpr int kLdsPad = 8;
+__device__ __forceinline__ float rowDotLds8(const float* val, const int* col, const float* x, int k0, int k1)
+{
+    float s = 0.0f;
+    for (int k = k0; k < k1; k += 8)
+    {
+        float v[8], xv[8];
+        int c[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+        {
+            const float vv = val[k + u];
+            const int cc = col[k + u];
+            const bool ok = k + u < k1;
+            v[u] = ok ? vv : 0.0f;
+            c[u] = ok ? cc : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xv[u] = x[c[u]];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+        {
+            const float t = s + v[u] * xv[u];
+            s = k + u < k1 ? t : s;
         }
     }
     return s;
@@ -537,23 +597,23 @@ __global__ __launch_bounds__(kDense) void k_mg_dense_apply(int n, const float* _
 // holds b of tail level 0 at entry; the correction of level 0 is written to
 // eOut.
 template <typename Acc>
-__device__ __forceinline__ void tailCycle(const TailArgs& a, const Acc& acc, float* v, float* eOut)
+__device__ __forceinline__ void tailCycle(const Acc& acc, float* v, float* eOut)
 {
     const int t = threadIdx.x, nt = blockDim.x;
-    const int last = a.count - 1;
+    const int count = acc.count(), last = count - 1;
     // level m: b at off[m], x at +n, r (later z) at +2n; coarsest: b, e (and the Jacobi ping-pong buffer)
     int off[kTailLevels];
     {
         int o = 0;
-        for (int m = 0; m < a.count; ++m)
+        for (int m = 0; m < count; ++m)
         {
             off[m] = o;
-            o += 3 * a.L[m].n;
+            o += 3 * acc.n(m);
         }
     }
     for (int m = 0; m < last; ++m)
     {
-        const int n = a.L[m].n, nc = a.L[m + 1].n;
+        const int n = acc.n(m), nc = acc.n(m + 1);
         const int* rowPtr = acc.template get<int>(m, kTRowPtr);
         const int* col = acc.template get<int>(m, kTCol);
         const float* valS = acc.template get<float>(m, kTValS);
@@ -561,7 +621,7 @@ __device__ __forceinline__ void tailCycle(const TailArgs& a, const Acc& acc, flo
         float *b = v + off[m], *x = b + n, *r = x + n;
         for (int i = t; i < n; i += nt)
         {
-            const float s = rowDot8(valS, col, b, rowPtr[i], rowPtr[i + 1]);
+            const float s = acc.dot(valS, col, b, rowPtr[i], rowPtr[i + 1]);
             const float bi = b[i];
             x[i] = wPre[i] * bi;
             r[i] = bi - s;
@@ -571,12 +631,12 @@ __device__ __forceinline__ void tailCycle(const TailArgs& a, const Acc& acc, flo
         const int* rCol = acc.template get<int>(m, kTRCol);
         const float* rval = acc.template get<float>(m, kTRval);
         float* bc = v + off[m + 1];
-        for (int c = t; c < nc; c += nt) bc[c] = rowDot8(rval, rCol, r, rRowPtr[c], rRowPtr[c + 1]);
+        for (int c = t; c < nc; c += nt) bc[c] = acc.dot(rval, rCol, r, rRowPtr[c], rRowPtr[c + 1]);
         __syncthreads();
     }
     const float* ec;
     {
-        const int n = a.L[last].n;
+        const int n = acc.n(last);
         float *b = v + off[last], *e = b + n, *w = e + n;
         const float* dense = acc.template get<float>(last, kTDense);
         if (dense)
@@ -601,12 +661,14 @@ __device__ __forceinline__ void tailCycle(const TailArgs& a, const Acc& acc, flo
             for (int i = t; i < n; i += nt) e[i] = 0.0f;
             __syncthreads();
             float *cur = e, *nxt = w;
-            for (int it = 0; it < a.sweeps; ++it)
+            const int sweeps = acc.sweeps();
+            const float omega = acc.omega();
+            for (int it = 0; it < sweeps; ++it)
             {
                 for (int i = t; i < n; i += nt)
                 {
-                    const float s = rowDot8(val, col, cur, rowPtr[i], rowPtr[i + 1]);
-                    nxt[i] = cur[i] + a.omega * invD[i] * (b[i] - s);
+                    const float s = acc.dot(val, col, cur, rowPtr[i], rowPtr[i + 1]);
+                    nxt[i] = cur[i] + omega * invD[i] * (b[i] - s);
                 }
                 __syncthreads();
                 float* tmp = cur;
@@ -618,7 +680,7 @@ __device__ __forceinline__ void tailCycle(const TailArgs& a, const Acc& acc, flo
     }
     for (int m = last - 1; m >= 0; --m)
     {
-        const int n = a.L[m].n;
+        const int n = acc.n(m);
         const int* rowPtr = acc.template get<int>(m, kTRowPtr);
         const int* col = acc.template get<int>(m, kTCol);
         const float* val = acc.template get<float>(m, kTVal);
@@ -631,7 +693,7 @@ __device__ __forceinline__ void tailCycle(const TailArgs& a, const Acc& acc, flo
         __syncthreads();
         for (int i = t; i < n; i += nt)
         {
-            const float s = rowDot8(val, col, x, rowPtr[i], rowPtr[i + 1]);
+            const float s = acc.dot(val, col, x, rowPtr[i], rowPtr[i + 1]);
             z[i] = x[i] + wPost[i] * (b[i] - s);
         }
         __syncthreads();
@@ -644,6 +706,14 @@ struct TailGlobalAcc
 {
     const TailArgs& a;
     const float* dense;
+    __device__ __forceinline__ int count() const { return a.count; }
+    __device__ __forceinline__ int n(int m) const { return a.L[m].n; }
+    __device__ __forceinline__ int sweeps() const { return a.sweeps; }
+    __device__ __forceinline__ float omega() const { return a.omega; }
+    __device__ __forceinline__ float dot(const float* v, const int* c, const float* x, int k0, int k1) const
+    {
+        return rowDot8(v, c, x, k0, k1);
+    }
     template <typename T>
     __device__ __forceinline__ const T* get(int m, int k) const
     {
@@ -652,16 +722,48 @@ struct TailGlobalAcc
     }
 };
 
-// tail arrays staged in LDS at the word offsets slot[level][array]
+// The tail's sizes and staging slots, copied from the kernel arguments into
+// LDS in one round at kernel entry (read per level from the arguments, each
+// level's first use waits on a scalar load).
+struct TailMeta
+{
+    int count, sweeps;
+    float omega;
+    int n[kTailLevels];
+    int slot[kTailLevels * kTailArrays];
+};
+__device__ __forceinline__ void loadTailMeta(const TailArgs& a, const TailStage& st, TailMeta* mt)
+{
+    const int t = threadIdx.x;
+    if (t < kTailLevels * kTailArrays) mt->slot[t] = st.slot[t / kTailArrays][t % kTailArrays];
+    if (t < kTailLevels) mt->n[t] = a.L[t].n;
+    if (t == 0)
+    {
+        mt->count = a.count;
+        mt->sweeps = a.sweeps;
+        mt->omega = a.omega;
+    }
+}
+
+// tail arrays staged in LDS at the word offsets of the slot table
 struct TailLdsAcc
 {
     const float* arena;
-    const TailStage& st;
+    const TailMeta& mt;
+    __device__ __forceinline__ int count() const { return mt.count; }
+    __device__ __forceinline__ int n(int m) const { return mt.n[m]; }
+    __device__ __forceinline__ int sweeps() const { return mt.sweeps; }
+    __device__ __forceinline__ float omega() const { return mt.omega; }
+    __device__ __forceinline__ float dot(const float* v, const int* c, const float* x, int k0, int k1) const
+    {
+        return rowDotLds8(v, c, x, k0, k1);
+    }
     template <typename T>
     __device__ __forceinline__ const T* get(int m, int k) const
     {
         // every array the cycle reads is staged; the coarsest inverse exists only where it was formed
-        return st.slot[m][k] >= 0 ? reinterpret_cast<const T*>(arena + st.slot[m][k]) : nullptr;
+        const int w = mt.slot[m * kTailArrays + k];
+        return w >= 0 ? reinterpret_cast<const T*>(arena + w) : nullptr;
     }
 };
 
@@ -681,7 +783,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const floa
         for (int k = t; k < nn; k += kTailThreads) dl[k] = a.L[last].dense[k];
     }
     __syncthreads();
-    tailCycle(a, TailGlobalAcc{a, dl}, v, eOut);
+    tailCycle(TailGlobalAcc{a, dl}, v, eOut);
 }
 
 // Copies the staged tail arrays into LDS.  The chunk table is read into LDS
@@ -817,6 +919,7 @@ __device__ __forceinline__ void copyTwo(float* da, const float* __restrict__ sa,
 // theirs; the launches and the tail's chains of dependent global loads go.
 constexpr int kTailUpThreads = 1024;
 constexpr int kTailUpBlocks = kTailUpThreads / B;  // row blocks per workgroup
+constexpr int kXpAhead = 16;                        // prolongation rows per thread loaded at kernel entry
 __global__ __launch_bounds__(kTailUpThreads) void k_mg_tail_up(
     CsrView A, const float* __restrict__ val32, const float* __restrict__ x, const float* __restrict__ b,
     const float* __restrict__ wD, const int* __restrict__ agg, const float* __restrict__ sc,
@@ -827,9 +930,35 @@ __global__ __launch_bounds__(kTailUpThreads) void k_mg_tail_up(
     __shared__ float arena[kTailUpArena];
     __shared__ int cs4[kTailChunks + 1], cw[kTailChunks], cd[kTailChunks];
     __shared__ const float* csrc[kTailChunks];
+    __shared__ TailMeta mt;
     if (done && *done) return;
     const int t = threadIdx.x;
     const int n = A.n, n0 = ta.L[0].n;
+    // loads that depend on nothing the tail computes, issued first so that they arrive while the tail's
+    // arrays are staged and its cycle runs: the prolongation's x, S and aggregate of the first
+    // kXpAhead * 1024 rows, and this workgroup's post-smoothing row block
+    float xv[kXpAhead], sv[kXpAhead];
+    int av[kXpAhead];
+#pragma unroll
+    for (int u = 0; u < kXpAhead; ++u)
+    {
+        const int j = t + u * kTailUpThreads;
+        const bool ok = j < n;
+        xv[u] = ok ? x[j] : 0.0f;
+        sv[u] = ok ? sc[j] : 0.0f;
+        av[u] = ok ? agg[j] : 0;
+    }
+    const int q = t / B, lt = t % B;
+    const int blk = blockIdx.x * kTailUpBlocks + q;
+    int r0 = 0, r1 = 0, k0 = 0, nnz = 0;
+    if (blk < A.nRowBlocks)
+    {
+        r0 = A.rowBlocks[blk];
+        r1 = A.rowBlocks[blk + 1];
+        k0 = A.rowPtr[r0];
+        nnz = A.rowPtr[r1] - k0;
+    }
+    loadTailMeta(ta, st, &mt);
     // restricted: the tail's right-hand side was formed by l's pre-smoothing and is a staged chunk
     if (!restricted)
     {
@@ -869,10 +998,16 @@ __global__ __launch_bounds__(kTailUpThreads) void k_mg_tail_up(
     stageTail(st, arena, cs4, cw, cd, csrc);
     __syncthreads();
     float* e = arena + 2 * n0;  // the tail's level-0 residual slot, free once restricted
-    tailCycle(ta, TailLdsAcc{arena, st}, arena, e);
+    tailCycle(TailLdsAcc{arena, mt}, arena, e);
     // xp of every row of level l, over the staged arrays (no longer read)
     float* xp = arena + xpOff;
-    for (int j0 = t; j0 < n; j0 += 16 * kTailUpThreads)
+#pragma unroll
+    for (int u = 0; u < kXpAhead; ++u)
+    {
+        const int j = t + u * kTailUpThreads;
+        if (j < n) xp[j] = xv[u] + sv[u] * e[av[u]];
+    }
+    for (int j0 = t + kXpAhead * kTailUpThreads; j0 < n; j0 += 16 * kTailUpThreads)
     {
         float xv[16], sv[16];
         int av[16];
@@ -894,17 +1029,7 @@ __global__ __launch_bounds__(kTailUpThreads) void k_mg_tail_up(
     }
     __syncthreads();
     // post-smoothing of this workgroup's row blocks, one per 256 threads
-    const int q = t / B, lt = t % B;
-    const int blk = blockIdx.x * kTailUpBlocks + q;
     float* lq = arena + prodOff + q * kSpmvLds;
-    int r0 = 0, r1 = 0, k0 = 0, nnz = 0;
-    if (blk < A.nRowBlocks)
-    {
-        r0 = A.rowBlocks[blk];
-        r1 = A.rowBlocks[blk + 1];
-        k0 = A.rowPtr[r0];
-        nnz = A.rowPtr[r1] - k0;
-    }
     const bool staged = nnz <= kSpmvLds;
     if (staged)
     {
@@ -1477,7 +1602,7 @@ Multigrid::Multigrid(Context& ctx, const DevPattern& fine, int maxLevels, int co
         if (l > 0) L.val64.alloc(sizeof(double) * nnz);
         L.val32.alloc(sizeof(float) * nnz);
         L.valS32.alloc(sizeof(float) * nnz);
-        for (DevBuf* b : {&L.invD32, &L.wPre32, &L.wPost32, &L.s32, &L.t32, &L.b, &L.x, &L.r, &L.e})
+        for (DevBuf* b : {&L.invD32, &L.wPre32, &L.wPost32, &L.s32, &L.t32, &L.b, &L.x, &L.r, &L.t2, &L.e})
             b->alloc(sizeof(float) * n);
         const int nrb = l == 0 ? fine_.nRowBlocks : L.pat.nRowBlocks;
         L.G = std::max(1, std::min(nrb, kPartMax));
@@ -1505,7 +1630,7 @@ void Multigrid::setValues(const double* fineVal64, const MgParams& p)
 {
     hipStream_t s = ctx_.stream();
     if (p.kcycleDepth != p_.kcycleDepth || p.fusion != p_.fusion || p.coarseSweeps != p_.coarseSweeps ||
-        p.coarsestOmega != p_.coarsestOmega)
+        p.coarsestOmega != p_.coarsestOmega || p.smoothFine != p_.smoothFine || p.smoothCoarse != p_.smoothCoarse)
         ++structVersion_;
     p_ = p;
     fineVal64_ = fineVal64;
@@ -1613,12 +1738,9 @@ void Multigrid::launchTail(int l, const int* done)
     k_mg_tail<<<1, kTailThreads, 0, ctx_.stream()>>>(tailArgs(l), lv_[l].b.as<float>(), lv_[l].e.as<float>(), done);
 }
 
-bool Multigrid::launchTailUp(int l, const float* in, float* out, const float* q, double* partZQ, int Gs,
-                             int rbPerWg, const int* done, bool restricted)
+bool Multigrid::launchTailUp(int l, const float* in, float* out, const float* res, const double* partZQ,
+                             const int* done, bool restricted)
 {
-    (void)q;
-    (void)Gs;
-    (void)rbPerWg;
     // z.q partials follow the grid of the separate post-smoothing: with them the separate kernels run
     if (!(p_.fusion & 4) || partZQ || !tailAt(l + 1)) return false;
     const MgLevel& L = lv_[l];
@@ -1671,15 +1793,16 @@ bool Multigrid::launchTailUp(int l, const float* in, float* out, const float* q,
             want = {{kTRowPtr, n + 1}, {kTCol, nnz}, {kTVal, nnz}, {kTInvD, n}};
         for (const Want& w : want)
         {
+            // each array is followed by kLdsPad words that the cycle's unconditional LDS loads may read
             const int words4 = (w.words + 3) / 4;
-            if (off + 4 * words4 > kTailUpArena) return false;  // the tail does not fit in LDS
+            if (off + 4 * words4 + kLdsPad > kTailUpArena) return false;  // the tail does not fit in LDS
             const int c = st.chunks++;
             st.src[c] = tailGlobal(T, w.k);
             st.words[c] = w.words;
             st.dst[c] = off;
             st.start4[c + 1] = st.start4[c] + words4;
             st.slot[m][w.k] = off;
-            off += 4 * words4;
+            off += 4 * words4 + kLdsPad;
         }
     }
     st.total4 = st.start4[st.chunks];
@@ -1687,7 +1810,7 @@ bool Multigrid::launchTailUp(int l, const float* in, float* out, const float* q,
     // off the finest level the restriction scale is 1 (k_restrict_vals): its products are exact
     k_mg_tail_up<<<G, kTailUpThreads, 0, ctx_.stream()>>>(
         A, L.val32.as<float>(), L.x.as<float>(), in, L.wPost32.as<float>(), L.agg.as<int>(), L.s32.as<float>(),
-        L.r.as<float>(), R.rowPtr, R.col, L.rval.as<float>(), l > 0 ? 1 : 0, restricted ? 1 : 0, ta, st, rOff, cOff,
+        res, R.rowPtr, R.col, L.rval.as<float>(), l > 0 ? 1 : 0, restricted ? 1 : 0, ta, st, rOff, cOff,
         xpOff, prodOff, out, done);
     return true;
 }
@@ -1752,49 +1875,83 @@ void Multigrid::visit(int l, const float* in, float* out, const float* q, double
     hipStream_t s = ctx_.stream();
     MgLevel& L = lv_[l];
     const CsrView A = l == 0 ? fine_.view() : L.pat.view();
-    // fusion bit 5: a coarse level numbered aggregate by aggregate restricts inside its pre-smoothing (its
-    // restriction weights are 1)
+    // smoothing sweeps before and after the coarse correction (the same number): smoothFine on the finest
+    // level, smoothCoarse on the K-cycle levels, one below them
+    const int sweeps = std::max(1, l == 0 ? p_.smoothFine : (l <= p_.kcycleDepth ? p_.smoothCoarse : 1));
+    // fusion bit 5: a coarse level numbered aggregate by aggregate restricts inside its last pre-smoothing
+    // sweep (its restriction weights are 1)
     const bool fuseR = (p_.fusion & 32) && l >= 1 && L.groupedRows && l + 1 < levels();
     float* bc = fuseR ? lv_[l + 1].b.as<float>() : nullptr;
+    float* rb[2] = {L.r.as<float>(), L.t2.as<float>()};  // residuals of successive sweeps
     if (kcB)
         k_mg_down_kc<<<A.nRowBlocks, B, 0, s>>>(A, L.valS32.as<float>(), kcB, kcV, L.part.as<double>(), L.G,
-                                                L.wPre32.as<float>(), const_cast<float*>(in), L.x.as<float>(),
-                                                L.r.as<float>(), L.agg.as<int>(), L.aggStart.as<int>(), bc, done);
+                                                L.wPre32.as<float>(), const_cast<float*>(in), L.x.as<float>(), rb[0],
+                                                L.agg.as<int>(), L.aggStart.as<int>(), sweeps == 1 ? bc : nullptr,
+                                                done);
     else
         k_mg_down<<<A.nRowBlocks, B, 0, s>>>(A, L.valS32.as<float>(), in, L.wPre32.as<float>(), L.x.as<float>(),
-                                             L.r.as<float>(), L.agg.as<int>(), L.aggStart.as<int>(), bc, done);
+                                             rb[0], L.agg.as<int>(), L.aggStart.as<int>(),
+                                             sweeps == 1 ? bc : nullptr, done);
+    int cur = 0;
+    for (int k = 1; k < sweeps; ++k, cur ^= 1)
+        k_mg_down_more<<<A.nRowBlocks, B, 0, s>>>(A, L.valS32.as<float>(), rb[cur], L.wPre32.as<float>(),
+                                                  L.x.as<float>(), rb[cur ^ 1], L.agg.as<int>(),
+                                                  L.aggStart.as<int>(), k + 1 == sweeps ? bc : nullptr, done);
+    float* res = rb[cur];
+    // the first post-smoothing sweep writes z1: out itself, or (further sweeps follow) the residual buffer
+    // that the restriction has consumed
+    float* z1 = sweeps == 1 ? out : rb[cur ^ 1];
+    // the z.q partials belong to the last sweep
+    const float* q1 = sweeps == 1 ? q : nullptr;
+    double* part1 = sweeps == 1 ? partZQ : nullptr;
+    bool first = false;  // the first post-smoothing sweep already ran (fused)
     if (l + 1 < levels())
     {
-        if (launchTailUp(l, in, out, q, partZQ, Gs, rbPerWg, done, fuseR)) return;
-        MgLevel& C = lv_[l + 1];
-        if (!fuseR)
-            k_mg_restrict_csr<<<L.rpat.nRowBlocks, B, 0, s>>>(L.rpat.view(), L.rval.as<float>(), L.r.as<float>(),
-                                                              C.b.as<float>(), done);
-        // fused only where the level is small: on a large level forming S P e per nonzero costs more
-        // gathers than the separate prolongation saves
-        const bool fuse = (p_.fusion & 2) && l >= 1 && L.n <= kFuseRowsMax;
-        // a K-cycle below may leave its combination to the prolongation (its coefficients are reduced by
-        // every workgroup there, so only for a level of few workgroups)
-        const bool deferKc = fuse || ((p_.fusion & 16) && C.G <= kDeferGroupsMax);
-        const bool deferred = solveLevel(l + 1, done, deferKc);
-        if (fuse)
-        {
-            k_mg_prolong_up<<<Gs, B, 0, s>>>(A, rbPerWg, L.val32.as<float>(), L.x.as<float>(), in,
-                                             L.wPost32.as<float>(), L.agg.as<int>(), L.s32.as<float>(),
-                                             C.e.as<float>(), C.c1.as<float>(), C.c2.as<float>(),
-                                             deferred ? C.part.as<double>() : nullptr, C.G, out, done);
-            return;
-        }
-        if (deferred)
-            k_mg_prolong_kc<<<std::max(1, std::min(1024, divUp(L.n, B))), B, 0, s>>>(
-                L.n, L.agg.as<int>(), L.s32.as<float>(), C.c1.as<float>(), C.c2.as<float>(), C.part.as<double>(), C.G,
-                L.x.as<float>(), done);
+        if (launchTailUp(l, in, z1, res, part1, done, fuseR))
+            first = true;
         else
-            k_mg_prolong<<<divUp(L.n, B), B, 0, s>>>(L.n, L.agg.as<int>(), L.s32.as<float>(), C.e.as<float>(),
-                                                      L.x.as<float>(), done);
+        {
+            MgLevel& C = lv_[l + 1];
+            if (!fuseR)
+                k_mg_restrict_csr<<<L.rpat.nRowBlocks, B, 0, s>>>(L.rpat.view(), L.rval.as<float>(), res,
+                                                                  C.b.as<float>(), done);
+            // fused only where the level is small: on a large level forming S P e per nonzero costs more
+            // gathers than the separate prolongation saves
+            const bool fuse = (p_.fusion & 2) && l >= 1 && L.n <= kFuseRowsMax;
+            // a K-cycle below may leave its combination to the prolongation (its coefficients are reduced by
+            // every workgroup there, so only for a level of few workgroups)
+            const bool deferKc = fuse || ((p_.fusion & 16) && C.G <= kDeferGroupsMax);
+            const bool deferred = solveLevel(l + 1, done, deferKc);
+            if (fuse)
+            {
+                k_mg_prolong_up<<<Gs, B, 0, s>>>(A, rbPerWg, L.val32.as<float>(), L.x.as<float>(), in,
+                                                 L.wPost32.as<float>(), L.agg.as<int>(), L.s32.as<float>(),
+                                                 C.e.as<float>(), C.c1.as<float>(), C.c2.as<float>(),
+                                                 deferred ? C.part.as<double>() : nullptr, C.G, z1, done);
+                first = true;
+            }
+            else if (deferred)
+                k_mg_prolong_kc<<<std::max(1, std::min(1024, divUp(L.n, B))), B, 0, s>>>(
+                    L.n, L.agg.as<int>(), L.s32.as<float>(), C.c1.as<float>(), C.c2.as<float>(),
+                    C.part.as<double>(), C.G, L.x.as<float>(), done);
+            else
+                k_mg_prolong<<<divUp(L.n, B), B, 0, s>>>(L.n, L.agg.as<int>(), L.s32.as<float>(), C.e.as<float>(),
+                                                          L.x.as<float>(), done);
+        }
     }
-    k_mg_up<<<Gs, B, 0, s>>>(A, rbPerWg, L.val32.as<float>(), L.x.as<float>(), in, L.wPost32.as<float>(), out, q,
-                             partZQ, done);
+    if (!first)
+        k_mg_up<<<Gs, B, 0, s>>>(A, rbPerWg, L.val32.as<float>(), L.x.as<float>(), in, L.wPost32.as<float>(), z1,
+                                 q1, part1, done);
+    // further sweeps z = z' + W (b - A z'), alternating between the two residual buffers, the last into out
+    float* zc = z1;
+    for (int k = 1; k < sweeps; ++k)
+    {
+        const bool last = k + 1 == sweeps;
+        float* zn = last ? out : (zc == rb[0] ? rb[1] : rb[0]);
+        k_mg_up<<<Gs, B, 0, s>>>(A, rbPerWg, L.val32.as<float>(), zc, in, L.wPost32.as<float>(), zn,
+                                 last ? q : nullptr, last ? partZQ : nullptr, done);
+        zc = zn;
+    }
 }
 
 void Multigrid::apply(const float* r, float* z, const float* q, double* partZQ, int Gs, int rbPerWg, const int* done)
@@ -1807,6 +1964,7 @@ double Multigrid::lossGrad(const float* e, const float* a, const float* t, float
 {
     if (levels() < 2) throw std::invalid_argument("lossGrad: the hierarchy has no coarse level");
     if (p_.kcycleDepth != 0) throw std::invalid_argument("lossGrad: needs the linear cycle (kcycleDepth = 0)");
+    if (p_.smoothFine != 1) throw std::invalid_argument("lossGrad: needs one smoothing sweep (smoothFine = 1)");
     hipStream_t s = ctx_.stream();
     MgLevel& L = lv_[0];
     MgLevel& C = lv_[1];

@@ -88,6 +88,8 @@ class Params:
     maxLevels: int = 10
     coarseMax: int = 128
     kcycleDepth: int = 2
+    smoothFine: int = 3     # smoothing sweeps before and after the coarse correction on the finest level,
+    smoothCoarse: int = 2   # on the K-cycle levels (1 below them)
 
 
 DENSE_MAX = 128
@@ -188,8 +190,11 @@ def solve_level(H, b, p: Params, level, exact_coarse=False):
 
 
 def vcycle(H, r, p: Params, pre=None, scale=None, post=None, exact_coarse=False, level=0, restrict=None):
-    """z = M r: x = Wpre b; r = b - A x; bc = P^T T r; e = coarse correction
-    (solve_level); x += S P e; z = x + Wpost (b - A x).  pre/scale/post/
+    """z = M r: x = Wpre b, then (sweeps - 1) times x += Wpre (b - A x);
+    r = b - A x; bc = P^T T r; e = coarse correction (solve_level);
+    x += S P e; then sweeps times x += Wpost (b - A x); z = x.  sweeps is
+    smoothFine on the finest level, smoothCoarse on the K-cycle levels, 1
+    below them.  pre/scale/post/
     restrict are per-row finest-level a, s, c, t (None -> constant)."""
     A = H.A[level]
     b = np.asarray(r, np.float64)
@@ -204,14 +209,19 @@ def vcycle(H, r, p: Params, pre=None, scale=None, post=None, exact_coarse=False,
         a = c = p.coarseOmega
         s = p.coarseScale
         t = 1.0
+    sweeps = p.smoothFine if level == 0 else (p.smoothCoarse if level <= p.kcycleDepth else 1)
     x = a * dinv * b
+    for _ in range(sweeps - 1):
+        x = x + a * dinv * (b - A @ x)
     if level < last:
         res = b - A @ x
         agg = H.agg[level]
         bc = np.bincount(agg, weights=t * res, minlength=H.A[level + 1].shape[0])
         ec = solve_level(H, bc, p, level + 1, exact_coarse)
         x = x + s * ec[agg]
-    return x + c * dinv * (b - A @ x)
+    for _ in range(sweeps):
+        x = x + c * dinv * (b - A @ x)
+    return x
 
 
 def fcg(A, b, precond, tol=1e-8, maxIter=1000, x0=None):

@@ -152,10 +152,10 @@ def backward(params, cache, ga, gt):
 def objective(H_, e, a, t, p: MG.Params):
     """L = 1 - (z.A e)^2 / ((e.A e)(z.A z)), z = M A e, for the cycle with
     tied finest-level weights (pre = post = a, restriction = prolongation =
-    t) and the linear V-cycle below level 1: the A-norm error left by one
-    preconditioned step with the optimal step length, min_alpha
-    |e - alpha z|_A^2 / |e|_A^2."""
-    q = MG.Params(**{**p.__dict__, "kcycleDepth": 0})
+    t), one smoothing sweep on each side and the linear V-cycle below level
+    1: the A-norm error left by one preconditioned step with the optimal step
+    length, min_alpha |e - alpha z|_A^2 / |e|_A^2."""
+    q = MG.Params(**{**p.__dict__, "kcycleDepth": 0, "smoothFine": 1, "smoothCoarse": 1})
     A = H_.A[0]
     b = A @ e
     z = MG.vcycle(H_, b, q, pre=a, scale=t, post=a, restrict=t)
@@ -167,7 +167,7 @@ def objective(H_, e, a, t, p: MG.Params):
 def objective_grad(H_, e, a, t, p: MG.Params):
     """(L, dL/da, dL/dt) by the adjoint of the finest level (the coarse
     correction Q = P (coarse cycle) P^T is symmetric)."""
-    q = MG.Params(**{**p.__dict__, "kcycleDepth": 0})
+    q = MG.Params(**{**p.__dict__, "kcycleDepth": 0, "smoothFine": 1, "smoothCoarse": 1})
     A = H_.A[0]
     dinv = 1.0 / A.diagonal()
     agg = H_.agg[0]

@@ -62,10 +62,13 @@ def test_hierarchy_matches_oracle(ctx, systems, name):
             assert np.array_equal(agg, H.agg[l])
 
 
-@pytest.mark.parametrize("name,maxLevels", [("lap40", 10), ("fv32", 10), ("fv32", 3), ("fv24n", 10)])
-def test_vcycle_fp32_matches_oracle(ctx, systems, name, maxLevels):
+@pytest.mark.parametrize("name,maxLevels,fine,coarse", [("lap40", 10, 2, 1), ("fv32", 10, 2, 1), ("fv32", 3, 2, 1),
+                                                        ("fv24n", 10, 2, 1), ("lap40", 10, 1, 1),
+                                                        ("lap40", 10, 3, 2)])
+def test_vcycle_fp32_matches_oracle(ctx, systems, name, maxLevels, fine, coarse):
     """Dense exact coarsest solve (<= 128 rows; pure Neumann in fv24n) and,
-    with 3 levels, Jacobi sweeps on a 253-row coarsest level."""
+    with 3 levels, Jacobi sweeps on a 253-row coarsest level; one to three
+    smoothing sweeps on the finest level, one or two on the K-cycle levels."""
     A, _ = systems[name]
     N, G = make(ctx, A)
     n = A.shape[0]
@@ -75,13 +78,13 @@ def test_vcycle_fp32_matches_oracle(ctx, systems, name, maxLevels):
                                   rng.uniform(0.4, 1.0, n).astype(np.float32),
                                   rng.uniform(0.5, 1.5, n).astype(np.float32))
     dpre, dscale, dpost, drestrict = ctx.array(pre), ctx.array(scale), ctx.array(post), ctx.array(restrict)
-    G.set_mg(coarseScale=1.7, coarseSweeps=20, maxLevels=maxLevels)
+    G.set_mg(coarseScale=1.7, coarseSweeps=20, maxLevels=maxLevels, smoothFine=fine, smoothCoarse=coarse)
     G.set_mg_weights(dpre, dscale, dpost, drestrict)
     r = rng.standard_normal(n)
     r = (r - r.mean()).astype(np.float32)
     z = ctx.empty(n, np.float32)
     G.mg_apply(ctx.array(r), z)
-    p = MG.Params(coarseScale=1.7, coarseSweeps=20)
+    p = MG.Params(coarseScale=1.7, coarseSweeps=20, smoothFine=fine, smoothCoarse=coarse)
     H = MG.hierarchy(A, maxLevels=maxLevels)
     assert len(H.A) == len(G.mg_info()[0])
     ref = MG.vcycle(H, r.astype(np.float64), p, pre.astype(np.float64), scale.astype(np.float64),
@@ -122,10 +125,12 @@ def test_identity_network_limit_equals_jacobi_fcg(ctx, systems):
     N, G = make(ctx, A)
     n = A.shape[0]
     one, zero = ctx.array(np.ones(n, np.float32)), ctx.array(np.zeros(n, np.float32))
+    # with one smoothing sweep, a = 1, s = 0, c = 0 makes the cycle z = D^-1 r
+    G.set_mg(smoothFine=1)
     G.set_mg_weights(one, zero, zero)
     _, rl = G.solve(b, None, 1e-8, 5000, N.SOLVER_CG, N.PRECOND_MG, N.FP32_IR)
     N2, J = make(ctx, A)
-    J.set_mg(maxLevels=1, omegaPre=1.0, omegaPost=0.0)
+    J.set_mg(maxLevels=1, omegaPre=1.0, omegaPost=0.0, smoothFine=1)
     _, rj = J.solve(b, None, 1e-8, 5000, N.SOLVER_CG, N.PRECOND_MG, N.FP32_IR)
     # FP64 Jacobi-PCG: FCG with a fixed SPD preconditioner has PCG's iterates in exact arithmetic
     _, rp = J.solve(b, None, 1e-8, 5000, N.SOLVER_CG, N.PRECOND_JACOBI, N.FP64)
@@ -179,7 +184,7 @@ def test_graph_replay_equals_eager_launches(ctx, systems):
     assert np.array_equal(xs[0], xs[1]) and np.array_equal(xs[0], xs[2])
 
 
-@pytest.mark.parametrize("case", ["lap40-k1", "fv24n-k0", "lap30-jacobi", "fv32-k2"])
+@pytest.mark.parametrize("case", ["lap40-k1", "fv24n-k0", "lap30-jacobi", "fv32-k2", "fv32-k2-sweeps"])
 def test_fused_kernels_equal_separate_launches(ctx, systems, case):
     """Launch fusion changes no bit: the small coarsest levels' cycle in one
     workgroup (fusion bit 0; dense and Jacobi-swept coarsest level), the
@@ -200,9 +205,12 @@ def test_fused_kernels_equal_separate_launches(ctx, systems, case):
     elif case == "lap30-jacobi":
         A, b = laplacian3d(30, 30, 30, seed=8), None
         kw = dict(kcycleDepth=0, coarseMax=1000, coarseSweeps=12)
-    else:
+    elif case == "fv32-k2":
         A, b = systems["fv32"]
         kw = dict(kcycleDepth=2)
+    else:
+        A, b = systems["fv32"]
+        kw = dict(kcycleDepth=2, smoothFine=3, smoothCoarse=2)
     n = A.shape[0]
     if b is None:
         b = np.random.default_rng(9).standard_normal(n)
@@ -211,7 +219,7 @@ def test_fused_kernels_equal_separate_launches(ctx, systems, case):
     rows, _ = G.mg_info()
     tail = G.mg_tail_level()
     print(f"[{case}] levels {rows}, tail from level {tail}", flush=True)
-    if case != "fv32-k2":
+    if not case.startswith("fv32-k2"):
         assert 1 <= tail < len(rows) - 1 and tail > kw["kcycleDepth"]
     else:
         assert len(rows) >= 4  # a K-cycle level below level 1: its combination is deferred

@@ -1,6 +1,7 @@
 """The multigrid / flexible-CG oracle pinned by invariants and closed forms
 (the preconditioner has no counterpart in the reference)."""
 import numpy as np
+import pytest
 import scipy.sparse as sp
 
 import mg_oracle as MG
@@ -106,7 +107,10 @@ def test_hierarchy_galerkin_is_ptap_and_preserves_row_sums():
         np.testing.assert_allclose(coarse_rs, np.bincount(H.agg[l], weights=fine_rs), atol=1e-12)
 
 
-def test_vcycle_two_level_matches_closed_form_error_propagation():
+@pytest.mark.parametrize("sweeps", [1, 2, 3])
+def test_vcycle_two_level_matches_closed_form_error_propagation(sweeps):
+    """E = (I - Wpost A)^k (I - S P Ac^-1 P^T T A) (I - Wpre A)^k with k
+    smoothing sweeps on each side, M = (I - E) A^-1."""
     A = laplacian3d(6, 5, 4, seed=4)
     H = MG.hierarchy(A, maxLevels=2, coarseMax=10)
     assert len(H.A) == 2
@@ -114,14 +118,16 @@ def test_vcycle_two_level_matches_closed_form_error_propagation():
     rng = np.random.default_rng(5)
     pre, scale, post = rng.uniform(0.3, 1.0, n), rng.uniform(0.5, 2.5, n), rng.uniform(0.3, 1.0, n)
     restrict = rng.uniform(0.5, 1.5, n)
-    p = MG.Params()
+    p = MG.Params(smoothFine=sweeps)
     Ad = A.toarray()
     Dinv = np.diag(1.0 / np.diag(Ad))
     P = H.P[0].toarray()
     Ac = P.T @ Ad @ P
     I = np.eye(n)
-    E = (I - np.diag(post) @ Dinv @ Ad) @ (
-        I - np.diag(scale) @ P @ np.linalg.solve(Ac, P.T @ np.diag(restrict) @ Ad)) @ (I - np.diag(pre) @ Dinv @ Ad)
+    mp = np.linalg.matrix_power
+    E = mp(I - np.diag(post) @ Dinv @ Ad, sweeps) @ (
+        I - np.diag(scale) @ P @ np.linalg.solve(Ac, P.T @ np.diag(restrict) @ Ad)) @ mp(
+        I - np.diag(pre) @ Dinv @ Ad, sweeps)
     M = (I - E) @ np.linalg.inv(Ad)
     r = rng.standard_normal(n)
     z = MG.vcycle(H, r, p, pre, scale, post, exact_coarse=True, restrict=restrict)
@@ -174,7 +180,7 @@ def test_jacobi_limit_of_the_cycle():
     H = MG.hierarchy(A, coarseMax=30)
     r = np.random.default_rng(9).standard_normal(A.shape[0])
     n = A.shape[0]
-    z = MG.vcycle(H, r, MG.Params(), np.ones(n), np.zeros(n), np.zeros(n))
+    z = MG.vcycle(H, r, MG.Params(smoothFine=1), np.ones(n), np.zeros(n), np.zeros(n))
     np.testing.assert_array_equal(z, (1.0 / A.diagonal()) * r)
 
 
