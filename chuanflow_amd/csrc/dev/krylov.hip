@@ -81,6 +81,27 @@ void DevPattern::build(std::vector<int> rowPtrH, std::vector<int> colH, hipStrea
             if (colH[k] == r) { diag[r] = k; break; }
     if (blocks.empty()) blocks = buildRowBlocks(rowPtrH);
     nRowBlocks = static_cast<int>(blocks.size()) - 1;
+    // the columns as int16 offsets from each block's first row, where all fit
+    {
+        std::vector<short> c16(static_cast<std::size_t>(nnz));
+        bool fits = true;
+        for (int b = 0; b < nRowBlocks && fits; ++b)
+        {
+            const int r0 = blocks[b];
+            for (int k = rowPtrH[r0]; k < rowPtrH[blocks[b + 1]]; ++k)
+            {
+                const int o = colH[k] - r0;
+                if (o < -32768 || o > 32767)
+                {
+                    fits = false;
+                    break;
+                }
+                c16[k] = static_cast<short>(o);
+            }
+        }
+        if (fits && nnz > 0) col16 = upload(c16, s);
+        else col16 = DevBuf();
+    }
     rowPtr = upload(rowPtrH, s);
     col = upload(colH, s);
     diagSlot = upload(diag, s);
@@ -111,7 +132,7 @@ __device__ __forceinline__ bool spmvRowSum(const CsrView& A, const T* __restrict
     row = r0 + static_cast<int>(threadIdx.x);
     if (nnz <= kSpmvLds)
     {
-        stageProducts(lds, val, A.col, x, k0, nnz);
+        stageProducts(lds, val, A, x, k0, r0, nnz);
         __syncthreads();
         if (row < r1)
         {

@@ -209,7 +209,7 @@ __device__ __forceinline__ bool rowSum(const CsrView& A, const float* __restrict
     float s = 0.0f;
     if (nnz <= LDS)
     {
-        stageProducts(lds, valM, A.col, v, k0, nnz);
+        stageProducts(lds, valM, A, v, k0, r0, nnz);
         __syncthreads();
         if (row < r1)
             for (int k = A.rowPtr[row] - k0, e = A.rowPtr[row + 1] - k0; k < e; ++k) s += lds[k];

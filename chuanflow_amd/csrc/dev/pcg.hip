@@ -113,7 +113,7 @@ __device__ __forceinline__ void body_pcg_residual(CsrView A, int rbPerWg, const 
         double s = 0;
         if (nnz <= kSpmvLds)
         {
-            stageProducts(lds, val, A.col, x, k0, nnz);
+            stageProducts(lds, val, A, x, k0, r0, nnz);
             __syncthreads();
             if (row < r1)
                 for (int k = A.rowPtr[row] - k0, e = A.rowPtr[row + 1] - k0; k < e; ++k) s += lds[k];
@@ -180,7 +180,7 @@ __device__ __forceinline__ void body_pcg_A(CsrView A, int rbPerWg, const T* __re
         T s = T(0);
         if (nnz <= kSpmvLds)
         {
-            stageProducts(lds, val, A.col, p, k0, nnz);
+            stageProducts(lds, val, A, p, k0, r0, nnz);
             __syncthreads();
             if (row < r1)
                 for (int kk = A.rowPtr[row] - k0, e = A.rowPtr[row + 1] - k0; kk < e; ++kk) s += lds[kk];
@@ -404,7 +404,7 @@ __global__ __launch_bounds__(kB) void k_fcg_A(CsrView A, int rbPerWg, const doub
         double s = 0.0;
         if (nnz <= kSpmvLds)
         {
-            stageProducts(lds, val, A.col, p, k0, nnz);
+            stageProducts(lds, val, A, p, k0, r0, nnz);
             __syncthreads();
             if (row < r1)
                 for (int kk = A.rowPtr[row] - k0, e = A.rowPtr[row + 1] - k0; kk < e; ++kk) s += lds[kk];

@@ -87,18 +87,56 @@ struct CsrView
     const int* col = nullptr;
     const int* diagSlot = nullptr;
     const int* rowBlocks = nullptr;
+    // each column less the first row of its row block, as int16 (null unless every offset fits): the
+    // CSR-stream staging reads 2 bytes per nonzero instead of 4
+    const short* col16 = nullptr;
 };
+
+// The same for a row block starting at row r0 of a pattern, through its int16
+// column offsets where it has them (the same columns, so the same products).
+template <typename TL, typename TV, typename TX>
+__device__ __forceinline__ void stageProducts(TL* lds, const TV* __restrict__ val, const CsrView& A,
+                                              const TX* __restrict__ x, int k0, int r0, int nnz)
+{
+    if (!A.col16)
+    {
+        stageProducts(lds, val, A.col, x, k0, nnz);
+        return;
+    }
+    const short* __restrict__ c16 = A.col16;
+    const int bd = static_cast<int>(blockDim.x);
+    int k = static_cast<int>(threadIdx.x);
+    for (; k + 3 * bd < nnz; k += 4 * bd)
+    {
+        TV v[4];
+        int c[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+        {
+            v[q] = val[k0 + k + q * bd];
+            c[q] = r0 + c16[k0 + k + q * bd];
+        }
+        TX xv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) xv[q] = x[c[q]];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) lds[k + q * bd] = static_cast<TL>(v[q]) * static_cast<TL>(xv[q]);
+    }
+    for (; k < nnz; k += bd) lds[k] = static_cast<TL>(val[k0 + k]) * static_cast<TL>(x[r0 + c16[k0 + k]]);
+}
+
 
 // Owning CSR pattern (int32 columns, rows sorted by column).
 struct DevPattern
 {
     int n = 0, nnz = 0, nRowBlocks = 0;
     std::uint64_t uid = 0;  // unique per build (caches keyed on a pattern compare this, not the address)
-    DevBuf rowPtr, col, diagSlot, rowBlocks;
+    DevBuf rowPtr, col, diagSlot, rowBlocks, col16;
     std::vector<int> hRowPtr, hCol;
     CsrView view() const
     {
-        return {n, nnz, nRowBlocks, rowPtr.as<int>(), col.as<int>(), diagSlot.as<int>(), rowBlocks.as<int>()};
+        return {n,           nnz, nRowBlocks, rowPtr.as<int>(), col.as<int>(), diagSlot.as<int>(), rowBlocks.as<int>(),
+                col16.p ? col16.as<short>() : nullptr};
     }
     // blocks: CSR-stream row blocks to use (empty: buildRowBlocks)
     void build(std::vector<int> rowPtrH, std::vector<int> colH, hipStream_t s, std::vector<int> blocks = {});
