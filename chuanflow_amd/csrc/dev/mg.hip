@@ -81,7 +81,8 @@ int aggregate(const std::vector<int>& rp, const std::vector<int>& col, std::vect
 
 // The coarse pattern of aggregation `agg` (na aggregates) of the pattern (rp, cl): coarse rows in
 // aggregate order, each row's columns ascending.  With gStart/gPerm also the Galerkin map: coarse
-// nonzero c sums the fine nonzeros gPerm[gStart[c] .. gStart[c+1]), in ascending slot order.
+// nonzero c sums the fine nonzeros gPerm[gStart[c] .. gStart[c+1]), in ascending slot order (the order
+// in which the members, ascending, and their slots are visited, so the slots need no sort).
 void coarsePattern(const std::vector<int>& rp, const std::vector<int>& cl, const std::vector<int>& agg, int na,
                    std::vector<int>& crp, std::vector<int>& ccol, std::vector<int>* gStart, std::vector<int>* gPerm)
 {
@@ -95,37 +96,16 @@ void coarsePattern(const std::vector<int>& rp, const std::vector<int>& cl, const
     }
     crp.assign(na + 1, 0);
     ccol.clear();
+    ccol.reserve(static_cast<std::size_t>(na) * 16);
     if (gPerm)
     {
         gStart->clear();
-        gPerm->clear();
-        gPerm->reserve(rp[n]);
-        std::vector<std::pair<int, int>> pairs;
-        for (int c = 0; c < na; ++c)
-        {
-            pairs.clear();
-            for (int t = aStart[c]; t < aStart[c + 1]; ++t)
-            {
-                const int i = aMem[t];
-                for (int k = rp[i]; k < rp[i + 1]; ++k) pairs.emplace_back(agg[cl[k]], k);
-            }
-            std::sort(pairs.begin(), pairs.end());
-            for (std::size_t t = 0; t < pairs.size(); ++t)
-            {
-                if (t == 0 || pairs[t].first != pairs[t - 1].first)
-                {
-                    gStart->push_back(static_cast<int>(gPerm->size()));
-                    ccol.push_back(pairs[t].first);
-                }
-                gPerm->push_back(pairs[t].second);
-            }
-            crp[c + 1] = static_cast<int>(ccol.size());
-        }
-        gStart->push_back(static_cast<int>(gPerm->size()));
-        return;
+        gStart->reserve(static_cast<std::size_t>(na) * 16 + 1);
+        gPerm->assign(rp[n], 0);
     }
-    // the pattern alone: distinct coarse columns per coarse row (a marker per column), then sorted
-    std::vector<int> mark(na, -1), row;
+    // per coarse column: the coarse row that last saw it, its fine-nonzero count and next output slot
+    std::vector<int> mark(na, -1), cnt(na), pos(na), row;
+    int out = 0;
     for (int c = 0; c < na; ++c)
     {
         row.clear();
@@ -138,14 +118,32 @@ void coarsePattern(const std::vector<int>& rp, const std::vector<int>& cl, const
                 if (mark[cc] != c)
                 {
                     mark[cc] = c;
+                    cnt[cc] = 0;
                     row.push_back(cc);
                 }
+                ++cnt[cc];
             }
         }
         std::sort(row.begin(), row.end());
-        ccol.insert(ccol.end(), row.begin(), row.end());
+        for (int cc : row)
+        {
+            if (gPerm)
+            {
+                gStart->push_back(out);
+                pos[cc] = out;
+                out += cnt[cc];
+            }
+            ccol.push_back(cc);
+        }
         crp[c + 1] = static_cast<int>(ccol.size());
+        if (gPerm)
+            for (int t = aStart[c]; t < aStart[c + 1]; ++t)
+            {
+                const int i = aMem[t];
+                for (int k = rp[i]; k < rp[i + 1]; ++k) (*gPerm)[pos[agg[cl[k]]]++] = k;
+            }
     }
+    if (gPerm) gStart->push_back(out);
 }
 
 // ------------------------------------------------------------------ kernels
@@ -1510,28 +1508,32 @@ Multigrid::Multigrid(Context& ctx, const DevPattern& fine, int maxLevels, int co
     hipStream_t s = ctx.stream();
     coarseMax = std::min(coarseMax, kCoarseMax);
     // 1. the aggregation of every level, each coarse level numbered in the order its aggregates were made
-    std::vector<std::vector<int>> RP{fine.hRowPtr}, CL{fine.hCol}, AG;
+    //    (RP[l], CL[l]: the pattern of level l + 1; level 0 is the fine pattern itself)
+    std::vector<std::vector<int>> RP, CL, AG;
+    auto rpOf = [&](int l) -> const std::vector<int>& { return l == 0 ? fine.hRowPtr : RP[l - 1]; };
+    auto clOf = [&](int l) -> const std::vector<int>& { return l == 0 ? fine.hCol : CL[l - 1]; };
     while (true)
     {
-        const int n = static_cast<int>(RP.back().size()) - 1;
-        if (n <= coarseMax || static_cast<int>(RP.size()) >= maxLevels) break;
+        const int l = static_cast<int>(RP.size());
+        const int n = static_cast<int>(rpOf(l).size()) - 1;
+        if (n <= coarseMax || l + 1 >= maxLevels) break;
         std::vector<int> agg;
-        const int na = aggregate(RP.back(), CL.back(), agg);
+        const int na = aggregate(rpOf(l), clOf(l), agg);
         if (na >= n) break;  // no coarsening possible
         std::vector<int> crp, ccol;
-        coarsePattern(RP.back(), CL.back(), agg, na, crp, ccol, nullptr, nullptr);
+        coarsePattern(rpOf(l), clOf(l), agg, na, crp, ccol, nullptr, nullptr);
         AG.push_back(std::move(agg));
         RP.push_back(std::move(crp));
         CL.push_back(std::move(ccol));
     }
-    const int last = static_cast<int>(RP.size()) - 1;
+    const int last = static_cast<int>(RP.size());
     // 2. every coarse level above the coarsest renumbered aggregate by aggregate (rows sorted by their
     //    aggregate, then by their first number), from the top down so that each level is sorted by the
     //    final numbers of the level above it: the same hierarchy, whose restriction from such a level sums
     //    contiguous rows and whose aggregates follow its rows in order
     for (int l = last - 1; l >= 1; --l)
     {
-        const int n = static_cast<int>(AG[l].size()), na = static_cast<int>(RP[l + 1].size()) - 1;
+        const int n = static_cast<int>(AG[l].size()), na = static_cast<int>(RP[l].size()) - 1;
         std::vector<int> start(na + 1, 0), pi(n);
         for (int i = 0; i < n; ++i) ++start[AG[l][i] + 1];
         for (int c = 0; c < na; ++c) start[c + 1] += start[c];
@@ -1541,8 +1543,8 @@ Multigrid::Multigrid(Context& ctx, const DevPattern& fine, int maxLevels, int co
         for (int i = 0; i < n; ++i) moved[pi[i]] = AG[l][i];
         AG[l].swap(moved);
     }
-    // 3. patterns, transfers and Galerkin maps in the final numbering
-    std::vector<int> rp = fine.hRowPtr, cl = fine.hCol;
+    // 3. patterns, transfers and Galerkin maps in the final numbering (rp, cl: level l's, after level 0)
+    std::vector<int> rp, cl;
     lv_.emplace_back();
     lv_[0].n = fine.n;
     lv_[0].nnz = fine.nnz;
@@ -1551,7 +1553,9 @@ Multigrid::Multigrid(Context& ctx, const DevPattern& fine, int maxLevels, int co
         MgLevel& L = lv_.back();
         const int n = L.n;
         const std::vector<int>& agg = AG[l];
-        const int na = static_cast<int>(RP[l + 1].size()) - 1;
+        const int na = static_cast<int>(RP[l].size()) - 1;
+        const std::vector<int>& lrp = l == 0 ? fine.hRowPtr : rp;
+        const std::vector<int>& lcl = l == 0 ? fine.hCol : cl;
         // members of each aggregate
         std::vector<int> aStart(na + 1, 0), aMem(n);
         for (int i = 0; i < n; ++i) ++aStart[agg[i] + 1];
@@ -1561,7 +1565,7 @@ Multigrid::Multigrid(Context& ctx, const DevPattern& fine, int maxLevels, int co
             for (int i = 0; i < n; ++i) aMem[fill[agg[i]]++] = i;
         }
         std::vector<int> crp, ccol, gStart, gPerm;
-        coarsePattern(rp, cl, agg, na, crp, ccol, &gStart, &gPerm);
+        coarsePattern(lrp, lcl, agg, na, crp, ccol, &gStart, &gPerm);
         L.agg = upload(agg, s);
         L.aggStart = upload(aStart, s);
         L.aggMem = upload(aMem, s);
