@@ -63,13 +63,13 @@ def canonical_fcg_bytes(n, nnz):
     return 12 * nnz + 4 * (n + 1) + 12 * n
 
 
-def fine_level_bytes(n, nnz):
-    # per FCG iteration on the finest level: k_fcg_A, then pre-smoothing (FP32 column-scaled values + columns,
-    # row pointer, b, w read; x, r written) and post-smoothing (FP32 values + columns, row pointer, x, b, w
-    # read; z written)
+def fine_level_bytes(n, nnz, sweeps=1):
+    # per FCG iteration on the finest level: k_fcg_A, then `sweeps` pre-smoothing passes (FP32 column-scaled
+    # values + int32 columns, row pointer, r and w read; x and r written) and as many post-smoothing passes
+    # (FP32 values + columns, row pointer, x, b, w read; z written)
     down = 8 * nnz + 4 * (n + 1) + 16 * n
     up = 8 * nnz + 4 * (n + 1) + 16 * n
-    return canonical_fcg_bytes(n, nnz) + down + up
+    return canonical_fcg_bytes(n, nnz) + sweeps * (down + up)
 
 
 def canonical_spmv_bytes(n, nnz, s):
@@ -483,7 +483,7 @@ def bench_c3(args):
         Ubc, pbc = cavity_bcs(arrays["patchStart"], arrays["patchSize"], len(arrays["neighbour"]), dm.nB)
         dU, dp, dphi = ctx.zeros((3, dm.nCells)), ctx.zeros(dm.nCells), ctx.zeros(dm.nFaces)
         dm.set_mg(scale=args.mg_scale, coarseScale=args.mg_scale, kcycleDepth=args.kcycle_depth,
-                  fusion=args.mg_fusion, smoothFine=args.smooth_fine, smoothCoarse=args.smooth_coarse)
+                  fusion=args.mg_fusion, smoothFine=args.c3_smooth_fine, smoothCoarse=args.c3_smooth_coarse)
         if learned:
             dm.set_mpnn(LN.load(ctx))
         return hm, arrays, geom, dm, Ubc, pbc, dU, dp, dphi
@@ -646,6 +646,10 @@ def main():
                          "weights, faster on these tets: 92.7 against 100.1 ms per step over 40 steps)")
     ap.add_argument("--c3-steps", type=int, default=500)
     ap.add_argument("--c3-pressure", default="mg", choices=["fp64", "fp32ir", "mg"])
+    # the cavity's pressure solves start from the last step's pressure and take a few iterations each: one
+    # smoothing sweep per side costs less there (61.1 against 66.0 ms per step over 60 steps with 3/2)
+    ap.add_argument("--c3-smooth-fine", type=int, default=1)
+    ap.add_argument("--c3-smooth-coarse", type=int, default=1)
     ap.add_argument("--c3-check-n", type=int, default=12)
     ap.add_argument("--c3-check-steps", type=int, default=20)
     ap.add_argument("--c3-laplacian", default="uncorrected", choices=["uncorrected", "corrected"])
@@ -820,10 +824,11 @@ def main():
         # hierarchy on the host and captures the FCG iteration as a graph (later solves on the pattern reuse both)
         "setup_ms": round(warm_mg[0] - tmg, 1) if warm_mg else None,
         "first_solve_ms": round(warm_mg[0], 1) if warm_mg else None,
-        # the three fine-level matrix passes of every learned FCG iteration (k_fcg_A, pre- and post-smoothing)
-        # at their canonical bytes, over the whole learned solve's wall time: a floor on what the solve streams
+        # the fine-level matrix passes of every learned FCG iteration (k_fcg_A, the pre- and post-smoothing
+        # sweeps) at their canonical bytes, over the whole learned solve's wall time: a floor on what it streams
         "learned_fine_level_GBps": round(
-            (rln[-1][1]["iterations"] + rln[-1][1]["outerIterations"]) * fine_level_bytes(n, nnz) / (tln * 1e-3) / 1e9,
+            (rln[-1][1]["iterations"] + rln[-1][1]["outerIterations"]) * fine_level_bytes(n, nnz, args.smooth_fine)
+            / (tln * 1e-3) / 1e9,
             1),
     }
     if not args.no_cpu:
