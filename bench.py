@@ -45,11 +45,12 @@ sys.path.insert(0, REPO)
 METRIC = "p-solve time to 1e-8 FP64 residual: learned+mixed PCG vs plain PCG; SpMV GB/s"
 HBM_PEAK_GBPS = 8000.0
 # HBM bytes per FP64 SpMV launch on the 128^3 system from rocprofv3 PMC (profiles/spmv_128cube_hbm_traffic.md):
-# FETCH_SIZE 100,831.1 KiB / 0.50005 (the copy-kernel calibration) + WRITE_SIZE 16,384 KiB
-SPMV_FP64_TRAFFIC_128 = 220712939
-# HBM bytes per k_fcg_A launch on the 128^3 system, the same way: FETCH_SIZE 105,493.0 KiB / 0.50005 + WRITE_SIZE
-# 8,320.1 KiB (profiles/spmv_128cube_hbm_traffic.md)
-FCG_A_TRAFFIC_128 = 224547574
+# FETCH_SIZE 84,851.1 KiB / 0.50005 (the copy-kernel calibration) + WRITE_SIZE 16,384 KiB; the staging reads
+# int16 column offsets, so the traffic is below the canonical bytes, which count 4-byte columns
+SPMV_FP64_TRAFFIC_128 = 190534903
+# HBM bytes per k_fcg_A launch on the 128^3 system, the same way: FETCH_SIZE 90,122.8 KiB / 0.50005 + WRITE_SIZE
+# 8,320.0 KiB (profiles/spmv_128cube_hbm_traffic.md)
+FCG_A_TRAFFIC_128 = 193072729
 SEED_MESH, SEED_SHUFFLE, SEED_RHS = 128, 7, 2024
 
 
