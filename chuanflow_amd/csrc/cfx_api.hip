@@ -1527,10 +1527,8 @@ int cfx_csr_mg_loss_grad(cfx_csr* A, const float* e, const float* a, const float
 {
     if (!A || !e || !a || !t || !ga || !gt || !loss) return CFX_E_INVALID;
     return guard(A->owner, [&] {
-        MgParams p = A->mg.p;
-        p.kcycleDepth = 0;
-        p.smoothFine = p.smoothCoarse = 1;  // the objective's cycle: one sweep each side
-        Multigrid mg(*A->owner->ctx, A->pattern, p.maxLevels, p.coarseMax);
+        const MgParams& p = A->mg.p;
+            Multigrid mg(*A->owner->ctx, A->pattern, p.maxLevels, p.coarseMax);
         mg.setMatrix(A->val.as<double>(), a, t, a, t, p);
         *loss = mg.lossGrad(e, a, t, ga, gt);
     });

@@ -43,12 +43,14 @@ namespace cfx
 
 struct MgParams
 {
-    float omegaPre = 2.0f / 3.0f;   // finest-level constants (used where no per-row values are given)
-    float scale = 1.0f;
-    float omegaPost = 2.0f / 3.0f;
-    float restrictScale = 1.0f;
-    float coarseOmega = 2.0f / 3.0f;  // coarse levels: pre = post smoother weight
-    float coarseScale = 1.0f;
+    // defaults tuned on a 128^3 jittered pressure system with the 3/2 sweeps (a grid over the weights, the
+    // transfer scales and the sweeps: 9 flexible-CG iterations against 13 with a = 2/3, s = 1.8, t = 1)
+    float omegaPre = 0.8f;            // finest-level constants (used where no per-row values are given)
+    float scale = 1.6f;
+    float omegaPost = 0.8f;
+    float restrictScale = 1.6f;
+    float coarseOmega = 0.9f;         // coarse levels: pre = post smoother weight
+    float coarseScale = 2.2f;
     int coarseSweeps = 30;            // damped Jacobi sweeps on a coarsest level above 128 rows
     float coarsestOmega = 2.0f / 3.0f;
     int maxLevels = 10, coarseMax = 128;
@@ -176,8 +178,10 @@ public:
                const int* done) override;
 
     // Training objective of the finest-level weights.  Requires setMatrix
-    // with pre = post = a, scale = restr = t and kcycleDepth = 0 (so the
-    // coarse correction Q is linear and symmetric).  For an error sample e
+    // with pre = post = a, scale = restr = t.  The coarse correction Q is the
+    // cycle's own below level 1 (the K-cycle on levels 1..kcycleDepth); the
+    // adjoint takes it as the linear symmetric operator it approximates
+    // (exact for kcycleDepth = 0).  For an error sample e
     // and z = M A e, L = 1 - (z.A e)^2 / ((e.A e)(z.A z)) = min over alpha of
     // |e - alpha z|_A^2 / |e|_A^2: the error left by one preconditioned step
     // with the best step length (invariant to the scale of M, as the Krylov
@@ -218,6 +222,7 @@ private:
     bool launchTailUp(int l, const float* in, float* out, const float* res, const double* partZQ, const int* done,
                       bool restricted);
     DevBuf tv_[14];          // finest-level FP32 scratch of lossGrad / smoothSample
+    std::vector<DevBuf> tk_; // lossGrad: the sweeps' D^-1 (b - A x) (2 (smoothFine - 1) vectors)
     DevBuf tpart_;
 };
 

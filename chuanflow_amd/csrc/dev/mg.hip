@@ -1486,6 +1486,30 @@ __global__ void k_lg_end(int n, const float* __restrict__ gx2, const float* __re
     if (i < n) ga[i] += (gx2[i] - atp[i]) * db[i];
 }
 
+// one smoothing sweep of the objective's cycle: p = D^-1 (b - A x), x' = x + a p (x' may be x); p is kept
+// for the adjoint
+__global__ void k_lg_sweep(int n, const float* __restrict__ b, const float* __restrict__ ax,
+                           const float* __restrict__ invD, const float* __restrict__ a, const float* x, float* xo,
+                           float* __restrict__ pj)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    const float p = invD[i] * (b[i] - ax[i]);
+    pj[i] = p;
+    xo[i] = x[i] + a[i] * p;
+}
+
+// the adjoint of that sweep, first half: ga += g p, w = D^-1 (a g) (the gradient passed back is g - A w)
+__global__ void k_lg_back(int n, const float* __restrict__ g, const float* __restrict__ pj,
+                          const float* __restrict__ invD, const float* __restrict__ a, float* __restrict__ ga,
+                          float* __restrict__ w)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    ga[i] += g[i] * pj[i];
+    w[i] = invD[i] * (a[i] * g[i]);
+}
+
 __global__ void k_lg_isub(int n, float* __restrict__ e, const float* __restrict__ z)
 {
     const int i = blockIdx.x * B + threadIdx.x;
@@ -1967,17 +1991,24 @@ void Multigrid::apply(const float* r, float* z, const float* q, double* partZQ, 
 double Multigrid::lossGrad(const float* e, const float* a, const float* t, float* ga, float* gt)
 {
     if (levels() < 2) throw std::invalid_argument("lossGrad: the hierarchy has no coarse level");
-    if (p_.kcycleDepth != 0) throw std::invalid_argument("lossGrad: needs the linear cycle (kcycleDepth = 0)");
-    if (p_.smoothFine != 1) throw std::invalid_argument("lossGrad: needs one smoothing sweep (smoothFine = 1)");
     hipStream_t s = ctx_.stream();
     MgLevel& L = lv_[0];
     MgLevel& C = lv_[1];
     const int n = L.n;
+    const int k = std::max(1, p_.smoothFine);  // smoothing sweeps on each side
     if (!tv_[0].p)
     {
         for (DevBuf& v : tv_) v.alloc(sizeof(float) * n);
         tpart_.alloc(sizeof(double) * (2 * kPartMax + 16));
     }
+    // D^-1 (b - A x) of the pre-smoothing sweeps 2..k and of the post-smoothing sweeps 1..k-1
+    while (static_cast<int>(tk_.size()) < 2 * (k - 1))
+    {
+        tk_.emplace_back();
+        tk_.back().alloc(sizeof(float) * n);
+    }
+    auto pre = [&](int j) { return tk_[j].as<float>(); };           // j = 0 .. k-2
+    auto post = [&](int j) { return tk_[k - 1 + j].as<float>(); };  // j = 0 .. k-2
     float *b = tv_[0].as<float>(), *db = tv_[1].as<float>(), *x1 = tv_[2].as<float>(), *ax = tv_[3].as<float>(),
           *r1 = tv_[4].as<float>(), *u = tv_[5].as<float>(), *x2 = tv_[6].as<float>(), *pd = tv_[7].as<float>(),
           *eps = tv_[8].as<float>(), *aeps = tv_[9].as<float>(), *gz = tv_[10].as<float>(),
@@ -1991,12 +2022,22 @@ double Multigrid::lossGrad(const float* e, const float* a, const float* t, float
     // forward: the cycle applied to b = A e
     k_spmv_f<<<A.nRowBlocks, B, 0, s>>>(A, v32, e, b);
     k_lg_start<<<G, B, 0, s>>>(n, invD, a, b, db, x1);
+    for (int j = 0; j + 1 < k; ++j)
+    {
+        k_spmv_f<<<A.nRowBlocks, B, 0, s>>>(A, v32, x1, ax);
+        k_lg_sweep<<<G, B, 0, s>>>(n, b, ax, invD, a, x1, x1, pre(j));
+    }
     k_spmv_f<<<A.nRowBlocks, B, 0, s>>>(A, v32, x1, ax);
     k_lg_sub<<<G, B, 0, s>>>(n, b, ax, r1);
     k_mg_restrict_csr<<<L.rpat.nRowBlocks, B, 0, s>>>(L.rpat.view(), L.rval.as<float>(), r1, C.b.as<float>(), nullptr);
     solveLevel(1, nullptr);
     k_lg_gather<<<G, B, 0, s>>>(n, L.agg.as<int>(), C.e.as<float>(), u);
     k_lg_x2<<<G, B, 0, s>>>(n, x1, t, u, x2);
+    for (int j = 0; j + 1 < k; ++j)
+    {
+        k_spmv_f<<<A.nRowBlocks, B, 0, s>>>(A, v32, x2, ax);
+        k_lg_sweep<<<G, B, 0, s>>>(n, b, ax, invD, a, x2, x2, post(j));
+    }
     k_spmv_f<<<A.nRowBlocks, B, 0, s>>>(A, v32, x2, ax);
     k_lg_z<<<G, B, 0, s>>>(n, b, ax, invD, a, x2, e, pd, eps);
     k_spmv_f<<<A.nRowBlocks, B, 0, s>>>(A, v32, eps, aeps);
@@ -2006,15 +2047,30 @@ double Multigrid::lossGrad(const float* e, const float* a, const float* t, float
     k_final<<<1, B, 0, s>>>(Gd, part + kPartMax, sc + 1, 0);
     k_dot_part<<<Gd, B, 0, s>>>(n, eps, aeps, part);
     k_final<<<1, B, 0, s>>>(Gd, part, sc + 2, 0);
-    // adjoint
+    // adjoint: back through the post-smoothing sweeps (the last in k_lg_g), each passing g - A D^-1 (a g)
     k_lg_g<<<G, B, 0, s>>>(n, b, aeps, sc, a, invD, pd, gz, w1, ga);
+    for (int j = k - 2; j >= 0; --j)
+    {
+        k_spmv_f<<<A.nRowBlocks, B, 0, s>>>(A, v32, w1, ax);
+        k_lg_isub<<<G, B, 0, s>>>(n, gz, ax);
+        k_lg_back<<<G, B, 0, s>>>(n, gz, post(j), invD, a, ga, w1);
+    }
     k_spmv_f<<<A.nRowBlocks, B, 0, s>>>(A, v32, w1, ax);
     k_lg_gx2<<<G, B, 0, s>>>(n, gz, ax, u, gx2, gt);
+    // the coarse correction (symmetric) and the restriction
     k_mg_restrict_csr<<<L.rpat.nRowBlocks, B, 0, s>>>(L.rpat.view(), L.rval.as<float>(), gx2, C.b.as<float>(), nullptr);
     solveLevel(1, nullptr);
     k_lg_gather<<<G, B, 0, s>>>(n, L.agg.as<int>(), C.e.as<float>(), py);
     k_lg_tail<<<G, B, 0, s>>>(n, r1, py, t, gt, x1);
     k_spmv_f<<<A.nRowBlocks, B, 0, s>>>(A, v32, x1, ax);
+    // back through the pre-smoothing sweeps k..2, then x1 = a D^-1 b
+    if (k > 1) k_lg_isub<<<G, B, 0, s>>>(n, gx2, ax);
+    for (int j = k - 2; j >= 0; --j)
+    {
+        k_lg_back<<<G, B, 0, s>>>(n, gx2, pre(j), invD, a, ga, w1);
+        k_spmv_f<<<A.nRowBlocks, B, 0, s>>>(A, v32, w1, ax);
+        if (j > 0) k_lg_isub<<<G, B, 0, s>>>(n, gx2, ax);
+    }
     k_lg_end<<<G, B, 0, s>>>(n, gx2, ax, db, ga);
     CFX_CHECK_LAUNCH();
     double h[3];

@@ -111,8 +111,6 @@ std::vector<double> MpnnTrainer::train(const std::vector<TrainSystem>& systems, 
     if (systems.empty()) throw std::invalid_argument("train: no systems");
     hipStream_t s = ctx_.stream();
     MgParams p = opt.mg;
-    p.kcycleDepth = 0;
-    p.smoothFine = p.smoothCoarse = 1;  // the objective's cycle: one sweep each side
     std::vector<double> hist;
     for (int step = 0; step < opt.steps; ++step)
     {
@@ -151,8 +149,6 @@ double MpnnTrainer::evaluate(const TrainSystem& sys, const TrainOptions& opt, in
 {
     hipStream_t s = ctx_.stream();
     MgParams p = opt.mg;
-    p.kcycleDepth = 0;
-    p.smoothFine = p.smoothCoarse = 1;  // the objective's cycle: one sweep each side
     const int n = sys.pat->n;
     reserve(n);
     Multigrid mg(ctx_, *sys.pat, p.maxLevels, p.coarseMax);

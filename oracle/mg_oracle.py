@@ -77,12 +77,12 @@ def prolongation(agg, na):
 class Params:
     """Finest-level a (pre), s (scale), c (post) constants, coarse-level
     constants (pre = post weight), coarsest sweeps, hierarchy caps."""
-    omegaPre: float = 2.0 / 3.0
-    scale: float = 1.0
-    omegaPost: float = 2.0 / 3.0
-    restrictScale: float = 1.0
-    coarseOmega: float = 2.0 / 3.0
-    coarseScale: float = 1.0
+    omegaPre: float = 0.800000011920929    # the defaults of the device cycle (FP32 constants)
+    scale: float = 1.600000023841858
+    omegaPost: float = 0.800000011920929
+    restrictScale: float = 1.600000023841858
+    coarseOmega: float = 0.8999999761581421
+    coarseScale: float = 2.200000047683716
     coarseSweeps: int = 30
     coarsestOmega: float = 2.0 / 3.0
     maxLevels: int = 10

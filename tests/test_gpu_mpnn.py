@@ -87,13 +87,17 @@ def test_backward_fp16_close_to_oracle(ctx):
     assert rel(g, ga) < 5e-2, rel(g, ga)
 
 
-def test_objective_gradient_matches_oracle(ctx):
-    print("\n[case] multigrid objective adjoint", flush=True)
+@pytest.mark.parametrize("sweeps,kcycle", [(1, 0), (3, 0), (3, 2)])
+def test_objective_gradient_matches_oracle(ctx, sweeps, kcycle):
+    """The device adjoint against the oracle's: the exact gradient with the
+    V-cycle below level 1 (pinned by central differences on the CPU), and the
+    same symmetric-coarse-correction gradient with the K-cycle."""
+    print(f"\n[case] multigrid objective adjoint, {sweeps} smoothing sweeps, K-cycle depth {kcycle}", flush=True)
     from chuanflow_amd import native as N
     A = laplacian3d(20, 18, 16, seed=8)
     n = A.shape[0]
     G = N.Csr(ctx, A.indptr.astype(np.int64), A.indices.astype(np.int64), A.data)
-    G.set_mg(coarseScale=1.6)
+    G.set_mg(coarseScale=1.6, smoothFine=sweeps, kcycleDepth=kcycle)
     rng = np.random.default_rng(9)
     a = rng.uniform(0.5, 0.8, n).astype(np.float32)
     t = rng.uniform(1.1, 1.5, n).astype(np.float32)
@@ -101,7 +105,7 @@ def test_objective_gradient_matches_oracle(ctx):
     L, ga, gt = G.mg_loss_grad(ctx.array(e), ctx.array(a), ctx.array(t))
     Hh = MG.hierarchy(A)
     Lo, gao, gto = NN.objective_grad(Hh, e.astype(np.float64), a.astype(np.float64), t.astype(np.float64),
-                                     MG.Params(coarseScale=1.6))
+                                     MG.Params(coarseScale=1.6, smoothFine=sweeps, kcycleDepth=kcycle))
     print(f"L gpu {L:.6e} oracle {Lo:.6e}", flush=True)
     assert abs(L - Lo) <= 1e-4 * Lo
     assert rel(ga.get(), gao) < 1e-3 and rel(gt.get(), gto) < 1e-3, (rel(ga.get(), gao), rel(gt.get(), gto))
