@@ -126,27 +126,42 @@ __global__ void k_hfree(int n, double relaxU, const double* __restrict__ UOld, c
     H[i] = (1.0 - relaxU) * UOld[i] + relaxU * x[i];
 }
 
-// adjustPhi (simpleSolver.cpp:211-247): patch sums in face order; one thread.
-__global__ void k_adjust_phi(int nPatch, const long long* __restrict__ pStart, const long long* __restrict__ pSize,
-                             const int* __restrict__ fixedU, double* __restrict__ phi)
+// adjustPhi (simpleSolver.cpp:211-247), in two parallel passes.  Each patch's
+// flux sum is one workgroup's: lane t adds faces t, t + B, ... in order, then a
+// fixed tree over the lanes (a sum of the same terms as the reference's face-order
+// loop, grouped differently).
+__global__ __launch_bounds__(B) void k_patch_sums(const long long* __restrict__ pStart,
+                                                  const long long* __restrict__ pSize, const double* __restrict__ phi,
+                                                  double* __restrict__ psum)
 {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    __shared__ double red[B / kWave];
+    const int pi = blockIdx.x;
+    const long long st = pStart[pi], sz = pSize[pi];
+    double s = 0.0;
+    for (long long f = threadIdx.x; f < sz; f += B) s += phi[st + f];
+    s = blockSum(s, red);
+    if (threadIdx.x == 0) psum[pi] = s;
+}
+// ... then the net and free sums in patch order (the reference's), and every
+// face of a patch whose velocity is not fixed scaled by (free - net) / free
+__global__ void k_adjust_phi(int nPatch, const long long* __restrict__ pStart, const long long* __restrict__ pSize,
+                             const int* __restrict__ fixedU, const double* __restrict__ psum, double* __restrict__ phi)
+{
     double netSum = 0, freeSum = 0;
     for (int pi = 0; pi < nPatch; ++pi)
     {
-        double s = 0;
-        for (long long f = pStart[pi]; f < pStart[pi] + pSize[pi]; ++f) s += phi[f];
-        netSum += s;
-        if (!fixedU[pi]) freeSum += s;
+        netSum += psum[pi];
+        if (!fixedU[pi]) freeSum += psum[pi];
     }
-    if (fabs(freeSum) > 1e-30)
+    if (!(fabs(freeSum) > 1e-30)) return;
+    const double scale = (freeSum - netSum) / freeSum;
+    for (int pi = 0; pi < nPatch; ++pi)
     {
-        const double scale = (freeSum - netSum) / freeSum;
-        for (int pi = 0; pi < nPatch; ++pi)
-        {
-            if (fixedU[pi]) continue;
-            for (long long f = pStart[pi]; f < pStart[pi] + pSize[pi]; ++f) phi[f] *= scale;
-        }
+        if (fixedU[pi]) continue;
+        const long long st = pStart[pi], sz = pSize[pi];
+        for (long long f = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; f < sz;
+             f += static_cast<long long>(gridDim.x) * blockDim.x)
+            phi[st + f] *= scale;
     }
 }
 
@@ -612,9 +627,13 @@ SimpleResult simpleSolve(DevMesh& m, Krylov& K, double nu, const Schemes& sch, c
         k_hfree<<<grid(3LL * n), B, 0, s>>>(n, c.relaxU, w.UOld.as<double>(), w.xU.as<double>(), w.H.as<double>());
         updatePhiInternal(m, w.H.as<double>(), phi);
         correctBoundaryFlux(m, w.H.as<double>(), w.Hlc.as<double>(), w.Hlk.as<double>(), phi);
-        if (needReference)
-            k_adjust_phi<<<1, 1, 0, s>>>(m.nPatches, w.patchStart.as<long long>(), w.patchSize.as<long long>(),
-                                         w.fixedU.as<int>(), phi);
+        if (needReference && m.nPatches > 0)
+        {
+            k_patch_sums<<<m.nPatches, B, 0, s>>>(w.patchStart.as<long long>(), w.patchSize.as<long long>(), phi,
+                                                  part);
+            k_adjust_phi<<<gridCap(m.nB), B, 0, s>>>(m.nPatches, w.patchStart.as<long long>(),
+                                                     w.patchSize.as<long long>(), w.fixedU.as<int>(), part, phi);
+        }
         continuityError(m, phi, w.divPhi.as<double>(), sc + kContErr, part);
         // pressure
         interpolateToFaces(m, w.rAU.as<double>(), w.rAUf.as<double>());
