@@ -51,6 +51,9 @@ SPMV_FP64_TRAFFIC_128 = 190534903
 # HBM bytes per k_fcg_A launch on the 128^3 system, the same way: FETCH_SIZE 90,122.8 KiB / 0.50005 + WRITE_SIZE
 # 8,320.0 KiB (profiles/spmv_128cube_hbm_traffic.md)
 FCG_A_TRAFFIC_128 = 193072729
+# HBM bytes per finest-level k_mg_up launch on the 128^3 system, the same way: FETCH_SIZE 59,431.1 KiB / 0.50005 +
+# WRITE_SIZE 8,192.0 KiB (profiles/spmv_128cube_hbm_traffic.md)
+MG_UP_TRAFFIC_128 = 130091802
 SEED_MESH, SEED_SHUFFLE, SEED_RHS = 128, 7, 2024
 
 
@@ -71,6 +74,12 @@ def fine_level_bytes(n, nnz, sweeps=1):
     down = 8 * nnz + 4 * (n + 1) + 16 * n
     up = 8 * nnz + 4 * (n + 1) + 16 * n
     return canonical_fcg_bytes(n, nnz) + sweeps * (down + up)
+
+
+def canonical_up_bytes(n, nnz):
+    # one finest-level post-smoothing sweep z = x + W (b - A x): FP32 value + int32 column per nonzero, int32
+    # row pointer, x, b and W read once, z written once
+    return 8 * nnz + 4 * (n + 1) + 16 * n
 
 
 def canonical_spmv_bytes(n, nnz, s):
@@ -585,8 +594,8 @@ def bench_c3(args):
 
 
 def bench_spmv(args):
-    """SpMV alone on the 128^3 pressure system (FP64 and FP32 values, and the flexible CG's fine-level
-    k_fcg_A; args.spmv_reps launches each) and a
+    """SpMV alone on the 128^3 pressure system (FP64 and FP32 values, the flexible CG's fine-level
+    k_fcg_A and the multigrid's finest-level post-smoothing sweep k_mg_up; args.spmv_reps launches each) and a
     device-to-device copy of a known byte count: the workload the HBM traffic counters are read on
     (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE; the copy's counters give the scale to bytes)."""
     from chuanflow_amd import native as N
@@ -600,6 +609,8 @@ def bench_spmv(args):
     ys32 = ctx.empty(n, np.float32)
     ms32 = A.time_spmv(0, xs32, ys32, args.spmv_reps, N.FP32_IR)
     msf = A.time_fcg_spmv(args.spmv_reps)
+    # the finest-level post-smoothing sweep (the call builds the multigrid hierarchy on the pattern; no solve)
+    msu = A.time_smooth(args.spmv_reps)
     copy_bytes = 256 << 20
     src, dst = ctx.zeros(copy_bytes // 8), ctx.empty(copy_bytes // 8)
     for _ in range(args.spmv_reps):
@@ -613,6 +624,8 @@ def bench_spmv(args):
                       "spmv_fp32_GBps": round(b32 / (ms32 * 1e-3) / 1e9, 1),
                       "fcg_a_us": round(msf * 1e3, 2), "fcg_a_bytes": canonical_fcg_bytes(n, nnz),
                       "fcg_a_GBps": round(canonical_fcg_bytes(n, nnz) / (msf * 1e-3) / 1e9, 1),
+                      "mg_up_us": round(msu * 1e3, 2), "mg_up_bytes": canonical_up_bytes(n, nnz),
+                      "mg_up_GBps": round(canonical_up_bytes(n, nnz) / (msu * 1e-3) / 1e9, 1),
                       "copy_bytes_read": copy_bytes, "copy_bytes_written": copy_bytes}), flush=True)
 
 
@@ -735,7 +748,13 @@ def main():
     ctx.sync()
     ctx.set_fcg_probe(False)
     ms_fcg, n_fcg = A.fcg_probe()
+    ms_up, n_up = A.smooth_probe()
     ms_fcg_loop = A.time_fcg_spmv(args.spmv_reps)
+    ms_up_loop = A.time_smooth(args.spmv_reps)
+    b_up = canonical_up_bytes(n, nnz)
+    gbs_up = b_up / (ms_up * 1e-3) / 1e9
+    log(f"k_mg_up (finest post-smoothing) in the solve {ms_up*1e3:.1f} us ({n_up} launches) = {gbs_up:.0f} GB/s "
+        f"(canonical bytes); back to back {ms_up_loop*1e3:.1f} us")
     b_fcg = canonical_fcg_bytes(n, nnz)
     gbs_fcg = b_fcg / (ms_fcg * 1e-3) / 1e9
     log(f"k_fcg_A in the solve {ms_fcg*1e3:.1f} us ({n_fcg} launches) = {gbs_fcg:.0f} GB/s (canonical bytes); "
@@ -826,13 +845,21 @@ def main():
         "spmv_fp64_vs_stream_read": round(gbs64 / read_gbs, 3),
         "device_stream_read_spmv_bytes_GBps": round(read_ws_gbs, 1),
         "spmv_fp64_vs_stream_read_same_bytes": round(gbs64 / read_ws_gbs, 3),
-        "roofline": {"bound": "hbm", "achieved": round(gbs_fcg, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(gbs_fcg / HBM_PEAK_GBPS, 4),
-                     "traffic": FCG_A_TRAFFIC_128 if args.n == 128 else None,
-                     "kernel": "k_fcg_A (the learned solve's fine-level q = A p: FP64 values, FP32 vectors, "
-                               "p.q and p.r partials; once per FCG iteration)",
-                     "bytes_per_launch": b_fcg, "us_per_launch": round(ms_fcg * 1e3, 2),
-                     "launches_timed": n_fcg, "us_back_to_back": round(ms_fcg_loop * 1e3, 2)},
+        # the solve's most-launched fine-level kernel: the post-smoothing sweep runs smoothFine times per FCG
+        # iteration (the pre-smoothing as many, k_fcg_A once)
+        "roofline": {"bound": "hbm", "achieved": round(gbs_up, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(gbs_up / HBM_PEAK_GBPS, 4),
+                     "traffic": MG_UP_TRAFFIC_128 if args.n == 128 else None,
+                     "kernel": "k_mg_up (the learned solve's finest-level post-smoothing sweep z = x + W (b - A x): "
+                               "FP32 values, FP32 vectors; smoothFine launches per FCG iteration)",
+                     "bytes_per_launch": b_up, "us_per_launch": round(ms_up * 1e3, 2),
+                     "launches_timed": n_up, "us_back_to_back": round(ms_up_loop * 1e3, 2)},
+        "fcg_a_roofline": {"achieved": round(gbs_fcg, 1), "frac": round(gbs_fcg / HBM_PEAK_GBPS, 4),
+                           "traffic": FCG_A_TRAFFIC_128 if args.n == 128 else None,
+                           "kernel": "k_fcg_A (the learned solve's fine-level q = A p: FP64 values, FP32 vectors, "
+                                     "p.q and p.r partials; once per FCG iteration)",
+                           "bytes_per_launch": b_fcg, "us_per_launch": round(ms_fcg * 1e3, 2),
+                           "launches_timed": n_fcg, "us_back_to_back": round(ms_fcg_loop * 1e3, 2)},
         "spmv_fp64_roofline": {"achieved": round(gbs64, 1), "frac": round(gbs64 / HBM_PEAK_GBPS, 4),
                                "traffic": SPMV_FP64_TRAFFIC_128 if args.n == 128 else None,
                                "kernel": "k_spmv<double> (CSR-stream SpMV, FP64; the plain PCG's product)",

@@ -1266,6 +1266,30 @@ int cfx_matrix_fcg_probe(cfx_matrix* A, double* avgMs, int* launches)
     });
 }
 
+int cfx_matrix_smooth_probe(cfx_matrix* A, double* avgMs, int* launches)
+{
+    if (!A || !avgMs || !launches) return CFX_E_INVALID;
+    return guard(A->mesh->owner, [&] {
+        DevMesh& m = *A->mesh->m;
+        Krylov& k = A->mesh->owner->solver(m.pattern.n, m.pattern.nRowBlocks);
+        *avgMs = k.probeSmoothMs;
+        *launches = k.probeSmoothCount;
+    });
+}
+
+int cfx_matrix_time_smooth(cfx_matrix* A, int reps, double* ms)
+{
+    if (!A || !ms) return CFX_E_INVALID;
+    return guard(A->mesh->owner, [&] {
+        need(A->ncomp == 1, "the multigrid preconditioner solves scalar matrices");
+        DevMesh& m = *A->mesh->m;
+        // the hierarchy of the last multigrid solve, else one built on the pattern now (the time depends on
+        // the pattern alone)
+        Multigrid& mg = A->mg.mg ? *A->mg.mg : A->mg.ready(*m.ctx, m.pattern, A->val.as<double>());
+        *ms = mg.timeFineSmooth(reps);
+    });
+}
+
 int cfx_linearise(cfx_mesh* m, int ncomp, const int32_t* type, const double* value, double* linc, double* link)
 {
     if (!m) return CFX_E_INVALID;

@@ -126,6 +126,9 @@ public:
     bool probeFcgA = false;
     double probeMs = 0;
     int probeCount = 0;
+    // the same probe on the preconditioner's finest-level post-smoothing launches (multigrid)
+    double probeSmoothMs = 0;
+    int probeSmoothCount = 0;
     // the drivers' FP64 vector solves: all components in shared launches (true) or one by one
     bool batchComponents = true;
 

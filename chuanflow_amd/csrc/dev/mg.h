@@ -190,6 +190,10 @@ public:
     double lossGrad(const float* e, const float* a, const float* t, float* ga, float* gt);
     // e <- (I - M A) e, `sweeps` times, then e / max|e| (smooth error samples)
     void smoothSample(float* e, int sweeps);
+    void setProbe(bool on) override;
+    bool probeResult(double& avgMs, int& count) override;
+    // average device time of one finest-level post-smoothing sweep over `reps` back-to-back launches (ms)
+    double timeFineSmooth(int reps);
 
     static constexpr int kPartMax = 2048;
 
@@ -221,6 +225,12 @@ private:
     // side (fusion bit 5); out: l's first post-smoothing sweep
     bool launchTailUp(int l, const float* in, float* out, const float* res, const double* partZQ, const int* done,
                       bool restricted);
+    // launches the post-smoothing z = x + W (b - A x) of level l (with the probe's events on the finest level)
+    void launchUp(int l, const CsrView& A, int Gs, int rbPerWg, const float* x, const float* b, float* z,
+                  const float* q, double* partZQ, const int* done);
+    bool probeOn_ = false;
+    std::vector<hipEvent_t> probeEv_;  // start/stop pairs of the probed launches
+    int probeUsed_ = 0;
     DevBuf tv_[14];          // finest-level FP32 scratch of lossGrad / smoothSample
     std::vector<DevBuf> tk_; // lossGrad: the sweeps' D^-1 (b - A x) (2 (smoothFine - 1) vectors)
     DevBuf tpart_;

@@ -2,6 +2,8 @@
 // values and the FP32 V-cycle kernels.
 #include "mg.h"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -1967,19 +1969,93 @@ void Multigrid::visit(int l, const float* in, float* out, const float* q, double
                                                           L.x.as<float>(), done);
         }
     }
-    if (!first)
-        k_mg_up<<<Gs, B, 0, s>>>(A, rbPerWg, L.val32.as<float>(), L.x.as<float>(), in, L.wPost32.as<float>(), z1,
-                                 q1, part1, done);
+    if (!first) launchUp(l, A, Gs, rbPerWg, L.x.as<float>(), in, z1, q1, part1, done);
     // further sweeps z = z' + W (b - A z'), alternating between the two residual buffers, the last into out
     float* zc = z1;
     for (int k = 1; k < sweeps; ++k)
     {
         const bool last = k + 1 == sweeps;
         float* zn = last ? out : (zc == rb[0] ? rb[1] : rb[0]);
-        k_mg_up<<<Gs, B, 0, s>>>(A, rbPerWg, L.val32.as<float>(), zc, in, L.wPost32.as<float>(), zn,
-                                 last ? q : nullptr, last ? partZQ : nullptr, done);
+        launchUp(l, A, Gs, rbPerWg, zc, in, zn, last ? q : nullptr, last ? partZQ : nullptr, done);
         zc = zn;
     }
+}
+
+void Multigrid::launchUp(int l, const CsrView& A, int Gs, int rbPerWg, const float* x, const float* b, float* z,
+                         const float* q, double* partZQ, const int* done)
+{
+    hipStream_t s = ctx_.stream();
+    const MgLevel& L = lv_[l];
+    if (probeOn_ && l == 0)
+    {
+        // the command processor records the events at the kernel's own start and end on this stream
+        while (static_cast<int>(probeEv_.size()) < 2 * (probeUsed_ + 1))
+        {
+            hipEvent_t e;
+            CFX_HIP(hipEventCreate(&e));
+            probeEv_.push_back(e);
+        }
+        hipExtLaunchKernelGGL(k_mg_up, dim3(Gs), dim3(B), 0, s, probeEv_[2 * probeUsed_], probeEv_[2 * probeUsed_ + 1],
+                              0, A, rbPerWg, L.val32.as<float>(), x, b, L.wPost32.as<float>(), z, q, partZQ, done);
+        ++probeUsed_;
+        return;
+    }
+    k_mg_up<<<Gs, B, 0, s>>>(A, rbPerWg, L.val32.as<float>(), x, b, L.wPost32.as<float>(), z, q, partZQ, done);
+}
+
+void Multigrid::setProbe(bool on)
+{
+    probeOn_ = on;
+    probeUsed_ = 0;
+}
+
+bool Multigrid::probeResult(double& avgMs, int& count)
+{
+    if (probeUsed_ == 0) return false;
+    CFX_HIP(hipStreamSynchronize(ctx_.stream()));
+    // launches after the stop decision return at once: average over those that ran (> 20 % of the longest)
+    std::vector<float> t(probeUsed_);
+    float mx = 0;
+    for (int i = 0; i < probeUsed_; ++i)
+    {
+        CFX_HIP(hipEventElapsedTime(&t[i], probeEv_[2 * i], probeEv_[2 * i + 1]));
+        mx = std::max(mx, t[i]);
+    }
+    double sum = 0;
+    int cnt = 0;
+    for (float v : t)
+        if (v > 0.2f * mx)
+        {
+            sum += v;
+            ++cnt;
+        }
+    avgMs = cnt ? sum / cnt : 0.0;
+    count = cnt;
+    return cnt > 0;
+}
+
+double Multigrid::timeFineSmooth(int reps)
+{
+    hipStream_t s = ctx_.stream();
+    MgLevel& L = lv_[0];
+    const CsrView A = fine_.view();
+    hipEvent_t e0, e1;
+    CFX_HIP(hipEventCreate(&e0));
+    CFX_HIP(hipEventCreate(&e1));
+    // z = x + W (b - A x) on the level's own vectors (their contents do not matter for the time)
+    k_mg_up<<<A.nRowBlocks, B, 0, s>>>(A, 1, L.val32.as<float>(), L.x.as<float>(), L.b.as<float>(),
+                                       L.wPost32.as<float>(), L.e.as<float>(), nullptr, nullptr, nullptr);
+    CFX_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < reps; ++i)
+        k_mg_up<<<A.nRowBlocks, B, 0, s>>>(A, 1, L.val32.as<float>(), L.x.as<float>(), L.b.as<float>(),
+                                           L.wPost32.as<float>(), L.e.as<float>(), nullptr, nullptr, nullptr);
+    CFX_HIP(hipEventRecord(e1, s));
+    CFX_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    CFX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    CFX_HIP(hipEventDestroy(e0));
+    CFX_HIP(hipEventDestroy(e1));
+    return ms / std::max(1, reps);
 }
 
 void Multigrid::apply(const float* r, float* z, const float* q, double* partZQ, int Gs, int rbPerWg, const int* done)

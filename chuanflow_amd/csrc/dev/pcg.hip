@@ -1092,6 +1092,7 @@ SolveReport Krylov::fcgMixed(const CsrView& A, const double* val, Preconditioner
     double* partRZ = partials_.as<double>();  // written by the residual kernel, unused here
     CgState* st = reinterpret_cast<CgState*>(state_.as<char>() + 2 * sizeof(SolveState));
     probeUsed_ = 0;
+    mg.setProbe(probeFcgA);
     CFX_HIP(hipMemsetAsync(st, 0, sizeof(CgState), s));
     CFX_HIP(hipMemsetAsync(d32_.p, 0, sizeof(float) * n_, s));
     float *p = p32_.as<float>(), *q = q32_.as<float>(), *r = r32_.as<float>(), *z = z32_.as<float>(),
@@ -1225,6 +1226,13 @@ SolveReport Krylov::fcgMixed(const CsrView& A, const double* val, Preconditioner
             continue;
         }
         break;
+    }
+    if (probeFcgA)
+    {
+        probeSmoothMs = 0;
+        probeSmoothCount = 0;
+        mg.probeResult(probeSmoothMs, probeSmoothCount);
+        mg.setProbe(false);
     }
     if (probeFcgA && probeUsed_ > 0)
     {

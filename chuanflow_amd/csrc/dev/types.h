@@ -221,6 +221,10 @@ public:
     virtual const float* invD32() const = 0;      // FP32 inverse diagonal
     virtual void apply(const float* r, float* z, const float* q, double* partZQ, int Gs, int rbPerWg,
                        const int* done) = 0;
+    // measurement: while on, the finest-level post-smoothing launches carry HIP start and stop events (no
+    // graph capture then); the result is the average device time of those that ran and their count
+    virtual void setProbe(bool) {}
+    virtual bool probeResult(double& avgMs, int& count) { return false; }
 };
 
 } // namespace cfx

@@ -222,6 +222,8 @@ _sig("cfx_matrix_time_spmv", C.c_int, _vp, C.c_int, _vp, _vp, C.c_int, C.c_int, 
 _sig("cfx_matrix_time_fcg_spmv", C.c_int, _vp, C.c_int, C.POINTER(C.c_double))
 _sig("cfx_set_fcg_probe", C.c_int, _vp, C.c_int)
 _sig("cfx_matrix_fcg_probe", C.c_int, _vp, C.POINTER(C.c_double), C.POINTER(C.c_int))
+_sig("cfx_matrix_smooth_probe", C.c_int, _vp, C.POINTER(C.c_double), C.POINTER(C.c_int))
+_sig("cfx_matrix_time_smooth", C.c_int, _vp, C.c_int, C.POINTER(C.c_double))
 # host mesh
 _sig("cfx_hmesh_box", C.c_int, _i64, _i64, _i64, _vp, _vp, C.c_double, C.c_uint64, C.c_int, C.POINTER(_vp))
 _sig("cfx_hmesh_channel", C.c_int, _i64, _i64, _i64, _vp, _vp, C.c_double, C.c_uint64, C.POINTER(_vp))
@@ -744,6 +746,19 @@ class Matrix:
         ms, cnt = C.c_double(0), C.c_int(0)
         self.ctx.check(_L.cfx_matrix_fcg_probe(self.h, C.byref(ms), C.byref(cnt)))
         return ms.value, cnt.value
+
+    def smooth_probe(self):
+        """(average ms, launches) of the multigrid's finest-level post-smoothing sweep in the last probed
+        flexible-CG solve."""
+        ms, cnt = C.c_double(0), C.c_int(0)
+        self.ctx.check(_L.cfx_matrix_smooth_probe(self.h, C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value
+
+    def time_smooth(self, reps=20):
+        """Average ms of the finest-level post-smoothing sweep over reps back-to-back launches."""
+        ms = C.c_double(0)
+        self.ctx.check(_L.cfx_matrix_time_smooth(self.h, reps, C.byref(ms)))
+        return ms.value
 
     def set_relaxation(self, relaxation):
         """Schemes::relaxation of this fvMatrix: applied after the final pass of solve()."""

@@ -509,6 +509,12 @@ int cfx_matrix_time_fcg_spmv(cfx_matrix* A, int reps, double* ms);
  * queued after the stop decision return at once and are not counted). */
 int cfx_set_fcg_probe(cfx_ctx* ctx, int enable);
 int cfx_matrix_fcg_probe(cfx_matrix* A, double* avgMs, int* launches);
+/* the same probe, on the multigrid's finest-level post-smoothing sweeps of the last flexible-CG solve (set
+ * by cfx_set_fcg_probe); and that sweep timed over reps back-to-back launches (on the hierarchy of the last
+ * multigrid solve, else on one built on the matrix pattern by this call).  Measurement helpers with no
+ * counterpart in the reference. */
+int cfx_matrix_smooth_probe(cfx_matrix* A, double* avgMs, int* launches);
+int cfx_matrix_time_smooth(cfx_matrix* A, int reps, double* ms);
 
 #ifdef __cplusplus
 }
