@@ -48,12 +48,12 @@ HBM_PEAK_GBPS = 8000.0
 # FETCH_SIZE 84,851.1 KiB / 0.50005 (the copy-kernel calibration) + WRITE_SIZE 16,384 KiB; the staging reads
 # int16 column offsets, so the traffic is below the canonical bytes, which count 4-byte columns
 SPMV_FP64_TRAFFIC_128 = 190534903
-# HBM bytes per k_fcg_A launch on the 128^3 system, the same way: FETCH_SIZE 90,122.8 KiB / 0.50005 + WRITE_SIZE
-# 8,320.0 KiB (profiles/spmv_128cube_hbm_traffic.md)
-FCG_A_TRAFFIC_128 = 193072729
-# HBM bytes per finest-level k_mg_up launch on the 128^3 system, the same way: FETCH_SIZE 59,431.1 KiB / 0.50005 +
-# WRITE_SIZE 8,192.0 KiB (profiles/spmv_128cube_hbm_traffic.md)
-MG_UP_TRAFFIC_128 = 130091802
+# HBM bytes per k_fcg_A launch on the 128^3 system (on the multigrid's sliced copy of the values), the same way:
+# FETCH_SIZE 86,891.2 KiB / 0.50005 + WRITE_SIZE 8,322.2 KiB (profiles/spmv_128cube_hbm_traffic.md)
+FCG_A_TRAFFIC_128 = 186457796
+# HBM bytes per finest-level k_mg_up launch on the 128^3 system (64-row slices, int16 columns), the same way:
+# FETCH_SIZE 55,747.9 KiB / 0.50005 + WRITE_SIZE 8,192.0 KiB (profiles/spmv_128cube_hbm_traffic.md)
+MG_UP_TRAFFIC_128 = 122549474
 SEED_MESH, SEED_SHUFFLE, SEED_RHS = 128, 7, 2024
 
 
@@ -215,6 +215,7 @@ def bench_c5(args):
     from chuanflow_amd import detail_batch as DB
     from chuanflow_amd import native as N
     ctx = N.Context(0)
+    ctx.set_graphs(bool(args.graphs))
     t0 = time.perf_counter()
     specs = DB.block_specs(count=args.blocks, seed=5)
     blocks = DB.build_blocks(ctx, specs, simple_iterations=3, log=log, own_context=True)
@@ -381,6 +382,7 @@ def bench_c1(args):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O  # noqa: E402  (checker and CPU baseline only)
     ctx = N.Context(0)
+    ctx.set_graphs(bool(args.graphs))
     hm = N.HostMesh.box(40, 20, 20, (0, 0, 0), (1.0, 0.5, 0.5), 0.0, 0, False)
     arrays, geom = hm.arrays(), hm.geometry()
     dm = N.Mesh(ctx, arrays, geom)
@@ -483,6 +485,7 @@ def bench_c3(args):
     from chuanflow_amd import learned as LN
     from chuanflow_amd import native as N
     ctx = N.Context(0)
+    ctx.set_graphs(bool(args.graphs))
     prec = {"fp64": N.FP64, "fp32ir": N.FP32_IR, "mg": N.FP32_MG}[args.c3_pressure]
 
     def setup(n, learned):
@@ -600,17 +603,21 @@ def bench_spmv(args):
     (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE; the copy's counters give the scale to bytes)."""
     from chuanflow_amd import native as N
     ctx = N.Context(0)
+    ctx.set_graphs(bool(args.graphs))
     hm, arrays, geom, dm, A, b, bw = build_system(N, ctx, args.n)
     n, nnz = dm.nCells, dm.sizes()["nnz"]
     xs = ctx.array(np.random.default_rng(1).standard_normal(n))
     ys = ctx.empty(n)
-    ms64 = A.time_spmv(0, xs, ys, args.spmv_reps, N.FP64)
     xs32 = ctx.array(np.random.default_rng(1).standard_normal(n).astype(np.float32))
     ys32 = ctx.empty(n, np.float32)
+    ctx.sync()  # the uploads stay out of the timed loops
+    A.time_spmv(0, xs, ys, args.spmv_reps, N.FP64)  # the device's first work since the assembly: warm-up
+    ms64 = A.time_spmv(0, xs, ys, args.spmv_reps, N.FP64)
     ms32 = A.time_spmv(0, xs32, ys32, args.spmv_reps, N.FP32_IR)
-    msf = A.time_fcg_spmv(args.spmv_reps)
-    # the finest-level post-smoothing sweep (the call builds the multigrid hierarchy on the pattern; no solve)
+    # the finest-level post-smoothing sweep (the call builds the multigrid hierarchy on the pattern; no solve),
+    # then k_fcg_A on the hierarchy's sliced copy of the values, as the learned solve runs both
     msu = A.time_smooth(args.spmv_reps)
+    msf = A.time_fcg_spmv(args.spmv_reps)
     copy_bytes = 256 << 20
     src, dst = ctx.zeros(copy_bytes // 8), ctx.empty(copy_bytes // 8)
     for _ in range(args.spmv_reps):
@@ -645,7 +652,9 @@ def main():
     ap.add_argument("--mg-coarse-omega", type=float, default=None)
     ap.add_argument("--mg-coarse-scale", type=float, default=None)
     ap.add_argument("--kcycle-depth", type=int, default=2)
-    ap.add_argument("--mg-fusion", type=int, default=61)
+    ap.add_argument("--mg-fusion", type=int, default=125)
+    ap.add_argument("--graphs", type=int, default=0, choices=[0, 1],
+                    help="flexible CG: replay a captured iteration (1) or launch kernel by kernel (0)")
     ap.add_argument("--smooth-fine", type=int, default=3, help="multigrid smoothing sweeps on the finest level")
     ap.add_argument("--smooth-coarse", type=int, default=2, help="multigrid smoothing sweeps on the K-cycle levels")
     ap.add_argument("--inner-tol", type=float, default=0.0,
@@ -703,6 +712,7 @@ def main():
         return bench_spmv(args)
     from chuanflow_amd import native as N
     ctx = N.Context(0)
+    ctx.set_graphs(bool(args.graphs))
     arch, cus = ctx.device_info()
     log(f"device {arch}, {cus} CUs")
     hm, arrays, geom, dm, A, b, bw_shuffled = build_system(N, ctx, args.n)

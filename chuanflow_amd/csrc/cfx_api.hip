@@ -24,7 +24,7 @@ struct cfx_ctx
     std::unique_ptr<Context> ctx;
     std::string err;
     std::map<int, std::unique_ptr<Krylov>> krylov;  // workspace per system size
-    bool graphs = true;
+    bool graphs = false;  // flexible CG launches eagerly (measured faster than graph replay)
     bool batchComponents = true;
     bool probeFcgA = false;
     std::unique_ptr<BatchedPcg> batch;
@@ -1114,6 +1114,15 @@ int cfx_csr_mg_tail_level(cfx_csr* A, int* level)
     });
 }
 
+int cfx_csr_mg_slices(cfx_csr* A, int64_t* slots)
+{
+    if (!A || !slots) return CFX_E_INVALID;
+    return guard(A->owner, [&] {
+        Multigrid& mg = A->mg.ready(*A->owner->ctx, A->pattern, A->val.as<double>());
+        *slots = mg.fineSlots();
+    });
+}
+
 int cfx_csr_mg_level(cfx_csr* A, int l, int* rowPtr, int* col, double* val, int* agg)
 {
     if (!A) return CFX_E_INVALID;
@@ -1242,7 +1251,7 @@ int cfx_matrix_time_fcg_spmv(cfx_matrix* A, int reps, double* ms)
         need(A->ncomp == 1, "the flexible CG solves scalar matrices");
         DevMesh& m = *A->mesh->m;
         *ms = A->mesh->owner->solver(m.pattern.n, m.pattern.nRowBlocks)
-                  .timeFcgSpmv(m.pattern.view(), A->val.as<double>(), reps);
+                  .timeFcgSpmv(m.pattern.view(), A->val.as<double>(), reps, A->mg.mg.get());
     });
 }
 

@@ -107,8 +107,9 @@ public:
 
     // Average device time (ms) of the flexible CG's fine-level kernel (q = A p with FP64 values on FP32
     // vectors, partials p.q and p.r) over `reps` launches on this solver's vectors, HIP events on the
-    // context's stream: the roofline yardstick of the multigrid solves.
-    double timeFcgSpmv(const CsrView& A, const double* val, int reps);
+    // context's stream: the roofline yardstick of the multigrid solves (on the preconditioner's sliced
+    // values of `val` where it holds them, as the solve does).
+    double timeFcgSpmv(const CsrView& A, const double* val, int reps, const Preconditioner* pre = nullptr);
 
     // True residual ||b - A x|| in FP64: returns {inf-norm rel, 2-norm rel}.
     void trueResidual(const CsrView& A, const double* val, const double* b, const double* x,
@@ -120,7 +121,7 @@ public:
     // Batch control: iterations launched between host polls.
     int minBatch = 8, maxBatch = 256;
     // flexible CG: replay one captured iteration (true) or launch its kernels one by one
-    bool useGraphs = true;
+    bool useGraphs = false;
     // measurement: flexible CG launches eagerly and times every fine-level k_fcg_A launch with HIP events;
     // probeMs / probeCount: average device time and count of those that ran (not the ones after the stop)
     bool probeFcgA = false;

@@ -57,7 +57,7 @@ struct MgParams
     int kcycleDepth = 2;              // coarse levels 1..kcycleDepth use the K-cycle (0: V-cycle)
     int smoothFine = 3;               // smoothing sweeps before and after the coarse correction: finest level,
     int smoothCoarse = 2;             // K-cycle levels (the levels below them take one)
-    int fusion = 61;                  // bit 0: the small coarse levels' cycle in one workgroup;
+    int fusion = 125;                 // bit 0: the small coarse levels' cycle in one workgroup;
                                       // bit 1: prolongation (and K-cycle combination) fused into the
                                       // post-smoothing of coarse levels of at most 65536 rows;
                                       // bit 2: that cycle (the tail) run by every workgroup of the
@@ -67,7 +67,10 @@ struct MgParams
                                       // bit 4: the K-cycle combination of a level of at most 256
                                       // workgroups formed in the prolongation to the level above;
                                       // bit 5: the restriction from a coarse level formed by its
-                                      // pre-smoothing (rows numbered aggregate by aggregate)
+                                      // pre-smoothing (rows numbered aggregate by aggregate);
+                                      // bit 6: the finest level's smoothing sweeps, and the flexible
+                                      // CG's product, on copies of its values in 64-row slices,
+                                      // slot-major (where its pattern allows them)
 };
 
 // One level of the single-workgroup coarse tail (device pointers).
@@ -148,6 +151,15 @@ public:
     std::uintptr_t id() const override { return (id_ << 16) | (structVersion_ & 0xffffu); }
     // first level whose cycle (with all coarser levels) runs in one workgroup; -1: none
     int tailLevel() const { return tailFrom_; }
+    // slots of the finest level's sliced layout (0: none)
+    long long fineSlots() const { return sl_.ok ? sl_.slots : 0; }
+    bool fineSlices(const double* val, SliceView& S, const double*& sliceVal64) const override
+    {
+        if (!sliced(0) || val != fineVal64_) return false;
+        S = sl_.view();
+        sliceVal64 = sl_.val64.as<double>();
+        return true;
+    }
     int rows() const override { return lv_[0].n; }
     const float* val32() const override { return lv_[0].val32.as<float>(); }
     const float* invD32() const override { return lv_[0].invD32.as<float>(); }
@@ -228,6 +240,18 @@ private:
     // launches the post-smoothing z = x + W (b - A x) of level l (with the probe's events on the finest level)
     void launchUp(int l, const CsrView& A, int Gs, int rbPerWg, const float* x, const float* b, float* z,
                   const float* q, double* partZQ, const int* done);
+    // the finest level in 64-row slices (fusion bit 6): its FP32 and column-scaled values in slot order, the
+    // CSR slot of every slot (-1: padding) to form them; ok where the pattern's offsets fit int16 and the
+    // padding stays under a quarter of the nonzeros
+    struct FineSlices
+    {
+        bool ok = false;
+        long long slots = 0;
+        DevBuf slicePtr, blockSlice, col, idx, val, valS, val64;
+        SliceView view() const { return {slicePtr.as<int>(), blockSlice.as<int>(), col.as<short>()}; }
+    } sl_;
+    void buildSlices();
+    bool sliced(int l) const { return l == 0 && sl_.ok && (p_.fusion & 64); }
     bool probeOn_ = false;
     std::vector<hipEvent_t> probeEv_;  // start/stop pairs of the probed launches
     int probeUsed_ = 0;

@@ -383,13 +383,15 @@ __global__ void k_inv_diag32(CsrView A, const double* __restrict__ val, float* _
 // ------------------------------------------------------- flexible CG (FP32)
 // FCG A(k): q = A p with the FP64 matrix values (FP32 values would perturb
 // the near-null modes of a (nearly) singular pressure matrix by more than
-// their eigenvalues), q rounded to FP32; partials p.q and p.r
-__global__ __launch_bounds__(kB) void k_fcg_A(CsrView A, int rbPerWg, const double* __restrict__ val,
+// their eigenvalues), q rounded to FP32; partials p.q and p.r.  kSliced: the
+// values in the matrix's 64-row slices (S), the same sums bit for bit
+template <bool kSliced>
+__global__ __launch_bounds__(kB) void k_fcg_A(CsrView A, SliceView S, int rbPerWg, const double* __restrict__ val,
                                               const float* __restrict__ p, const float* __restrict__ r,
                                               float* __restrict__ q, const CgState* st, double* __restrict__ partPQ,
                                               double* __restrict__ partPR)
 {
-    __shared__ double lds[kSpmvLds];
+    __shared__ double lds[kSliced ? 1 : kSpmvLds];
     __shared__ double red[kB / kWave];
     if (st->done) return;
     const int w = xcdRemap(blockIdx.x, gridDim.x);
@@ -399,28 +401,36 @@ __global__ __launch_bounds__(kB) void k_fcg_A(CsrView A, int rbPerWg, const doub
         const int blk = w * rbPerWg + j;
         if (blk >= A.nRowBlocks) break;
         const int r0 = A.rowBlocks[blk], r1 = A.rowBlocks[blk + 1];
-        const int k0 = A.rowPtr[r0], nnz = A.rowPtr[r1] - k0;
-        const int row = r0 + static_cast<int>(threadIdx.x);
+        int row = r0 + static_cast<int>(threadIdx.x);
         double s = 0.0;
-        if (nnz <= kSpmvLds)
+        // the row's own operands, loaded before its sum (in flight with the value loads)
+        const float pOwn = row < r1 ? p[row] : 0.0f, rOwn = row < r1 ? r[row] : 0.0f;
+        if constexpr (kSliced)
+            sliceRowSum<double>(A, S, val, p, blk, row, s);
+        else
         {
-            stageProducts(lds, val, A, p, k0, r0, nnz);
-            __syncthreads();
-            if (row < r1)
-                for (int kk = A.rowPtr[row] - k0, e = A.rowPtr[row + 1] - k0; kk < e; ++kk) s += lds[kk];
-            __syncthreads();
-        }
-        else if (row < r1)
-        {
-            for (int kk = A.rowPtr[row]; kk < A.rowPtr[row + 1]; ++kk) s += val[kk] * static_cast<double>(p[A.col[kk]]);
+            const int k0 = A.rowPtr[r0], nnz = A.rowPtr[r1] - k0;
+            if (nnz <= kSpmvLds)
+            {
+                stageProducts(lds, val, A, p, k0, r0, nnz);
+                __syncthreads();
+                if (row < r1)
+                    for (int kk = A.rowPtr[row] - k0, e = A.rowPtr[row + 1] - k0; kk < e; ++kk) s += lds[kk];
+                __syncthreads();
+            }
+            else if (row < r1)
+            {
+                for (int kk = A.rowPtr[row]; kk < A.rowPtr[row + 1]; ++kk)
+                    s += val[kk] * static_cast<double>(p[A.col[kk]]);
+            }
         }
         if (row < r1)
         {
             const float qs = static_cast<float>(s);
             q[row] = qs;
-            const double pi = p[row];
+            const double pi = pOwn;
             pq += pi * static_cast<double>(qs);
-            pr += pi * static_cast<double>(r[row]);
+            pr += pi * static_cast<double>(rOwn);
         }
     }
     pq = blockSum(pq, red);
@@ -1002,7 +1012,49 @@ SolveReport pcgSolveWith(Context& ctx, const CsrView& A, const double* val, cons
     return pcgCore<double>(ctx, A.n, w, 8, 256, A, val, val, invD, b, x, x, r, tol, maxIter, nullptr, false, 0.0);
 }
 
-double Krylov::timeFcgSpmv(const CsrView& A, const double* val, int reps)
+namespace
+{
+
+// The flexible CG's product q = A p: on the preconditioner's sliced copy of the values where it holds one
+// for `val`, else on the CSR values; with start/stop events (measurement) when given.
+struct FcgProduct
+{
+    CsrView A;
+    SliceView S;
+    const double* val = nullptr;
+    bool sliced = false;
+    void launch(const PcgLaunch& g, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const float* p, const float* r,
+                float* q, const CgState* st, double* partPQ, double* partPR) const
+    {
+        if (e0 && sliced)
+            hipExtLaunchKernelGGL(k_fcg_A<true>, dim3(g.Gs), dim3(kB), 0, s, e0, e1, 0, A, S, g.rbPerWg, val, p, r, q,
+                                  st, partPQ, partPR);
+        else if (e0)
+            hipExtLaunchKernelGGL(k_fcg_A<false>, dim3(g.Gs), dim3(kB), 0, s, e0, e1, 0, A, S, g.rbPerWg, val, p, r,
+                                  q, st, partPQ, partPR);
+        else if (sliced)
+            k_fcg_A<true><<<g.Gs, kB, 0, s>>>(A, S, g.rbPerWg, val, p, r, q, st, partPQ, partPR);
+        else
+            k_fcg_A<false><<<g.Gs, kB, 0, s>>>(A, S, g.rbPerWg, val, p, r, q, st, partPQ, partPR);
+    }
+};
+FcgProduct fcgProduct(const CsrView& A, const double* val, const Preconditioner* pre)
+{
+    FcgProduct f;
+    f.A = A;
+    f.val = val;
+    const double* sv = nullptr;
+    if (pre && pre->fineSlices(val, f.S, sv))
+    {
+        f.val = sv;
+        f.sliced = true;
+    }
+    return f;
+}
+
+} // namespace
+
+double Krylov::timeFcgSpmv(const CsrView& A, const double* val, int reps, const Preconditioner* pre)
 {
     hipStream_t s = ctx_.stream();
     const PcgLaunch g = pcgGrid(A, n_);
@@ -1015,12 +1067,13 @@ double Krylov::timeFcgSpmv(const CsrView& A, const double* val, int reps)
     double* part = fcgPart_.as<double>();
     CgState* st = reinterpret_cast<CgState*>(state_.as<char>() + 2 * sizeof(SolveState));
     k_clear_words<<<1, kB, 0, s>>>(reinterpret_cast<unsigned int*>(st), sizeof(CgState) / 4);  // not done
-    k_fcg_A<<<g.Gs, kB, 0, s>>>(A, g.rbPerWg, val, p32_.as<float>(), r32_.as<float>(), q32_.as<float>(), st, part,
-                                part + kRedMax);  // warm
-    CFX_HIP(hipEventRecord(ev0_, s));
-    for (int i = 0; i < reps; ++i)
-        k_fcg_A<<<g.Gs, kB, 0, s>>>(A, g.rbPerWg, val, p32_.as<float>(), r32_.as<float>(), q32_.as<float>(), st,
-                                    part, part + kRedMax);
+    FcgProduct fa = fcgProduct(A, val, pre);
+    for (int i = 0; i <= reps; ++i)  // the first launch warms
+    {
+        if (i == 1) CFX_HIP(hipEventRecord(ev0_, s));
+        fa.launch(g, s, nullptr, nullptr, p32_.as<float>(), r32_.as<float>(), q32_.as<float>(), st, part,
+                  part + kRedMax);
+    }
     CFX_HIP(hipEventRecord(ev1_, s));
     CFX_HIP(hipEventSynchronize(ev1_));
     CFX_CHECK_LAUNCH();
@@ -1100,6 +1153,7 @@ SolveReport Krylov::fcgMixed(const CsrView& A, const double* val, Preconditioner
     const float* val32 = mg.val32();
     const float* invD32 = mg.invD32();
     const int* done = &st->done;
+    const FcgProduct fa = fcgProduct(A, val, &mg);
 
     k_pcg_residual<float><<<g.Gs, kB, 0, s>>>(A, g.rbPerWg, val, x, b, nullptr, r, invD32, partRZ, partRM, partBM);
     mg.apply(r, z, nullptr, nullptr, g.Gs, g.rbPerWg, done);
@@ -1113,17 +1167,32 @@ SolveReport Krylov::fcgMixed(const CsrView& A, const double* val, Preconditioner
     int k = 0, outer = 1;
     int k0Ref = 0;  // iteration of the last true residual
     double r0 = -1;
+    // the device's reliable-update thresholds as of the last true residual, and the last poll's state
+    double updAbs = 0, updFast = 0;
+    int kRef = 0, lastIter = -1;
+    double lastRnorm = -1;
     for (;;)
     {
+        // iterations to the next stop the device will make: convergence of the recurrence residual or a
+        // reliable update (after a drop to updFast, or to updAbs once kFcgSegment iterations have passed;
+        // a fixed factor: updAbs), at the rate of the last batch (since the start while that ran fewer than
+        // two iterations).  An iteration launched past the stop costs its launches (about 0.1 ms at 2 M rows),
+        // a batch that ends before it one more poll, which is cheaper: the batch ends at the predicted stop.
         int batch = 2;
         if (r0 > 0 && hs->iter > 0 && hs->rnorm > 0 && hs->rnorm < r0)
         {
-            const double rate = std::log(hs->rnorm / r0) / hs->iter;
-            const double target = std::max(tol * hs->bnorm, hs->updAbs);
-            const double need = std::log(target / hs->rnorm) / rate;
-            if (std::isfinite(need)) batch = static_cast<int>(0.9 * need);
+            double rate = std::log(hs->rnorm / r0) / hs->iter;
+            if (lastIter >= 0 && hs->iter - lastIter >= 2 && lastRnorm > hs->rnorm)
+                rate = std::log(hs->rnorm / lastRnorm) / (hs->iter - lastIter);
+            auto itersTo = [&](double level) {
+                return std::log(std::max(level, tol * hs->bnorm) / hs->rnorm) / rate;
+            };
+            const double need = delta > 0 ? itersTo(updAbs)
+                                          : std::min(itersTo(updFast), std::max(itersTo(updAbs),
+                                                                                 double(kRef + kFcgSegment - k)));
+            if (std::isfinite(need)) batch = static_cast<int>(std::ceil(need - 0.25));
         }
-        batch = std::max(2, std::min(64, batch));
+        batch = std::max(1, std::min(64, batch));
         batch = std::min(batch, maxIter - k);
         if (batch <= 0) break;
         if (!useGraphs || probeFcgA)
@@ -1140,13 +1209,12 @@ SolveReport Krylov::fcgMixed(const CsrView& A, const double* val, Preconditioner
                         CFX_HIP(hipEventCreate(&e));
                         probeEv_.push_back(e);
                     }
-                    hipExtLaunchKernelGGL(k_fcg_A, dim3(g.Gs), dim3(kB), 0, s, probeEv_[2 * probeUsed_],
-                                          probeEv_[2 * probeUsed_ + 1], 0, A, g.rbPerWg, val, p, r, q,
-                                          static_cast<const CgState*>(st), partPQ, partPR);
+                    fa.launch(g, s, probeEv_[2 * probeUsed_], probeEv_[2 * probeUsed_ + 1], p, r, q, st, partPQ,
+                              partPR);
                     ++probeUsed_;
                 }
                 else
-                    k_fcg_A<<<g.Gs, kB, 0, s>>>(A, g.rbPerWg, val, p, r, q, st, partPQ, partPR);
+                    fa.launch(g, s, nullptr, nullptr, p, r, q, st, partPQ, partPR);
                 k_fcg_B<<<g.Gv, kB, 0, s>>>(n_, x32, r, p, q, st, partPQ, partPR, g.Gs, partRM);
                 mg.apply(r, z, q, partZQ, g.Gs, g.rbPerWg, done);
                 k_fcg_C<<<g.Gv, kB, 0, s>>>(n_, z, p, st, partZQ, g.Gs, partRM, partBM, g.Gv, -1, tol, maxIter,
@@ -1162,6 +1230,7 @@ SolveReport Krylov::fcgMixed(const CsrView& A, const double* val, Preconditioner
             const std::vector<const void*> key{&mg,
                                                reinterpret_cast<const void*>(mg.id()),
                                                val,
+                                               fa.val,
                                                resHist,
                                                A.rowPtr,
                                                A.col,
@@ -1183,7 +1252,7 @@ SolveReport Krylov::fcgMixed(const CsrView& A, const double* val, Preconditioner
                 fcgExec_ = nullptr;
                 hipGraph_t graph;
                 CFX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                k_fcg_A<<<g.Gs, kB, 0, s>>>(A, g.rbPerWg, val, p, r, q, st, partPQ, partPR);
+                fa.launch(g, s, nullptr, nullptr, p, r, q, st, partPQ, partPR);
                 k_fcg_B<<<g.Gv, kB, 0, s>>>(n_, x32, r, p, q, st, partPQ, partPR, g.Gs, partRM);
                 mg.apply(r, z, q, partZQ, g.Gs, g.rbPerWg, done);
                 k_fcg_C<<<g.Gv, kB, 0, s>>>(n_, z, p, st, partZQ, g.Gs, partRM, partBM, g.Gv, -1, tol, maxIter, delta, 2,
@@ -1201,8 +1270,18 @@ SolveReport Krylov::fcgMixed(const CsrView& A, const double* val, Preconditioner
         k += batch;
         CFX_CHECK_LAUNCH();
         pollState();
-        if (r0 < 0) r0 = hs->initialResidual;
-        if (!hs->done) continue;
+        if (r0 < 0)
+        {
+            r0 = hs->initialResidual;
+            updAbs = hs->updAbs;
+            updFast = hs->updFast;
+        }
+        if (!hs->done)
+        {
+            lastIter = hs->iter;
+            lastRnorm = hs->rnorm;
+            continue;
+        }
         if (hs->paused && !hs->converged)
         {
             // reliable update at iteration hs->iter: the direction p_{k-1} and q = A p_{k-1} are kept; r
@@ -1223,6 +1302,19 @@ SolveReport Krylov::fcgMixed(const CsrView& A, const double* val, Preconditioner
                                         resHist);
             CFX_CHECK_LAUNCH();
             ++outer;
+            // the thresholds the device takes from this true residual (close to the recurrence residual)
+            updAbs = (delta > 0 ? delta : kPcgDelta) * hs->rnorm;
+            updFast = kFcgDelta * hs->rnorm;
+            kRef = k;
+            lastIter = hs->iter;
+            lastRnorm = hs->rnorm;
+            if (hs->rnorm < tol * hs->bnorm)
+            {
+                // the pause came from the recurrence residual's convergence: the true residual most likely
+                // converges too, and then any iteration launched now would only return
+                pollState();
+                if (hs->done) break;
+            }
             continue;
         }
         break;

@@ -92,6 +92,69 @@ struct CsrView
     const short* col16 = nullptr;
 };
 
+// A pattern's CSR-stream row blocks cut into slices of 64 consecutive rows
+// (one wavefront), each slice stored slot-major: slot j of every row of the
+// slice together, padded to the slice's longest row.  A wavefront's loads of
+// one slot are contiguous, and its gathers of x fall on the neighbours of 64
+// consecutive rows.  Columns are int16 offsets from the row block's first row;
+// padding slots carry kSlicePad.
+constexpr short kSlicePad = -32768;
+struct SliceView
+{
+    const int* slicePtr = nullptr;    // [slices + 1] first slot of each slice (multiples of 64)
+    const int* blockSlice = nullptr;  // [nRowBlocks + 1] first slice of each row block
+    const short* col = nullptr;       // [slots]
+};
+// readable slots past the last slice (padding) in every slot array, for the
+// unconditional loads of sliceRowSum
+constexpr int kSliceTail = 8 * 64;
+
+// Row sum of y = M v for this thread's row of row block blk, over the block's
+// slices: the wave of threads 64 w .. 64 w + 63 sums the rows of the block's
+// slice w, slot j of a row being its j-th nonzero, so each sum runs in column
+// order from zero with the products TL(val) * TL(v), as the CSR-stream sum over
+// staged products does (bit for bit); padding slots are skipped by a select.
+// Eight slots in flight per lane, every load issued unconditionally and only
+// the use masked.
+template <typename TL, typename TV, typename TX>
+__device__ __forceinline__ bool sliceRowSum(const CsrView& A, const SliceView& S, const TV* __restrict__ valM,
+                                            const TX* __restrict__ v, int blk, int& row, TL& sum)
+{
+    const int r0 = A.rowBlocks[blk], r1 = A.rowBlocks[blk + 1];
+    row = r0 + static_cast<int>(threadIdx.x);
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+    TL s = TL(0);
+    if (r0 + wave * kWave < r1)
+    {
+        const int sl = S.blockSlice[blk] + wave;
+        const int p0 = S.slicePtr[sl], w = (S.slicePtr[sl + 1] - p0) / kWave;  // the slice's width
+        for (int j = 0; j < w; j += 8)
+        {
+            const int k = p0 + j * kWave + static_cast<int>(threadIdx.x & (kWave - 1));
+            TV a[8];
+            TX xv[8];
+            int c[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+            {
+                a[u] = valM[k + u * kWave];
+                const int cu = S.col[k + u * kWave];
+                c[u] = j + u < w ? cu : static_cast<int>(kSlicePad);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) xv[u] = v[r0 + (c[u] == kSlicePad ? 0 : c[u])];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+            {
+                const TL t = s + static_cast<TL>(a[u]) * static_cast<TL>(xv[u]);
+                s = c[u] != kSlicePad ? t : s;
+            }
+        }
+    }
+    sum = s;
+    return row < r1;
+}
+
 // The same for a row block starting at row r0 of a pattern, through its int16
 // column offsets where it has them (the same columns, so the same products).
 template <typename TL, typename TV, typename TX>
@@ -225,6 +288,9 @@ public:
     // graph capture then); the result is the average device time of those that ran and their count
     virtual void setProbe(bool) {}
     virtual bool probeResult(double& avgMs, int& count) { return false; }
+    // the matrix in 64-row slices with its FP64 values in slot order, where the preconditioner holds them
+    // for exactly the values `val` (the pointer its setup read); false: none
+    virtual bool fineSlices(const double* val, SliceView& S, const double*& sliceVal64) const { return false; }
 };
 
 } // namespace cfx

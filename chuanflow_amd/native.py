@@ -208,6 +208,7 @@ _sig("cfx_matrix_set_mg", C.c_int, _vp, C.POINTER(MgParams))
 _sig("cfx_matrix_set_mg_weights", C.c_int, _vp, _vp, _vp, _vp, _vp)
 _sig("cfx_csr_mg_info", C.c_int, _vp, C.POINTER(C.c_int), _vp, _vp, C.c_int)
 _sig("cfx_csr_mg_tail_level", C.c_int, _vp, C.POINTER(C.c_int))
+_sig("cfx_csr_mg_slices", C.c_int, _vp, C.POINTER(C.c_int64))
 _sig("cfx_csr_mg_level", C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp)
 _sig("cfx_csr_mg_apply", C.c_int, _vp, _vp, _vp)
 _sig("cfx_csr_dic_info", C.c_int, _vp, C.POINTER(C.c_int), _vp)
@@ -869,6 +870,12 @@ class Csr:
         lv = C.c_int(0)
         self.ctx.check(_L.cfx_csr_mg_tail_level(self.h, C.byref(lv)))
         return lv.value
+
+    def mg_slices(self):
+        """Slots of the finest level's sliced layout (0: the pattern does not allow it)."""
+        n = C.c_int64(0)
+        self.ctx.check(_L.cfx_csr_mg_slices(self.h, C.byref(n)))
+        return n.value
 
     def mg_level(self, l):
         """(rowPtr, col, FP64 values, agg or None) of hierarchy level l."""

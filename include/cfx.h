@@ -261,16 +261,19 @@ typedef struct cfx_mg_params {
                                            formed inside the pre-smoothing; bit 4: the K-cycle combination of
                                            a level of at most 256 workgroups formed inside the prolongation;
                                            bit 5: the restriction from a coarse level (whose rows are
-                                           numbered aggregate by aggregate) formed by its pre-smoothing
-                                           (default 61 = bits 0, 2, 3, 4, 5; bit 1 measured slower);
+                                           numbered aggregate by aggregate) formed by its pre-smoothing;
+                                           bit 6: the finest level's smoothing sweeps and the flexible CG's
+                                           product on copies of its values in 64-row slices, stored
+                                           slot-major (where cfx_csr_mg_slices is not 0)
+                                           (default 125 = bits 0, 2, 3, 4, 5, 6; bit 1 measured slower);
                                            results are bit for bit those of fusion 0 */
     int smoothFine, smoothCoarse;       /* smoothing sweeps before and after the coarse correction (the same
                                            number, 1..8) on the finest level and on the K-cycle levels (default
                                            3 and 2; the levels below the K-cycle levels take one) */
 } cfx_mg_params;
 int cfx_mg_default_params(cfx_mg_params* p);
-/* flexible CG replays one captured iteration per step (hipGraph, default 1)
- * or launches its kernels one by one (0); both give identical results */
+/* flexible CG replays one captured iteration per step (hipGraph, 1) or launches
+ * its kernels one by one (0, the default: measured faster); both give identical results */
 int cfx_set_graphs(cfx_ctx* ctx, int enable);
 /* the drivers' and fvMatrix solves of vector equations with FP64 BiCGStab
  * (fvMatrix.cpp:93-122 solves component after component): all components in
@@ -301,6 +304,9 @@ int cfx_csr_mg_info(cfx_csr* A, int* levels, int64_t* rows, int64_t* nnz, int ma
 /* first level whose cycle, with every coarser level, fits one workgroup (cfx_mg_params.fusion bit 0
  * runs it as one launch below the K-cycle levels); -1: none */
 int cfx_csr_mg_tail_level(cfx_csr* A, int* level);
+/* slots of the finest level's 64-row sliced layout (cfx_mg_params.fusion bit 6; nonzeros plus padding), 0
+ * where its pattern does not allow it (an int16 column offset overflows, or padding above a quarter) */
+int cfx_csr_mg_slices(cfx_csr* A, int64_t* slots);
 
 /* precond = CFX_PRECOND_DIC (solver CG, precision CFX_FP32_IR): flexible CG
  * preconditioned by diagonal incomplete Cholesky, D_ii = a_ii - sum_{j<i}
