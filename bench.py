@@ -655,7 +655,7 @@ def main():
     ap.add_argument("--mg-fusion", type=int, default=125)
     ap.add_argument("--graphs", type=int, default=0, choices=[0, 1],
                     help="flexible CG: replay a captured iteration (1) or launch kernel by kernel (0)")
-    ap.add_argument("--smooth-fine", type=int, default=3, help="multigrid smoothing sweeps on the finest level")
+    ap.add_argument("--smooth-fine", type=int, default=4, help="multigrid smoothing sweeps on the finest level")
     ap.add_argument("--smooth-coarse", type=int, default=2, help="multigrid smoothing sweeps on the K-cycle levels")
     ap.add_argument("--inner-tol", type=float, default=0.0,
                     help="reliable-update factor of the FP32-refined solves (0: each solver's default, 1e-2 for "
@@ -860,8 +860,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(gbs_up, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(gbs_up / HBM_PEAK_GBPS, 4),
                      "traffic": MG_UP_TRAFFIC_128 if args.n == 128 else None,
-                     "kernel": "k_mg_up (the learned solve's finest-level post-smoothing sweep z = x + W (b - A x): "
-                               "FP32 values, FP32 vectors; smoothFine launches per FCG iteration)",
+                     "kernel": "k_mg_up (the learned solve's finest-level post-smoothing sweep z = x + W (b - A x) on "
+                               "the values in 64-row slices: FP32 values, FP32 vectors; smoothFine launches per FCG "
+                               "iteration)",
                      "bytes_per_launch": b_up, "us_per_launch": round(ms_up * 1e3, 2),
                      "launches_timed": n_up, "us_back_to_back": round(ms_up_loop * 1e3, 2)},
         "fcg_a_roofline": {"achieved": round(gbs_fcg, 1), "frac": round(gbs_fcg / HBM_PEAK_GBPS, 4),

@@ -43,19 +43,20 @@ namespace cfx
 
 struct MgParams
 {
-    // defaults tuned on a 128^3 jittered pressure system with the 3/2 sweeps (a grid over the weights, the
-    // transfer scales and the sweeps: 9 flexible-CG iterations against 13 with a = 2/3, s = 1.8, t = 1)
+    // defaults tuned on a 128^3 jittered pressure system with the 4/2 sweeps (a grid over the weights, the
+    // transfer scales and the sweeps: 8 flexible-CG iterations against 13 with a = 2/3, s = 1.8, t = 1 and
+    // 3/2 sweeps)
     float omegaPre = 0.8f;            // finest-level constants (used where no per-row values are given)
-    float scale = 1.6f;
+    float scale = 1.8f;
     float omegaPost = 0.8f;
-    float restrictScale = 1.6f;
+    float restrictScale = 1.8f;
     float coarseOmega = 0.9f;         // coarse levels: pre = post smoother weight
-    float coarseScale = 2.2f;
+    float coarseScale = 3.0f;
     int coarseSweeps = 30;            // damped Jacobi sweeps on a coarsest level above 128 rows
     float coarsestOmega = 2.0f / 3.0f;
     int maxLevels = 10, coarseMax = 128;
     int kcycleDepth = 2;              // coarse levels 1..kcycleDepth use the K-cycle (0: V-cycle)
-    int smoothFine = 3;               // smoothing sweeps before and after the coarse correction: finest level,
+    int smoothFine = 4;               // smoothing sweeps before and after the coarse correction: finest level,
     int smoothCoarse = 2;             // K-cycle levels (the levels below them take one)
     int fusion = 125;                 // bit 0: the small coarse levels' cycle in one workgroup;
                                       // bit 1: prolongation (and K-cycle combination) fused into the

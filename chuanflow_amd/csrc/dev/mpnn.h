@@ -38,7 +38,7 @@ struct MpnnShape
     static constexpr int nParams = offBout + NOUT;
 };
 
-constexpr float kMpnnAMax = 1.6f;  // a zero output gives a = 0.8, t = 1.6: the multigrid's default constants
+constexpr float kMpnnAMax = 1.6f;  // a zero output gives a = 0.8, t = 1.6 (the centre the network was trained at)
 constexpr float kMpnnTMax = 3.2f;
 
 class Mpnn

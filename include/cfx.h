@@ -269,7 +269,7 @@ typedef struct cfx_mg_params {
                                            results are bit for bit those of fusion 0 */
     int smoothFine, smoothCoarse;       /* smoothing sweeps before and after the coarse correction (the same
                                            number, 1..8) on the finest level and on the K-cycle levels (default
-                                           3 and 2; the levels below the K-cycle levels take one) */
+                                           4 and 2; the levels below the K-cycle levels take one) */
 } cfx_mg_params;
 int cfx_mg_default_params(cfx_mg_params* p);
 /* flexible CG replays one captured iteration per step (hipGraph, 1) or launches

@@ -78,17 +78,17 @@ class Params:
     """Finest-level a (pre), s (scale), c (post) constants, coarse-level
     constants (pre = post weight), coarsest sweeps, hierarchy caps."""
     omegaPre: float = 0.800000011920929    # the defaults of the device cycle (FP32 constants)
-    scale: float = 1.600000023841858
+    scale: float = 1.7999999523162842
     omegaPost: float = 0.800000011920929
-    restrictScale: float = 1.600000023841858
+    restrictScale: float = 1.7999999523162842
     coarseOmega: float = 0.8999999761581421
-    coarseScale: float = 2.200000047683716
+    coarseScale: float = 3.0
     coarseSweeps: int = 30
     coarsestOmega: float = 2.0 / 3.0
     maxLevels: int = 10
     coarseMax: int = 128
     kcycleDepth: int = 2
-    smoothFine: int = 3     # smoothing sweeps before and after the coarse correction on the finest level,
+    smoothFine: int = 4     # smoothing sweeps before and after the coarse correction on the finest level,
     smoothCoarse: int = 2   # on the K-cycle levels (1 below them)
 
 

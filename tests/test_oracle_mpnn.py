@@ -38,7 +38,9 @@ def test_interior_laplacian_row_features_closed_form():
     assert np.all(ew[A.indptr[i]:A.indptr[i + 1]][A.indices[A.indptr[i]:A.indptr[i + 1]] != i] == 1 / 6)
 
 
-def test_untrained_network_gives_the_tuned_constants():
+def test_untrained_network_gives_the_head_centre():
+    """A zero output layer gives a = 0.8 (the multigrid's tuned smoother
+    weight) and t = 1.6, the transfer scale the network was trained around."""
     A, geo, r0 = small_graph(1)
     p = NN.random_params(3)
     d = NN.unpack(p)
