@@ -199,19 +199,33 @@ __global__ void k_scale_vals(CsrView A, const double* __restrict__ val64, float*
     }
 }
 
-// the finest values in slot order: out[t] = val32[k] (times wD[col[k]] when
-// wD is given, the column-scaled values) for the CSR slot k of slot t, 0 in
-// padding -- the same FP32 numbers as k_scale_vals' arrays
+// the finest FP32 values in slot order: out[t] = val32[k] for the CSR slot k of slot t, 0 in padding
 __global__ void k_slice_vals(long long slots, const int* __restrict__ idx, const float* __restrict__ val32,
-                             const int* __restrict__ col, const float* __restrict__ wD, float* __restrict__ out)
+                             float* __restrict__ out)
 {
     for (long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; t < slots;
          t += static_cast<long long>(gridDim.x) * blockDim.x)
     {
         const int k = idx[t];
-        float v = 0.0f;
-        if (k >= 0) v = wD ? val32[k] * wD[col[k]] : val32[k];
-        out[t] = v;
+        out[t] = k >= 0 ? val32[k] : 0.0f;
+    }
+}
+
+// the column-scaled values in slot order from the slot-order FP32 values: out[t] = val[t] * wD[col of t], 0
+// in padding (the same FP32 products as k_scale_vals'); one workgroup per row block, one wave per slice
+__global__ __launch_bounds__(B) void k_slice_scale(CsrView A, SliceView S, const float* __restrict__ val,
+                                                   const float* __restrict__ wD, float* __restrict__ out)
+{
+    const int blk = blockIdx.x;
+    const int r0 = A.rowBlocks[blk], r1 = A.rowBlocks[blk + 1];
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+    if (r0 + wave * kWave >= r1) return;
+    const int sl = S.blockSlice[blk] + wave;
+    const int p1 = S.slicePtr[sl + 1];
+    for (int t = S.slicePtr[sl] + static_cast<int>(threadIdx.x & (kWave - 1)); t < p1; t += kWave)
+    {
+        const int c = S.col[t];
+        out[t] = c == kSlicePad ? 0.0f : val[t] * wD[r0 + c];
     }
 }
 
@@ -1791,8 +1805,7 @@ void Multigrid::setValues(const double* fineVal64, const MgParams& p)
             if (sliced(0))
             {
                 const int g = static_cast<int>(std::min<long long>(8192, divUp(sl_.slots, B)));
-                k_slice_vals<<<g, B, 0, s>>>(sl_.slots, sl_.idx.as<int>(), L.val32.as<float>(), A.col, nullptr,
-                                             sl_.val.as<float>());
+                k_slice_vals<<<g, B, 0, s>>>(sl_.slots, sl_.idx.as<int>(), L.val32.as<float>(), sl_.val.as<float>());
                 // the FP64 values too, for the flexible CG's product on this level
                 k_slice_vals64<<<g, B, 0, s>>>(sl_.slots, sl_.idx.as<int>(), v64, sl_.val64.as<double>());
             }
@@ -1838,8 +1851,8 @@ void Multigrid::setWeights(const float* pre, const float* scale, const float* po
                                               p.omegaPre, scale, p.scale, post, p.omegaPost, restr, p.restrictScale);
     if (sliced(0))
         // the column-scaled values in slot order only (the finest pre-smoothing is their one reader)
-        k_slice_vals<<<static_cast<int>(std::min<long long>(8192, divUp(sl_.slots, B))), B, 0, s>>>(
-            sl_.slots, sl_.idx.as<int>(), L.val32.as<float>(), A.col, L.wPre32.as<float>(), sl_.valS.as<float>());
+        k_slice_scale<<<A.nRowBlocks, B, 0, s>>>(A, sl_.view(), sl_.val.as<float>(), L.wPre32.as<float>(),
+                                                 sl_.valS.as<float>());
     else
         k_scale_vals<<<std::max(1, std::min(8192, divUp(L.nnz, B))), B, 0, s>>>(A, fineVal64_, nullptr,
                                                                                L.wPre32.as<float>(),
