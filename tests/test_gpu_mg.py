@@ -243,3 +243,27 @@ def test_fused_kernels_equal_separate_launches(ctx, systems, case):
     for k in range(1, len(fusions)):
         assert np.array_equal(zs[0], zs[k]), fusions[k]
         assert its[0] == its[k] and np.array_equal(xs[0], xs[k]), fusions[k]
+
+
+def test_patterns_without_slices_keep_the_csr_stream_kernels(ctx):
+    """A finest pattern whose column offsets leave int16 (couplings 40,000 rows
+    apart) has no sliced layout: fusion bit 6 then changes nothing, and the
+    solve equals fusion 61's bit for bit."""
+    A = laplacian3d(40, 40, 40, seed=21).tolil()
+    n = A.shape[0]
+    for i in range(0, n - 40000, 997):
+        A[i, i + 40000] = A[i + 40000, i] = -0.05
+        A[i, i] += 0.05
+        A[i + 40000, i + 40000] += 0.05
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    b = np.random.default_rng(22).standard_normal(n)
+    N, G = make(ctx, A)
+    assert G.mg_slices() == 0
+    xs = []
+    for fusion in (61, 125):
+        G.set_mg(fusion=fusion)
+        x, rep = G.solve(b, None, 1e-8, 500, N.SOLVER_CG, N.PRECOND_MG, N.FP32_IR)
+        assert rep.converged and np.abs(b - A @ x).max() / np.abs(b).max() < 1e-8
+        xs.append(x)
+    assert np.array_equal(xs[0], xs[1])
