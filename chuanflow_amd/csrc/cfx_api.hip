@@ -1018,7 +1018,7 @@ int cfx_mg_default_params(cfx_mg_params* p)
     const MgParams d;
     *p = cfx_mg_params{d.omegaPre, d.scale, d.omegaPost, d.restrictScale, d.coarseOmega, d.coarseScale,
                        d.coarseSweeps, d.coarsestOmega, d.maxLevels, d.coarseMax, d.kcycleDepth, d.fusion,
-                       d.smoothFine, d.smoothCoarse};
+                       d.smoothFine, d.smoothCoarse, d.smoothDeep};
     return CFX_OK;
 }
 
@@ -1044,10 +1044,12 @@ void setMgParams(MgCache& mg, const cfx_mg_params* p)
     need(p->kcycleDepth >= 0, "kcycleDepth < 0");
     m.kcycleDepth = p->kcycleDepth;
     m.fusion = p->fusion;
-    need(p->smoothFine >= 1 && p->smoothFine <= 8 && p->smoothCoarse >= 1 && p->smoothCoarse <= 8,
-         "smoothFine and smoothCoarse must be in [1,8]");
+    need(p->smoothFine >= 1 && p->smoothFine <= 8 && p->smoothCoarse >= 1 && p->smoothCoarse <= 8 &&
+             p->smoothDeep >= 1 && p->smoothDeep <= 8,
+         "smoothFine, smoothCoarse and smoothDeep must be in [1,8]");
     m.smoothFine = p->smoothFine;
     m.smoothCoarse = p->smoothCoarse;
+    m.smoothDeep = p->smoothDeep;
     mg.valid = false;
 }
 

@@ -57,7 +57,8 @@ struct MgParams
     int maxLevels = 10, coarseMax = 128;
     int kcycleDepth = 2;              // coarse levels 1..kcycleDepth use the K-cycle (0: V-cycle)
     int smoothFine = 4;               // smoothing sweeps before and after the coarse correction: finest level,
-    int smoothCoarse = 2;             // K-cycle levels (the levels below them take one)
+    int smoothCoarse = 2;             // level 1 of the K-cycle levels,
+    int smoothDeep = 2;               // the K-cycle levels below level 1 (the levels below them take one)
     int fusion = 125;                 // bit 0: the small coarse levels' cycle in one workgroup;
                                       // bit 1: prolongation (and K-cycle combination) fused into the
                                       // post-smoothing of coarse levels of at most 65536 rows;

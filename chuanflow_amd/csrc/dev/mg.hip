@@ -1788,7 +1788,8 @@ void Multigrid::setValues(const double* fineVal64, const MgParams& p)
 {
     hipStream_t s = ctx_.stream();
     if (p.kcycleDepth != p_.kcycleDepth || p.fusion != p_.fusion || p.coarseSweeps != p_.coarseSweeps ||
-        p.coarsestOmega != p_.coarsestOmega || p.smoothFine != p_.smoothFine || p.smoothCoarse != p_.smoothCoarse)
+        p.coarsestOmega != p_.coarsestOmega || p.smoothFine != p_.smoothFine || p.smoothCoarse != p_.smoothCoarse ||
+        p.smoothDeep != p_.smoothDeep)
         ++structVersion_;
     p_ = p;
     fineVal64_ = fineVal64;
@@ -2047,8 +2048,9 @@ void Multigrid::visit(int l, const float* in, float* out, const float* q, double
     MgLevel& L = lv_[l];
     const CsrView A = l == 0 ? fine_.view() : L.pat.view();
     // smoothing sweeps before and after the coarse correction (the same number): smoothFine on the finest
-    // level, smoothCoarse on the K-cycle levels, one below them
-    const int sweeps = std::max(1, l == 0 ? p_.smoothFine : (l <= p_.kcycleDepth ? p_.smoothCoarse : 1));
+    // level, smoothCoarse on level 1 and smoothDeep on the further K-cycle levels, one below them
+    const int sweeps =
+        std::max(1, l == 0 ? p_.smoothFine : (l > p_.kcycleDepth ? 1 : (l == 1 ? p_.smoothCoarse : p_.smoothDeep)));
     // fusion bit 5: a coarse level numbered aggregate by aggregate restricts inside its last pre-smoothing
     // sweep (its restriction weights are 1)
     const bool fuseR = (p_.fusion & 32) && l >= 1 && L.groupedRows && l + 1 < levels();

@@ -80,7 +80,8 @@ class MgParams(C.Structure):
     _fields_ = [("omegaPre", C.c_float), ("scale", C.c_float), ("omegaPost", C.c_float),
                 ("restrictScale", C.c_float), ("coarseOmega", C.c_float), ("coarseScale", C.c_float), ("coarseSweeps", C.c_int),
                 ("coarsestOmega", C.c_float), ("maxLevels", C.c_int), ("coarseMax", C.c_int),
-                ("kcycleDepth", C.c_int), ("fusion", C.c_int), ("smoothFine", C.c_int), ("smoothCoarse", C.c_int)]
+                ("kcycleDepth", C.c_int), ("fusion", C.c_int), ("smoothFine", C.c_int), ("smoothCoarse", C.c_int),
+                ("smoothDeep", C.c_int)]
 
 
 class TrainOptions(C.Structure):

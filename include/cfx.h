@@ -268,8 +268,10 @@ typedef struct cfx_mg_params {
                                            (default 125 = bits 0, 2, 3, 4, 5, 6; bit 1 measured slower);
                                            results are bit for bit those of fusion 0 */
     int smoothFine, smoothCoarse;       /* smoothing sweeps before and after the coarse correction (the same
-                                           number, 1..8) on the finest level and on the K-cycle levels (default
-                                           4 and 2; the levels below the K-cycle levels take one) */
+                                           number, 1..8) on the finest level and on level 1 of the K-cycle
+                                           levels (default 4 and 2; the levels below the K-cycle levels take
+                                           one) */
+    int smoothDeep;                     /* the same on the K-cycle levels below level 1 (1..8) */
 } cfx_mg_params;
 int cfx_mg_default_params(cfx_mg_params* p);
 /* flexible CG replays one captured iteration per step (hipGraph, 1) or launches

@@ -89,7 +89,8 @@ class Params:
     coarseMax: int = 128
     kcycleDepth: int = 2
     smoothFine: int = 4     # smoothing sweeps before and after the coarse correction on the finest level,
-    smoothCoarse: int = 2   # on the K-cycle levels (1 below them)
+    smoothCoarse: int = 2   # on level 1 of the K-cycle levels,
+    smoothDeep: int = 2     # on the K-cycle levels below level 1 (1 below them)
 
 
 DENSE_MAX = 128
@@ -193,8 +194,8 @@ def vcycle(H, r, p: Params, pre=None, scale=None, post=None, exact_coarse=False,
     """z = M r: x = Wpre b, then (sweeps - 1) times x += Wpre (b - A x);
     r = b - A x; bc = P^T T r; e = coarse correction (solve_level);
     x += S P e; then sweeps times x += Wpost (b - A x); z = x.  sweeps is
-    smoothFine on the finest level, smoothCoarse on the K-cycle levels, 1
-    below them.  pre/scale/post/
+    smoothFine on the finest level, smoothCoarse on level 1 and smoothDeep on
+    the further K-cycle levels, 1 below them.  pre/scale/post/
     restrict are per-row finest-level a, s, c, t (None -> constant)."""
     A = H.A[level]
     b = np.asarray(r, np.float64)
@@ -209,7 +210,8 @@ def vcycle(H, r, p: Params, pre=None, scale=None, post=None, exact_coarse=False,
         a = c = p.coarseOmega
         s = p.coarseScale
         t = 1.0
-    sweeps = p.smoothFine if level == 0 else (p.smoothCoarse if level <= p.kcycleDepth else 1)
+    sweeps = p.smoothFine if level == 0 else (
+        1 if level > p.kcycleDepth else (p.smoothCoarse if level == 1 else p.smoothDeep))
     x = a * dinv * b
     for _ in range(sweeps - 1):
         x = x + a * dinv * (b - A @ x)

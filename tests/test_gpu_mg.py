@@ -62,13 +62,15 @@ def test_hierarchy_matches_oracle(ctx, systems, name):
             assert np.array_equal(agg, H.agg[l])
 
 
-@pytest.mark.parametrize("name,maxLevels,fine,coarse", [("lap40", 10, 2, 1), ("fv32", 10, 2, 1), ("fv32", 3, 2, 1),
-                                                        ("fv24n", 10, 2, 1), ("lap40", 10, 1, 1),
-                                                        ("lap40", 10, 3, 2)])
-def test_vcycle_fp32_matches_oracle(ctx, systems, name, maxLevels, fine, coarse):
+@pytest.mark.parametrize("name,maxLevels,fine,coarse,deep", [("lap40", 10, 2, 1, 1), ("fv32", 10, 2, 1, 1),
+                                                             ("fv32", 3, 2, 1, 1), ("fv24n", 10, 2, 1, 1),
+                                                             ("lap40", 10, 1, 1, 1), ("lap40", 10, 3, 2, 2),
+                                                             ("lap40", 10, 4, 2, 1), ("lap40", 10, 3, 1, 3)])
+def test_vcycle_fp32_matches_oracle(ctx, systems, name, maxLevels, fine, coarse, deep):
     """Dense exact coarsest solve (<= 128 rows; pure Neumann in fv24n) and,
-    with 3 levels, Jacobi sweeps on a 253-row coarsest level; one to three
-    smoothing sweeps on the finest level, one or two on the K-cycle levels."""
+    with 3 levels, Jacobi sweeps on a 253-row coarsest level; one to four
+    smoothing sweeps on the finest level, one to three on level 1 and on the
+    K-cycle levels below it."""
     A, _ = systems[name]
     N, G = make(ctx, A)
     n = A.shape[0]
@@ -78,20 +80,22 @@ def test_vcycle_fp32_matches_oracle(ctx, systems, name, maxLevels, fine, coarse)
                                   rng.uniform(0.4, 1.0, n).astype(np.float32),
                                   rng.uniform(0.5, 1.5, n).astype(np.float32))
     dpre, dscale, dpost, drestrict = ctx.array(pre), ctx.array(scale), ctx.array(post), ctx.array(restrict)
-    G.set_mg(coarseScale=1.7, coarseSweeps=20, maxLevels=maxLevels, smoothFine=fine, smoothCoarse=coarse)
+    G.set_mg(coarseScale=1.7, coarseSweeps=20, maxLevels=maxLevels, smoothFine=fine, smoothCoarse=coarse,
+             smoothDeep=deep)
     G.set_mg_weights(dpre, dscale, dpost, drestrict)
     r = rng.standard_normal(n)
     r = (r - r.mean()).astype(np.float32)
     z = ctx.empty(n, np.float32)
     G.mg_apply(ctx.array(r), z)
-    p = MG.Params(coarseScale=1.7, coarseSweeps=20, smoothFine=fine, smoothCoarse=coarse)
+    p = MG.Params(coarseScale=1.7, coarseSweeps=20, smoothFine=fine, smoothCoarse=coarse, smoothDeep=deep)
     H = MG.hierarchy(A, maxLevels=maxLevels)
     assert len(H.A) == len(G.mg_info()[0])
     ref = MG.vcycle(H, r.astype(np.float64), p, pre.astype(np.float64), scale.astype(np.float64),
                     post.astype(np.float64), restrict=restrict.astype(np.float64))
     err = np.abs(z.get() - ref).max() / np.abs(ref).max()
     # an FP32 cycle against FP64: on the pinned (nearly singular) fv32 system its rounding, amplified by the
-    # exact coarsest solve, spans 4e-5 to 1.2e-4 of max|z| over weight seeds, for every launch fusion
+    # exact coarsest solve, spans 4e-5 to 1.2e-4 of max|z| over weight seeds, for every launch fusion, with
+    # two sweeps on the finest level (three on it and on the deep K-cycle levels reach 2.1e-4)
     assert err < 2e-4, err
 
 
