@@ -496,10 +496,11 @@ def bench_c3(args):
         Ubc, pbc = cavity_bcs(arrays["patchStart"], arrays["patchSize"], len(arrays["neighbour"]), dm.nB)
         dU, dp, dphi = ctx.zeros((3, dm.nCells)), ctx.zeros(dm.nCells), ctx.zeros(dm.nFaces)
         # the cavity's constants: measured at 61 ms per step with these (the 128^3 defaults are tuned for cold
-        # solves to 1e-8 with three sweeps)
+        # solves to 1e-8 with four sweeps); the coarse sweeps on every K-cycle level
         dm.set_mg(omegaPre=2.0 / 3.0, omegaPost=2.0 / 3.0, scale=1.8, restrictScale=1.0, coarseOmega=2.0 / 3.0,
                   coarseScale=1.8, kcycleDepth=args.kcycle_depth, fusion=args.mg_fusion,
-                  smoothFine=args.c3_smooth_fine, smoothCoarse=args.c3_smooth_coarse)
+                  smoothFine=args.c3_smooth_fine, smoothCoarse=args.c3_smooth_coarse,
+                  smoothDeep=args.c3_smooth_coarse)
         if learned:
             dm.set_mpnn(LN.load(ctx))
         return hm, arrays, geom, dm, Ubc, pbc, dU, dp, dphi

@@ -278,7 +278,7 @@ def to_device(ctx, systems, mg=None):
 
 # the multigrid's default constants (tuned on a 128^3 jittered pressure system), written out
 TUNED_MG = dict(omegaPre=0.8, omegaPost=0.8, scale=1.8, restrictScale=1.8, coarseOmega=0.9, coarseScale=3.0,
-                smoothFine=4, smoothCoarse=2)
+                smoothFine=4, smoothCoarse=2, smoothDeep=2)
 
 
 def train(ctx, systems, steps=300, lr=1e-3, seed=1, samples=2, smooth=1, fp16=True, mg=TUNED_MG):
