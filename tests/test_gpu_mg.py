@@ -249,16 +249,20 @@ def test_fused_kernels_equal_separate_launches(ctx, systems, case):
         assert its[0] == its[k] and np.array_equal(xs[0], xs[k]), fusions[k]
 
 
-def test_patterns_without_slices_keep_the_csr_stream_kernels(ctx):
+@pytest.mark.parametrize("case", ["far", "padded"])
+def test_patterns_without_slices_keep_the_csr_stream_kernels(ctx, case):
     """A finest pattern whose column offsets leave int16 (couplings 40,000 rows
-    apart) has no sliced layout: fusion bit 6 then changes nothing, and the
-    solve equals fusion 61's bit for bit."""
+    apart), or whose 64-row slices would pad beyond a quarter of its nonzeros
+    (every 64th row coupled to 40 rows), has no sliced layout: fusion bit 6
+    then changes nothing, and the solve equals fusion 61's bit for bit."""
     A = laplacian3d(40, 40, 40, seed=21).tolil()
     n = A.shape[0]
-    for i in range(0, n - 40000, 997):
-        A[i, i + 40000] = A[i + 40000, i] = -0.05
+    pairs = ([(i, i + 40000) for i in range(0, n - 40000, 997)] if case == "far" else
+             [(i, i + d) for i in range(0, n - 400, 64) for d in range(2, 402, 10)])
+    for i, j in pairs:
+        A[i, j] = A[j, i] = A[i, j] - 0.05
         A[i, i] += 0.05
-        A[i + 40000, i + 40000] += 0.05
+        A[j, j] += 0.05
     A = sp.csr_matrix(A)
     A.sort_indices()
     b = np.random.default_rng(22).standard_normal(n)
